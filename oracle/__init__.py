@@ -1,0 +1,234 @@
+"""CPU oracle for the gravis force-and-integration path (test infrastructure).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this package, and only as the checker or the timed CPU baseline. The product
+(gravis_amd) never imports it.
+
+The C restatement lives in gravis_oracle.c (reference citations per function
+there); this module is a thin ctypes layer over liboracle.so returning numpy
+arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+BF, BH_REF, BH_LINEAR = 0, 1, 2
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_sz, _d, _u64, _u32, _i = C.c_size_t, C.c_double, C.c_uint64, C.c_uint32, C.c_int
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the Makefile beside this file."""
+    path = os.path.join(_HERE, "liboracle.so")
+    if force or not os.path.exists(path):
+        subprocess.run(["make", "-C", _HERE, "-B" if force else "liboracle.so"],
+                       check=True, stdout=subprocess.DEVNULL)
+    return path
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        L = C.CDLL(build())
+        sig = {
+            "orc_chacha20_block": (None, [_u32p, _u64, _u64, _u32p]),
+            "orc_rng_stream_u64": (None, [_u64, _sz, _u64p]),
+            "orc_rng_seed_key": (None, [_u64, _u32p]),
+            "orc_plummer": (None, [_sz, _d, _d, _u64] + [_dp] * 7),
+            "orc_plummer_suggested_dt": (_d, [_d, _d]),
+            "orc_plummer_suggested_softening": (_d, [_sz, _d]),
+            "orc_two_body": (None, [_d, _d, _d] + [_dp] * 7),
+            "orc_two_body_period": (_d, [_d]),
+            "orc_shift_to_com": (None, [_sz] + [_dp] * 7),
+            "orc_bf_accel": (None, [_sz, _d] + [_dp] * 7),
+            "orc_bf_targets": (None, [_sz, _d] + [_dp] * 4 + [_sz, _i64p] + [_dp] * 4 + [_i]),
+            "orc_bh_accel": (None, [_sz, _d, _d] + [_dp] * 7 + [_i]),
+            "orc_ref_octree_summary": (_sz, [_sz] + [_dp] * 5),
+            "orc_root_box": (None, [_sz, _dp, _dp, _dp, _dp]),
+            "orc_morton63": (None, [_sz, _dp, _dp, _dp, _dp, _u64p]),
+            "orc_sort_keys": (None, [_sz, _u64p, _u64p, _u32p]),
+            "orc_linear_tree": (_sz, [_sz] + [_dp] * 4 + [_u32, _sz, _u64p, _u32p] + [_u32p] * 5
+                                + [_dp, _dp]),
+            "orc_bh_group_accel": (_u64, [_sz, _d, _d, _u32, _u32] + [_dp] * 7 + [_i]),
+            "orc_kdk_steps": (None, [_sz, _i, _d, _d] + [_dp] * 10 + [_d, _u64, _i]),
+            "orc_run": (_sz, [_sz, _i, _d, _d] + [_dp] * 7 + [_d, _u64, _u64, _dp, _sz, _i]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        f = L.orc_diagnostics
+        f.restype = None
+        f.argtypes = [_sz, _d, _i] + [_dp] * 7 + [C.c_void_p, C.c_void_p, _d, _u64, _dp]
+        _LIB = L
+    return _LIB
+
+
+def _z(n):
+    return np.zeros(n, dtype=np.float64)
+
+
+class State:
+    """SoA particle state mirroring sim-core's Particles (particle.rs:21-64),
+    gravity fields only."""
+
+    def __init__(self, n: int):
+        self.n = n
+        for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "m"):
+            setattr(self, k, _z(n))
+
+    def copy(self) -> "State":
+        s = State(self.n)
+        for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "m"):
+            setattr(s, k, getattr(self, k).copy())
+        return s
+
+    def pos(self):
+        return np.stack([self.x, self.y, self.z], axis=1)
+
+    def acc(self):
+        return np.stack([self.ax, self.ay, self.az], axis=1)
+
+
+def chacha20_block(key, counter: int, stream: int = 0) -> np.ndarray:
+    out = np.zeros(16, dtype=np.uint32)
+    lib().orc_chacha20_block(np.ascontiguousarray(key, dtype=np.uint32), counter, stream, out)
+    return out
+
+
+def rng_stream(seed: int, n: int) -> np.ndarray:
+    out = np.zeros(n, dtype=np.uint64)
+    lib().orc_rng_stream_u64(seed, n, out)
+    return out
+
+
+def rng_seed_key(seed: int) -> np.ndarray:
+    out = np.zeros(8, dtype=np.uint32)
+    lib().orc_rng_seed_key(seed, out)
+    return out
+
+
+def plummer(n: int, seed: int = 42, a: float = 1.0, total_mass: float = 1.0) -> State:
+    s = State(n)
+    lib().orc_plummer(n, a, total_mass, seed, s.x, s.y, s.z, s.vx, s.vy, s.vz, s.m)
+    return s
+
+
+def plummer_suggested_dt(a: float = 1.0, total_mass: float = 1.0) -> float:
+    return lib().orc_plummer_suggested_dt(a, total_mass)
+
+
+def plummer_suggested_softening(n: int, a: float = 1.0) -> float:
+    return lib().orc_plummer_suggested_softening(n, a)
+
+
+def two_body(e: float = 0.5, mass_ratio: float = 0.5, a: float = 1.0) -> State:
+    s = State(2)
+    lib().orc_two_body(e, mass_ratio, a, s.x, s.y, s.z, s.vx, s.vy, s.vz, s.m)
+    return s
+
+
+def two_body_period(a: float = 1.0) -> float:
+    return lib().orc_two_body_period(a)
+
+
+def bf_accel(s: State, eps: float) -> None:
+    """Accumulate brute-force accelerations into s.ax/ay/az."""
+    lib().orc_bf_accel(s.n, eps, s.x, s.y, s.z, s.m, s.ax, s.ay, s.az)
+
+
+def bf_targets(s: State, eps: float, targets, nthreads: int = 0):
+    t = np.ascontiguousarray(targets, dtype=np.int64)
+    ax, ay, az, ab = _z(len(t)), _z(len(t)), _z(len(t)), _z(len(t))
+    lib().orc_bf_targets(s.n, eps, s.x, s.y, s.z, s.m, len(t), t, ax, ay, az, ab, nthreads)
+    return np.stack([ax, ay, az], axis=1), ab
+
+
+def bh_accel(s: State, eps: float, theta: float, nthreads: int = 0) -> None:
+    lib().orc_bh_accel(s.n, eps, theta, s.x, s.y, s.z, s.m, s.ax, s.ay, s.az, nthreads)
+
+
+def bh_group_accel(s: State, eps: float, theta: float, leaf: int = 16, group: int = 64,
+                   nthreads: int = 0) -> int:
+    return lib().orc_bh_group_accel(s.n, eps, theta, leaf, group, s.x, s.y, s.z, s.m,
+                                    s.ax, s.ay, s.az, nthreads)
+
+
+def root_box(x, y, z) -> np.ndarray:
+    out = _z(4)
+    lib().orc_root_box(len(x), np.ascontiguousarray(x), np.ascontiguousarray(y),
+                       np.ascontiguousarray(z), out)
+    return out
+
+
+def morton63(x, y, z, box) -> np.ndarray:
+    keys = np.zeros(len(x), dtype=np.uint64)
+    lib().orc_morton63(len(x), np.ascontiguousarray(x), np.ascontiguousarray(y),
+                       np.ascontiguousarray(z), np.ascontiguousarray(box, dtype=np.float64), keys)
+    return keys
+
+
+def sort_keys(keys):
+    keys = np.ascontiguousarray(keys, dtype=np.uint64)
+    sk = np.zeros_like(keys)
+    perm = np.zeros(len(keys), dtype=np.uint32)
+    lib().orc_sort_keys(len(keys), keys, sk, perm)
+    return sk, perm
+
+
+def linear_tree(s: State, leaf: int = 16):
+    """Return dict with the leaf-`leaf` linear octree of s (see gravis_oracle.c)."""
+    n = s.n
+    cap = 2 * n + 64
+    out = {k: np.zeros(cap, dtype=np.uint32) for k in ("level", "first", "cnt", "child", "nchild")}
+    com = np.zeros(4 * cap, dtype=np.float64)
+    sk = np.zeros(n, dtype=np.uint64)
+    perm = np.zeros(n, dtype=np.uint32)
+    box = _z(4)
+    nn = lib().orc_linear_tree(n, s.x, s.y, s.z, s.m, leaf, cap, sk, perm, out["level"],
+                               out["first"], out["cnt"], out["child"], out["nchild"], com, box)
+    assert nn > 0
+    res = {k: v[:nn].copy() for k, v in out.items()}
+    res["com"] = com[: 4 * nn].reshape(nn, 4).copy()
+    res["keys"], res["perm"], res["box"] = sk, perm, box
+    return res
+
+
+def kdk_steps(s: State, dt: float, nsteps: int, eps: float, algo: int = BF,
+              theta: float = 0.5, nthreads: int = 0) -> None:
+    lib().orc_kdk_steps(s.n, algo, eps, theta, s.x, s.y, s.z, s.vx, s.vy, s.vz, s.m,
+                        s.ax, s.ay, s.az, dt, nsteps, nthreads)
+
+
+def run(s: State, dt: float, nsteps: int, eps: float, diag_interval: int, algo: int = BF,
+        theta: float = 0.5, nthreads: int = 0) -> np.ndarray:
+    """headless-style run: returns rows of diagnostics (see diagnostics())."""
+    max_rows = 2 + (nsteps // diag_interval if diag_interval else 0)
+    rows = np.zeros(19 * max_rows, dtype=np.float64)
+    nr = lib().orc_run(s.n, algo, eps, theta, s.x, s.y, s.z, s.vx, s.vy, s.vz, s.m, dt,
+                       nsteps, diag_interval, rows, max_rows, nthreads)
+    return rows[: 19 * nr].reshape(nr, 19)
+
+
+DIAG_FIELDS = ("K", "U", "Eth", "E", "px", "py", "pz", "Lx", "Ly", "Lz", "cx", "cy", "cz",
+               "cvx", "cvy", "cvz", "virial", "time", "step")
+
+
+def diagnostics(s: State, eps: float, with_potential: bool = True, time: float = 0.0,
+                step: int = 0) -> dict:
+    out = _z(19)
+    lib().orc_diagnostics(s.n, eps, 1 if with_potential else 0, s.x, s.y, s.z, s.vx, s.vy,
+                          s.vz, s.m, None, None, time, step, out)
+    return dict(zip(DIAG_FIELDS, out.tolist()))
