@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""Benchmark of the gravis force-and-integration hot path on one MI355X.
+
+Headline workload (the reference-style KDK run the metric is quoted on):
+Plummer sphere N = 262,144 (seed 42, sim-core's ChaCha20 ICs), eps = 0.01,
+dt = 1e-4, leapfrog KDK with mixed-precision brute-force gravity. One "step"
+is one full KDK step (kick+drift, Morton sort, force, kick) with the state
+resident in HBM. value = N(N-1) directed interactions per step / step time.
+
+Also reported: the plain FP64 GPU path on the same state, the mixed-vs-FP64
+RMS/max relative force error, the live roofline of the dominant kernel
+(HIP events on the stream it runs on) and a CPU baseline (the oracle's
+restatement of sim-core's BruteForce, 1 thread, on a bounded sample).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "G-interactions/s at N=256K (mixed vs FP64, err<=1e-6); BH steps/s at N=4M"
+FLOP_PER_INTERACTION = 20.0  # one softened pair force (GPU Gems 3 ch. 31 convention)
+PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector/matrix dense peak (datasheet)
+PEAK_FP64_TFLOPS = 78.6
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=262144)
+    ap.add_argument("--eps", type=float, default=0.01)
+    ap.add_argument("--dt", type=float, default=1e-4)
+    ap.add_argument("--fp64-steps", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=65536)
+    return ap.parse_args()
+
+
+def traffic_from_profiles(name):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+    path = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        d = json.load(open(path))
+        return float(d["hbm_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def cpu_baseline(n_sample, eps):
+    """Oracle restatement of BruteForce::compute_accelerations (gravity.rs:
+    47-80: i<j pairs, Newton's third law, 1 thread) on the first n_sample
+    particles of the same Plummer ICs; one force evaluation."""
+    import oracle as O
+    s = O.plummer(262144, 42)
+    sub = O.State(n_sample)
+    for f in ("x", "y", "z", "m"):
+        setattr(sub, f, getattr(s, f)[:n_sample].copy())
+    t0 = time.perf_counter()
+    O.bf_accel(sub, eps)
+    dt = time.perf_counter() - t0
+    inter = n_sample * (n_sample - 1)
+    return {"value": inter / dt / 1e9, "unit": "G-interactions/s", "cores": 1, "kind": "port",
+            "sample": f"one brute-force evaluation (sim-core i<j algorithm, f64, 1 thread) on the "
+                      f"first {n_sample} particles of the N=262144 Plummer ICs, eps={eps}: "
+                      f"{dt:.2f} s"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU layout is a design in DESIGN.md, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import gravis_amd as g
+
+    n, eps, dt = args.n, args.eps, args.dt
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(dev)  # default priority; every op below runs on it
+    torch.cuda.set_stream(stream)
+    log(f"bench: Plummer N={n} eps={eps} dt={dt}; generating ICs")
+    p = g.scenarios.plummer(n, 42)
+    T = {f: torch.from_numpy(getattr(p, f)).to(dev) for f in ("x", "y", "z", "vx", "vy", "vz",
+                                                              "mass")}
+    for f in ("ax", "ay", "az"):
+        T[f] = torch.zeros(n, dtype=torch.float64, device=dev)
+
+    torch.cuda.synchronize()
+    ctx = g.Context(0)
+    ctx.set_stream(stream.cuda_stream)
+    mixed = g.BruteForce(eps, "mixed", ctx=ctx)
+    fp64 = g.BruteForce(eps, "fp64", ctx=ctx)
+    import ctypes as C
+    L = ctx._L
+
+    def ptrs(keys):
+        return [C.c_void_p(T[k].data_ptr()) for k in keys]
+
+    state_keys = ("x", "y", "z", "vx", "vy", "vz", "mass", "ax", "ay", "az")
+
+    def kdk(solver, steps):
+        d = solver.desc()
+        ctx.check(L.gravis_kdk_device(ctx.handle, C.byref(d), n, *ptrs(state_keys), dt, steps))
+
+    def accel(solver, out):
+        d = solver.desc()
+        ctx.check(L.gravis_accel_device(ctx.handle, C.byref(d), n,
+                                        *[C.c_void_p(T[k].data_ptr()) for k in
+                                          ("x", "y", "z", "mass")],
+                                        *[C.c_void_p(o.data_ptr()) for o in out]))
+
+    # a(t0), as headless does before the loop (main.rs:151-152)
+    accel(mixed, (T["ax"], T["ay"], T["az"]))
+    log(f"bench: warmup {args.warmup} mixed KDK steps")
+    for _ in range(args.warmup):
+        kdk(mixed, 1)
+    torch.cuda.synchronize()
+
+    log(f"bench: timing {args.steps} mixed KDK steps")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        kdk(mixed, 1)
+    torch.cuda.synchronize()
+    t_mixed = (time.perf_counter() - t0) / args.steps
+    inter = float(n) * float(n - 1)
+    value = inter / t_mixed / 1e9
+
+    # Live kernel timing of the dominant kernel (events on the same stream).
+    prof_steps = max(3, min(10, args.steps))
+    ctx.set_profiling(True)
+    for _ in range(prof_steps):
+        kdk(mixed, 1)
+    ctx.set_profiling(False)
+    torch.cuda.synchronize()
+    ms_total, launches = ctx.kernel_time("bf_mixed")
+    k_mixed_s = ms_total / 1e3 / max(1, launches)
+    achieved = FLOP_PER_INTERACTION * inter / k_mixed_s / 1e12
+
+    # Accuracy on the current state: mixed vs FP64 GPU path.
+    am = [torch.zeros(n, dtype=torch.float64, device=dev) for _ in range(3)]
+    a6 = [torch.zeros(n, dtype=torch.float64, device=dev) for _ in range(3)]
+    accel(mixed, am)
+    accel(fp64, a6)
+    torch.cuda.synchronize()
+    A6 = torch.stack(a6, 1)
+    AM = torch.stack(am, 1)
+    m2 = (A6 * A6).sum(1)
+    e2 = ((AM - A6) ** 2).sum(1)
+    ok = m2 >= 1e-30
+    rel2 = e2[ok] / m2[ok]
+    err_rms = float(torch.sqrt(rel2.sum() / n))
+    err_max = float(torch.sqrt(rel2.max()))
+
+    # Plain FP64 GPU path on the same state (its own KDK steps).
+    log(f"bench: timing {args.fp64_steps} FP64 KDK steps")
+    kdk(fp64, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.fp64_steps):
+        kdk(fp64, 1)
+    torch.cuda.synchronize()
+    t_fp64 = (time.perf_counter() - t0) / args.fp64_steps
+    ctx.set_profiling(True)
+    kdk(fp64, 1)
+    ctx.set_profiling(False)
+    ms64, l64 = ctx.kernel_time("bf_fp64")
+    k_fp64_s = ms64 / 1e3 / max(1, l64)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log(f"bench: CPU baseline on {args.cpu_sample} particles")
+        cpu = cpu_baseline(args.cpu_sample, eps)
+
+    traffic = traffic_from_profiles(f"bf_mixed_n{n}")
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "G-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_mixed * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair forces / f64 state",
+        "data": "synthetic: Plummer sphere, sim-core ChaCha20 ICs, seed 42",
+        "config": {"workload": f"Plummer N={n}, eps={eps}, dt={dt}, leapfrog KDK, mixed-precision "
+                               "brute-force gravity (FP32 pair forces on FP64 tile-relative "
+                               "separations, compensated accumulation)",
+                   "n": n, "softening": eps, "dt": dt, "parallelism": "single GPU"},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": traffic, "kernel": "bf_mixed_kernel",
+                     "kernel_ms": round(k_mixed_s * 1e3, 4),
+                     "flop_per_interaction": FLOP_PER_INTERACTION},
+        "cpu_baseline": cpu,
+        "fp64": {"value": round(inter / t_fp64 / 1e9, 3), "unit": "G-interactions/s",
+                 "ms_per_step": round(t_fp64 * 1e3, 4), "kernel_ms": round(k_fp64_s * 1e3, 4),
+                 "achieved_tflops": round(FLOP_PER_INTERACTION * inter / k_fp64_s / 1e12, 3),
+                 "frac_fp64_peak": round(FLOP_PER_INTERACTION * inter / k_fp64_s / 1e12
+                                         / PEAK_FP64_TFLOPS, 4)},
+        "mixed_over_fp64": round(t_fp64 / t_mixed, 3),
+        "force_error_vs_fp64": {"rms": err_rms, "max": err_max},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

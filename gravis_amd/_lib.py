@@ -1,0 +1,101 @@
+"""ctypes binding of libgravis_hip.so (the C ABI declared in include/gravis.h).
+
+Loading fails loudly: there is no CPU fallback anywhere in this package. If
+the shared library is missing, build it with `python -c "import
+__graft_entry__ as g; g.build()"` (or `make -C gravis_amd/csrc`).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgravis_hip.so")
+
+# gravis_status
+OK, INVALID_ARG, HIP_ERROR, OUT_OF_MEMORY, UNSUPPORTED, INTERNAL = range(6)
+# gravis_algorithm / gravis_precision
+BRUTE_FORCE, BARNES_HUT, NO_GRAVITY = 0, 1, 2
+FP64, MIXED = 0, 1
+
+EXPORTS = (
+    "gravis_version", "gravis_status_string", "gravis_create", "gravis_destroy",
+    "gravis_last_error", "gravis_set_stream", "gravis_synchronize", "gravis_set_profiling",
+    "gravis_kernel_time", "gravis_accel", "gravis_accel_device", "gravis_kdk",
+    "gravis_kdk_device", "gravis_diagnostics", "gravis_diagnostics_device",
+    "gravis_morton_sort", "gravis_scenario_plummer", "gravis_plummer_suggested_dt",
+    "gravis_plummer_suggested_softening", "gravis_scenario_two_body", "gravis_two_body_period",
+)
+
+
+class Solver(C.Structure):
+    _fields_ = [("algorithm", C.c_int32), ("precision", C.c_int32), ("softening", C.c_double),
+                ("theta", C.c_double), ("leaf_size", C.c_uint32), ("group_size", C.c_uint32)]
+
+
+class Diag(C.Structure):
+    _fields_ = [("time", C.c_double), ("step", C.c_uint64), ("kinetic_energy", C.c_double),
+                ("potential_energy", C.c_double), ("total_energy", C.c_double),
+                ("momentum", C.c_double * 3), ("angular_momentum", C.c_double * 3),
+                ("center_of_mass", C.c_double * 3), ("center_of_mass_velocity", C.c_double * 3),
+                ("virial_ratio", C.c_double), ("thermal_energy", C.c_double)]
+
+
+_LIB = None
+
+
+class GravisError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"gravis status {status}: {message}")
+        self.status = status
+
+
+def load() -> C.CDLL:
+    """Load the HIP library (raises if it is not built)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is not built; run make -C gravis_amd/csrc "
+                          "(this package has no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    vp, dp, sz, d, i32 = C.c_void_p, C.POINTER(C.c_double), C.c_size_t, C.c_double, C.c_int
+    st = C.c_int
+    spec = {
+        "gravis_version": (C.c_char_p, []),
+        "gravis_status_string": (C.c_char_p, [st]),
+        "gravis_create": (st, [i32, C.POINTER(vp)]),
+        "gravis_destroy": (None, [vp]),
+        "gravis_last_error": (C.c_char_p, [vp]),
+        "gravis_set_stream": (st, [vp, vp]),
+        "gravis_synchronize": (st, [vp]),
+        "gravis_set_profiling": (st, [vp, i32]),
+        "gravis_kernel_time": (st, [vp, C.c_char_p, dp, C.POINTER(C.c_uint64)]),
+        "gravis_accel": (st, [vp, C.POINTER(Solver), sz] + [vp] * 7),
+        "gravis_accel_device": (st, [vp, C.POINTER(Solver), sz] + [vp] * 7),
+        "gravis_kdk": (st, [vp, C.POINTER(Solver), sz] + [vp] * 10 + [d, C.c_uint64]),
+        "gravis_kdk_device": (st, [vp, C.POINTER(Solver), sz] + [vp] * 10 + [d, C.c_uint64]),
+        "gravis_diagnostics": (st, [vp, sz, d, i32] + [vp] * 9 + [d, C.c_uint64,
+                                                                  C.POINTER(Diag)]),
+        "gravis_diagnostics_device": (st, [vp, sz, d, i32] + [vp] * 9 + [d, C.c_uint64,
+                                                                         C.POINTER(Diag)]),
+        "gravis_morton_sort": (st, [vp, sz, vp, vp, vp, vp, vp, vp]),
+        "gravis_scenario_plummer": (st, [sz, d, d, C.c_uint64] + [vp] * 7),
+        "gravis_plummer_suggested_dt": (d, [d, d]),
+        "gravis_plummer_suggested_softening": (d, [sz, d]),
+        "gravis_scenario_two_body": (st, [d, d, d] + [vp] * 7),
+        "gravis_two_body_period": (d, [d]),
+    }
+    for name, (res, args) in spec.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def check(ctx, status: int) -> None:
+    if status != OK:
+        L = load()
+        msg = L.gravis_last_error(ctx).decode() if ctx else ""
+        raise GravisError(status, msg or L.gravis_status_string(status).decode())

@@ -1,0 +1,429 @@
+// api.hip -- the extern "C" boundary (include/gravis.h) over the gfx950
+// kernels. Validates arguments the way the reference's CLI and asserts do
+// (crates/headless/src/main.rs:73-95, octree.rs:16), converts every failure
+// into a gravis_status plus a message, and never throws across the ABI.
+#include <cstdarg>
+#include <cstring>
+#include <new>
+
+#include "internal.h"
+
+namespace gravis {
+
+gravis_status fail(Ctx* c, gravis_status s, const char* fmt, ...) {
+    if (c) {
+        char msg[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(msg, sizeof msg, fmt, ap);
+        va_end(ap);
+        c->err = msg;
+    }
+    return s;
+}
+
+gravis_status buffer(Ctx* c, const char* name, size_t bytes, void** out) {
+    if (bytes == 0) bytes = 256;
+    DevBuf& b = c->bufs[name];
+    if (b.bytes < bytes) {
+        if (b.p) {
+            // The old buffer may still be read by queued work.
+            hipError_t e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess)
+                return fail(c, GRAVIS_HIP_ERROR, "sync before realloc: %s", hipGetErrorString(e));
+            (void)hipFree(b.p);
+            b.p = nullptr;
+            b.bytes = 0;
+        }
+        size_t want = bytes + bytes / 8;  // headroom for growing N
+        hipError_t e = hipMalloc(&b.p, want);
+        if (e != hipSuccess) {
+            b.p = nullptr;
+            (void)hipGetLastError();
+            return fail(c, GRAVIS_OUT_OF_MEMORY, "hipMalloc(%zu) for %s: %s", want, name,
+                        hipGetErrorString(e));
+        }
+        b.bytes = want;
+    }
+    *out = b.p;
+    return GRAVIS_OK;
+}
+
+gravis_status check_launch(Ctx* c, const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess)
+        return fail(c, GRAVIS_HIP_ERROR, "%s launch failed: %s", what, hipGetErrorString(e));
+    return GRAVIS_OK;
+}
+
+void timer_begin(Ctx* c) {
+    if (c->profiling) (void)hipEventRecord(c->ev0, c->stream);
+}
+
+void timer_end(Ctx* c, const char* name) {
+    if (!c->profiling) return;
+    (void)hipEventRecord(c->ev1, c->stream);
+    if (hipEventSynchronize(c->ev1) == hipSuccess) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev0, c->ev1) == hipSuccess) {
+            KernelTimer& t = c->timers[name];
+            t.ms += ms;
+            t.launches += 1;
+        }
+    }
+}
+
+}  // namespace gravis
+
+using gravis::Ctx;
+using gravis::fail;
+
+struct gravis_ctx {
+    Ctx c;
+};
+
+namespace {
+
+gravis_status check_solver(Ctx* c, const gravis_solver* s) {
+    if (!s) return fail(c, GRAVIS_INVALID_ARG, "solver is NULL");
+    if (!(s->softening >= 0.0))
+        return fail(c, GRAVIS_INVALID_ARG, "softening must be non-negative, got %g", s->softening);
+    if (s->algorithm != GRAVIS_BRUTE_FORCE && s->algorithm != GRAVIS_BARNES_HUT &&
+        s->algorithm != GRAVIS_NO_GRAVITY)
+        return fail(c, GRAVIS_INVALID_ARG, "unknown algorithm %d", s->algorithm);
+    if (s->precision != GRAVIS_FP64 && s->precision != GRAVIS_MIXED)
+        return fail(c, GRAVIS_INVALID_ARG, "unknown precision %d", s->precision);
+    if (s->algorithm == GRAVIS_BARNES_HUT && !(s->theta >= 0.0))
+        return fail(c, GRAVIS_INVALID_ARG, "theta must be non-negative, got %g", s->theta);
+    return GRAVIS_OK;
+}
+
+bool any_null(std::initializer_list<const void*> ps) {
+    for (const void* p : ps)
+        if (!p) return true;
+    return false;
+}
+
+gravis_status force_device(Ctx* c, const gravis_solver* s, size_t n, const double* x,
+                           const double* y, const double* z, const double* m,
+                           const gravis::FinishArgs& fa) {
+    switch (s->algorithm) {
+        case GRAVIS_BRUTE_FORCE:
+            return gravis::bf_force(c, s, n, x, y, z, m, fa);
+        case GRAVIS_BARNES_HUT:
+            return gravis::bh_force(c, s, n, x, y, z, m, fa);
+        default:
+            if (fa.mode == gravis::kAssignKick) {
+                GRAVIS_HIP(c, hipMemsetAsync(fa.ax, 0, n * sizeof(double), c->stream));
+                GRAVIS_HIP(c, hipMemsetAsync(fa.ay, 0, n * sizeof(double), c->stream));
+                GRAVIS_HIP(c, hipMemsetAsync(fa.az, 0, n * sizeof(double), c->stream));
+            }
+            return GRAVIS_OK;
+    }
+}
+
+struct HostState {
+    double *x, *y, *z, *vx, *vy, *vz, *m, *ax, *ay, *az;
+};
+
+gravis_status stage(Ctx* c, size_t n, HostState* d) {
+    double* base;
+    GRAVIS_TRY(gravis::buf(c, "host_state", 10 * n, &base));
+    d->x = base; d->y = base + n; d->z = base + 2 * n;
+    d->vx = base + 3 * n; d->vy = base + 4 * n; d->vz = base + 5 * n;
+    d->m = base + 6 * n;
+    d->ax = base + 7 * n; d->ay = base + 8 * n; d->az = base + 9 * n;
+    return GRAVIS_OK;
+}
+
+gravis_status h2d(Ctx* c, double* d, const double* h, size_t n) {
+    GRAVIS_HIP(c, hipMemcpyAsync(d, h, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    return GRAVIS_OK;
+}
+
+gravis_status d2h(Ctx* c, double* h, const double* d, size_t n) {
+    GRAVIS_HIP(c, hipMemcpyAsync(h, d, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    return GRAVIS_OK;
+}
+
+void fill_diag(const double r[19], double time, uint64_t step, gravis_diag* out) {
+    out->time = time;
+    out->step = step;
+    out->kinetic_energy = r[0];
+    out->potential_energy = r[1];
+    out->thermal_energy = r[2];
+    out->total_energy = r[3];
+    for (int d = 0; d < 3; ++d) {
+        out->momentum[d] = r[4 + d];
+        out->angular_momentum[d] = r[7 + d];
+        out->center_of_mass[d] = r[10 + d];
+        out->center_of_mass_velocity[d] = r[13 + d];
+    }
+    out->virial_ratio = r[16];
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gravis_version(void) { return "gravis-mi355x 0.1 (gfx950)"; }
+
+const char* gravis_status_string(gravis_status s) {
+    switch (s) {
+        case GRAVIS_OK: return "ok";
+        case GRAVIS_INVALID_ARG: return "invalid argument";
+        case GRAVIS_HIP_ERROR: return "HIP error";
+        case GRAVIS_OUT_OF_MEMORY: return "out of device memory";
+        case GRAVIS_UNSUPPORTED: return "unsupported";
+        case GRAVIS_INTERNAL: return "internal capacity exceeded";
+    }
+    return "unknown status";
+}
+
+gravis_status gravis_create(int device, gravis_ctx** out) {
+    if (!out) return GRAVIS_INVALID_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+        (void)hipGetLastError();
+        return GRAVIS_HIP_ERROR;
+    }
+    if (device < 0 || device >= count) return GRAVIS_INVALID_ARG;
+    if (hipSetDevice(device) != hipSuccess) return GRAVIS_HIP_ERROR;
+    gravis_ctx* g = new (std::nothrow) gravis_ctx();
+    if (!g) return GRAVIS_OUT_OF_MEMORY;
+    Ctx* c = &g->c;
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        c->num_cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete g;
+        return GRAVIS_HIP_ERROR;
+    }
+    c->stream = c->own_stream;
+    *out = g;
+    return GRAVIS_OK;
+}
+
+void gravis_destroy(gravis_ctx* g) {
+    if (!g) return;
+    Ctx* c = &g->c;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    if (c->stream != c->own_stream) (void)hipStreamSynchronize(c->own_stream);
+    for (auto& kv : c->bufs)
+        if (kv.second.p) (void)hipFree(kv.second.p);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+    delete g;
+}
+
+const char* gravis_last_error(const gravis_ctx* g) { return g ? g->c.err.c_str() : "no context"; }
+
+gravis_status gravis_set_stream(gravis_ctx* g, void* stream) {
+    if (!g) return GRAVIS_INVALID_ARG;
+    g->c.stream = stream ? static_cast<hipStream_t>(stream) : g->c.own_stream;
+    return GRAVIS_OK;
+}
+
+gravis_status gravis_synchronize(gravis_ctx* g) {
+    if (!g) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    GRAVIS_HIP(c, hipStreamSynchronize(c->stream));
+    return GRAVIS_OK;
+}
+
+gravis_status gravis_set_profiling(gravis_ctx* g, int enable) {
+    if (!g) return GRAVIS_INVALID_ARG;
+    g->c.profiling = enable != 0;
+    return GRAVIS_OK;
+}
+
+gravis_status gravis_kernel_time(gravis_ctx* g, const char* name, double* ms_total,
+                                 uint64_t* launches) {
+    if (!g || !name || !ms_total || !launches) return GRAVIS_INVALID_ARG;
+    auto it = g->c.timers.find(name);
+    if (it == g->c.timers.end()) {
+        *ms_total = 0.0;
+        *launches = 0;
+        return GRAVIS_OK;
+    }
+    *ms_total = it->second.ms;
+    *launches = it->second.launches;
+    g->c.timers.erase(it);
+    return GRAVIS_OK;
+}
+
+gravis_status gravis_accel_device(gravis_ctx* g, const gravis_solver* s, size_t n,
+                                  const double* x, const double* y, const double* z,
+                                  const double* m, double* ax, double* ay, double* az) {
+    if (!g) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    GRAVIS_TRY(check_solver(c, s));
+    if (n == 0) return GRAVIS_OK;  // barnes_hut.rs:141-143: empty system, nothing to do
+    if (any_null({x, y, z, m, ax, ay, az})) return fail(c, GRAVIS_INVALID_ARG, "null array");
+    GRAVIS_HIP(c, hipSetDevice(c->device));
+    gravis::FinishArgs fa;
+    fa.mode = gravis::kAccumulate;
+    fa.ax = ax; fa.ay = ay; fa.az = az;
+    return force_device(c, s, n, x, y, z, m, fa);
+}
+
+gravis_status gravis_accel(gravis_ctx* g, const gravis_solver* s, size_t n, const double* x,
+                           const double* y, const double* z, const double* m, double* ax,
+                           double* ay, double* az) {
+    if (!g) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    GRAVIS_TRY(check_solver(c, s));
+    if (n == 0) return GRAVIS_OK;
+    if (any_null({x, y, z, m, ax, ay, az})) return fail(c, GRAVIS_INVALID_ARG, "null array");
+    GRAVIS_HIP(c, hipSetDevice(c->device));
+    HostState d;
+    GRAVIS_TRY(stage(c, n, &d));
+    GRAVIS_TRY(h2d(c, d.x, x, n)); GRAVIS_TRY(h2d(c, d.y, y, n)); GRAVIS_TRY(h2d(c, d.z, z, n));
+    GRAVIS_TRY(h2d(c, d.m, m, n));
+    GRAVIS_TRY(h2d(c, d.ax, ax, n)); GRAVIS_TRY(h2d(c, d.ay, ay, n)); GRAVIS_TRY(h2d(c, d.az, az, n));
+    gravis::FinishArgs fa;
+    fa.mode = gravis::kAccumulate;
+    fa.ax = d.ax; fa.ay = d.ay; fa.az = d.az;
+    GRAVIS_TRY(force_device(c, s, n, d.x, d.y, d.z, d.m, fa));
+    GRAVIS_TRY(d2h(c, ax, d.ax, n)); GRAVIS_TRY(d2h(c, ay, d.ay, n)); GRAVIS_TRY(d2h(c, az, d.az, n));
+    GRAVIS_HIP(c, hipStreamSynchronize(c->stream));
+    return GRAVIS_OK;
+}
+
+gravis_status gravis_kdk_device(gravis_ctx* g, const gravis_solver* s, size_t n, double* x,
+                                double* y, double* z, double* vx, double* vy, double* vz,
+                                const double* m, double* ax, double* ay, double* az, double dt,
+                                uint64_t nsteps) {
+    if (!g) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    GRAVIS_TRY(check_solver(c, s));
+    if (!(dt > 0.0)) return fail(c, GRAVIS_INVALID_ARG, "dt must be positive, got %g", dt);
+    if (n == 0 || nsteps == 0) return GRAVIS_OK;
+    if (any_null({x, y, z, vx, vy, vz, m, ax, ay, az}))
+        return fail(c, GRAVIS_INVALID_ARG, "null array");
+    GRAVIS_HIP(c, hipSetDevice(c->device));
+    const double h = 0.5 * dt;
+    gravis::FinishArgs fa;
+    fa.mode = gravis::kAssignKick;
+    fa.ax = ax; fa.ay = ay; fa.az = az;
+    fa.vx = vx; fa.vy = vy; fa.vz = vz;
+    fa.half_dt = h;
+    for (uint64_t step = 0; step < nsteps; ++step) {
+        GRAVIS_TRY(gravis::kick_drift(c, n, x, y, z, vx, vy, vz, ax, ay, az, h, dt));
+        GRAVIS_TRY(force_device(c, s, n, x, y, z, m, fa));
+    }
+    return GRAVIS_OK;
+}
+
+gravis_status gravis_kdk(gravis_ctx* g, const gravis_solver* s, size_t n, double* x, double* y,
+                         double* z, double* vx, double* vy, double* vz, const double* m,
+                         double* ax, double* ay, double* az, double dt, uint64_t nsteps) {
+    if (!g) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    GRAVIS_TRY(check_solver(c, s));
+    if (!(dt > 0.0)) return fail(c, GRAVIS_INVALID_ARG, "dt must be positive, got %g", dt);
+    if (n == 0 || nsteps == 0) return GRAVIS_OK;
+    if (any_null({x, y, z, vx, vy, vz, m, ax, ay, az}))
+        return fail(c, GRAVIS_INVALID_ARG, "null array");
+    GRAVIS_HIP(c, hipSetDevice(c->device));
+    HostState d;
+    GRAVIS_TRY(stage(c, n, &d));
+    const double* hin[10] = {x, y, z, vx, vy, vz, m, ax, ay, az};
+    double* din[10] = {d.x, d.y, d.z, d.vx, d.vy, d.vz, d.m, d.ax, d.ay, d.az};
+    for (int k = 0; k < 10; ++k) GRAVIS_TRY(h2d(c, din[k], hin[k], n));
+    GRAVIS_TRY(gravis_kdk_device(g, s, n, d.x, d.y, d.z, d.vx, d.vy, d.vz, d.m, d.ax, d.ay,
+                                 d.az, dt, nsteps));
+    double* hout[9] = {x, y, z, vx, vy, vz, ax, ay, az};
+    double* dout[9] = {d.x, d.y, d.z, d.vx, d.vy, d.vz, d.ax, d.ay, d.az};
+    for (int k = 0; k < 9; ++k) GRAVIS_TRY(d2h(c, hout[k], dout[k], n));
+    GRAVIS_HIP(c, hipStreamSynchronize(c->stream));
+    return GRAVIS_OK;
+}
+
+gravis_status gravis_diagnostics_device(gravis_ctx* g, size_t n, double softening,
+                                        int compute_potential, const double* x,
+                                        const double* y, const double* z, const double* vx,
+                                        const double* vy, const double* vz, const double* m,
+                                        const double* u, const uint8_t* ptype, double time,
+                                        uint64_t step, gravis_diag* out) {
+    if (!g || !out) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    if (!(softening >= 0.0)) return fail(c, GRAVIS_INVALID_ARG, "softening must be >= 0");
+    double r[19] = {0};
+    if (n > 0) {
+        if (any_null({x, y, z, vx, vy, vz, m})) return fail(c, GRAVIS_INVALID_ARG, "null array");
+        GRAVIS_HIP(c, hipSetDevice(c->device));
+        GRAVIS_TRY(gravis::diag_device(c, n, softening, compute_potential, x, y, z, vx, vy, vz,
+                                       m, u, ptype, r));
+    }
+    fill_diag(r, time, step, out);
+    return GRAVIS_OK;
+}
+
+gravis_status gravis_diagnostics(gravis_ctx* g, size_t n, double softening,
+                                 int compute_potential, const double* x, const double* y,
+                                 const double* z, const double* vx, const double* vy,
+                                 const double* vz, const double* m, const double* u,
+                                 const uint8_t* ptype, double time, uint64_t step,
+                                 gravis_diag* out) {
+    if (!g || !out) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    if (!(softening >= 0.0)) return fail(c, GRAVIS_INVALID_ARG, "softening must be >= 0");
+    if (n == 0) {
+        double r[19] = {0};
+        fill_diag(r, time, step, out);
+        return GRAVIS_OK;
+    }
+    if (any_null({x, y, z, vx, vy, vz, m})) return fail(c, GRAVIS_INVALID_ARG, "null array");
+    GRAVIS_HIP(c, hipSetDevice(c->device));
+    HostState d;
+    GRAVIS_TRY(stage(c, n, &d));
+    const double* hin[7] = {x, y, z, vx, vy, vz, m};
+    double* din[7] = {d.x, d.y, d.z, d.vx, d.vy, d.vz, d.m};
+    for (int k = 0; k < 7; ++k) GRAVIS_TRY(h2d(c, din[k], hin[k], n));
+    double* du = nullptr;
+    uint8_t* dt = nullptr;
+    if (u && ptype) {
+        GRAVIS_TRY(gravis::buf(c, "host_u", n, &du));
+        GRAVIS_TRY(gravis::buf(c, "host_type", n, &dt));
+        GRAVIS_TRY(h2d(c, du, u, n));
+        GRAVIS_HIP(c, hipMemcpyAsync(dt, ptype, n, hipMemcpyHostToDevice, c->stream));
+    }
+    double r[19] = {0};
+    GRAVIS_TRY(gravis::diag_device(c, n, softening, compute_potential, d.x, d.y, d.z, d.vx, d.vy,
+                                   d.vz, d.m, du, dt, r));
+    fill_diag(r, time, step, out);
+    return GRAVIS_OK;
+}
+
+gravis_status gravis_morton_sort(gravis_ctx* g, size_t n, const double* x, const double* y,
+                                 const double* z, uint64_t* keys_sorted, uint32_t* perm,
+                                 double* box_out) {
+    if (!g) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    if (n == 0) return fail(c, GRAVIS_INVALID_ARG, "n must be > 0 (octree.rs:16)");
+    if (any_null({x, y, z, keys_sorted, perm, box_out}))
+        return fail(c, GRAVIS_INVALID_ARG, "null array");
+    GRAVIS_HIP(c, hipSetDevice(c->device));
+    HostState d;
+    GRAVIS_TRY(stage(c, n, &d));
+    GRAVIS_TRY(h2d(c, d.x, x, n)); GRAVIS_TRY(h2d(c, d.y, y, n)); GRAVIS_TRY(h2d(c, d.z, z, n));
+    double* box;
+    GRAVIS_TRY(gravis::buf(c, "ms_box", 4, &box));
+    GRAVIS_TRY(gravis::root_box(c, n, d.x, d.y, d.z, box));
+    uint64_t* k;
+    uint32_t* p;
+    GRAVIS_TRY(gravis::morton_sort(c, n, d.x, d.y, d.z, box, &k, &p));
+    GRAVIS_HIP(c, hipMemcpyAsync(keys_sorted, k, n * 8, hipMemcpyDeviceToHost, c->stream));
+    GRAVIS_HIP(c, hipMemcpyAsync(perm, p, n * 4, hipMemcpyDeviceToHost, c->stream));
+    GRAVIS_HIP(c, hipMemcpyAsync(box_out, box, 32, hipMemcpyDeviceToHost, c->stream));
+    GRAVIS_HIP(c, hipStreamSynchronize(c->stream));
+    return GRAVIS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,375 @@
+// bf.hip -- brute-force gravity on gfx950 (FP64 and mixed precision).
+//
+// Reference: BruteForce::compute_accelerations (crates/sim-core/src/
+// gravity.rs:47-80): a_i = sum_{j != i} G m_j d_ij / (|d_ij|^2 + eps^2)^(3/2),
+// d_ij = x_j - x_i, accumulated into the caller's acceleration arrays.
+//
+// The reference visits pairs i<j once (Newton's third law, sequential). Here
+// every target i gathers over all j (no write conflicts, no atomics): a
+// workgroup owns BLOCK*IPT targets of the Morton-sorted particle list and a
+// contiguous j range; j records are staged through LDS in TJ-wide tiles and
+// read back as wavefront-uniform broadcasts. Pairwise gravity is not a dense
+// contraction, so this is VALU work, not MFMA work. The j range is split in
+// S chunks so the grid holds many workgroups per CU; the chunk partials are
+// summed in a fixed order by the finishing kernel (deterministic).
+//
+// Mixed precision: separations are formed in FP64 relative to the centre of
+// the workgroup's target box, rounded to FP32, and the pair force is FP32
+// (v_rsq_f32). Each thread sums a tile of TJ pair forces in FP32 and adds the
+// tile sum into a Kahan-compensated FP32 accumulator; the result leaves as
+// FP64 sum - compensation.
+#include "internal.h"
+
+namespace gravis {
+namespace {
+
+// Pair force factor in FP32: m * r^-3 with r^2 = d.d + eps^2.
+template <bool MASK>
+__device__ __forceinline__ float pair_f32(float dx, float dy, float dz, float m, float eps2) {
+    const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+    float ri = __builtin_amdgcn_rsqf(r2);
+    if (MASK) ri = r2 > 0.0f ? ri : 0.0f;
+    const float mr = m * ri;
+    const float ri2 = ri * ri;
+    return mr * ri2;
+}
+
+template <bool MASK>
+__device__ __forceinline__ double pair_f64(double dx, double dy, double dz, double m,
+                                           double eps2) {
+    const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, eps2)));
+    double ri = rsqrt(r2);
+    if (MASK) ri = r2 > 0.0 ? ri : 0.0;
+    const double mr = m * ri;
+    const double ri2 = ri * ri;
+    return mr * ri2;
+}
+
+__device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
+    const float y = v - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// Block-wide bounding-box centre of the targets (FP64).
+template <int BLOCK, int IPT>
+__device__ __forceinline__ void block_centre(const double (&x)[IPT], const double (&y)[IPT],
+                                             const double (&z)[IPT], double& cx, double& cy,
+                                             double& cz) {
+    double lo[3] = {x[0], y[0], z[0]}, hi[3] = {x[0], y[0], z[0]};
+#pragma unroll
+    for (int k = 1; k < IPT; ++k) {
+        lo[0] = fmin(lo[0], x[k]); hi[0] = fmax(hi[0], x[k]);
+        lo[1] = fmin(lo[1], y[k]); hi[1] = fmax(hi[1], y[k]);
+        lo[2] = fmin(lo[2], z[k]); hi[2] = fmax(hi[2], z[k]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], __shfl_xor(lo[d], off));
+            hi[d] = fmax(hi[d], __shfl_xor(hi[d], off));
+        }
+    }
+    __shared__ double s[BLOCK / kWave][6];
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0)
+        for (int d = 0; d < 3; ++d) { s[w][d] = lo[d]; s[w][3 + d] = hi[d]; }
+    __syncthreads();
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = s[0][d];
+        hi[d] = s[0][3 + d];
+    }
+    for (int k = 1; k < BLOCK / kWave; ++k)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], s[k][d]);
+            hi[d] = fmax(hi[d], s[k][3 + d]);
+        }
+    cx = 0.5 * (lo[0] + hi[0]);
+    cy = 0.5 * (lo[1] + hi[1]);
+    cz = 0.5 * (lo[2] + hi[2]);
+}
+
+template <int BLOCK, int IPT, int TJ, bool MASK>
+__global__ __launch_bounds__(BLOCK) void bf_mixed_kernel(const double4* __restrict__ pm, int n,
+                                                         int jchunk, float eps2,
+                                                         double* __restrict__ part) {
+    static_assert(TJ % BLOCK == 0, "tile must be a multiple of the block");
+    constexpr int LPT = TJ / BLOCK;  // j records each thread stages per tile
+    __shared__ float4 tile[2][TJ];
+
+    const int ibase = blockIdx.x * BLOCK * IPT;
+    double xd[IPT], yd[IPT], zd[IPT];
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+        int i = ibase + k * BLOCK + threadIdx.x;
+        i = i < n ? i : ibase;  // out-of-range targets mirror a valid one
+        const double4 p = pm[i];
+        xd[k] = p.x; yd[k] = p.y; zd[k] = p.z;
+    }
+    double cx, cy, cz;
+    block_centre<BLOCK, IPT>(xd, yd, zd, cx, cy, cz);
+    float xi[IPT], yi[IPT], zi[IPT];
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+        xi[k] = (float)(xd[k] - cx);
+        yi[k] = (float)(yd[k] - cy);
+        zi[k] = (float)(zd[k] - cz);
+    }
+    float sx[IPT], sy[IPT], sz[IPT], kx[IPT], ky[IPT], kz[IPT];
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) { sx[k] = sy[k] = sz[k] = 0.f; kx[k] = ky[k] = kz[k] = 0.f; }
+
+    const int j0 = blockIdx.y * jchunk;
+    const int j1 = min(n, j0 + jchunk);
+    const int ntiles = (j1 - j0 + TJ - 1) / TJ;
+
+    float4 stage[LPT];
+    auto load = [&](int t) {
+#pragma unroll
+        for (int e = 0; e < LPT; ++e) {
+            const int j = j0 + t * TJ + e * BLOCK + threadIdx.x;
+            if (j < j1) {
+                const double4 p = pm[j];
+                stage[e] = make_float4((float)(p.x - cx), (float)(p.y - cy), (float)(p.z - cz),
+                                       (float)p.w);
+            } else {
+                stage[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+        }
+    };
+    auto store = [&](int b) {
+#pragma unroll
+        for (int e = 0; e < LPT; ++e) tile[b][e * BLOCK + threadIdx.x] = stage[e];
+    };
+
+    if (ntiles > 0) {
+        load(0);
+        store(0);
+    }
+    __syncthreads();
+    for (int t = 0; t < ntiles; ++t) {
+        const int cur = t & 1;
+        if (t + 1 < ntiles) load(t + 1);
+        float ax[IPT], ay[IPT], az[IPT];
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) ax[k] = ay[k] = az[k] = 0.f;
+#pragma unroll 8
+        for (int j = 0; j < TJ; ++j) {
+            const float4 s = tile[cur][j];
+#pragma unroll
+            for (int k = 0; k < IPT; ++k) {
+                const float dx = s.x - xi[k], dy = s.y - yi[k], dz = s.z - zi[k];
+                const float f = pair_f32<MASK>(dx, dy, dz, s.w, eps2);
+                ax[k] = __builtin_fmaf(f, dx, ax[k]);
+                ay[k] = __builtin_fmaf(f, dy, ay[k]);
+                az[k] = __builtin_fmaf(f, dz, az[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) {
+            kahan_add(sx[k], kx[k], ax[k]);
+            kahan_add(sy[k], ky[k], ay[k]);
+            kahan_add(sz[k], kz[k], az[k]);
+        }
+        if (t + 1 < ntiles) store(cur ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+        const int i = ibase + k * BLOCK + threadIdx.x;
+        if (i < n) {
+            const size_t base = (size_t)blockIdx.y * 3 * n;
+            part[base + i] = (double)sx[k] - (double)kx[k];
+            part[base + n + i] = (double)sy[k] - (double)ky[k];
+            part[base + 2 * (size_t)n + i] = (double)sz[k] - (double)kz[k];
+        }
+    }
+}
+
+template <int BLOCK, int IPT, int TJ, bool MASK>
+__global__ __launch_bounds__(BLOCK) void bf_fp64_kernel(const double4* __restrict__ pm, int n,
+                                                        int jchunk, double eps2,
+                                                        double* __restrict__ part) {
+    static_assert(TJ % BLOCK == 0, "tile must be a multiple of the block");
+    constexpr int LPT = TJ / BLOCK;
+    __shared__ double4 tile[2][TJ];
+
+    const int ibase = blockIdx.x * BLOCK * IPT;
+    double xi[IPT], yi[IPT], zi[IPT], sx[IPT], sy[IPT], sz[IPT];
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+        int i = ibase + k * BLOCK + threadIdx.x;
+        i = i < n ? i : ibase;
+        const double4 p = pm[i];
+        xi[k] = p.x; yi[k] = p.y; zi[k] = p.z;
+        sx[k] = sy[k] = sz[k] = 0.0;
+    }
+    const int j0 = blockIdx.y * jchunk;
+    const int j1 = min(n, j0 + jchunk);
+    const int ntiles = (j1 - j0 + TJ - 1) / TJ;
+    // Padding records: zero mass placed far away (no contribution, r^2 > 0).
+    const double4 pad = make_double4(1e150, 1e150, 1e150, 0.0);
+
+    double4 stage[LPT];
+    auto load = [&](int t) {
+#pragma unroll
+        for (int e = 0; e < LPT; ++e) {
+            const int j = j0 + t * TJ + e * BLOCK + threadIdx.x;
+            stage[e] = j < j1 ? pm[j] : pad;
+        }
+    };
+    auto store = [&](int b) {
+#pragma unroll
+        for (int e = 0; e < LPT; ++e) tile[b][e * BLOCK + threadIdx.x] = stage[e];
+    };
+    if (ntiles > 0) {
+        load(0);
+        store(0);
+    }
+    __syncthreads();
+    for (int t = 0; t < ntiles; ++t) {
+        const int cur = t & 1;
+        if (t + 1 < ntiles) load(t + 1);
+        double ax[IPT], ay[IPT], az[IPT];
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) ax[k] = ay[k] = az[k] = 0.0;
+#pragma unroll 4
+        for (int j = 0; j < TJ; ++j) {
+            const double4 s = tile[cur][j];
+#pragma unroll
+            for (int k = 0; k < IPT; ++k) {
+                const double dx = s.x - xi[k], dy = s.y - yi[k], dz = s.z - zi[k];
+                const double f = pair_f64<MASK>(dx, dy, dz, s.w, eps2);
+                ax[k] = __builtin_fma(f, dx, ax[k]);
+                ay[k] = __builtin_fma(f, dy, ay[k]);
+                az[k] = __builtin_fma(f, dz, az[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) { sx[k] += ax[k]; sy[k] += ay[k]; sz[k] += az[k]; }
+        if (t + 1 < ntiles) store(cur ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < IPT; ++k) {
+        const int i = ibase + k * BLOCK + threadIdx.x;
+        if (i < n) {
+            const size_t base = (size_t)blockIdx.y * 3 * n;
+            part[base + i] = sx[k];
+            part[base + n + i] = sy[k];
+            part[base + 2 * (size_t)n + i] = sz[k];
+        }
+    }
+}
+
+// Sum the S chunk partials in chunk order (G = 1) and deliver in the
+// caller's particle order.
+__global__ __launch_bounds__(256) void bf_finish_kernel(const double* __restrict__ part,
+                                                        int nchunks, int n,
+                                                        const uint32_t* __restrict__ perm,
+                                                        FinishArgs fa) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double a[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double s = 0.0;
+        for (int q = 0; q < nchunks; ++q) s += part[((size_t)q * 3 + c) * n + k];
+        a[c] = s;
+    }
+    const uint32_t o = perm ? perm[k] : (uint32_t)k;
+    if (fa.mode == kAccumulate) {
+        fa.ax[o] += a[0];
+        fa.ay[o] += a[1];
+        fa.az[o] += a[2];
+    } else {
+        fa.ax[o] = a[0];
+        fa.ay[o] = a[1];
+        fa.az[o] = a[2];
+        fa.vx[o] = fa.vx[o] + a[0] * fa.half_dt;
+        fa.vy[o] = fa.vy[o] + a[1] * fa.half_dt;
+        fa.vz[o] = fa.vz[o] + a[2] * fa.half_dt;
+    }
+}
+
+constexpr size_t kSortMinN = 2048;
+
+template <int BLOCK, int IPT, int TJ>
+void launch_mixed(dim3 g, hipStream_t st, bool mask, const double4* pm, int n, int jchunk,
+                  float eps2, double* part) {
+    if (mask)
+        bf_mixed_kernel<BLOCK, IPT, TJ, true><<<g, BLOCK, 0, st>>>(pm, n, jchunk, eps2, part);
+    else
+        bf_mixed_kernel<BLOCK, IPT, TJ, false><<<g, BLOCK, 0, st>>>(pm, n, jchunk, eps2, part);
+}
+
+template <int BLOCK, int IPT, int TJ>
+void launch_fp64(dim3 g, hipStream_t st, bool mask, const double4* pm, int n, int jchunk,
+                 double eps2, double* part) {
+    if (mask)
+        bf_fp64_kernel<BLOCK, IPT, TJ, true><<<g, BLOCK, 0, st>>>(pm, n, jchunk, eps2, part);
+    else
+        bf_fp64_kernel<BLOCK, IPT, TJ, false><<<g, BLOCK, 0, st>>>(pm, n, jchunk, eps2, part);
+}
+
+}  // namespace
+
+gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x,
+                       const double* y, const double* z, const double* m, const FinishArgs& fa) {
+    if (n == 0) return GRAVIS_OK;
+    if (n > 0x7FFFFFFFull) return fail(c, GRAVIS_INVALID_ARG, "n too large");
+    const int ni = (int)n;
+    const bool mixed = s->precision == GRAVIS_MIXED;
+    // Mixed precision forms separations relative to each workgroup's target
+    // box, so targets are Morton sorted to make those boxes compact. FP64
+    // needs no ordering and runs on the caller's order.
+    uint32_t* perm = nullptr;
+    if (mixed && n >= kSortMinN) {
+        double* box;
+        GRAVIS_TRY(buf(c, "bf_box", 4, &box));
+        GRAVIS_TRY(root_box(c, n, x, y, z, box));
+        uint64_t* keys;
+        GRAVIS_TRY(morton_sort(c, n, x, y, z, box, &keys, &perm));
+    }
+    double4* pm;
+    GRAVIS_TRY(buf(c, "bf_pm", n, &pm));
+    GRAVIS_TRY(gather_sorted(c, n, x, y, z, m, perm, pm));
+
+    const int block = 256;
+    const int ipt = mixed ? 2 : 1;
+    const int tj = 256;
+    const int gx = (ni + block * ipt - 1) / (block * ipt);
+    // Enough workgroups for 8 per CU, but at least one tile per chunk.
+    const int target = 8 * c->num_cus;
+    int S = (target + gx - 1) / gx;
+    const int max_s = (ni + tj - 1) / tj;
+    if (S > max_s) S = max_s;
+    if (S < 1) S = 1;
+    int jchunk = (ni + S - 1) / S;
+    jchunk = (jchunk + tj - 1) / tj * tj;
+    S = (ni + jchunk - 1) / jchunk;
+
+    double* part;
+    GRAVIS_TRY(buf(c, "bf_part", (size_t)S * 3 * n, &part));
+    const dim3 grid(gx, S);
+    const double eps2 = s->softening * s->softening;
+    timer_begin(c);
+    if (mixed) {
+        const float eps2f = (float)eps2;
+        const bool mask = !(eps2f >= 1.17549435e-38f);
+        launch_mixed<256, 2, 256>(grid, c->stream, mask, pm, ni, jchunk, eps2f, part);
+        timer_end(c, "bf_mixed");
+    } else {
+        const bool mask = !(eps2 > 0.0);
+        launch_fp64<256, 1, 256>(grid, c->stream, mask, pm, ni, jchunk, eps2, part);
+        timer_end(c, "bf_fp64");
+    }
+    GRAVIS_TRY(check_launch(c, "bf kernel"));
+    bf_finish_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(part, S, ni, perm, fa);
+    return check_launch(c, "bf_finish");
+}
+
+}  // namespace gravis

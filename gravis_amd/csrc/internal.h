@@ -1,0 +1,139 @@
+// internal.h -- shared definitions of the gfx950 engine (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/gravis.h"
+
+namespace gravis {
+
+constexpr int kWave = 64;  // CDNA wavefront
+
+// Device buffer that only grows; freed with the context.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+struct KernelTimer {
+    double ms = 0.0;
+    uint64_t launches = 0;
+};
+
+// One Barnes-Hut tree arena (device), kept between calls as sim-core keeps
+// its Octree in BarnesHut::tree_cache (barnes_hut.rs:27,146-156).
+struct TreeArena {
+    size_t node_cap = 0;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::map<std::string, DevBuf> bufs;
+    bool profiling = false;
+    std::map<std::string, KernelTimer> timers;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int num_cus = 256;
+};
+
+// Status helpers ----------------------------------------------------------
+gravis_status fail(Ctx* c, gravis_status s, const char* fmt, ...);
+
+#define GRAVIS_HIP(ctx, call)                                                             \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            return ::gravis::fail((ctx), GRAVIS_HIP_ERROR, "%s failed: %s (%s:%d)", #call, \
+                                  hipGetErrorString(e_), __FILE__, __LINE__);             \
+    } while (0)
+
+#define GRAVIS_TRY(expr)                  \
+    do {                                  \
+        gravis_status s_ = (expr);        \
+        if (s_ != GRAVIS_OK) return s_;   \
+    } while (0)
+
+// Returns a device buffer of at least `bytes` under `name` (grow only).
+gravis_status buffer(Ctx* c, const char* name, size_t bytes, void** out);
+
+template <typename T>
+inline gravis_status buf(Ctx* c, const char* name, size_t count, T** out) {
+    void* p = nullptr;
+    gravis_status s = buffer(c, name, count * sizeof(T), &p);
+    *out = static_cast<T*>(p);
+    return s;
+}
+
+// Kernel launch check (launch errors are reported by hipGetLastError).
+gravis_status check_launch(Ctx* c, const char* what);
+
+// Optional event bracket around the dominant kernel of a call.
+void timer_begin(Ctx* c);
+void timer_end(Ctx* c, const char* name);
+
+// ---- spatial sort (sort.hip) --------------------------------------------
+// Root cube (octree.rs:14-44) of n positions, written to d_box[4] =
+// (cx, cy, cz, half_width) on the device.
+gravis_status root_box(Ctx* c, size_t n, const double* x, const double* y, const double* z,
+                       double* d_box);
+// 63-bit Morton keys over d_box and the stable sort permutation.
+// Outputs: d_keys_sorted[n], d_perm[n] (sorted position -> original index).
+gravis_status morton_sort(Ctx* c, size_t n, const double* x, const double* y, const double* z,
+                          const double* d_box, uint64_t** d_keys_sorted, uint32_t** d_perm);
+// Gather x,y,z,m into double4 records in d_perm order (identity if null).
+gravis_status gather_sorted(Ctx* c, size_t n, const double* x, const double* y,
+                            const double* z, const double* m, const uint32_t* d_perm,
+                            double4* d_out);
+
+// ---- forces (bf.hip, bh.hip) --------------------------------------------
+// How the finishing kernel writes the result in the caller's order.
+enum FinishMode : int {
+    kAccumulate = 0,  // a += F  (GravitySolver contract)
+    kAssignKick = 1,  // a = F; v += a * half_dt  (second half-kick of KDK)
+};
+
+struct FinishArgs {
+    int mode = kAccumulate;
+    double* ax = nullptr;
+    double* ay = nullptr;
+    double* az = nullptr;
+    double* vx = nullptr;
+    double* vy = nullptr;
+    double* vz = nullptr;
+    double half_dt = 0.0;
+};
+
+// Brute-force force on device SoA arrays, result delivered per FinishArgs.
+gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x,
+                       const double* y, const double* z, const double* m, const FinishArgs& fa);
+
+// Barnes-Hut force (linear octree, group walk).
+gravis_status bh_force(Ctx* c, const gravis_solver* s, size_t n, const double* x,
+                       const double* y, const double* z, const double* m, const FinishArgs& fa);
+
+// ---- integrator / diagnostics -------------------------------------------
+gravis_status kick_drift(Ctx* c, size_t n, double* x, double* y, double* z, double* vx,
+                         double* vy, double* vz, const double* ax, const double* ay,
+                         const double* az, double half_dt, double dt);
+
+gravis_status diag_device(Ctx* c, size_t n, double eps, int with_potential, const double* x,
+                          const double* y, const double* z, const double* vx, const double* vy,
+                          const double* vz, const double* m, const double* u,
+                          const uint8_t* ptype, double out[19]);
+
+inline unsigned grid_for(size_t n, unsigned block, unsigned cap = 1u << 30) {
+    size_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return static_cast<unsigned>(g);
+}
+
+}  // namespace gravis

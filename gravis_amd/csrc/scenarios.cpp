@@ -1,0 +1,201 @@
+// scenarios.cpp -- initial-condition generators of sim-core, host side.
+//
+// Random streams reproduce ChaCha20Rng::seed_from_u64 (rand_chacha 0.9.0 /
+// rand_core 0.9.5) and rand 0.9.2's f64 conversions, so a given seed gives
+// the reference's particles:
+//   - seed expansion: PCG32 (XSH-RR) words fill the 256-bit key;
+//   - ChaCha20 block function (20 rounds), 64-bit block counter from 0,
+//     stream id 0, output consumed as consecutive little-endian u32 words,
+//     u64 = low word | high word << 32;
+//   - random::<f64>()      = (u64 >> 11) * 2^-53;
+//   - random_range(lo..hi) = ((u64 >> 12 | 1.0 bits) - 1) * (hi - lo) + lo,
+//     redrawn with a one-ulp smaller scale if the result reaches hi.
+// Scenario code: scenarios/plummer_sphere.rs:13-152, scenarios/two_body.rs:
+// 49-105, particle.rs:189-226 (COM shift). Compiled without contraction so
+// the arithmetic rounds as the Rust code does.
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+
+#include "../../include/gravis.h"
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+
+class ChaChaStream {
+  public:
+    explicit ChaChaStream(uint64_t seed) {
+        // rand_core seed_from_u64
+        uint64_t st = seed;
+        for (int i = 0; i < 8; ++i) {
+            st = st * 6364136223846793005ull + 11634580027462260723ull;
+            const uint32_t xs = static_cast<uint32_t>(((st >> 18) ^ st) >> 27);
+            const uint32_t rot = static_cast<uint32_t>(st >> 59);
+            key_[i] = (xs >> rot) | (xs << ((32u - rot) & 31u));
+        }
+    }
+
+    uint64_t next_u64() {
+        const uint64_t lo = next_u32();
+        const uint64_t hi = next_u32();
+        return lo | (hi << 32);
+    }
+
+    double uniform01() { return static_cast<double>(next_u64() >> 11) * 0x1.0p-53; }
+
+    double range(double lo, double hi) {
+        double scale = hi - lo;
+        for (;;) {
+            const uint64_t bits = (next_u64() >> 12) | 0x3FF0000000000000ull;
+            double one_two;
+            std::memcpy(&one_two, &bits, sizeof one_two);
+            const double res = (one_two - 1.0) * scale + lo;
+            if (res < hi) return res;
+            uint64_t sb;
+            std::memcpy(&sb, &scale, sizeof sb);
+            --sb;
+            std::memcpy(&scale, &sb, sizeof sb);
+        }
+    }
+
+  private:
+    static uint32_t rotl(uint32_t v, int s) { return (v << s) | (v >> (32 - s)); }
+
+    static void quarter(uint32_t* w, int a, int b, int c, int d) {
+        w[a] += w[b]; w[d] = rotl(w[d] ^ w[a], 16);
+        w[c] += w[d]; w[b] = rotl(w[b] ^ w[c], 12);
+        w[a] += w[b]; w[d] = rotl(w[d] ^ w[a], 8);
+        w[c] += w[d]; w[b] = rotl(w[b] ^ w[c], 7);
+    }
+
+    void block(uint64_t counter, uint32_t* out) const {
+        uint32_t in[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u};
+        for (int i = 0; i < 8; ++i) in[4 + i] = key_[i];
+        in[12] = static_cast<uint32_t>(counter);
+        in[13] = static_cast<uint32_t>(counter >> 32);
+        in[14] = 0u;
+        in[15] = 0u;
+        uint32_t w[16];
+        std::memcpy(w, in, sizeof w);
+        for (int r = 0; r < 10; ++r) {
+            quarter(w, 0, 4, 8, 12); quarter(w, 1, 5, 9, 13);
+            quarter(w, 2, 6, 10, 14); quarter(w, 3, 7, 11, 15);
+            quarter(w, 0, 5, 10, 15); quarter(w, 1, 6, 11, 12);
+            quarter(w, 2, 7, 8, 13); quarter(w, 3, 4, 9, 14);
+        }
+        for (int i = 0; i < 16; ++i) out[i] = w[i] + in[i];
+    }
+
+    uint32_t next_u32() {
+        if (pos_ == 16) {
+            block(counter_++, words_);
+            pos_ = 0;
+        }
+        return words_[pos_++];
+    }
+
+    uint32_t key_[8];
+    uint32_t words_[16] = {};
+    uint64_t counter_ = 0;
+    int pos_ = 16;
+};
+
+void com_shift(size_t n, double* x, double* y, double* z, double* vx, double* vy, double* vz,
+               const double* m) {
+    double mt = 0.0;
+    for (size_t i = 0; i < n; ++i) mt += m[i];
+    if (mt == 0.0) return;
+    double c[6] = {0, 0, 0, 0, 0, 0};
+    for (size_t i = 0; i < n; ++i) {
+        c[0] += m[i] * x[i]; c[1] += m[i] * y[i]; c[2] += m[i] * z[i];
+        c[3] += m[i] * vx[i]; c[4] += m[i] * vy[i]; c[5] += m[i] * vz[i];
+    }
+    for (double& v : c) v /= mt;
+    for (size_t i = 0; i < n; ++i) {
+        x[i] -= c[0]; y[i] -= c[1]; z[i] -= c[2];
+        vx[i] -= c[3]; vy[i] -= c[4]; vz[i] -= c[5];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+gravis_status gravis_scenario_plummer(size_t n, double scale_radius, double total_mass,
+                                      uint64_t seed, double* x, double* y, double* z,
+                                      double* vx, double* vy, double* vz, double* m) {
+    if (n == 0 || !(scale_radius > 0.0) || !(total_mass > 0.0)) return GRAVIS_INVALID_ARG;
+    if (!x || !y || !z || !vx || !vy || !vz || !m) return GRAVIS_INVALID_ARG;
+    ChaChaStream rng(seed);
+    const double a = scale_radius;
+    const double mp = total_mass / static_cast<double>(n);
+    for (size_t i = 0; i < n; ++i) {
+        // position: inverse CDF of M(<r) = r^3 / (r^2 + a^2)^(3/2)
+        const double xm = rng.range(0.001, 1.0);
+        const double r = a / std::sqrt(std::pow(xm, -2.0 / 3.0) - 1.0);
+        const double ct = rng.range(-1.0, 1.0);
+        const double st = std::sqrt(1.0 - ct * ct);
+        const double ph = rng.range(0.0, 2.0 * kPi);
+        x[i] = r * st * std::cos(ph);
+        y[i] = r * st * std::sin(ph);
+        z[i] = r * ct;
+        // speed: rejection from q^2 (1 - q^2)^(7/2) with bound 0.1
+        const double vesc = std::sqrt(2.0 * 1.0 * total_mass / std::sqrt(r * r + a * a));
+        double q;
+        for (;;) {
+            q = rng.range(0.0, 1.0);
+            const double gq = q * q * std::pow(1.0 - q * q, 3.5);
+            if (rng.uniform01() < gq / 0.1) break;
+        }
+        const double v = q * vesc;
+        const double cv = rng.range(-1.0, 1.0);
+        const double sv = std::sqrt(1.0 - cv * cv);
+        const double pv = rng.range(0.0, 2.0 * kPi);
+        vx[i] = v * sv * std::cos(pv);
+        vy[i] = v * sv * std::sin(pv);
+        vz[i] = v * cv;
+        m[i] = mp;
+    }
+    com_shift(n, x, y, z, vx, vy, vz, m);
+    return GRAVIS_OK;
+}
+
+double gravis_plummer_suggested_dt(double scale_radius, double total_mass) {
+    const double a3 = scale_radius * scale_radius * scale_radius;
+    return std::sqrt(a3 / (1.0 * total_mass)) / 100.0;
+}
+
+double gravis_plummer_suggested_softening(size_t n, double scale_radius) {
+    const double rh = 1.305 * scale_radius;
+    const double spacing = std::pow(4.0 / 3.0 * kPi * (rh * rh * rh) / static_cast<double>(n),
+                                    1.0 / 3.0);
+    return spacing * 0.5;
+}
+
+gravis_status gravis_scenario_two_body(double e, double mass_ratio, double a, double* x,
+                                       double* y, double* z, double* vx, double* vy,
+                                       double* vz, double* m) {
+    if (!(e >= 0.0 && e < 1.0) || !(a > 0.0) || !(mass_ratio > 0.0 && mass_ratio < 1.0))
+        return GRAVIS_INVALID_ARG;
+    if (!x || !y || !z || !vx || !vy || !vz || !m) return GRAVIS_INVALID_ARG;
+    const double mt = 1.0;
+    const double m1 = mt * mass_ratio, m2 = mt * (1.0 - mass_ratio);
+    const double rp = a * (1.0 - e);
+    const double vp = std::sqrt(1.0 * mt * (2.0 / rp - 1.0 / a));
+    x[0] = -m2 / mt * rp;
+    x[1] = m1 / mt * rp;
+    y[0] = y[1] = z[0] = z[1] = 0.0;
+    vx[0] = vx[1] = vz[0] = vz[1] = 0.0;
+    vy[0] = -m2 / mt * vp;
+    vy[1] = m1 / mt * vp;
+    m[0] = m1;
+    m[1] = m2;
+    return GRAVIS_OK;
+}
+
+double gravis_two_body_period(double a) {
+    return 2.0 * kPi * std::sqrt(a * a * a / (1.0 * 1.0));
+}
+
+}  // extern "C"

@@ -1,0 +1,298 @@
+// sort.hip -- root cube, 63-bit Morton keys and a stable LSD radix sort.
+//
+// The root cube restates BoundingBox::containing (crates/sim-core/src/
+// octree.rs:14-44) bit for bit: min/max are exact and the cube arithmetic
+// runs in one thread in the same operation order with contraction off.
+// The key quantises each axis to 21 bits over that cube with the x/y/z bit
+// order of BoundingBox::octant (octree.rs:67-79), so key prefixes are the
+// octree cells level by level.
+//
+// The radix sort is stable (LSD, 8-bit digits, 8 passes): equal keys keep
+// their input order, so the permutation is bit-exact with any stable CPU
+// sort of the same keys.
+#include "internal.h"
+
+namespace gravis {
+namespace {
+
+constexpr int kBoxBlock = 256;
+constexpr int kBoxMaxBlocks = 1024;
+
+__global__ __launch_bounds__(kBoxBlock) void box_partial_kernel(const double* __restrict__ x,
+                                                                const double* __restrict__ y,
+                                                                const double* __restrict__ z,
+                                                                size_t n,
+                                                                double* __restrict__ part) {
+    double mn[3] = {INFINITY, INFINITY, INFINITY};
+    double mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (size_t i = blockIdx.x * (size_t)kBoxBlock + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * kBoxBlock) {
+        double p[3] = {x[i], y[i], z[i]};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = fmin(mn[d], p[d]);
+            mx[d] = fmax(mx[d], p[d]);
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = fmin(mn[d], __shfl_xor(mn[d], off));
+            mx[d] = fmax(mx[d], __shfl_xor(mx[d], off));
+        }
+    }
+    __shared__ double s[kBoxBlock / kWave][6];
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        for (int d = 0; d < 3; ++d) {
+            s[w][d] = mn[d];
+            s[w][3 + d] = mx[d];
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < kBoxBlock / kWave; ++k)
+            for (int d = 0; d < 3; ++d) {
+                s[0][d] = fmin(s[0][d], s[k][d]);
+                s[0][3 + d] = fmax(s[0][3 + d], s[k][3 + d]);
+            }
+        for (int d = 0; d < 6; ++d) part[blockIdx.x * 6 + d] = s[0][d];
+    }
+}
+
+__global__ void box_final_kernel(const double* __restrict__ part, int nparts,
+                                 double* __restrict__ box) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double mn[3] = {INFINITY, INFINITY, INFINITY};
+    double mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int b = 0; b < nparts; ++b)
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = fmin(mn[d], part[b * 6 + d]);
+            mx[d] = fmax(mx[d], part[b * 6 + 3 + d]);
+        }
+    // octree.rs:29-41, same operation order (no contraction in this build).
+    box[0] = 0.5 * (mn[0] + mx[0]);
+    box[1] = 0.5 * (mn[1] + mx[1]);
+    box[2] = 0.5 * (mn[2] + mx[2]);
+    double ext = fmax(fmax(mx[0] - mn[0], mx[1] - mn[1]), mx[2] - mn[2]);
+    box[3] = 0.5 * ext * 1.001 + 1e-10;
+}
+
+__device__ __forceinline__ uint64_t spread3(uint32_t v) {
+    uint64_t x = v & 0x1FFFFFu;
+    x = (x | (x << 32)) & 0x1F00000000FFFFull;
+    x = (x | (x << 16)) & 0x1F0000FF0000FFull;
+    x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+    x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t quantise(double v, double lo, double inv) {
+    double q = (v - lo) * inv;
+    if (!(q >= 0.0)) return 0u;
+    if (q >= 2097151.0) return 2097151u;
+    return static_cast<uint32_t>(q);
+}
+
+__global__ __launch_bounds__(256) void morton_kernel(const double* __restrict__ x,
+                                                     const double* __restrict__ y,
+                                                     const double* __restrict__ z, size_t n,
+                                                     const double* __restrict__ box,
+                                                     uint64_t* __restrict__ keys,
+                                                     uint32_t* __restrict__ vals) {
+    const double hw = box[3];
+    const double inv = 2097152.0 / (2.0 * hw);
+    const double lx = box[0] - hw, ly = box[1] - hw, lz = box[2] - hw;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        uint32_t qx = quantise(x[i], lx, inv), qy = quantise(y[i], ly, inv),
+                 qz = quantise(z[i], lz, inv);
+        keys[i] = spread3(qx) | (spread3(qy) << 1) | (spread3(qz) << 2);
+        vals[i] = static_cast<uint32_t>(i);
+    }
+}
+
+// ---- radix sort -----------------------------------------------------------
+constexpr int kRsBlock = 256;               // 4 wavefronts
+constexpr int kRsWaves = kRsBlock / kWave;  // 4
+constexpr int kRsRounds = 16;               // 64 keys per wave per round
+constexpr int kRsWaveTile = kWave * kRsRounds;      // 1024
+constexpr int kRsTile = kRsWaveTile * kRsWaves;     // 4096 keys per block
+constexpr int kRsDigits = 256;
+
+__global__ __launch_bounds__(kRsBlock) void radix_hist_kernel(const uint64_t* __restrict__ keys,
+                                                              size_t n, int shift,
+                                                              uint32_t* __restrict__ counts,
+                                                              int nblocks) {
+    __shared__ uint32_t h[kRsDigits];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const size_t base = (size_t)blockIdx.x * kRsTile;
+    for (int e = threadIdx.x; e < kRsTile; e += kRsBlock) {
+        size_t i = base + e;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    counts[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of m counts in one block of 1024 threads.
+__global__ __launch_bounds__(1024) void radix_scan_kernel(uint32_t* __restrict__ v, size_t m) {
+    __shared__ uint32_t s[1024];
+    const size_t per = (m + 1023) / 1024;
+    const size_t b = threadIdx.x * per;
+    const size_t e = b + per < m ? b + per : m;
+    uint32_t sum = 0;
+    for (size_t i = b; i < e; ++i) sum += v[i];
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        uint32_t t = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t run = s[threadIdx.x] - sum;  // exclusive
+    for (size_t i = b; i < e; ++i) {
+        uint32_t c = v[i];
+        v[i] = run;
+        run += c;
+    }
+}
+
+__device__ __forceinline__ uint64_t digit_match(uint32_t d, uint64_t active) {
+    uint64_t mask = active;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bal = __ballot(bit);
+        mask &= bit ? bal : ~bal;
+    }
+    return mask;
+}
+
+__global__ __launch_bounds__(kRsBlock) void radix_scatter_kernel(
+    const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, size_t n, int shift,
+    const uint32_t* __restrict__ offsets, int nblocks, uint64_t* __restrict__ kout,
+    uint32_t* __restrict__ vout) {
+    __shared__ uint32_t cnt[kRsWaves][kRsDigits];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = threadIdx.x / kWave;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (int e = threadIdx.x; e < kRsWaves * kRsDigits; e += kRsBlock) (&cnt[0][0])[e] = 0u;
+    __syncthreads();
+
+    const size_t wbase = (size_t)blockIdx.x * kRsTile + (size_t)w * kRsWaveTile;
+    uint64_t key[kRsRounds];
+    uint32_t dig[kRsRounds];
+#pragma unroll
+    for (int r = 0; r < kRsRounds; ++r) {
+        const size_t i = wbase + (size_t)r * kWave + lane;
+        key[r] = i < n ? kin[i] : 0ull;
+        dig[r] = static_cast<uint32_t>((key[r] >> shift) & 255u);
+    }
+    // Phase A: per-wave digit counts (each digit has one leader lane per round).
+#pragma unroll
+    for (int r = 0; r < kRsRounds; ++r) {
+        const size_t i = wbase + (size_t)r * kWave + lane;
+        const uint64_t active = __ballot(i < n);
+        const uint64_t mask = digit_match(dig[r], active);
+        if (i < n && (mask & lt) == 0ull) cnt[w][dig[r]] += (uint32_t)__popcll(mask);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // Phase B: global base of each (wave, digit): offsets + earlier waves.
+    {
+        const int d = threadIdx.x;
+        uint32_t run = offsets[(size_t)d * nblocks + blockIdx.x];
+        for (int k = 0; k < kRsWaves; ++k) {
+            const uint32_t c = cnt[k][d];
+            cnt[k][d] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+    // Phase C: stable ranks in element order.
+#pragma unroll
+    for (int r = 0; r < kRsRounds; ++r) {
+        const size_t i = wbase + (size_t)r * kWave + lane;
+        const uint64_t active = __ballot(i < n);
+        const uint64_t mask = digit_match(dig[r], active);
+        uint32_t pos = 0;
+        if (i < n) {
+            pos = cnt[w][dig[r]] + (uint32_t)__popcll(mask & lt);
+            kout[pos] = key[r];
+            vout[pos] = vin[i];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        if (i < n && (mask & lt) == 0ull) cnt[w][dig[r]] += (uint32_t)__popcll(mask);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+__global__ __launch_bounds__(256) void gather_kernel(const double* __restrict__ x,
+                                                     const double* __restrict__ y,
+                                                     const double* __restrict__ z,
+                                                     const double* __restrict__ m,
+                                                     const uint32_t* __restrict__ perm, size_t n,
+                                                     double4* __restrict__ out) {
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+         k += (size_t)gridDim.x * blockDim.x) {
+        const size_t p = perm ? perm[k] : k;
+        out[k] = make_double4(x[p], y[p], z[p], m[p]);
+    }
+}
+
+}  // namespace
+
+gravis_status root_box(Ctx* c, size_t n, const double* x, const double* y, const double* z,
+                       double* d_box) {
+    double* part = nullptr;
+    GRAVIS_TRY(buf(c, "box_part", kBoxMaxBlocks * 6, &part));
+    const unsigned g = grid_for(n, kBoxBlock, kBoxMaxBlocks);
+    box_partial_kernel<<<g, kBoxBlock, 0, c->stream>>>(x, y, z, n, part);
+    box_final_kernel<<<1, 64, 0, c->stream>>>(part, (int)g, d_box);
+    return check_launch(c, "root_box");
+}
+
+gravis_status morton_sort(Ctx* c, size_t n, const double* x, const double* y, const double* z,
+                          const double* d_box, uint64_t** d_keys_sorted, uint32_t** d_perm) {
+    if (n > 0xFFFFFFFFull) return fail(c, GRAVIS_INVALID_ARG, "n too large for 32-bit indices");
+    uint64_t *k0, *k1;
+    uint32_t *v0, *v1, *counts;
+    const int nblocks = (int)((n + kRsTile - 1) / kRsTile);
+    const size_t m = (size_t)kRsDigits * nblocks;
+    GRAVIS_TRY(buf(c, "sort_k0", n, &k0));
+    GRAVIS_TRY(buf(c, "sort_k1", n, &k1));
+    GRAVIS_TRY(buf(c, "sort_v0", n, &v0));
+    GRAVIS_TRY(buf(c, "sort_v1", n, &v1));
+    GRAVIS_TRY(buf(c, "sort_counts", m, &counts));
+    morton_kernel<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(x, y, z, n, d_box, k0, v0);
+    for (int pass = 0; pass < 8; ++pass) {
+        const int shift = 8 * pass;
+        radix_hist_kernel<<<nblocks, kRsBlock, 0, c->stream>>>(k0, n, shift, counts, nblocks);
+        radix_scan_kernel<<<1, 1024, 0, c->stream>>>(counts, m);
+        radix_scatter_kernel<<<nblocks, kRsBlock, 0, c->stream>>>(k0, v0, n, shift, counts,
+                                                                  nblocks, k1, v1);
+        std::swap(k0, k1);
+        std::swap(v0, v1);
+    }
+    // 8 passes: the result is back in sort_k0/sort_v0.
+    *d_keys_sorted = k0;
+    *d_perm = v0;
+    return check_launch(c, "morton_sort");
+}
+
+gravis_status gather_sorted(Ctx* c, size_t n, const double* x, const double* y,
+                            const double* z, const double* m, const uint32_t* d_perm,
+                            double4* d_out) {
+    gather_kernel<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(x, y, z, m, d_perm, n, d_out);
+    return check_launch(c, "gather_sorted");
+}
+
+}  // namespace gravis

@@ -1,0 +1,181 @@
+/*
+ * gravis.h -- C ABI of the MI355X (gfx950) force-and-integration engine.
+ *
+ * This is the drop-in boundary for sim-core's hot path. Every entry point
+ * is extern "C", takes plain pointers and sizes, returns a gravis_status and
+ * never aborts or throws across the ABI. The reference interface each entry
+ * replaces is cited beside it (paths relative to the gravis repository).
+ *
+ * Conventions
+ *   - Particle arrays are structure-of-arrays f64, as sim-core's Particles
+ *     (crates/sim-core/src/particle.rs:21-64); no alignment requirement.
+ *   - Plain entry points take HOST pointers borrowed for the call; the
+ *     *_device variants take device pointers and run on the context's
+ *     stream (gravis_set_stream) without synchronising.
+ *   - Accelerations follow GravitySolver's accumulate contract
+ *     (crates/sim-core/src/gravity.rs:6-10): results are ADDED to ax/ay/az;
+ *     the caller zeroes them first.
+ *   - A context is used by one host thread at a time (sim-core guards its
+ *     Barnes-Hut tree cache with a Mutex, barnes_hut.rs:27,146).
+ *   - G = 1 (crates/sim-core/src/units.rs:11).
+ */
+#ifndef GRAVIS_H
+#define GRAVIS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gravis_status {
+    GRAVIS_OK = 0,
+    GRAVIS_INVALID_ARG = 1,   /* bad size, null pointer, eps < 0, dt <= 0 ... */
+    GRAVIS_HIP_ERROR = 2,     /* a HIP runtime call failed (see gravis_last_error) */
+    GRAVIS_OUT_OF_MEMORY = 3, /* device allocation failed */
+    GRAVIS_UNSUPPORTED = 4,   /* valid request this build does not implement */
+    GRAVIS_INTERNAL = 5       /* device-side capacity exceeded (e.g. walk frontier) */
+} gravis_status;
+
+/* --algorithm (crates/headless/src/main.rs:61-67,114-121) */
+typedef enum gravis_algorithm {
+    GRAVIS_BRUTE_FORCE = 0, /* gravity.rs:36-80 */
+    GRAVIS_BARNES_HUT = 1,  /* barnes_hut.rs:21-192 (linear-octree form) */
+    GRAVIS_NO_GRAVITY = 2   /* gravity.rs:13-19 */
+} gravis_algorithm;
+
+typedef enum gravis_precision {
+    GRAVIS_FP64 = 0,  /* f64 throughout, as sim-core */
+    GRAVIS_MIXED = 1  /* FP32 pair forces on FP64 tile-relative separations,
+                         compensated FP32 accumulation, FP64 state */
+} gravis_precision;
+
+/* Solver description: BruteForce::new(softening) (gravity.rs:41-45) or
+ * BarnesHut::new(softening, theta) (barnes_hut.rs:32-40). */
+typedef struct gravis_solver {
+    int32_t algorithm;   /* gravis_algorithm */
+    int32_t precision;   /* gravis_precision */
+    double softening;    /* Plummer softening eps >= 0 */
+    double theta;        /* Barnes-Hut opening angle >= 0 */
+    uint32_t leaf_size;  /* Barnes-Hut leaf capacity (0 = 16) */
+    uint32_t group_size; /* Barnes-Hut target group (0 = 64, one wavefront) */
+} gravis_solver;
+
+/* Diagnostics (crates/sim-core/src/diagnostics.rs:5-30). */
+typedef struct gravis_diag {
+    double time;
+    uint64_t step;
+    double kinetic_energy;
+    double potential_energy;
+    double total_energy;
+    double momentum[3];
+    double angular_momentum[3];
+    double center_of_mass[3];
+    double center_of_mass_velocity[3];
+    double virial_ratio;
+    double thermal_energy;
+} gravis_diag;
+
+typedef struct gravis_ctx gravis_ctx;
+
+/* Context: owns device buffers, the stream, the cached tree arena (the
+ * analogue of BarnesHut::tree_cache, barnes_hut.rs:27) and tuning choices. */
+gravis_status gravis_create(int device, gravis_ctx** out);
+void gravis_destroy(gravis_ctx* ctx);
+const char* gravis_last_error(const gravis_ctx* ctx);
+const char* gravis_status_string(gravis_status s);
+/* Stream for the *_device entry points (a hipStream_t; NULL = the
+ * context's own stream). */
+gravis_status gravis_set_stream(gravis_ctx* ctx, void* stream);
+gravis_status gravis_synchronize(gravis_ctx* ctx);
+/* Library version string. */
+const char* gravis_version(void);
+
+/* GravitySolver::compute_accelerations (gravity.rs:6-10, :47-80;
+ * barnes_hut.rs:139-192). Accumulates into ax/ay/az. Host pointers. */
+gravis_status gravis_accel(gravis_ctx* ctx, const gravis_solver* solver, size_t n,
+                           const double* x, const double* y, const double* z,
+                           const double* m, double* ax, double* ay, double* az);
+
+/* Same with device pointers, asynchronous on the context stream. */
+gravis_status gravis_accel_device(gravis_ctx* ctx, const gravis_solver* solver, size_t n,
+                                  const double* x, const double* y, const double* z,
+                                  const double* m, double* ax, double* ay, double* az);
+
+/* nsteps of LeapfrogKDK::step (integrator.rs:41-77) with the given solver:
+ * v += a dt/2; x += v dt; a = F(x); v += a dt/2. On entry a must hold
+ * a(t) (as after sim-core's initial clear + compute_accelerations,
+ * headless/src/main.rs:151-152); on return it holds a(t + nsteps dt).
+ * Host pointers; state is device resident across the nsteps. */
+gravis_status gravis_kdk(gravis_ctx* ctx, const gravis_solver* solver, size_t n, double* x,
+                         double* y, double* z, double* vx, double* vy, double* vz,
+                         const double* m, double* ax, double* ay, double* az, double dt,
+                         uint64_t nsteps);
+
+/* Same with device pointers, asynchronous on the context stream. */
+gravis_status gravis_kdk_device(gravis_ctx* ctx, const gravis_solver* solver, size_t n,
+                                double* x, double* y, double* z, double* vx, double* vy,
+                                double* vz, const double* m, double* ax, double* ay, double* az,
+                                double dt, uint64_t nsteps);
+
+/* diagnostics::compute / compute_fast (diagnostics.rs:37-48): FP64.
+ * compute_potential = 0 gives compute_fast (U, E_pot and virial = 0).
+ * u and ptype may be NULL (no gas: thermal energy 0); ptype 4 = gas
+ * (particle.rs:4-10). Host pointers. */
+gravis_status gravis_diagnostics(gravis_ctx* ctx, size_t n, double softening,
+                                 int compute_potential, const double* x, const double* y,
+                                 const double* z, const double* vx, const double* vy,
+                                 const double* vz, const double* m, const double* u,
+                                 const uint8_t* ptype, double time, uint64_t step,
+                                 gravis_diag* out);
+
+/* Same with device pointers; synchronises the stream to return *out. */
+gravis_status gravis_diagnostics_device(gravis_ctx* ctx, size_t n, double softening,
+                                        int compute_potential, const double* x,
+                                        const double* y, const double* z, const double* vx,
+                                        const double* vy, const double* vz, const double* m,
+                                        const double* u, const uint8_t* ptype, double time,
+                                        uint64_t step, gravis_diag* out);
+
+/* ---- introspection (no reference counterpart: test and bench hooks) ---- */
+
+/* Kernel timing: when enabled, the dominant force kernel of every call is
+ * bracketed by HIP events on the stream it runs on. gravis_kernel_time
+ * returns the summed milliseconds and launch count for `name`
+ * ("bf_fp64", "bf_mixed", "bh_walk", ...) and resets them. */
+gravis_status gravis_set_profiling(gravis_ctx* ctx, int enable);
+gravis_status gravis_kernel_time(gravis_ctx* ctx, const char* name, double* ms_total,
+                                 uint64_t* launches);
+
+/* Morton keys (63-bit, 21 bits per axis over the root cube of
+ * octree.rs:14-44) and their stable sort permutation, device computed,
+ * copied to host. box_out = (cx, cy, cz, half_width). */
+gravis_status gravis_morton_sort(gravis_ctx* ctx, size_t n, const double* x, const double* y,
+                                 const double* z, uint64_t* keys_sorted, uint32_t* perm,
+                                 double* box_out);
+
+
+/* ---- initial conditions (host; sim-core's scenario generators) ---------- */
+
+/* PlummerSphere::generate (crates/sim-core/src/scenarios/plummer_sphere.rs:
+ * 106-133): n equal masses M/n, inverse-CDF radii, Aarseth-Henon-Wielen
+ * velocities, ChaCha20Rng::seed_from_u64(seed) stream, COM frame. */
+gravis_status gravis_scenario_plummer(size_t n, double scale_radius, double total_mass,
+                                      uint64_t seed, double* x, double* y, double* z,
+                                      double* vx, double* vy, double* vz, double* m);
+/* plummer_sphere.rs:135-152 */
+double gravis_plummer_suggested_dt(double scale_radius, double total_mass);
+double gravis_plummer_suggested_softening(size_t n, double scale_radius);
+/* TwoBody::generate (crates/sim-core/src/scenarios/two_body.rs:49-92):
+ * 2 particles at periapsis in the COM frame; period (two_body.rs:38-41). */
+gravis_status gravis_scenario_two_body(double eccentricity, double mass_ratio,
+                                       double semi_major_axis, double* x, double* y, double* z,
+                                       double* vx, double* vy, double* vz, double* m);
+double gravis_two_body_period(double semi_major_axis);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRAVIS_H */
