@@ -54,10 +54,12 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
 
 // Block-wide bounding-box centre of the targets (FP64).
 template <int BLOCK, int IPT>
-__device__ __forceinline__ void block_centre(const double (&x)[IPT], const double (&y)[IPT],
-                                             const double (&z)[IPT], double& cx, double& cy,
-                                             double& cz) {
-    double lo[3] = {x[0], y[0], z[0]}, hi[3] = {x[0], y[0], z[0]};
+__device__ __forceinline__ void block_box(const double (&x)[IPT], const double (&y)[IPT],
+                                          const double (&z)[IPT], double (&lo)[3],
+                                          double (&hi)[3]) {
+    lo[0] = hi[0] = x[0];
+    lo[1] = hi[1] = y[0];
+    lo[2] = hi[2] = z[0];
 #pragma unroll
     for (int k = 1; k < IPT; ++k) {
         lo[0] = fmin(lo[0], x[k]); hi[0] = fmax(hi[0], x[k]);
@@ -87,36 +89,102 @@ __device__ __forceinline__ void block_centre(const double (&x)[IPT], const doubl
             lo[d] = fmin(lo[d], s[k][d]);
             hi[d] = fmax(hi[d], s[k][3 + d]);
         }
-    cx = 0.5 * (lo[0] + hi[0]);
-    cy = 0.5 * (lo[1] + hi[1]);
-    cz = 0.5 * (lo[2] + hi[2]);
+}
+
+// Bounding box of each TJ-wide tile of the sorted records: tbox[6 t ..] =
+// (lo x, lo y, lo z, hi x, hi y, hi z).
+template <int TJ>
+__global__ __launch_bounds__(256) void tile_box_kernel(const double4* __restrict__ pm, int n,
+                                                       double* __restrict__ tbox) {
+    const int base = blockIdx.x * TJ;
+    const int last = min(n, base + TJ) - 1;
+    double lo[3], hi[3];
+    {
+        const double4 p = pm[base];
+        lo[0] = hi[0] = p.x; lo[1] = hi[1] = p.y; lo[2] = hi[2] = p.z;
+    }
+    for (int j = base + threadIdx.x; j <= last; j += blockDim.x) {
+        const double4 p = pm[j];
+        lo[0] = fmin(lo[0], p.x); hi[0] = fmax(hi[0], p.x);
+        lo[1] = fmin(lo[1], p.y); hi[1] = fmax(hi[1], p.y);
+        lo[2] = fmin(lo[2], p.z); hi[2] = fmax(hi[2], p.z);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], __shfl_xor(lo[d], off));
+            hi[d] = fmax(hi[d], __shfl_xor(hi[d], off));
+        }
+    __shared__ double sh[4][6];
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0)
+        for (int d = 0; d < 3; ++d) { sh[w][d] = lo[d]; sh[w][3 + d] = hi[d]; }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int d = threadIdx.x;
+        double v = sh[0][d];
+        for (int k = 1; k < (int)(blockDim.x / kWave); ++k)
+            v = d < 3 ? fmin(v, sh[k][d]) : fmax(v, sh[k][d]);
+        tbox[(size_t)blockIdx.x * 6 + d] = v;
+    }
+}
+
+// A j tile is "near" the target box when the gap between the two boxes is
+// smaller than the larger box diagonal. Near tiles get separations formed
+// in FP64 per pair (and a Newton step on rsqrt): there the FP32 rounding of
+// box-relative coordinates would be comparable to the pair separation. Far
+// tiles keep box-relative FP32 coordinates, whose rounding error is a few
+// ulp of the separation.
+__device__ __forceinline__ bool tile_is_near(const double* __restrict__ tb, const double (&lo)[3],
+                                             const double (&hi)[3]) {
+    double gap2 = 0.0, di2 = 0.0, dj2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double g = fmax(0.0, fmax(tb[d] - hi[d], lo[d] - tb[3 + d]));
+        gap2 = __builtin_fma(g, g, gap2);
+        const double ei = hi[d] - lo[d], ej = tb[3 + d] - tb[d];
+        di2 = __builtin_fma(ei, ei, di2);
+        dj2 = __builtin_fma(ej, ej, dj2);
+    }
+    return gap2 < fmax(di2, dj2);
 }
 
 template <int BLOCK, int IPT, int TJ, bool MASK>
 __global__ __launch_bounds__(BLOCK) void bf_mixed_kernel(const double4* __restrict__ pm, int n,
                                                          int jchunk, float eps2,
+                                                         const double* __restrict__ tbox,
                                                          double* __restrict__ part) {
     static_assert(TJ % BLOCK == 0, "tile must be a multiple of the block");
     constexpr int LPT = TJ / BLOCK;  // j records each thread stages per tile
-    __shared__ float4 tile[2][TJ];
+    // Near tiles are staged as FP64 records, far tiles as FP32 box-relative
+    // records in the first half of the same buffer.
+    __shared__ double4 tile[2][TJ];
 
     const int ibase = blockIdx.x * BLOCK * IPT;
-    double xd[IPT], yd[IPT], zd[IPT];
-#pragma unroll
-    for (int k = 0; k < IPT; ++k) {
-        int i = ibase + k * BLOCK + threadIdx.x;
-        i = i < n ? i : ibase;  // out-of-range targets mirror a valid one
-        const double4 p = pm[i];
-        xd[k] = p.x; yd[k] = p.y; zd[k] = p.z;
-    }
-    double cx, cy, cz;
-    block_centre<BLOCK, IPT>(xd, yd, zd, cx, cy, cz);
+    int iidx[IPT];
+    double lo[3], hi[3];
     float xi[IPT], yi[IPT], zi[IPT];
+    double cx, cy, cz;
+    {
+        double xd[IPT], yd[IPT], zd[IPT];
 #pragma unroll
-    for (int k = 0; k < IPT; ++k) {
-        xi[k] = (float)(xd[k] - cx);
-        yi[k] = (float)(yd[k] - cy);
-        zi[k] = (float)(zd[k] - cz);
+        for (int k = 0; k < IPT; ++k) {
+            int i = ibase + k * BLOCK + threadIdx.x;
+            iidx[k] = i < n ? i : ibase;  // out-of-range targets mirror a valid one
+            const double4 p = pm[iidx[k]];
+            xd[k] = p.x; yd[k] = p.y; zd[k] = p.z;
+        }
+        block_box<BLOCK, IPT>(xd, yd, zd, lo, hi);
+        cx = 0.5 * (lo[0] + hi[0]);
+        cy = 0.5 * (lo[1] + hi[1]);
+        cz = 0.5 * (lo[2] + hi[2]);
+#pragma unroll
+        for (int k = 0; k < IPT; ++k) {
+            xi[k] = (float)(xd[k] - cx);
+            yi[k] = (float)(yd[k] - cy);
+            zi[k] = (float)(zd[k] - cz);
+        }
     }
     float sx[IPT], sy[IPT], sz[IPT], kx[IPT], ky[IPT], kz[IPT];
 #pragma unroll
@@ -125,47 +193,89 @@ __global__ __launch_bounds__(BLOCK) void bf_mixed_kernel(const double4* __restri
     const int j0 = blockIdx.y * jchunk;
     const int j1 = min(n, j0 + jchunk);
     const int ntiles = (j1 - j0 + TJ - 1) / TJ;
+    const int tile0 = j0 / TJ;
 
-    float4 stage[LPT];
+    double4 stage[LPT];
     auto load = [&](int t) {
 #pragma unroll
         for (int e = 0; e < LPT; ++e) {
             const int j = j0 + t * TJ + e * BLOCK + threadIdx.x;
-            if (j < j1) {
-                const double4 p = pm[j];
-                stage[e] = make_float4((float)(p.x - cx), (float)(p.y - cy), (float)(p.z - cz),
-                                       (float)p.w);
-            } else {
-                stage[e] = make_float4(0.f, 0.f, 0.f, 0.f);
-            }
+            // padding: zero mass at the box centre (no contribution)
+            stage[e] = j < j1 ? pm[j] : make_double4(cx, cy, cz, 0.0);
         }
     };
-    auto store = [&](int b) {
+    auto store = [&](int b, bool near) {
+        if (near) {
 #pragma unroll
-        for (int e = 0; e < LPT; ++e) tile[b][e * BLOCK + threadIdx.x] = stage[e];
+            for (int e = 0; e < LPT; ++e) tile[b][e * BLOCK + threadIdx.x] = stage[e];
+        } else {
+            float4* t32 = reinterpret_cast<float4*>(&tile[b][0]);
+#pragma unroll
+            for (int e = 0; e < LPT; ++e)
+                t32[e * BLOCK + threadIdx.x] =
+                    make_float4((float)(stage[e].x - cx), (float)(stage[e].y - cy),
+                                (float)(stage[e].z - cz), (float)stage[e].w);
+        }
     };
 
+    bool near_cur = false;
     if (ntiles > 0) {
+        near_cur = tile_is_near(tbox + (size_t)tile0 * 6, lo, hi);
         load(0);
-        store(0);
+        store(0, near_cur);
     }
     __syncthreads();
     for (int t = 0; t < ntiles; ++t) {
         const int cur = t & 1;
-        if (t + 1 < ntiles) load(t + 1);
+        bool near_next = false;
+        if (t + 1 < ntiles) {
+            near_next = tile_is_near(tbox + (size_t)(tile0 + t + 1) * 6, lo, hi);
+            load(t + 1);
+        }
         float ax[IPT], ay[IPT], az[IPT];
 #pragma unroll
         for (int k = 0; k < IPT; ++k) ax[k] = ay[k] = az[k] = 0.f;
+        if (!near_cur) {
+            const float4* t32 = reinterpret_cast<const float4*>(&tile[cur][0]);
 #pragma unroll 8
-        for (int j = 0; j < TJ; ++j) {
-            const float4 s = tile[cur][j];
+            for (int j = 0; j < TJ; ++j) {
+                const float4 s = t32[j];
+#pragma unroll
+                for (int k = 0; k < IPT; ++k) {
+                    const float dx = s.x - xi[k], dy = s.y - yi[k], dz = s.z - zi[k];
+                    const float f = pair_f32<MASK>(dx, dy, dz, s.w, eps2);
+                    ax[k] = __builtin_fmaf(f, dx, ax[k]);
+                    ay[k] = __builtin_fmaf(f, dy, ay[k]);
+                    az[k] = __builtin_fmaf(f, dz, az[k]);
+                }
+            }
+        } else {
+            double px[IPT], py[IPT], pz[IPT];
 #pragma unroll
             for (int k = 0; k < IPT; ++k) {
-                const float dx = s.x - xi[k], dy = s.y - yi[k], dz = s.z - zi[k];
-                const float f = pair_f32<MASK>(dx, dy, dz, s.w, eps2);
-                ax[k] = __builtin_fmaf(f, dx, ax[k]);
-                ay[k] = __builtin_fmaf(f, dy, ay[k]);
-                az[k] = __builtin_fmaf(f, dz, az[k]);
+                const double4 p = pm[iidx[k]];
+                px[k] = p.x; py[k] = p.y; pz[k] = p.z;
+            }
+#pragma unroll 2
+            for (int j = 0; j < TJ; ++j) {
+                const double4 s = tile[cur][j];
+                const float m = (float)s.w;
+#pragma unroll
+                for (int k = 0; k < IPT; ++k) {
+                    const float dx = (float)(s.x - px[k]);
+                    const float dy = (float)(s.y - py[k]);
+                    const float dz = (float)(s.z - pz[k]);
+                    const float r2 =
+                        __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+                    float ri = __builtin_amdgcn_rsqf(r2);
+                    const float h = (r2 * ri) * ri;
+                    ri = ri * __builtin_fmaf(-0.5f, h, 1.5f);
+                    if (MASK) ri = r2 > 0.0f ? ri : 0.0f;
+                    const float f = (m * ri) * (ri * ri);
+                    ax[k] = __builtin_fmaf(f, dx, ax[k]);
+                    ay[k] = __builtin_fmaf(f, dy, ay[k]);
+                    az[k] = __builtin_fmaf(f, dz, az[k]);
+                }
             }
         }
 #pragma unroll
@@ -174,7 +284,8 @@ __global__ __launch_bounds__(BLOCK) void bf_mixed_kernel(const double4* __restri
             kahan_add(sy[k], ky[k], ay[k]);
             kahan_add(sz[k], kz[k], az[k]);
         }
-        if (t + 1 < ntiles) store(cur ^ 1);
+        if (t + 1 < ntiles) store(cur ^ 1, near_next);
+        near_cur = near_next;
         __syncthreads();
     }
 #pragma unroll
@@ -265,9 +376,33 @@ __global__ __launch_bounds__(BLOCK) void bf_fp64_kernel(const double4* __restric
     }
 }
 
+constexpr size_t kSortMinN = 2048;
+
+template <int BLOCK, int IPT, int TJ>
+void launch_mixed(dim3 g, hipStream_t st, bool mask, const double4* pm, int n, int jchunk,
+                  float eps2, const double* tbox, double* part) {
+    if (mask)
+        bf_mixed_kernel<BLOCK, IPT, TJ, true><<<g, BLOCK, 0, st>>>(pm, n, jchunk, eps2, tbox, part);
+    else
+        bf_mixed_kernel<BLOCK, IPT, TJ, false><<<g, BLOCK, 0, st>>>(pm, n, jchunk, eps2, tbox,
+                                                                     part);
+}
+
+template <int BLOCK, int IPT, int TJ>
+void launch_fp64(dim3 g, hipStream_t st, bool mask, const double4* pm, int n, int jchunk,
+                 double eps2, double* part) {
+    if (mask)
+        bf_fp64_kernel<BLOCK, IPT, TJ, true><<<g, BLOCK, 0, st>>>(pm, n, jchunk, eps2, part);
+    else
+        bf_fp64_kernel<BLOCK, IPT, TJ, false><<<g, BLOCK, 0, st>>>(pm, n, jchunk, eps2, part);
+}
+
+}  // namespace
+
+namespace {
 // Sum the S chunk partials in chunk order (G = 1) and deliver in the
 // caller's particle order.
-__global__ __launch_bounds__(256) void bf_finish_kernel(const double* __restrict__ part,
+__global__ __launch_bounds__(256) void finish_kernel(const double* __restrict__ part,
                                                         int nchunks, int n,
                                                         const uint32_t* __restrict__ perm,
                                                         FinishArgs fa) {
@@ -295,27 +430,13 @@ __global__ __launch_bounds__(256) void bf_finish_kernel(const double* __restrict
     }
 }
 
-constexpr size_t kSortMinN = 2048;
-
-template <int BLOCK, int IPT, int TJ>
-void launch_mixed(dim3 g, hipStream_t st, bool mask, const double4* pm, int n, int jchunk,
-                  float eps2, double* part) {
-    if (mask)
-        bf_mixed_kernel<BLOCK, IPT, TJ, true><<<g, BLOCK, 0, st>>>(pm, n, jchunk, eps2, part);
-    else
-        bf_mixed_kernel<BLOCK, IPT, TJ, false><<<g, BLOCK, 0, st>>>(pm, n, jchunk, eps2, part);
-}
-
-template <int BLOCK, int IPT, int TJ>
-void launch_fp64(dim3 g, hipStream_t st, bool mask, const double4* pm, int n, int jchunk,
-                 double eps2, double* part) {
-    if (mask)
-        bf_fp64_kernel<BLOCK, IPT, TJ, true><<<g, BLOCK, 0, st>>>(pm, n, jchunk, eps2, part);
-    else
-        bf_fp64_kernel<BLOCK, IPT, TJ, false><<<g, BLOCK, 0, st>>>(pm, n, jchunk, eps2, part);
-}
-
 }  // namespace
+
+gravis_status finish_forces(Ctx* c, const double* part, int nchunks, int n,
+                            const uint32_t* perm, const FinishArgs& fa) {
+    finish_kernel<<<grid_for((size_t)n, 256), 256, 0, c->stream>>>(part, nchunks, n, perm, fa);
+    return check_launch(c, "finish_forces");
+}
 
 gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x,
                        const double* y, const double* z, const double* m, const FinishArgs& fa) {
@@ -356,11 +477,18 @@ gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
     GRAVIS_TRY(buf(c, "bf_part", (size_t)S * 3 * n, &part));
     const dim3 grid(gx, S);
     const double eps2 = s->softening * s->softening;
+    double* tbox = nullptr;
+    if (mixed) {
+        const int ntile = (ni + tj - 1) / tj;
+        GRAVIS_TRY(buf(c, "bf_tbox", (size_t)ntile * 6, &tbox));
+        tile_box_kernel<256><<<ntile, 256, 0, c->stream>>>(pm, ni, tbox);
+        GRAVIS_TRY(check_launch(c, "tile_box"));
+    }
     timer_begin(c);
     if (mixed) {
         const float eps2f = (float)eps2;
         const bool mask = !(eps2f >= 1.17549435e-38f);
-        launch_mixed<256, 2, 256>(grid, c->stream, mask, pm, ni, jchunk, eps2f, part);
+        launch_mixed<256, 2, 256>(grid, c->stream, mask, pm, ni, jchunk, eps2f, tbox, part);
         timer_end(c, "bf_mixed");
     } else {
         const bool mask = !(eps2 > 0.0);
@@ -368,8 +496,7 @@ gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
         timer_end(c, "bf_fp64");
     }
     GRAVIS_TRY(check_launch(c, "bf kernel"));
-    bf_finish_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>(part, S, ni, perm, fa);
-    return check_launch(c, "bf_finish");
+    return finish_forces(c, part, S, ni, perm, fa);
 }
 
 }  // namespace gravis

@@ -1,11 +1,435 @@
-// bh.hip -- Barnes-Hut on gfx950 (placeholder until the linear octree lands).
+// bh.hip -- Barnes-Hut gravity on gfx950: Morton-ordered linear octree and
+// a wavefront-coherent group walk.
+//
+// Reference: BarnesHut::compute_accelerations (crates/sim-core/src/
+// barnes_hut.rs:139-192) over the arena octree of octree.rs:145-239. The
+// reference inserts particles one by one (one particle per leaf) and walks
+// the tree per particle with a 512-entry stack, accepting a node when
+// s^2 < theta^2 d^2 (s = node side, d = distance to the node's COM).
+//
+// Here the same root cube (octree.rs:14-44) is cut by 63-bit Morton keys, so
+// every node is a run of the key-sorted particles and its cell is the
+// reference's cell at that depth:
+//   build   level by level: each internal node (more than `leaf` particles,
+//           level < 21) finds its child runs by binary search on the next
+//           3-bit digit; nodes are numbered breadth first, children of a
+//           node contiguous in digit order.
+//   COM     bottom up in FP64: leaves sum their particles in sorted order,
+//           internal nodes sum their children in order (m, m*x; no FMA);
+//           com = sum(m x) / sum(m).
+//   walk    one wavefront per group of `group` consecutive sorted targets.
+//           The frontier is processed breadth first, 64 nodes at a time
+//           (one per lane). A node is accepted for the whole group when
+//           s^2 < theta^2 d_min^2 with d_min the distance from its COM to the
+//           group's bounding box (so never less accurate than the
+//           reference's per-particle test); a rejected leaf contributes its
+//           particles directly; a rejected internal node pushes its children.
+//           Accepted nodes and opened leaves are appended, in frontier
+//           order, to a per-wave list in LDS that all 64 lanes then sweep
+//           with wave-uniform (scalar) loads. Pair arithmetic is
+//           barnes_hut.rs:114-135 in FP64 (sqrt and divide correctly
+//           rounded, no contraction); a source at r^2 = 0 is skipped.
+// The accumulation order is the list order, so results are bit-for-bit the
+// CPU oracle's group walk with the same groups.
+#include <algorithm>
+
 #include "internal.h"
 
 namespace gravis {
+namespace {
 
-gravis_status bh_force(Ctx* c, const gravis_solver*, size_t, const double*, const double*,
-                       const double*, const double*, const FinishArgs&) {
-    return fail(c, GRAVIS_UNSUPPORTED, "Barnes-Hut is not built yet");
+constexpr int kMaxLevel = 21;
+constexpr uint32_t kLeafFlag = 0x80000000u;
+constexpr int kTreeBlock = 256;
+constexpr int kTreeGrid = 1024;
+
+// meta = {first, count, first child, (nchild << 8) | level}
+
+__global__ void tree_init_kernel(uint4* __restrict__ meta, uint32_t* __restrict__ lvl,
+                                 uint32_t n) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    meta[0] = make_uint4(0u, n, 0u, 0u);
+    lvl[0] = 0u;
+    lvl[1] = 1u;
+    for (int L = 1; L <= kMaxLevel + 1; ++L) {
+        lvl[2 * L] = 1u;
+        lvl[2 * L + 1] = 1u;
+    }
+}
+
+__device__ __forceinline__ uint32_t digit_of(uint64_t key, int shift) {
+    return static_cast<uint32_t>((key >> shift) & 7ull);
+}
+
+// Number of non-empty children of each node of level L.
+__global__ __launch_bounds__(kTreeBlock) void tree_count_kernel(const uint64_t* __restrict__ keys,
+                                                               const uint4* __restrict__ meta,
+                                                               const uint32_t* __restrict__ lvl,
+                                                               int L, uint32_t leaf,
+                                                               uint32_t* __restrict__ nch) {
+    const uint32_t b = lvl[2 * L], e = lvl[2 * L + 1];
+    const int shift = 3 * (kMaxLevel - (L + 1));
+    for (uint32_t v = b + blockIdx.x * kTreeBlock + threadIdx.x; v < e;
+         v += gridDim.x * kTreeBlock) {
+        const uint4 md = meta[v];
+        uint32_t c = 0;
+        if (md.y > leaf && L < kMaxLevel) {
+            uint32_t pos = md.x;
+            const uint32_t end = md.x + md.y;
+            while (pos < end) {
+                const uint32_t d = digit_of(keys[pos], shift);
+                uint32_t lo = pos + 1, hi = end;  // first index with a larger digit
+                while (lo < hi) {
+                    const uint32_t mid = lo + (hi - lo) / 2;
+                    if (digit_of(keys[mid], shift) > d) hi = mid;
+                    else lo = mid + 1;
+                }
+                ++c;
+                pos = lo;
+            }
+        }
+        nch[v - b] = c;
+    }
+}
+
+// Exclusive scan of the level's child counts (one block) and the range of
+// level L+1.
+__global__ __launch_bounds__(1024) void tree_scan_kernel(const uint32_t* __restrict__ nch,
+                                                         uint32_t* __restrict__ off,
+                                                         uint32_t* __restrict__ lvl, int L) {
+    __shared__ uint32_t s[1024];
+    const uint32_t b = lvl[2 * L], e = lvl[2 * L + 1];
+    const uint32_t m = e - b;
+    const uint32_t per = (m + 1023) / 1024;
+    const uint32_t lo = threadIdx.x * per;
+    const uint32_t hi = min(m, lo + per);
+    uint32_t sum = 0;
+    for (uint32_t i = lo; i < hi; ++i) sum += nch[i];
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const uint32_t t = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t run = s[threadIdx.x] - sum;
+    for (uint32_t i = lo; i < hi; ++i) {
+        off[i] = run;
+        run += nch[i];
+    }
+    if (threadIdx.x == 1023) {
+        lvl[2 * (L + 1)] = e;
+        lvl[2 * (L + 1) + 1] = e + s[1023];
+    }
+}
+
+__global__ __launch_bounds__(kTreeBlock) void tree_write_kernel(
+    const uint64_t* __restrict__ keys, uint4* __restrict__ meta, const uint32_t* __restrict__ lvl,
+    int L, const uint32_t* __restrict__ nch, const uint32_t* __restrict__ off, uint32_t cap,
+    uint32_t* __restrict__ flag) {
+    const uint32_t b = lvl[2 * L], e = lvl[2 * L + 1];
+    const int shift = 3 * (kMaxLevel - (L + 1));
+    for (uint32_t v = b + blockIdx.x * kTreeBlock + threadIdx.x; v < e;
+         v += gridDim.x * kTreeBlock) {
+        const uint32_t nc = nch[v - b];
+        if (nc == 0) continue;
+        const uint4 md = meta[v];
+        const uint32_t c0 = e + off[v - b];
+        if (c0 + nc > cap) {
+            atomicOr(flag, 2u);
+            continue;
+        }
+        uint32_t pos = md.x, k = 0;
+        const uint32_t end = md.x + md.y;
+        while (pos < end) {
+            const uint32_t d = digit_of(keys[pos], shift);
+            uint32_t lo = pos + 1, hi = end;
+            while (lo < hi) {
+                const uint32_t mid = lo + (hi - lo) / 2;
+                if (digit_of(keys[mid], shift) > d) hi = mid;
+                else lo = mid + 1;
+            }
+            meta[c0 + k] = make_uint4(pos, lo - pos, 0u, (uint32_t)(L + 1));
+            ++k;
+            pos = lo;
+        }
+        meta[v] = make_uint4(md.x, md.y, c0, (nc << 8) | (uint32_t)L);
+    }
+}
+
+// Raw FP64 sums (sum m x, sum m y, sum m z, sum m) of the nodes of level L.
+__global__ __launch_bounds__(kTreeBlock) void tree_sums_kernel(const double4* __restrict__ src,
+                                                              const uint4* __restrict__ meta,
+                                                              const uint32_t* __restrict__ lvl,
+                                                              int L,
+                                                              double4* __restrict__ sums) {
+    const uint32_t b = lvl[2 * L], e = lvl[2 * L + 1];
+    for (uint32_t v = b + blockIdx.x * kTreeBlock + threadIdx.x; v < e;
+         v += gridDim.x * kTreeBlock) {
+        const uint4 md = meta[v];
+        const uint32_t nc = md.w >> 8;
+        double M = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
+        if (nc == 0) {
+            for (uint32_t k = md.x; k < md.x + md.y; ++k) {
+                const double4 p = src[k];
+                M = M + p.w;
+                X = X + p.w * p.x;
+                Y = Y + p.w * p.y;
+                Z = Z + p.w * p.z;
+            }
+        } else {
+            for (uint32_t c = md.z; c < md.z + nc; ++c) {
+                const double4 q = sums[c];
+                M = M + q.w;
+                X = X + q.x;
+                Y = Y + q.y;
+                Z = Z + q.z;
+            }
+        }
+        sums[v] = make_double4(X, Y, Z, M);
+    }
+}
+
+// Node source records src[n + v] = (com, mass).
+__global__ __launch_bounds__(kTreeBlock) void tree_com_kernel(double4* __restrict__ src, int n,
+                                                             const uint4* __restrict__ meta,
+                                                             const uint32_t* __restrict__ lvl,
+                                                             const double4* __restrict__ sums) {
+    const uint32_t total = lvl[2 * (kMaxLevel + 1)];
+    for (uint32_t v = blockIdx.x * kTreeBlock + threadIdx.x; v < total;
+         v += gridDim.x * kTreeBlock) {
+        const double4 q = sums[v];
+        double4 r;
+        if (q.w != 0.0) {
+            r = make_double4(q.x / q.w, q.y / q.w, q.z / q.w, q.w);
+        } else {
+            const double4 p = src[meta[v].x];
+            r = make_double4(p.x, p.y, p.z, 0.0);
+        }
+        src[(size_t)n + v] = r;
+    }
+}
+
+__global__ void walk_reset_kernel(uint32_t* __restrict__ counter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0u;
+}
+
+constexpr int kWalkWaves = 4;
+constexpr int kListCap = 1024;
+
+__device__ __forceinline__ void pair_ref(const double4 s, double px, double py, double pz,
+                                         double eps2, double& ax, double& ay, double& az) {
+    const double dx = s.x - px, dy = s.y - py, dz = s.z - pz;
+    const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+    const double inv_r3 = 1.0 / (r2 * sqrt(r2));
+    const double f = s.w * inv_r3;
+    const bool use = r2 != 0.0;
+    ax = use ? ax + f * dx : ax;
+    ay = use ? ay + f * dy : ay;
+    az = use ? az + f * dz : az;
+}
+
+__global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
+    const double4* __restrict__ src, const uint4* __restrict__ meta, int n, int gs, int ngroups,
+    const double* __restrict__ box, double theta2, double eps2, uint32_t* __restrict__ frontier, uint32_t fcap,
+    uint32_t* __restrict__ counter, uint32_t* __restrict__ flag, double* __restrict__ part) {
+    __shared__ uint32_t lists[kWalkWaves][kListCap];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = threadIdx.x / kWave;
+    uint32_t* list = lists[w];
+    const uint64_t lt = (1ull << lane) - 1ull;
+    const size_t slot = (size_t)blockIdx.x * kWalkWaves + w;
+    uint32_t* fa0 = frontier + slot * 2 * (size_t)fcap;
+    uint32_t* fb0 = fa0 + fcap;
+    const double side0 = 2.0 * box[3];  // root side, octree.rs:81-85
+
+    for (;;) {
+        uint32_t g = 0;
+        if (lane == 0) g = atomicAdd(counter, 1u);
+        g = __shfl(g, 0);
+        if (g >= (uint32_t)ngroups) break;
+        const int i0 = (int)g * gs;
+        const int cnt = min(gs, n - i0);
+        const bool active = lane < cnt;
+        const double4 me = src[i0 + (active ? lane : 0)];
+        double lo[3] = {me.x, me.y, me.z}, hi[3] = {me.x, me.y, me.z};
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = fmin(lo[d], __shfl_xor(lo[d], o));
+                hi[d] = fmax(hi[d], __shfl_xor(hi[d], o));
+            }
+        double ax = 0.0, ay = 0.0, az = 0.0;
+        uint32_t nl = 0;  // list length (wave uniform)
+
+        auto flush = [&]() {
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            for (uint32_t q = 0; q < nl; ++q) {
+                const uint32_t e = __builtin_amdgcn_readfirstlane(list[q]);
+                if (e & kLeafFlag) {
+                    const uint4 md = meta[e & ~kLeafFlag];
+                    const uint32_t kb = __builtin_amdgcn_readfirstlane(md.x);
+                    const uint32_t ke = kb + __builtin_amdgcn_readfirstlane(md.y);
+                    for (uint32_t k = kb; k < ke; ++k) pair_ref(src[k], me.x, me.y, me.z, eps2, ax, ay, az);
+                } else {
+                    pair_ref(src[(size_t)n + e], me.x, me.y, me.z, eps2, ax, ay, az);
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            nl = 0;
+        };
+
+        uint32_t* fa = fa0;
+        uint32_t* fb = fb0;
+        if (lane == 0) fa[0] = 0u;
+        uint32_t nf = 1;
+        bool overflow = false;
+        __threadfence_block();
+        while (nf > 0 && !overflow) {
+            uint32_t nn = 0;
+            for (uint32_t base = 0; base < nf; base += kWave) {
+                if (nl > (uint32_t)(kListCap - kWave)) flush();
+                const uint32_t idx = base + lane;
+                const bool valid = idx < nf;
+                const uint32_t v = valid ? fa[idx] : 0u;
+                const uint4 md = meta[v];
+                const double4 c = src[(size_t)n + v];
+                const uint32_t level = md.w & 255u;
+                const uint32_t nchild = md.w >> 8;
+                const double s = ldexp(side0, -(int)level);
+                const double ex = fmax(fmax(lo[0] - c.x, c.x - hi[0]), 0.0);
+                const double ey = fmax(fmax(lo[1] - c.y, c.y - hi[1]), 0.0);
+                const double ez = fmax(fmax(lo[2] - c.z, c.z - hi[2]), 0.0);
+                const double d2 = ex * ex + ey * ey + ez * ez;
+                const bool accept = valid && (s * s < theta2 * d2);
+                const bool open = valid && !accept && nchild > 0;
+                const bool direct = valid && !accept && nchild == 0;
+                const uint64_t bl = __ballot(accept || direct);
+                if (accept || direct)
+                    list[nl + (uint32_t)__popcll(bl & lt)] = direct ? (v | kLeafFlag) : v;
+                nl += (uint32_t)__popcll(bl);
+                // children of opened internal nodes -> next frontier, lane order
+                const uint32_t k = open ? nchild : 0u;
+                uint32_t sc = k;
+#pragma unroll
+                for (int o = 1; o < kWave; o <<= 1) {
+                    const uint32_t t = __shfl_up(sc, o);
+                    if (lane >= o) sc += t;
+                }
+                const uint32_t total = __shfl(sc, kWave - 1);
+                if (nn + total > fcap) {
+                    overflow = true;
+                    break;
+                }
+                const uint32_t ex0 = nn + sc - k;
+                for (uint32_t q = 0; q < k; ++q) fb[ex0 + q] = md.z + q;
+                nn += total;
+            }
+            __threadfence_block();
+            __builtin_amdgcn_wave_barrier();
+            uint32_t* t = fa;
+            fa = fb;
+            fb = t;
+            nf = nn;
+        }
+        flush();
+        if (overflow && lane == 0) atomicOr(flag, 1u);
+        if (active) {
+            const size_t i = (size_t)i0 + lane;
+            part[i] = ax;
+            part[(size_t)n + i] = ay;
+            part[2 * (size_t)n + i] = az;
+        }
+    }
+}
+
+}  // namespace
+
+gravis_status bh_force(Ctx* c, const gravis_solver* s, size_t n, const double* x,
+                       const double* y, const double* z, const double* m, const FinishArgs& fa) {
+    if (n == 0) return GRAVIS_OK;
+    if (n > 0x7FFFFFFFull) return fail(c, GRAVIS_INVALID_ARG, "n too large");
+    const uint32_t leaf = s->leaf_size ? s->leaf_size : 16u;
+    const int gs = s->group_size ? (int)s->group_size : kWave;
+    if (gs < 1 || gs > kWave) return fail(c, GRAVIS_INVALID_ARG, "group_size must be 1..64");
+    if (leaf < 1 || leaf > 4096) return fail(c, GRAVIS_INVALID_ARG, "leaf_size must be 1..4096");
+    const int ni = (int)n;
+
+    double* box;
+    GRAVIS_TRY(buf(c, "bh_box", 4, &box));
+    GRAVIS_TRY(root_box(c, n, x, y, z, box));
+    uint64_t* keys;
+    uint32_t* perm;
+    GRAVIS_TRY(morton_sort(c, n, x, y, z, box, &keys, &perm));
+
+    // Internal nodes of one level are disjoint runs of more than `leaf`
+    // particles and have at most 8 children each.
+    const size_t cap = 1 + (size_t)kMaxLevel * 8 * (n / (leaf + 1) + 1);
+    if (cap > 0x7FFFFFFFull) return fail(c, GRAVIS_INVALID_ARG, "tree too large");
+    double4 *src, *sums;
+    uint4* meta;
+    uint32_t *lvl, *nch, *off, *flags;
+    GRAVIS_TRY(buf(c, "bh_src", n + cap, &src));
+    GRAVIS_TRY(buf(c, "bh_sums", cap, &sums));
+    GRAVIS_TRY(buf(c, "bh_meta", cap, &meta));
+    GRAVIS_TRY(buf(c, "bh_lvl", 2 * (kMaxLevel + 2), &lvl));
+    GRAVIS_TRY(buf(c, "bh_nch", cap, &nch));
+    GRAVIS_TRY(buf(c, "bh_off", cap, &off));
+    GRAVIS_TRY(buf(c, "bh_flags", 4, &flags));
+    GRAVIS_TRY(gather_sorted(c, n, x, y, z, m, perm, src));
+
+    hipStream_t st = c->stream;
+    GRAVIS_HIP(c, hipMemsetAsync(flags, 0, 4 * sizeof(uint32_t), st));
+    tree_init_kernel<<<1, 64, 0, st>>>(meta, lvl, (uint32_t)n);
+    const unsigned tg = grid_for(cap, kTreeBlock, kTreeGrid);
+    for (int L = 0; L < kMaxLevel; ++L) {
+        tree_count_kernel<<<tg, kTreeBlock, 0, st>>>(keys, meta, lvl, L, leaf, nch);
+        tree_scan_kernel<<<1, 1024, 0, st>>>(nch, off, lvl, L);
+        tree_write_kernel<<<tg, kTreeBlock, 0, st>>>(keys, meta, lvl, L, nch, off, (uint32_t)cap,
+                                                    flags + 1);
+    }
+    // level 21 holds leaves only (zero children); its scan closes the node range
+    tree_count_kernel<<<tg, kTreeBlock, 0, st>>>(keys, meta, lvl, kMaxLevel, leaf, nch);
+    tree_scan_kernel<<<1, 1024, 0, st>>>(nch, off, lvl, kMaxLevel);
+    GRAVIS_TRY(check_launch(c, "tree build"));
+    for (int L = kMaxLevel; L >= 0; --L)
+        tree_sums_kernel<<<tg, kTreeBlock, 0, st>>>(src, meta, lvl, L, sums);
+    tree_com_kernel<<<tg, kTreeBlock, 0, st>>>(src, ni, meta, lvl, sums);
+    GRAVIS_TRY(check_launch(c, "tree com"));
+
+    // walk
+    const int ngroups = (ni + gs - 1) / gs;
+    const int blocks = (int)std::min<long>((ngroups + kWalkWaves - 1) / kWalkWaves,
+                                           (long)c->num_cus * 4);
+    const uint32_t fcap = (uint32_t)std::min<size_t>(cap, 32768);
+    uint32_t *front, *counter;
+    GRAVIS_TRY(buf(c, "bh_front", (size_t)blocks * kWalkWaves * 2 * fcap, &front));
+    GRAVIS_TRY(buf(c, "bh_counter", 4, &counter));
+    double* part;
+    GRAVIS_TRY(buf(c, "bh_part", 3 * n, &part));
+    const double theta2 = s->theta * s->theta;
+    const double eps2 = s->softening * s->softening;
+    walk_reset_kernel<<<1, 64, 0, st>>>(counter);
+    timer_begin(c);
+    bh_walk_kernel<<<blocks, kWalkWaves * kWave, 0, st>>>(src, meta, ni, gs, ngroups,
+                                                          box, theta2, eps2, front, fcap,
+                                                          counter, flags, part);
+    timer_end(c, "bh_walk");
+    GRAVIS_TRY(check_launch(c, "bh_walk"));
+    uint32_t hflags[2] = {0, 0};
+    GRAVIS_HIP(c, hipMemcpyAsync(hflags, flags, sizeof hflags, hipMemcpyDeviceToHost, st));
+    GRAVIS_HIP(c, hipStreamSynchronize(st));
+    if (hflags[1])
+        return fail(c, GRAVIS_INTERNAL, "octree node capacity exceeded (%zu nodes)", cap);
+    if (hflags[0])
+        return fail(c, GRAVIS_INTERNAL,
+                    "Barnes-Hut walk frontier exceeded %u nodes (theta %g too small for n=%zu)",
+                    fcap, s->theta, n);
+    return finish_forces(c, part, 1, ni, perm, fa);
 }
 
 }  // namespace gravis

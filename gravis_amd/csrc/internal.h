@@ -111,6 +111,12 @@ struct FinishArgs {
     double half_dt = 0.0;
 };
 
+// Sums nchunks partial force arrays part[q][3][n] (sorted order, chunk order
+// fixed) and delivers them in the caller's order (perm: sorted -> caller
+// index, identity if null) per FinishArgs.
+gravis_status finish_forces(Ctx* c, const double* part, int nchunks, int n,
+                            const uint32_t* perm, const FinishArgs& fa);
+
 // Brute-force force on device SoA arrays, result delivered per FinishArgs.
 gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x,
                        const double* y, const double* z, const double* m, const FinishArgs& fa);

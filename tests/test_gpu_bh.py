@@ -1,0 +1,177 @@
+"""GPU parity of Barnes-Hut (linear octree, group walk) through the C ABI.
+
+Bars:
+  * Morton keys, sort permutation, tree and walk are deterministic and the
+    GPU forces equal the CPU oracle's group walk (oracle/gravis_oracle.c,
+    orc_bh_group_accel) BIT FOR BIT for the same theta, leaf and group.
+  * Accuracy vs brute force at theta 0.3/0.5/0.7/1.0 on the reference's
+    N=1000 seed-42 Plummer set is within the reference's own Barnes-Hut
+    error (m3_accuracy.csv) -- the north-star bar for BH forces.
+  * theta = 0 reduces to brute force (barnes_hut.rs:219-265), |a| = 0.25 for
+    two unit masses at distance 2 (barnes_hut.rs:268-283).
+  * KDK with BH conserves energy to 1% and keeps the virial ratio within
+    0.5 of 1 (tests/barnes_hut.rs:103-194).
+"""
+import csv
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def g():
+    import gravis_amd
+    return gravis_amd
+
+
+def to_particles(g, s):
+    return g.Particles.from_arrays(s.x, s.y, s.z, s.vx, s.vy, s.vz, s.m)
+
+
+def rel_errors(ref, got):
+    m2 = (ref ** 2).sum(1)
+    e2 = ((got - ref) ** 2).sum(1)
+    ok = m2 >= 1e-30
+    rel2 = e2[ok] / m2[ok]
+    return float(np.sqrt(rel2.sum() / len(ref))), float(np.sqrt(rel2.max()))
+
+
+@pytest.mark.parametrize("n,theta,eps", [(1000, 0.5, 0.05), (1000, 0.3, 0.05), (1000, 1.0, 0.05),
+                                         (257, 0.5, 0.05), (20000, 0.5, 0.01), (65, 0.7, 0.0)])
+def test_bh_bit_exact_with_oracle_group_walk(g, n, theta, eps):
+    say(f"bh bit-exact n={n} theta={theta} eps={eps}")
+    s = O.plummer(n, 42)
+    ref = s.copy()
+    O.bh_group_accel(ref, eps, theta, 16, 64)
+    p = to_particles(g, s)
+    g.BarnesHut(eps, theta).compute_accelerations(p)
+    diff = np.abs(p.acc() - ref.acc()).max()
+    say(f"  max |da| = {diff:.3e}")
+    assert np.array_equal(p.acc(), ref.acc())
+
+
+def test_bh_within_reference_error(g, golden_dir):
+    rows = list(csv.reader(open(os.path.join(golden_dir, "m3_accuracy.csv"))))[1:]
+    s = O.plummer(1000, 42)
+    bf = s.copy()
+    O.bf_accel(bf, 0.05)
+    for r in rows:
+        theta, rms_ref, max_ref = float(r[0]), float(r[2]), float(r[3])
+        say(f"bh accuracy theta={theta}")
+        p = to_particles(g, s)
+        g.BarnesHut(0.05, theta).compute_accelerations(p)
+        rms, mx = rel_errors(bf.acc(), p.acc())
+        say(f"  rms={rms:.3e} (ref {rms_ref:.3e}) max={mx:.3e} (ref {max_ref:.3e})")
+        assert rms <= rms_ref and mx <= max_ref
+
+
+def test_bh_theta_zero_and_pair(g):
+    say("bh closed forms")
+    pts = [(0.0, 0.0, 0.0, 1.0), (1.0, 0.0, 0.0, 2.0), (0.0, 1.0, 0.0, 1.5),
+           (-1.0, -1.0, 0.5, 0.5), (0.5, 0.5, -0.5, 3.0)]
+    s = O.State(5)
+    for i, (x, y, z, m) in enumerate(pts):
+        s.x[i], s.y[i], s.z[i], s.m[i] = x, y, z, m
+    bf = s.copy()
+    O.bf_accel(bf, 0.05)
+    p = to_particles(g, s)
+    g.BarnesHut(0.05, 0.0).compute_accelerations(p)
+    assert np.abs(p.acc() - bf.acc()).max() < 1e-12
+    q = g.Particles.from_arrays([0.0, 2.0], [0, 0], [0, 0], [0, 0], [0, 0], [0, 0], [1, 1])
+    g.BarnesHut(0.0, 0.5).compute_accelerations(q)
+    assert abs(q.ax[0] - 0.25) < 1e-14 and abs(q.ax[0] + q.ax[1]) < 1e-14
+
+
+def test_bh_degenerate_inputs(g):
+    say("bh n=1, coincident clusters")
+    p = g.Particles.from_arrays([0.5], [0.1], [0.2], [0], [0], [0], [1.0])
+    g.BarnesHut(0.05, 0.5).compute_accelerations(p)
+    assert (p.acc() == 0.0).all()
+    # 40 particles in two coincident clumps: level-21 leaves of 20 particles
+    s = O.State(40)
+    s.x[:20] = 0.25
+    s.x[20:] = -0.25
+    s.m[:] = 1.0 / 40
+    ref = s.copy()
+    O.bh_group_accel(ref, 0.05, 0.5)
+    p = to_particles(g, s)
+    g.BarnesHut(0.05, 0.5).compute_accelerations(p)
+    assert np.array_equal(p.acc(), ref.acc())
+    assert np.isfinite(p.acc()).all()
+
+
+def test_bh_leaf_and_group_sizes(g):
+    say("bh leaf 1/8/32, group 32")
+    s = O.plummer(3000, 8)
+    for leaf, grp in ((1, 64), (8, 64), (32, 64), (16, 32)):
+        ref = s.copy()
+        O.bh_group_accel(ref, 0.05, 0.5, leaf, grp)
+        p = to_particles(g, s)
+        g.BarnesHut(0.05, 0.5, leaf_size=leaf, group_size=grp).compute_accelerations(p)
+        assert np.array_equal(p.acc(), ref.acc()), (leaf, grp)
+
+
+def test_bh_accumulates_and_is_deterministic(g):
+    say("bh accumulate + determinism")
+    s = O.plummer(5000, 4)
+    a = to_particles(g, s)
+    b = to_particles(g, s)
+    b.ax[:] = 1.0
+    bh = g.BarnesHut(0.05, 0.5)
+    bh.compute_accelerations(a)
+    bh.compute_accelerations(b)
+    assert np.array_equal(b.ax - 1.0, a.ax) or np.abs(b.ax - 1.0 - a.ax).max() < 1e-15
+    c = to_particles(g, s)
+    bh.compute_accelerations(c)
+    assert np.array_equal(a.acc(), c.acc())
+
+
+def test_bh_kdk_plummer_conservation(g):
+    """tests/barnes_hut.rs:103-194: Plummer N=500 seed 42, suggested eps and
+    dt, 10 t_dyn with BH theta 0.5."""
+    say("bh kdk plummer 1000 steps")
+    s = O.plummer(500, 42)
+    eps = O.plummer_suggested_softening(500)
+    dt = O.plummer_suggested_dt()
+    p = to_particles(g, s)
+    bh = g.BarnesHut(eps, 0.5)
+    bh.compute_accelerations(p)
+    d0 = g.diagnostics.compute(p, eps)
+    kdk = g.LeapfrogKDK()
+    worst = 0.0
+    for _ in range(10):
+        kdk.run(p, bh, dt, 100)
+        d = g.diagnostics.compute(p, eps)
+        worst = max(worst, abs(d.virial_ratio - 1.0))
+    de = abs((d.total_energy - d0.total_energy) / d0.total_energy)
+    say(f"  dE/E={de:.3e} max|virial-1|={worst:.3f}")
+    assert de < 0.01
+    assert worst < 0.5
+
+
+def test_bh_large_n_against_brute_force_sample(g):
+    """N=200,000 Plummer, theta 0.5: forces on 512 random targets checked
+    against the oracle's f64 brute force (RMS within the reference's
+    theta=0.5 error) and the Morton permutation checked bit-exact."""
+    say("bh n=200000 sample vs brute force")
+    n = 200000
+    s = O.plummer(n, 42)
+    p = to_particles(g, s)
+    g.BarnesHut(0.01, 0.5).compute_accelerations(p)
+    rng = np.random.default_rng(42)
+    t = np.sort(rng.choice(n, 512, replace=False))
+    ref, _ = O.bf_targets(s, 0.01, t, nthreads=8)
+    rms, mx = rel_errors(ref, p.acc()[t])
+    say(f"  rms={rms:.3e} max={mx:.3e}")
+    assert rms <= 4.942351e-3 and mx <= 3.588391e-2
