@@ -232,3 +232,123 @@ def diagnostics(s: State, eps: float, with_potential: bool = True, time: float =
     lib().orc_diagnostics(s.n, eps, 1 if with_potential else 0, s.x, s.y, s.z, s.vx, s.vy,
                           s.vz, s.m, None, None, time, step, out)
     return dict(zip(DIAG_FIELDS, out.tolist()))
+
+
+# ---- SPH (gravis_oracle_sph.c) ---------------------------------------------
+
+SPH_FIELDS = ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "m", "rho", "p", "u", "dudt",
+              "h", "c", "alpha")
+
+
+class _SphStateC(C.Structure):
+    _fields_ = [("n", C.c_size_t)] + [(f, C.POINTER(C.c_double)) for f in SPH_FIELDS] + [
+        ("type", C.POINTER(C.c_uint8))]
+
+
+class SphParams(C.Structure):
+    """SphSolver::new + ForceParams::default (solver.rs:40-53, forces.rs:30-41)."""
+    _fields_ = [(k, C.c_double) for k in ("gamma", "eta", "h_max", "cfl", "force_factor",
+                                          "beta_visc", "alpha_min", "alpha_max", "alpha_cond",
+                                          "sigma_visc")]
+
+    @classmethod
+    def default(cls):
+        p = cls()
+        _sph_lib().orc_sph_default_params(C.byref(p))
+        return p
+
+
+class SphState:
+    """Particles with the SPH fields (particle.rs:47-63); type 4 = gas."""
+
+    def __init__(self, n: int):
+        self.n = n
+        for f in SPH_FIELDS:
+            setattr(self, f, np.zeros(n))
+        self.type = np.zeros(n, dtype=np.uint8)
+
+    def copy(self):
+        s = SphState(self.n)
+        for f in SPH_FIELDS + ("type",):
+            setattr(s, f, getattr(self, f).copy())
+        return s
+
+    def c_struct(self):
+        st = _SphStateC()
+        st.n = self.n
+        for f in SPH_FIELDS:
+            setattr(st, f, getattr(self, f).ctypes.data_as(C.POINTER(C.c_double)))
+        st.type = self.type.ctypes.data_as(C.POINTER(C.c_uint8))
+        return st
+
+    def acc(self):
+        return np.stack([self.ax, self.ay, self.az], axis=1)
+
+
+_SPH = None
+
+
+def _sph_lib():
+    global _SPH
+    if _SPH is None:
+        L = lib()
+        sp = C.POINTER(_SphStateC)
+        pp = C.POINTER(SphParams)
+        spec = {
+            "orc_sph_default_params": (None, [pp]),
+            "orc_w": (_d, [_d, _d]),
+            "orc_dw_dh": (_d, [_d, _d]),
+            "orc_grad_w": (None, [_d, _d, _d, _d, _d, _dp]),
+            "orc_sph_compute": (None, [sp, pp, _dp, _dp, _dp, _dp, _u32p]),
+            "orc_sph_timestep": (_d, [sp, pp, _dp, _dp]),
+            "orc_sph_step": (_d, [sp, pp, _i, _d, _d, _d]),
+            "orc_evrard": (None, [_sz, _d, _d, _d, _u64, sp]),
+            "orc_evrard_softening": (_d, [_sz, _d]),
+        }
+        for name, (res, args) in spec.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _SPH = L
+    return _SPH
+
+
+def evrard(n: int, seed: int = 42, total_mass: float = 1.0, radius: float = 1.0,
+           u0: float = 0.05) -> SphState:
+    s = SphState(n)
+    st = s.c_struct()
+    _sph_lib().orc_evrard(n, total_mass, radius, u0, seed, C.byref(st))
+    return s
+
+
+def evrard_softening(n: int, radius: float = 1.0) -> float:
+    return _sph_lib().orc_evrard_softening(n, radius)
+
+
+def sph_compute(s: SphState, params: SphParams | None = None):
+    """SphSolver::compute: accumulates SPH a and du/dt, updates rho/h/P/c;
+    returns dict(div_v, max_mu, dalpha_dt, omega, nneigh)."""
+    params = params or SphParams.default()
+    n = s.n
+    out = {k: np.zeros(n) for k in ("div_v", "max_mu", "dalpha_dt", "omega")}
+    nn = np.zeros(n, dtype=np.uint32)
+    st = s.c_struct()
+    _sph_lib().orc_sph_compute(C.byref(st), C.byref(params), out["div_v"], out["max_mu"],
+                               out["dalpha_dt"], out["omega"], nn)
+    out["nneigh"] = nn
+    return out
+
+
+def sph_timestep(s: SphState, inter: dict, params: SphParams | None = None) -> float:
+    params = params or SphParams.default()
+    st = s.c_struct()
+    return _sph_lib().orc_sph_timestep(C.byref(st), C.byref(params), inter["div_v"],
+                                       inter["max_mu"])
+
+
+def sph_step(s: SphState, dt: float, gravity: int = 1, eps: float = 0.0, theta: float = 0.5,
+             params: SphParams | None = None) -> float:
+    """step_with_sph; gravity 0 none, 1 brute force, 2 reference Barnes-Hut."""
+    params = params or SphParams.default()
+    st = s.c_struct()
+    return _sph_lib().orc_sph_step(C.byref(st), C.byref(params), gravity, eps, theta, dt)

@@ -43,6 +43,8 @@
 #include <omp.h>
 #endif
 
+#include "gravis_oracle.h"
+
 #define ORC_PI 3.14159265358979323846
 
 /* ------------------------------------------------------------------------ */
@@ -80,12 +82,7 @@ void orc_chacha20_block(const uint32_t key[8], uint64_t counter, uint64_t stream
     for (int i = 0; i < 16; ++i) out[i] = w[i] + s[i];
 }
 
-typedef struct {
-    uint32_t key[8];
-    uint64_t counter;   /* next block to generate */
-    uint32_t buf[64];   /* 4 blocks, as rand_chacha buffers them */
-    int idx;            /* next u32 word in buf; 64 = empty */
-} orc_rng;
+/* orc_rng: see gravis_oracle.h */
 
 /* rand_core 0.9 SeedableRng::seed_from_u64: PCG32 (XSH-RR) words fill the
  * 32-byte seed, little endian. */
