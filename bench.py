@@ -46,6 +46,11 @@ def parse():
     ap.add_argument("--fp64-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=65536)
+    ap.add_argument("--bh-n-each", type=int, default=2097152)
+    ap.add_argument("--bh-steps", type=int, default=20)
+    ap.add_argument("--no-bh", action="store_true")
+    ap.add_argument("--probe-copy", action="store_true",
+                    help="only run a 1 GiB device copy (PMC byte-counter calibration)")
     return ap.parse_args()
 
 
@@ -80,8 +85,82 @@ def cpu_baseline(n_sample, eps):
                       f"{dt:.2f} s"}
 
 
+def probe_copy():
+    """A plain device-to-device copy of known size (1 GiB read + 1 GiB
+    written per launch) for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    a = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"probe": "copy", "bytes_read": a.numel() * 8, "bytes_written": b.numel() * 8}))
+
+
+def run_bh(args, g, torch, ctx):
+    """Barnes-Hut KDK steps on two Plummer spheres of n_each particles (seeds
+    42/43, a = 1, M = 1 each) 10 a apart on a parabolic approach, eps 0.01,
+    dt 5e-4, theta 0.5, leaf 16, groups of 64."""
+    import ctypes as C
+    ne = args.bh_n_each
+    n = 2 * ne
+    dev = torch.device("cuda:0")
+    log(f"bench: BH two Plummer spheres N={n}; generating ICs")
+    p = g.scenarios.plummer_pair(ne, 42, 43, 10.0, 0.0)
+    T = {f: torch.from_numpy(getattr(p, f)).to(dev) for f in ("x", "y", "z", "vx", "vy", "vz",
+                                                              "mass")}
+    for f in ("ax", "ay", "az"):
+        T[f] = torch.zeros(n, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    bh = g.BarnesHut(0.01, 0.5, 16, 64, ctx=ctx)
+    L = ctx._L
+    keys = ("x", "y", "z", "vx", "vy", "vz", "mass", "ax", "ay", "az")
+    d = bh.desc()
+
+    def kdk(steps):
+        ctx.check(L.gravis_kdk_device(ctx.handle, C.byref(d), n,
+                                      *[C.c_void_p(T[k].data_ptr()) for k in keys], 5e-4, steps))
+
+    ctx.check(L.gravis_accel_device(ctx.handle, C.byref(d), n,
+                                    *[C.c_void_p(T[k].data_ptr()) for k in
+                                      ("x", "y", "z", "mass", "ax", "ay", "az")]))
+    kdk(2)  # warmup
+    torch.cuda.synchronize()
+    log(f"bench: timing {args.bh_steps} BH KDK steps")
+    t0 = time.perf_counter()
+    for _ in range(args.bh_steps):
+        kdk(1)
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / args.bh_steps
+    ctx.set_profiling(True)
+    kdk(1)
+    ctx.set_profiling(False)
+    ms, launches = ctx.kernel_time("bh_walk")
+    walk_s = ms / 1e3 / max(1, launches)
+    st = ctx.bh_stats()
+    # Algorithmic bytes of one walk: per frontier node test a 16 B node
+    # record + 32 B (com, mass); per accepted node and per leaf-particle
+    # source a 32 B record read by the wave; per target 32 B in + 24 B out.
+    walk_bytes = st["node_tests"] * 48 + (st["accepted_nodes"] + st["leaf_particles"]) * 32 \
+        + n * 56
+    return {"n": n, "steps_per_s": round(1.0 / t_step, 3), "ms_per_step": round(t_step * 1e3, 3),
+            "steps": args.bh_steps, "theta": 0.5, "leaf_size": 16, "group_size": 64,
+            "softening": 0.01, "dt": 5e-4,
+            "walk_kernel_ms": round(walk_s * 1e3, 3),
+            "walk_stats": st,
+            "walk_pairs_per_s": st["pairs"] / walk_s,
+            "walk_achieved_tflops": round(FLOP_PER_INTERACTION * st["pairs"] / walk_s / 1e12, 3),
+            "walk_algorithmic_GBps": round(walk_bytes / walk_s / 1e9, 1),
+            "walk_frac_hbm_peak": round(walk_bytes / walk_s / 8.0e12, 4),
+            "workload": "two Plummer spheres (seeds 42/43, a=1, M=1 each) 10a apart, parabolic "
+                        "approach, KDK with Barnes-Hut theta=0.5, leaf 16, FP64"}
+
+
 def main():
     args = parse()
+    if args.probe_copy:
+        probe_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the multi-GPU layout is a design in DESIGN.md, not built)", file=sys.stderr)
@@ -188,6 +267,10 @@ def main():
         log(f"bench: CPU baseline on {args.cpu_sample} particles")
         cpu = cpu_baseline(args.cpu_sample, eps)
 
+    bh = None
+    if not args.no_bh:
+        bh = run_bh(args, g, torch, ctx)
+
     traffic = traffic_from_profiles(f"bf_mixed_n{n}")
     out = {
         "metric": METRIC,
@@ -219,6 +302,7 @@ def main():
                                          / PEAK_FP64_TFLOPS, 4)},
         "mixed_over_fp64": round(t_fp64 / t_mixed, 3),
         "force_error_vs_fp64": {"rms": err_rms, "max": err_max},
+        "barnes_hut": bh,
     }
     print(json.dumps(out), flush=True)
 

@@ -63,6 +63,13 @@ class Context:
     def set_profiling(self, on: bool):
         self.check(self._L.gravis_set_profiling(self.handle, 1 if on else 0))
 
+    def bh_stats(self) -> dict:
+        """Work counters of the last Barnes-Hut call."""
+        out = (C.c_uint64 * 5)()
+        self.check(self._L.gravis_bh_stats(self.handle, C.byref(out)))
+        return dict(zip(("node_tests", "accepted_nodes", "leaf_particles", "pairs", "nodes"),
+                        list(out)))
+
     def kernel_time(self, name: str):
         ms = C.c_double()
         n = C.c_uint64()
@@ -290,6 +297,19 @@ class _Scenarios:
         a = [np.zeros(2) for _ in range(7)]
         _lib.check(None, L.gravis_scenario_two_body(eccentricity, mass_ratio, semi_major_axis,
                                                     *[_ptr(v) for v in a]))
+        return Particles.from_arrays(*a)
+
+    @staticmethod
+    def plummer_pair(n_each: int, seed_a: int = 42, seed_b: int = 43, separation: float = 10.0,
+                     impact: float = 0.0, scale_radius: float = 1.0,
+                     total_mass: float = 1.0) -> Particles:
+        """Two Plummer spheres on a parabolic approach (layout of
+        galaxy_collision.rs:134-154)."""
+        L = _lib.load()
+        a = [np.zeros(2 * n_each) for _ in range(7)]
+        _lib.check(None, L.gravis_scenario_plummer_pair(n_each, scale_radius, total_mass, seed_a,
+                                                        seed_b, separation, impact,
+                                                        *[_ptr(v) for v in a]))
         return Particles.from_arrays(*a)
 
     @staticmethod

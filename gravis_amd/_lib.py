@@ -25,6 +25,7 @@ EXPORTS = (
     "gravis_kdk_device", "gravis_diagnostics", "gravis_diagnostics_device",
     "gravis_morton_sort", "gravis_scenario_plummer", "gravis_plummer_suggested_dt",
     "gravis_plummer_suggested_softening", "gravis_scenario_two_body", "gravis_two_body_period",
+    "gravis_scenario_plummer_pair", "gravis_bh_stats",
 )
 
 
@@ -85,6 +86,8 @@ def load() -> C.CDLL:
         "gravis_plummer_suggested_softening": (d, [sz, d]),
         "gravis_scenario_two_body": (st, [d, d, d] + [vp] * 7),
         "gravis_two_body_period": (d, [d]),
+        "gravis_scenario_plummer_pair": (st, [sz, d, d, C.c_uint64, C.c_uint64, d, d] + [vp] * 7),
+        "gravis_bh_stats": (st, [vp, C.POINTER(C.c_uint64 * 5)]),
     }
     for name, (res, args) in spec.items():
         f = getattr(L, name)

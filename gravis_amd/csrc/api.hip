@@ -401,6 +401,12 @@ gravis_status gravis_diagnostics(gravis_ctx* g, size_t n, double softening,
     return GRAVIS_OK;
 }
 
+gravis_status gravis_bh_stats(gravis_ctx* g, uint64_t out[5]) {
+    if (!g || !out) return GRAVIS_INVALID_ARG;
+    for (int k = 0; k < 5; ++k) out[k] = g->c.bh_stats[k];
+    return GRAVIS_OK;
+}
+
 gravis_status gravis_morton_sort(gravis_ctx* g, size_t n, const double* x, const double* y,
                                  const double* z, uint64_t* keys_sorted, uint32_t* perm,
                                  double* box_out) {

@@ -211,8 +211,12 @@ __global__ __launch_bounds__(kTreeBlock) void tree_com_kernel(double4* __restric
     }
 }
 
-__global__ void walk_reset_kernel(uint32_t* __restrict__ counter) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0u;
+__global__ void walk_reset_kernel(uint32_t* __restrict__ counter,
+                                  unsigned long long* __restrict__ stats) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        counter[0] = 0u;
+        for (int k = 0; k < 4; ++k) stats[k] = 0ull;
+    }
 }
 
 constexpr int kWalkWaves = 4;
@@ -233,8 +237,11 @@ __device__ __forceinline__ void pair_ref(const double4 s, double px, double py, 
 __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
     const double4* __restrict__ src, const uint4* __restrict__ meta, int n, int gs, int ngroups,
     const double* __restrict__ box, double theta2, double eps2, uint32_t* __restrict__ frontier, uint32_t fcap,
-    uint32_t* __restrict__ counter, uint32_t* __restrict__ flag, double* __restrict__ part) {
+    uint32_t* __restrict__ counter, uint32_t* __restrict__ flag, double* __restrict__ part,
+    unsigned long long* __restrict__ stats) {
     __shared__ uint32_t lists[kWalkWaves][kListCap];
+    // per-wave tallies: node tests, accepted nodes, leaf particles, pair interactions
+    unsigned long long t_nodes = 0, t_acc = 0, t_leafp = 0, t_pairs = 0;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
     uint32_t* list = lists[w];
@@ -274,8 +281,12 @@ __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
                     const uint32_t kb = __builtin_amdgcn_readfirstlane(md.x);
                     const uint32_t ke = kb + __builtin_amdgcn_readfirstlane(md.y);
                     for (uint32_t k = kb; k < ke; ++k) pair_ref(src[k], me.x, me.y, me.z, eps2, ax, ay, az);
+                    t_leafp += ke - kb;
+                    t_pairs += (unsigned long long)(ke - kb) * (unsigned long long)cnt;
                 } else {
                     pair_ref(src[(size_t)n + e], me.x, me.y, me.z, eps2, ax, ay, az);
+                    t_acc += 1;
+                    t_pairs += (unsigned long long)cnt;
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -295,6 +306,7 @@ __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
                 if (nl > (uint32_t)(kListCap - kWave)) flush();
                 const uint32_t idx = base + lane;
                 const bool valid = idx < nf;
+                t_nodes += min(nf - base, (uint32_t)kWave);
                 const uint32_t v = valid ? fa[idx] : 0u;
                 const uint4 md = meta[v];
                 const double4 c = src[(size_t)n + v];
@@ -344,6 +356,12 @@ __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
             part[(size_t)n + i] = ay;
             part[2 * (size_t)n + i] = az;
         }
+    }
+    if (lane == 0) {
+        atomicAdd(&stats[0], t_nodes);
+        atomicAdd(&stats[1], t_acc);
+        atomicAdd(&stats[2], t_leafp);
+        atomicAdd(&stats[3], t_pairs);
     }
 }
 
@@ -413,16 +431,22 @@ gravis_status bh_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
     GRAVIS_TRY(buf(c, "bh_part", 3 * n, &part));
     const double theta2 = s->theta * s->theta;
     const double eps2 = s->softening * s->softening;
-    walk_reset_kernel<<<1, 64, 0, st>>>(counter);
+    unsigned long long* stats;
+    GRAVIS_TRY(buf(c, "bh_stats", 4, &stats));
+    walk_reset_kernel<<<1, 64, 0, st>>>(counter, stats);
     timer_begin(c);
     bh_walk_kernel<<<blocks, kWalkWaves * kWave, 0, st>>>(src, meta, ni, gs, ngroups,
                                                           box, theta2, eps2, front, fcap,
-                                                          counter, flags, part);
+                                                          counter, flags, part, stats);
     timer_end(c, "bh_walk");
     GRAVIS_TRY(check_launch(c, "bh_walk"));
     uint32_t hflags[2] = {0, 0};
     GRAVIS_HIP(c, hipMemcpyAsync(hflags, flags, sizeof hflags, hipMemcpyDeviceToHost, st));
+    GRAVIS_HIP(c, hipMemcpyAsync(c->bh_stats, stats, sizeof c->bh_stats, hipMemcpyDeviceToHost, st));
+    uint32_t hlvl[2 * (kMaxLevel + 2)];
+    GRAVIS_HIP(c, hipMemcpyAsync(hlvl, lvl, sizeof hlvl, hipMemcpyDeviceToHost, st));
     GRAVIS_HIP(c, hipStreamSynchronize(st));
+    c->bh_stats[4] = hlvl[2 * (kMaxLevel + 1)];  // node count
     if (hflags[1])
         return fail(c, GRAVIS_INTERNAL, "octree node capacity exceeded (%zu nodes)", cap);
     if (hflags[0])

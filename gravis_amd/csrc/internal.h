@@ -42,6 +42,9 @@ struct Ctx {
     std::map<std::string, KernelTimer> timers;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int num_cus = 256;
+    // last Barnes-Hut call: node tests, accepted nodes, leaf particles,
+    // pair interactions, tree nodes
+    unsigned long long bh_stats[5] = {0, 0, 0, 0, 0};
 };
 
 // Status helpers ----------------------------------------------------------

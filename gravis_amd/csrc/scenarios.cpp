@@ -199,3 +199,29 @@ double gravis_two_body_period(double a) {
 }
 
 }  // extern "C"
+
+extern "C" gravis_status gravis_scenario_plummer_pair(size_t n_each, double scale_radius,
+                                                      double total_mass, uint64_t seed_a,
+                                                      uint64_t seed_b, double separation,
+                                                      double impact, double* x, double* y,
+                                                      double* z, double* vx, double* vy,
+                                                      double* vz, double* m) {
+    if (!(separation > 0.0) || !(impact >= 0.0)) return GRAVIS_INVALID_ARG;
+    gravis_status s = gravis_scenario_plummer(n_each, scale_radius, total_mass, seed_a, x, y, z,
+                                              vx, vy, vz, m);
+    if (s != GRAVIS_OK) return s;
+    const size_t o = n_each;
+    s = gravis_scenario_plummer(n_each, scale_radius, total_mass, seed_b, x + o, y + o, z + o,
+                                vx + o, vy + o, vz + o, m + o);
+    if (s != GRAVIS_OK) return s;
+    // parabolic approach: E_orb = 0 at the initial separation
+    const double r0 = std::sqrt(separation * separation + impact * impact);
+    const double vrel = std::sqrt(2.0 * 1.0 * (2.0 * total_mass) / r0);
+    const double hs = separation / 2.0, hb = impact / 2.0, hv = vrel / 2.0;
+    for (size_t i = 0; i < n_each; ++i) {
+        x[i] -= hs; y[i] -= hb; vx[i] += hv;
+        x[o + i] += hs; y[o + i] += hb; vx[o + i] -= hv;
+    }
+    com_shift(2 * n_each, x, y, z, vx, vy, vz, m);
+    return GRAVIS_OK;
+}

@@ -61,15 +61,38 @@ __global__ __launch_bounds__(kBoxBlock) void box_partial_kernel(const double* __
     }
 }
 
-__global__ void box_final_kernel(const double* __restrict__ part, int nparts,
-                                 double* __restrict__ box) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One block of 256 threads folds the partial boxes, then thread 0 applies
+// the cube arithmetic.
+__global__ __launch_bounds__(kBoxBlock) void box_final_kernel(const double* __restrict__ part,
+                                                              int nparts,
+                                                              double* __restrict__ box) {
     double mn[3] = {INFINITY, INFINITY, INFINITY};
     double mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int b = 0; b < nparts; ++b)
+    for (int b = threadIdx.x; b < nparts; b += kBoxBlock)
         for (int d = 0; d < 3; ++d) {
             mn[d] = fmin(mn[d], part[b * 6 + d]);
             mx[d] = fmax(mx[d], part[b * 6 + 3 + d]);
+        }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = fmin(mn[d], __shfl_xor(mn[d], off));
+            mx[d] = fmax(mx[d], __shfl_xor(mx[d], off));
+        }
+    __shared__ double s[kBoxBlock / kWave][6];
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0)
+        for (int d = 0; d < 3; ++d) {
+            s[w][d] = mn[d];
+            s[w][3 + d] = mx[d];
+        }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    for (int k = 1; k < kBoxBlock / kWave; ++k)
+        for (int d = 0; d < 3; ++d) {
+            mn[d] = fmin(mn[d], s[k][d]);
+            mx[d] = fmax(mx[d], s[k][3 + d]);
         }
     // octree.rs:29-41, same operation order (no contraction in this build).
     box[0] = 0.5 * (mn[0] + mx[0]);
@@ -256,7 +279,7 @@ gravis_status root_box(Ctx* c, size_t n, const double* x, const double* y, const
     GRAVIS_TRY(buf(c, "box_part", kBoxMaxBlocks * 6, &part));
     const unsigned g = grid_for(n, kBoxBlock, kBoxMaxBlocks);
     box_partial_kernel<<<g, kBoxBlock, 0, c->stream>>>(x, y, z, n, part);
-    box_final_kernel<<<1, 64, 0, c->stream>>>(part, (int)g, d_box);
+    box_final_kernel<<<1, kBoxBlock, 0, c->stream>>>(part, (int)g, d_box);
     return check_launch(c, "root_box");
 }
 

@@ -148,6 +148,11 @@ gravis_status gravis_set_profiling(gravis_ctx* ctx, int enable);
 gravis_status gravis_kernel_time(gravis_ctx* ctx, const char* name, double* ms_total,
                                  uint64_t* launches);
 
+/* Work counters of the last Barnes-Hut force call: out[0] frontier node
+ * tests, out[1] accepted (monopole) nodes, out[2] leaf particles used as
+ * sources, out[3] target-source pair interactions, out[4] tree nodes. */
+gravis_status gravis_bh_stats(gravis_ctx* ctx, uint64_t out[5]);
+
 /* Morton keys (63-bit, 21 bits per axis over the root cube of
  * octree.rs:14-44) and their stable sort permutation, device computed,
  * copied to host. box_out = (cx, cy, cz, half_width). */
@@ -173,6 +178,18 @@ gravis_status gravis_scenario_two_body(double eccentricity, double mass_ratio,
                                        double semi_major_axis, double* x, double* y, double* z,
                                        double* vx, double* vy, double* vz, double* m);
 double gravis_two_body_period(double semi_major_axis);
+/* Two Plummer spheres (n_each particles, scale radius a, mass M each;
+ * seeds seed_a, seed_b) whose centres are `separation` apart along x with
+ * offset `impact` along y, on a parabolic approach: relative speed
+ * sqrt(2 G 2M / sqrt(separation^2 + impact^2)), centre-of-mass frame.
+ * Sphere A fills indices [0, n_each), B [n_each, 2 n_each). The layout
+ * follows GalaxyCollision (crates/sim-core/src/scenarios/
+ * galaxy_collision.rs:134-154), with Plummer spheres in place of disks. */
+gravis_status gravis_scenario_plummer_pair(size_t n_each, double scale_radius,
+                                           double total_mass, uint64_t seed_a, uint64_t seed_b,
+                                           double separation, double impact, double* x,
+                                           double* y, double* z, double* vx, double* vy,
+                                           double* vz, double* m);
 
 #ifdef __cplusplus
 }

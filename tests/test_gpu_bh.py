@@ -175,3 +175,27 @@ def test_bh_large_n_against_brute_force_sample(g):
     rms, mx = rel_errors(ref, p.acc()[t])
     say(f"  rms={rms:.3e} max={mx:.3e}")
     assert rms <= 4.942351e-3 and mx <= 3.588391e-2
+
+
+def test_bh_two_sphere_4m_parity(g):
+    """The Barnes-Hut benchmark workload at full size: two Plummer spheres of
+    2,097,152 particles (seeds 42/43) 10a apart. Morton keys and the sort
+    permutation are bit-exact with the oracle; forces on 256 random targets
+    are within the reference's theta=0.5 error of f64 brute force."""
+    say("bh two-sphere N=4194304: keys/permutation and force sample")
+    ne = 2097152
+    p = g.scenarios.plummer_pair(ne, 42, 43, 10.0, 0.0)
+    keys, perm, box = g.morton_sort(p.x, p.y, p.z)
+    rbox = O.root_box(p.x, p.y, p.z)
+    assert (box == rbox).all()
+    rk, rp = O.sort_keys(O.morton63(p.x, p.y, p.z, rbox))
+    assert (keys == rk).all() and (perm == rp).all()
+    g.BarnesHut(0.01, 0.5).compute_accelerations(p)
+    s = O.State(p.count)
+    s.x, s.y, s.z, s.m = p.x, p.y, p.z, p.mass
+    rng = np.random.default_rng(42)
+    t = np.sort(rng.choice(p.count, 256, replace=False))
+    ref, _ = O.bf_targets(s, 0.01, t, nthreads=16)
+    rms, mx = rel_errors(ref, p.acc()[t])
+    say(f"  rms={rms:.3e} max={mx:.3e} nodes={g.default_context().bh_stats()['nodes']}")
+    assert rms <= 4.942351e-3 and mx <= 3.588391e-2
