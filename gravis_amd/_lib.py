@@ -25,7 +25,8 @@ EXPORTS = (
     "gravis_kdk_device", "gravis_diagnostics", "gravis_diagnostics_device",
     "gravis_morton_sort", "gravis_scenario_plummer", "gravis_plummer_suggested_dt",
     "gravis_plummer_suggested_softening", "gravis_scenario_two_body", "gravis_two_body_period",
-    "gravis_scenario_plummer_pair", "gravis_bh_stats",
+    "gravis_scenario_plummer_pair", "gravis_bh_stats", "gravis_ensemble_kdk",
+    "gravis_ensemble_kdk_device", "gravis_ensemble_accel_device", "gravis_ensemble_adjoint", "gravis_ensemble_adjoint_device",
 )
 
 
@@ -88,6 +89,16 @@ def load() -> C.CDLL:
         "gravis_two_body_period": (d, [d]),
         "gravis_scenario_plummer_pair": (st, [sz, d, d, C.c_uint64, C.c_uint64, d, d] + [vp] * 7),
         "gravis_bh_stats": (st, [vp, C.POINTER(C.c_uint64 * 5)]),
+        "gravis_ensemble_kdk": (st, [vp, C.POINTER(Solver), i32, vp] + [vp] * 10
+                                + [d, C.c_uint64, i32]),
+        "gravis_ensemble_accel_device": (st, [vp, C.POINTER(Solver), i32, vp] + [vp] * 7),
+        "gravis_ensemble_kdk_device": (st, [vp, C.POINTER(Solver), i32, vp] + [vp] * 10
+                                       + [d, C.c_uint64, i32]),
+        "gravis_ensemble_adjoint": (st, [vp, C.POINTER(Solver), i32, vp] + [vp] * 7
+                                    + [d, C.c_uint64, C.c_uint32] + [vp] * 9 + [vp] * 9),
+        "gravis_ensemble_adjoint_device": (st, [vp, C.POINTER(Solver), i32, vp] + [vp] * 7
+                                           + [d, C.c_uint64, C.c_uint32] + [vp] * 9
+                                           + [vp] * 9),
     }
     for name, (res, args) in spec.items():
         f = getattr(L, name)

@@ -401,6 +401,130 @@ gravis_status gravis_diagnostics(gravis_ctx* g, size_t n, double softening,
     return GRAVIS_OK;
 }
 
+gravis_status gravis_ensemble_accel_device(gravis_ctx* g, const gravis_solver* s, int nsys,
+                                           const uint64_t* offsets, const double* x,
+                                           const double* y, const double* z, const double* m,
+                                           double* ax, double* ay, double* az) {
+    if (!g) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    if (any_null({offsets, x, y, z, m, ax, ay, az}))
+        return fail(c, GRAVIS_INVALID_ARG, "null array");
+    GRAVIS_HIP(c, hipSetDevice(c->device));
+    return gravis::ensemble_accel(c, s, nsys, offsets, x, y, z, m, ax, ay, az);
+}
+
+gravis_status gravis_ensemble_kdk_device(gravis_ctx* g, const gravis_solver* s, int nsys,
+                                         const uint64_t* offsets, double* x, double* y,
+                                         double* z, double* vx, double* vy, double* vz,
+                                         const double* m, double* ax, double* ay, double* az,
+                                         double dt, uint64_t nsteps, int use_graph) {
+    if (!g) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    if (any_null({offsets, x, y, z, vx, vy, vz, m, ax, ay, az}))
+        return fail(c, GRAVIS_INVALID_ARG, "null array");
+    GRAVIS_HIP(c, hipSetDevice(c->device));
+    return gravis::ensemble_kdk(c, s, nsys, offsets, x, y, z, vx, vy, vz, m, ax, ay, az, dt,
+                                nsteps, use_graph);
+}
+
+gravis_status gravis_ensemble_kdk(gravis_ctx* g, const gravis_solver* s, int nsys,
+                                  const uint64_t* offsets, double* x, double* y, double* z,
+                                  double* vx, double* vy, double* vz, const double* m,
+                                  double* ax, double* ay, double* az, double dt,
+                                  uint64_t nsteps, int use_graph) {
+    if (!g) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    if (any_null({offsets, x, y, z, vx, vy, vz, m, ax, ay, az}) || nsys <= 0)
+        return fail(c, GRAVIS_INVALID_ARG, "null array or empty ensemble");
+    const size_t n = offsets[nsys];
+    if (n == 0) return GRAVIS_OK;
+    GRAVIS_HIP(c, hipSetDevice(c->device));
+    HostState d;
+    GRAVIS_TRY(stage(c, n, &d));
+    const double* hin[10] = {x, y, z, vx, vy, vz, m, ax, ay, az};
+    double* din[10] = {d.x, d.y, d.z, d.vx, d.vy, d.vz, d.m, d.ax, d.ay, d.az};
+    for (int k = 0; k < 10; ++k) GRAVIS_TRY(h2d(c, din[k], hin[k], n));
+    GRAVIS_TRY(gravis::ensemble_kdk(c, s, nsys, offsets, d.x, d.y, d.z, d.vx, d.vy, d.vz, d.m,
+                                    d.ax, d.ay, d.az, dt, nsteps, use_graph));
+    double* hout[9] = {x, y, z, vx, vy, vz, ax, ay, az};
+    double* dout[9] = {d.x, d.y, d.z, d.vx, d.vy, d.vz, d.ax, d.ay, d.az};
+    for (int k = 0; k < 9; ++k) GRAVIS_TRY(d2h(c, hout[k], dout[k], n));
+    GRAVIS_HIP(c, hipStreamSynchronize(c->stream));
+    return GRAVIS_OK;
+}
+
+gravis_status gravis_ensemble_adjoint_device(
+    gravis_ctx* g, const gravis_solver* s, int nsys, const uint64_t* offsets, const double* x0,
+    const double* y0, const double* z0, const double* vx0, const double* vy0, const double* vz0,
+    const double* m, double dt, uint64_t nsteps, uint32_t ckpt, const double* tx,
+    const double* ty, const double* tz, const double* gx, const double* gy, const double* gz,
+    const double* gvx, const double* gvy, const double* gvz, double* loss, double* final_state,
+    double* ox, double* oy, double* oz, double* ovx, double* ovy, double* ovz, double* om) {
+    if (!g) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    if (any_null({offsets, x0, y0, z0, vx0, vy0, vz0, m, ox, oy, oz, ovx, ovy, ovz, om}))
+        return fail(c, GRAVIS_INVALID_ARG, "null array");
+    if (tx ? (!ty || !tz) : (!gx || !gy || !gz))
+        return fail(c, GRAVIS_INVALID_ARG, "need all three targets or all three cotangents");
+    if (gvx && (!gvy || !gvz)) return fail(c, GRAVIS_INVALID_ARG, "partial velocity cotangent");
+    GRAVIS_HIP(c, hipSetDevice(c->device));
+    return gravis::ensemble_adjoint(c, s, nsys, offsets, x0, y0, z0, vx0, vy0, vz0, m, dt, nsteps,
+                                    ckpt, tx, ty, tz, gx, gy, gz, gvx, gvy, gvz, loss,
+                                    final_state, ox, oy, oz, ovx, ovy, ovz, om);
+}
+
+gravis_status gravis_ensemble_adjoint(
+    gravis_ctx* g, const gravis_solver* s, int nsys, const uint64_t* offsets, const double* x0,
+    const double* y0, const double* z0, const double* vx0, const double* vy0, const double* vz0,
+    const double* m, double dt, uint64_t nsteps, uint32_t ckpt, const double* tx,
+    const double* ty, const double* tz, const double* gx, const double* gy, const double* gz,
+    const double* gvx, const double* gvy, const double* gvz, double* loss, double* final_state,
+    double* ox, double* oy, double* oz, double* ovx, double* ovy, double* ovz, double* om) {
+    if (!g) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    if (!offsets || nsys <= 0) return fail(c, GRAVIS_INVALID_ARG, "empty ensemble");
+    if (any_null({x0, y0, z0, vx0, vy0, vz0, m, ox, oy, oz, ovx, ovy, ovz, om}))
+        return fail(c, GRAVIS_INVALID_ARG, "null array");
+    if (tx ? (!ty || !tz) : (!gx || !gy || !gz))
+        return fail(c, GRAVIS_INVALID_ARG, "need all three targets or all three cotangents");
+    if (gvx && (!gvy || !gvz)) return fail(c, GRAVIS_INVALID_ARG, "partial velocity cotangent");
+    const size_t n = offsets[nsys];
+    if (n == 0) return GRAVIS_OK;
+    GRAVIS_HIP(c, hipSetDevice(c->device));
+    // inputs: 7 state + 3 target/cotangent + 3 velocity cotangent; outputs 7 + 6
+    double* base;
+    GRAVIS_TRY(gravis::buf(c, "adj_host", 26 * n + (size_t)nsys, &base));
+    double* din[7];
+    const double* hin[7] = {x0, y0, z0, vx0, vy0, vz0, m};
+    for (int k = 0; k < 7; ++k) {
+        din[k] = base + k * n;
+        GRAVIS_TRY(h2d(c, din[k], hin[k], n));
+    }
+    double* dt3 = base + 7 * n;
+    const double* hsrc[3] = {tx ? tx : gx, tx ? ty : gy, tx ? tz : gz};
+    for (int k = 0; k < 3; ++k) GRAVIS_TRY(h2d(c, dt3 + k * n, hsrc[k], n));
+    double* dgv = base + 10 * n;
+    if (gvx) {
+        const double* hv[3] = {gvx, gvy, gvz};
+        for (int k = 0; k < 3; ++k) GRAVIS_TRY(h2d(c, dgv + k * n, hv[k], n));
+    }
+    double* dout = base + 13 * n;       // 7 gradients
+    double* dfin = base + 20 * n;       // 6 final state
+    double* dloss = base + 26 * n;
+    GRAVIS_TRY(gravis::ensemble_adjoint(
+        c, s, nsys, offsets, din[0], din[1], din[2], din[3], din[4], din[5], din[6], dt, nsteps,
+        ckpt, tx ? dt3 : nullptr, tx ? dt3 + n : nullptr, tx ? dt3 + 2 * n : nullptr,
+        tx ? nullptr : dt3, tx ? nullptr : dt3 + n, tx ? nullptr : dt3 + 2 * n,
+        gvx ? dgv : nullptr, gvx ? dgv + n : nullptr, gvx ? dgv + 2 * n : nullptr, dloss, dfin,
+        dout, dout + n, dout + 2 * n, dout + 3 * n, dout + 4 * n, dout + 5 * n, dout + 6 * n));
+    double* hout[7] = {ox, oy, oz, ovx, ovy, ovz, om};
+    for (int k = 0; k < 7; ++k) GRAVIS_TRY(d2h(c, hout[k], dout + k * n, n));
+    if (final_state) GRAVIS_TRY(d2h(c, final_state, dfin, 6 * n));
+    if (loss && tx) GRAVIS_TRY(d2h(c, loss, dloss, (size_t)nsys));
+    GRAVIS_HIP(c, hipStreamSynchronize(c->stream));
+    return GRAVIS_OK;
+}
+
 gravis_status gravis_bh_stats(gravis_ctx* g, uint64_t out[5]) {
     if (!g || !out) return GRAVIS_INVALID_ARG;
     for (int k = 0; k < 5; ++k) out[k] = g->c.bh_stats[k];

@@ -442,7 +442,7 @@ gravis_status bh_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
     GRAVIS_TRY(check_launch(c, "bh_walk"));
     uint32_t hflags[2] = {0, 0};
     GRAVIS_HIP(c, hipMemcpyAsync(hflags, flags, sizeof hflags, hipMemcpyDeviceToHost, st));
-    GRAVIS_HIP(c, hipMemcpyAsync(c->bh_stats, stats, sizeof c->bh_stats, hipMemcpyDeviceToHost, st));
+    GRAVIS_HIP(c, hipMemcpyAsync(c->bh_stats, stats, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     uint32_t hlvl[2 * (kMaxLevel + 2)];
     GRAVIS_HIP(c, hipMemcpyAsync(hlvl, lvl, sizeof hlvl, hipMemcpyDeviceToHost, st));
     GRAVIS_HIP(c, hipStreamSynchronize(st));

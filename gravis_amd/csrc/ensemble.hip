@@ -1,0 +1,480 @@
+// ensemble.hip -- batched ensemble of independent N-body systems and the
+// discrete adjoint of its KDK trajectory (no counterpart in the reference).
+//
+// Forward: every system runs sim-core's LeapfrogKDK (crates/sim-core/src/
+// integrator.rs:41-77) with softened brute force (gravity.rs:47-80) in the
+// gather form. All systems advance together: one kick-drift launch over the
+// concatenated arrays and ONE force launch whose workgroups cover every
+// (system, target block); the two launches of a step are captured once as a
+// hipGraph and replayed nsteps times.
+//
+// Reverse: for cotangents (lx_T, lv_T) the adjoint of
+//   vh = v + h a(x);  x' = x + dt vh;  v' = vh + h a(x')       (h = dt/2)
+// is, per step from the last,
+//   lvh = lv' + dt lx'  ;  lx += J(x)^T (w lvh)  ;  lv = lvh
+// with w = 2h at interior time levels (the K1 of step t and the K2 of step
+// t-1 share a(x_t)), h at t = 0, and a first J^T(x_T)(h lv_T) term. With
+// d = x_j - x_k, s = |d|^2 + eps^2, w = s^{-3/2}, v = m_k u_j - m_j u_k:
+//   (J_x^T u)_k = sum_j w v - 3 s^{-5/2} d (d . v)
+//   (J_m^T u)_k = - sum_j w (u_j . d)
+// computed by ens_vjp_kernel (FP64). The trajectory is recomputed from
+// checkpoints stored every `ckpt` steps, one segment at a time.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "internal.h"
+
+namespace gravis {
+namespace {
+
+constexpr int kEnsBlock = 256;
+
+struct EnsBlock {
+    int i0, i1, j0, j1;  // targets [i0, i1), sources [j0, j1) of one system
+};
+
+template <bool MIXED>
+__global__ __launch_bounds__(kEnsBlock) void ens_force_kernel(
+    const EnsBlock* __restrict__ blocks, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ m,
+    double eps2, double h, int kick, double* __restrict__ ax, double* __restrict__ ay,
+    double* __restrict__ az, double* __restrict__ vx, double* __restrict__ vy,
+    double* __restrict__ vz) {
+    const EnsBlock b = blocks[blockIdx.x];
+    const int i = b.i0 + threadIdx.x;
+    const bool active = i < b.i1;
+    const int ii = active ? i : b.i0;
+    const double px = x[ii], py = y[ii], pz = z[ii];
+    if (MIXED) {
+        __shared__ float4 tile[kEnsBlock];
+        // block-relative FP32 coordinates (systems are centred on their COM)
+        double lo[3] = {px, py, pz}, hi[3] = {px, py, pz};
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = fmin(lo[d], __shfl_xor(lo[d], o));
+                hi[d] = fmax(hi[d], __shfl_xor(hi[d], o));
+            }
+        __shared__ double sb[kEnsBlock / kWave][6];
+        if ((threadIdx.x & (kWave - 1)) == 0)
+            for (int d = 0; d < 3; ++d) { sb[threadIdx.x / kWave][d] = lo[d]; sb[threadIdx.x / kWave][3 + d] = hi[d]; }
+        __syncthreads();
+        for (int d = 0; d < 3; ++d) { lo[d] = sb[0][d]; hi[d] = sb[0][3 + d]; }
+        for (int w = 1; w < kEnsBlock / kWave; ++w)
+            for (int d = 0; d < 3; ++d) { lo[d] = fmin(lo[d], sb[w][d]); hi[d] = fmax(hi[d], sb[w][3 + d]); }
+        const double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]),
+                     cz = 0.5 * (lo[2] + hi[2]);
+        const float xi = (float)(px - cx), yi = (float)(py - cy), zi = (float)(pz - cz);
+        const float e2 = (float)eps2;
+        float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
+        for (int jb = b.j0; jb < b.j1; jb += kEnsBlock) {
+            const int j = jb + threadIdx.x;
+            __syncthreads();
+            tile[threadIdx.x] = j < b.j1 ? make_float4((float)(x[j] - cx), (float)(y[j] - cy),
+                                                       (float)(z[j] - cz), (float)m[j])
+                                         : make_float4(0.f, 0.f, 0.f, 0.f);
+            __syncthreads();
+            const int cnt = min(kEnsBlock, b.j1 - jb);
+            float tx = 0.f, ty = 0.f, tz = 0.f;
+            for (int k = 0; k < cnt; ++k) {
+                const float4 s = tile[k];
+                const float dx = s.x - xi, dy = s.y - yi, dz = s.z - zi;
+                const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, e2)));
+                float ri = __builtin_amdgcn_rsqf(r2);
+                ri = r2 > 0.f ? ri : 0.f;
+                const float f = (s.w * ri) * (ri * ri);
+                tx = __builtin_fmaf(f, dx, tx);
+                ty = __builtin_fmaf(f, dy, ty);
+                tz = __builtin_fmaf(f, dz, tz);
+            }
+            float yk = tx - kx, t = sx + yk; kx = (t - sx) - yk; sx = t;
+            yk = ty - ky; t = sy + yk; ky = (t - sy) - yk; sy = t;
+            yk = tz - kz; t = sz + yk; kz = (t - sz) - yk; sz = t;
+        }
+        if (active) {
+            const double a0 = (double)sx - (double)kx, a1 = (double)sy - (double)ky,
+                         a2 = (double)sz - (double)kz;
+            ax[i] = a0; ay[i] = a1; az[i] = a2;
+            if (kick) {
+                vx[i] = vx[i] + a0 * h;
+                vy[i] = vy[i] + a1 * h;
+                vz[i] = vz[i] + a2 * h;
+            }
+        }
+    } else {
+        __shared__ double4 tile[kEnsBlock];
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int jb = b.j0; jb < b.j1; jb += kEnsBlock) {
+            const int j = jb + threadIdx.x;
+            __syncthreads();
+            tile[threadIdx.x] = j < b.j1 ? make_double4(x[j], y[j], z[j], m[j])
+                                         : make_double4(0, 0, 0, 0);
+            __syncthreads();
+            const int cnt = min(kEnsBlock, b.j1 - jb);
+            for (int k = 0; k < cnt; ++k) {
+                const double4 s = tile[k];
+                const double dx = s.x - px, dy = s.y - py, dz = s.z - pz;
+                const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, eps2)));
+                const double ri = (jb + k == i) ? 0.0 : rsqrt(r2);
+                const double f = (s.w * ri) * (ri * ri);
+                sx = __builtin_fma(f, dx, sx);
+                sy = __builtin_fma(f, dy, sy);
+                sz = __builtin_fma(f, dz, sz);
+            }
+        }
+        if (active) {
+            ax[i] = sx; ay[i] = sy; az[i] = sz;
+            if (kick) {
+                vx[i] = vx[i] + sx * h;
+                vy[i] = vy[i] + sy * h;
+                vz[i] = vz[i] + sz * h;
+            }
+        }
+    }
+}
+
+// lx += J_x^T u, lm += J_m^T u (FP64).
+__global__ __launch_bounds__(kEnsBlock) void ens_vjp_kernel(
+    const EnsBlock* __restrict__ blocks, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ m,
+    const double* __restrict__ ux, const double* __restrict__ uy, const double* __restrict__ uz,
+    double eps2, double* __restrict__ lx, double* __restrict__ ly, double* __restrict__ lz,
+    double* __restrict__ lm) {
+    __shared__ double4 tp[kEnsBlock];
+    __shared__ double4 tu[kEnsBlock];
+    const EnsBlock b = blocks[blockIdx.x];
+    const int k = b.i0 + threadIdx.x;
+    const bool active = k < b.i1;
+    const int kk = active ? k : b.i0;
+    const double px = x[kk], py = y[kk], pz = z[kk], mk = m[kk];
+    const double u0 = ux[kk], u1 = uy[kk], u2 = uz[kk];
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0, gm = 0.0;
+    for (int jb = b.j0; jb < b.j1; jb += kEnsBlock) {
+        const int j = jb + threadIdx.x;
+        __syncthreads();
+        if (j < b.j1) {
+            tp[threadIdx.x] = make_double4(x[j], y[j], z[j], m[j]);
+            tu[threadIdx.x] = make_double4(ux[j], uy[j], uz[j], 0.0);
+        } else {
+            tp[threadIdx.x] = make_double4(0, 0, 0, 0);
+            tu[threadIdx.x] = make_double4(0, 0, 0, 0);
+        }
+        __syncthreads();
+        const int cnt = min(kEnsBlock, b.j1 - jb);
+        for (int q = 0; q < cnt; ++q) {
+            if (jb + q == k) continue;
+            const double4 s = tp[q];
+            const double4 uj = tu[q];
+            const double dx = s.x - px, dy = s.y - py, dz = s.z - pz;
+            const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, eps2)));
+            const double ri = rsqrt(r2);
+            const double ri2 = ri * ri;
+            const double w = ri2 * ri;
+            const double qq = 3.0 * w * ri2;
+            const double v0 = mk * uj.x - s.w * u0;
+            const double v1 = mk * uj.y - s.w * u1;
+            const double v2 = mk * uj.z - s.w * u2;
+            const double dv = __builtin_fma(dx, v0, __builtin_fma(dy, v1, dz * v2));
+            const double c = qq * dv;
+            g0 += w * v0 - c * dx;
+            g1 += w * v1 - c * dy;
+            g2 += w * v2 - c * dz;
+            gm -= w * __builtin_fma(uj.x, dx, __builtin_fma(uj.y, dy, uj.z * dz));
+        }
+    }
+    if (active) {
+        lx[k] += g0;
+        ly[k] += g1;
+        lz[k] += g2;
+        lm[k] += gm;
+    }
+}
+
+// lv = lv + dt lx (skipped when dt == 0); u = wgt * lv
+__global__ __launch_bounds__(256) void adj_level_kernel(size_t n, const double* __restrict__ lx,
+                                                        const double* __restrict__ ly,
+                                                        const double* __restrict__ lz,
+                                                        double* __restrict__ lv, double dt,
+                                                        double wgt, double* __restrict__ u) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const double l[3] = {lx[i], ly[i], lz[i]};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            double v = lv[c * n + i];
+            if (dt != 0.0) v = v + dt * l[c];
+            lv[c * n + i] = v;
+            u[c * n + i] = wgt * v;
+        }
+    }
+}
+
+// cotangent of L = sum |x_T - target|^2 and per-system loss partials
+__global__ __launch_bounds__(256) void adj_loss_kernel(const EnsBlock* __restrict__ sys, int nsys,
+                                                       const double* __restrict__ x,
+                                                       const double* __restrict__ y,
+                                                       const double* __restrict__ z,
+                                                       const double* __restrict__ tx,
+                                                       const double* __restrict__ ty,
+                                                       const double* __restrict__ tz,
+                                                       double* __restrict__ gx,
+                                                       double* __restrict__ gy,
+                                                       double* __restrict__ gz,
+                                                       double* __restrict__ loss) {
+    const int s = blockIdx.x;
+    if (s >= nsys) return;
+    const EnsBlock b = sys[s];
+    double acc = 0.0;
+    for (int i = b.j0 + threadIdx.x; i < b.j1; i += blockDim.x) {
+        const double d0 = x[i] - tx[i], d1 = y[i] - ty[i], d2 = z[i] - tz[i];
+        gx[i] = 2.0 * d0;
+        gy[i] = 2.0 * d1;
+        gz[i] = 2.0 * d2;
+        acc += d0 * d0 + d1 * d1 + d2 * d2;
+    }
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < (unsigned)o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss[s] = red[0];
+}
+
+struct EnsPlan {
+    std::vector<EnsBlock> blocks;  // force/vjp work items, largest systems first
+    std::vector<EnsBlock> systems; // one per system (j0, j1)
+    size_t n = 0;
+};
+
+gravis_status make_plan(Ctx* c, int nsys, const uint64_t* offsets, EnsPlan* p) {
+    if (nsys <= 0 || !offsets) return fail(c, GRAVIS_INVALID_ARG, "empty ensemble");
+    if (offsets[0] != 0) return fail(c, GRAVIS_INVALID_ARG, "offsets[0] must be 0");
+    std::vector<int> order(nsys);
+    for (int s = 0; s < nsys; ++s) {
+        if (offsets[s + 1] < offsets[s])
+            return fail(c, GRAVIS_INVALID_ARG, "offsets must be non-decreasing");
+        order[s] = s;
+        p->systems.push_back({(int)offsets[s], (int)offsets[s + 1], (int)offsets[s],
+                              (int)offsets[s + 1]});
+    }
+    if (offsets[nsys] > 0x7FFFFFFFull) return fail(c, GRAVIS_INVALID_ARG, "ensemble too large");
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return offsets[a + 1] - offsets[a] > offsets[b + 1] - offsets[b];
+    });
+    for (int s : order) {
+        const int j0 = (int)offsets[s], j1 = (int)offsets[s + 1];
+        for (int i0 = j0; i0 < j1; i0 += kEnsBlock)
+            p->blocks.push_back({i0, std::min(i0 + kEnsBlock, j1), j0, j1});
+    }
+    p->n = offsets[nsys];
+    return GRAVIS_OK;
+}
+
+gravis_status upload(Ctx* c, const char* name, const std::vector<EnsBlock>& v, EnsBlock** out) {
+    GRAVIS_TRY(buf(c, name, std::max<size_t>(v.size(), 1), out));
+    if (!v.empty())
+        GRAVIS_HIP(c, hipMemcpyAsync(*out, v.data(), v.size() * sizeof(EnsBlock),
+                                     hipMemcpyHostToDevice, c->stream));
+    return GRAVIS_OK;
+}
+
+struct EnsArgs {
+    double *x, *y, *z, *vx, *vy, *vz, *ax, *ay, *az;
+    const double* m;
+};
+
+gravis_status ens_force(Ctx* c, bool mixed, const EnsBlock* d_blocks, int nblocks,
+                        const EnsArgs& a, double eps2, double h, int kick) {
+    if (nblocks == 0) return GRAVIS_OK;
+    if (mixed)
+        ens_force_kernel<true><<<nblocks, kEnsBlock, 0, c->stream>>>(
+            d_blocks, a.x, a.y, a.z, a.m, eps2, h, kick, a.ax, a.ay, a.az, a.vx, a.vy, a.vz);
+    else
+        ens_force_kernel<false><<<nblocks, kEnsBlock, 0, c->stream>>>(
+            d_blocks, a.x, a.y, a.z, a.m, eps2, h, kick, a.ax, a.ay, a.az, a.vx, a.vy, a.vz);
+    return check_launch(c, "ens_force");
+}
+
+gravis_status ens_step(Ctx* c, bool mixed, const EnsBlock* d_blocks, int nblocks, size_t n,
+                       const EnsArgs& a, double eps2, double dt) {
+    const double h = 0.5 * dt;
+    GRAVIS_TRY(kick_drift(c, n, a.x, a.y, a.z, a.vx, a.vy, a.vz, a.ax, a.ay, a.az, h, dt));
+    return ens_force(c, mixed, d_blocks, nblocks, a, eps2, h, 1);
+}
+
+gravis_status check_solver_ens(Ctx* c, const gravis_solver* s) {
+    if (!s) return fail(c, GRAVIS_INVALID_ARG, "solver is NULL");
+    if (s->algorithm != GRAVIS_BRUTE_FORCE)
+        return fail(c, GRAVIS_UNSUPPORTED, "the ensemble runs brute-force gravity only");
+    if (s->precision != GRAVIS_FP64 && s->precision != GRAVIS_MIXED)
+        return fail(c, GRAVIS_INVALID_ARG, "unknown precision %d", s->precision);
+    if (!(s->softening >= 0.0)) return fail(c, GRAVIS_INVALID_ARG, "softening must be >= 0");
+    return GRAVIS_OK;
+}
+
+}  // namespace
+
+gravis_status ensemble_kdk(Ctx* c, const gravis_solver* s, int nsys, const uint64_t* offsets,
+                           double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                           const double* m, double* ax, double* ay, double* az, double dt,
+                           uint64_t nsteps, int use_graph) {
+    GRAVIS_TRY(check_solver_ens(c, s));
+    if (!(dt > 0.0)) return fail(c, GRAVIS_INVALID_ARG, "dt must be positive");
+    EnsPlan plan;
+    GRAVIS_TRY(make_plan(c, nsys, offsets, &plan));
+    if (plan.n == 0 || nsteps == 0) return GRAVIS_OK;
+    EnsBlock* d_blocks;
+    GRAVIS_TRY(upload(c, "ens_blocks", plan.blocks, &d_blocks));
+    const EnsArgs a{x, y, z, vx, vy, vz, ax, ay, az, m};
+    const bool mixed = s->precision == GRAVIS_MIXED;
+    const double eps2 = s->softening * s->softening;
+    const int nb = (int)plan.blocks.size();
+    if (!use_graph) {
+        for (uint64_t t = 0; t < nsteps; ++t)
+            GRAVIS_TRY(ens_step(c, mixed, d_blocks, nb, plan.n, a, eps2, dt));
+        return GRAVIS_OK;
+    }
+    // Capture one step (kick-drift + force/kick) and replay it.
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    GRAVIS_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    gravis_status st = ens_step(c, mixed, d_blocks, nb, plan.n, a, eps2, dt);
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (st != GRAVIS_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    if (e != hipSuccess) return fail(c, GRAVIS_HIP_ERROR, "capture: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return fail(c, GRAVIS_HIP_ERROR, "graph instantiate: %s", hipGetErrorString(e));
+    }
+    for (uint64_t t = 0; t < nsteps && e == hipSuccess; ++t) e = hipGraphLaunch(exec, c->stream);
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(c, GRAVIS_HIP_ERROR, "graph launch: %s", hipGetErrorString(e));
+    return GRAVIS_OK;
+}
+
+gravis_status ensemble_accel(Ctx* c, const gravis_solver* s, int nsys, const uint64_t* offsets,
+                             const double* x, const double* y, const double* z,
+                             const double* m, double* ax, double* ay, double* az) {
+    GRAVIS_TRY(check_solver_ens(c, s));
+    EnsPlan plan;
+    GRAVIS_TRY(make_plan(c, nsys, offsets, &plan));
+    if (plan.n == 0) return GRAVIS_OK;
+    EnsBlock* d_blocks;
+    GRAVIS_TRY(upload(c, "ens_blocks", plan.blocks, &d_blocks));
+    const EnsArgs a{const_cast<double*>(x), const_cast<double*>(y), const_cast<double*>(z),
+                    nullptr, nullptr, nullptr, ax, ay, az, m};
+    return ens_force(c, s->precision == GRAVIS_MIXED, d_blocks, (int)plan.blocks.size(), a,
+                     s->softening * s->softening, 0.0, 0);
+}
+
+gravis_status ensemble_adjoint(Ctx* c, const gravis_solver* s, int nsys, const uint64_t* offsets,
+                               const double* x0, const double* y0, const double* z0,
+                               const double* vx0, const double* vy0, const double* vz0,
+                               const double* m, double dt, uint64_t nsteps, uint32_t ckpt,
+                               const double* tx, const double* ty, const double* tz,
+                               const double* gx, const double* gy, const double* gz,
+                               const double* gvx, const double* gvy, const double* gvz,
+                               double* loss, double* xT, double* ox, double* oy, double* oz,
+                               double* ovx, double* ovy, double* ovz, double* om) {
+    GRAVIS_TRY(check_solver_ens(c, s));
+    if (!(dt > 0.0)) return fail(c, GRAVIS_INVALID_ARG, "dt must be positive");
+    if (ckpt == 0) ckpt = 32;
+    if (!tx && !gx) return fail(c, GRAVIS_INVALID_ARG, "need targets or position cotangents");
+    EnsPlan plan;
+    GRAVIS_TRY(make_plan(c, nsys, offsets, &plan));
+    const size_t n = plan.n;
+    if (n == 0) return GRAVIS_OK;
+    EnsBlock *d_blocks, *d_sys;
+    GRAVIS_TRY(upload(c, "ens_blocks", plan.blocks, &d_blocks));
+    GRAVIS_TRY(upload(c, "ens_sys", plan.systems, &d_sys));
+    const int nb = (int)plan.blocks.size();
+    const bool mixed = s->precision == GRAVIS_MIXED;
+    const double eps2 = s->softening * s->softening, h = 0.5 * dt;
+    const size_t T = nsteps;
+    const size_t nseg = (T + ckpt - 1) / ckpt;
+    const size_t nck = nseg > 0 ? nseg : 1;
+
+    double *st, *ck, *traj, *lam, *u, *gbuf;
+    GRAVIS_TRY(buf(c, "adj_state", 9 * n, &st));             // x y z vx vy vz ax ay az
+    GRAVIS_TRY(buf(c, "adj_ckpt", nck * 6 * n, &ck));        // (x, v) at segment starts
+    GRAVIS_TRY(buf(c, "adj_traj", (size_t)(ckpt + 1) * 3 * n, &traj));
+    GRAVIS_TRY(buf(c, "adj_lambda", 7 * n, &lam));           // lx ly lz lvx lvy lvz lm
+    GRAVIS_TRY(buf(c, "adj_u", 3 * n, &u));
+    GRAVIS_TRY(buf(c, "adj_g", 3 * n, &gbuf));
+    EnsArgs a{st, st + n, st + 2 * n, st + 3 * n, st + 4 * n, st + 5 * n, st + 6 * n,
+              st + 7 * n, st + 8 * n, m};
+    hipStream_t S = c->stream;
+    auto dcopy = [&](double* d, const double* src, size_t cnt) -> gravis_status {
+        GRAVIS_HIP(c, hipMemcpyAsync(d, src, cnt * sizeof(double), hipMemcpyDeviceToDevice, S));
+        return GRAVIS_OK;
+    };
+    // forward with checkpoints
+    GRAVIS_TRY(dcopy(a.x, x0, n)); GRAVIS_TRY(dcopy(a.y, y0, n)); GRAVIS_TRY(dcopy(a.z, z0, n));
+    GRAVIS_TRY(dcopy(a.vx, vx0, n)); GRAVIS_TRY(dcopy(a.vy, vy0, n)); GRAVIS_TRY(dcopy(a.vz, vz0, n));
+    GRAVIS_TRY(ens_force(c, mixed, d_blocks, nb, a, eps2, h, 0));
+    for (size_t t = 0; t < T; ++t) {
+        if (t % ckpt == 0) GRAVIS_TRY(dcopy(ck + (t / ckpt) * 6 * n, st, 6 * n));
+        GRAVIS_TRY(ens_step(c, mixed, d_blocks, nb, n, a, eps2, dt));
+    }
+    if (T == 0) GRAVIS_TRY(dcopy(ck, st, 6 * n));
+    if (xT) GRAVIS_TRY(dcopy(xT, st, 6 * n));  // final x y z vx vy vz
+    // cotangents of the final state
+    double *lx = lam, *ly = lam + n, *lz = lam + 2 * n, *lv = lam + 3 * n, *lm = lam + 6 * n;
+    if (tx) {
+        double* dloss;
+        GRAVIS_TRY(buf(c, "adj_loss", (size_t)nsys, &dloss));
+        adj_loss_kernel<<<nsys, 256, 0, S>>>(d_sys, nsys, a.x, a.y, a.z, tx, ty, tz, lx, ly, lz,
+                                             dloss);
+        GRAVIS_TRY(check_launch(c, "adj_loss"));
+        if (loss) GRAVIS_TRY(dcopy(loss, dloss, (size_t)nsys));
+    } else {
+        GRAVIS_TRY(dcopy(lx, gx, n)); GRAVIS_TRY(dcopy(ly, gy, n)); GRAVIS_TRY(dcopy(lz, gz, n));
+    }
+    if (gvx) {
+        GRAVIS_TRY(dcopy(lv, gvx, n)); GRAVIS_TRY(dcopy(lv + n, gvy, n)); GRAVIS_TRY(dcopy(lv + 2 * n, gvz, n));
+    } else {
+        GRAVIS_HIP(c, hipMemsetAsync(lv, 0, 3 * n * sizeof(double), S));
+    }
+    GRAVIS_HIP(c, hipMemsetAsync(lm, 0, n * sizeof(double), S));
+    const unsigned eg = grid_for(n, 256, 4096);
+    // top: K2 of the last step at x_T
+    if (T > 0) {
+        adj_level_kernel<<<eg, 256, 0, S>>>(n, lx, ly, lz, lv, 0.0, h, u);
+        ens_vjp_kernel<<<nb, kEnsBlock, 0, S>>>(d_blocks, a.x, a.y, a.z, m, u, u + n, u + 2 * n,
+                                                eps2, lx, ly, lz, lm);
+        GRAVIS_TRY(check_launch(c, "adj top"));
+    }
+    for (size_t sg = nseg; sg-- > 0;) {
+        const size_t t0 = sg * ckpt, t1 = std::min(T, t0 + ckpt);
+        // recompute the segment, storing positions at every level
+        GRAVIS_TRY(dcopy(st, ck + sg * 6 * n, 6 * n));
+        GRAVIS_TRY(ens_force(c, mixed, d_blocks, nb, a, eps2, h, 0));
+        GRAVIS_TRY(dcopy(traj, st, 3 * n));
+        for (size_t t = t0; t < t1; ++t) {
+            GRAVIS_TRY(ens_step(c, mixed, d_blocks, nb, n, a, eps2, dt));
+            if (t + 1 < t1) GRAVIS_TRY(dcopy(traj + (t + 1 - t0) * 3 * n, st, 3 * n));
+        }
+        for (size_t t = t1; t-- > t0;) {
+            const double* xt = traj + (t - t0) * 3 * n;
+            adj_level_kernel<<<eg, 256, 0, S>>>(n, lx, ly, lz, lv, dt, t > 0 ? 2.0 * h : h, u);
+            ens_vjp_kernel<<<nb, kEnsBlock, 0, S>>>(d_blocks, xt, xt + n, xt + 2 * n, m, u, u + n,
+                                                    u + 2 * n, eps2, lx, ly, lz, lm);
+        }
+        GRAVIS_TRY(check_launch(c, "adj segment"));
+    }
+    GRAVIS_TRY(dcopy(ox, lx, n)); GRAVIS_TRY(dcopy(oy, ly, n)); GRAVIS_TRY(dcopy(oz, lz, n));
+    GRAVIS_TRY(dcopy(ovx, lv, n)); GRAVIS_TRY(dcopy(ovy, lv + n, n)); GRAVIS_TRY(dcopy(ovz, lv + 2 * n, n));
+    GRAVIS_TRY(dcopy(om, lm, n));
+    return GRAVIS_OK;
+}
+
+}  // namespace gravis

@@ -138,6 +138,24 @@ gravis_status diag_device(Ctx* c, size_t n, double eps, int with_potential, cons
                           const double* vz, const double* m, const double* u,
                           const uint8_t* ptype, double out[19]);
 
+// ---- ensemble (ensemble.hip), device pointers ---------------------------
+gravis_status ensemble_kdk(Ctx* c, const gravis_solver* s, int nsys, const uint64_t* offsets,
+                           double* x, double* y, double* z, double* vx, double* vy, double* vz,
+                           const double* m, double* ax, double* ay, double* az, double dt,
+                           uint64_t nsteps, int use_graph);
+gravis_status ensemble_accel(Ctx* c, const gravis_solver* s, int nsys, const uint64_t* offsets,
+                             const double* x, const double* y, const double* z,
+                             const double* m, double* ax, double* ay, double* az);
+gravis_status ensemble_adjoint(Ctx* c, const gravis_solver* s, int nsys, const uint64_t* offsets,
+                               const double* x0, const double* y0, const double* z0,
+                               const double* vx0, const double* vy0, const double* vz0,
+                               const double* m, double dt, uint64_t nsteps, uint32_t ckpt,
+                               const double* tx, const double* ty, const double* tz,
+                               const double* gx, const double* gy, const double* gz,
+                               const double* gvx, const double* gvy, const double* gvz,
+                               double* loss, double* xT, double* ox, double* oy, double* oz,
+                               double* ovx, double* ovy, double* ovz, double* om);
+
 inline unsigned grid_for(size_t n, unsigned block, unsigned cap = 1u << 30) {
     size_t g = (n + block - 1) / block;
     if (g < 1) g = 1;

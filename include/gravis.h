@@ -138,6 +138,64 @@ gravis_status gravis_diagnostics_device(gravis_ctx* ctx, size_t n, double soften
                                         const double* u, const uint8_t* ptype, double time,
                                         uint64_t step, gravis_diag* out);
 
+/* ---- batched ensemble and reverse mode (no reference counterpart) -------
+ * nsys independent systems stored back to back: system s owns particles
+ * [offsets[s], offsets[s+1]) of the SoA arrays (offsets is a HOST array of
+ * nsys+1 entries, offsets[0] = 0). Each system runs LeapfrogKDK
+ * (integrator.rs:41-77) with brute-force gravity among its own particles
+ * (solver->algorithm must be GRAVIS_BRUTE_FORCE; FP64 or MIXED). */
+
+/* a = F(x) for every system (assigns, unlike gravis_accel). Device
+ * pointers, asynchronous. */
+gravis_status gravis_ensemble_accel_device(gravis_ctx* ctx, const gravis_solver* solver,
+                                           int nsys, const uint64_t* offsets, const double* x,
+                                           const double* y, const double* z, const double* m,
+                                           double* ax, double* ay, double* az);
+
+/* nsteps of KDK for every system; a must hold a(t) on entry (as
+ * gravis_kdk). use_graph != 0 captures one step (kick-drift + force/kick
+ * launches) as a hipGraph and replays it. Host pointers. */
+gravis_status gravis_ensemble_kdk(gravis_ctx* ctx, const gravis_solver* solver, int nsys,
+                                  const uint64_t* offsets, double* x, double* y, double* z,
+                                  double* vx, double* vy, double* vz, const double* m,
+                                  double* ax, double* ay, double* az, double dt,
+                                  uint64_t nsteps, int use_graph);
+gravis_status gravis_ensemble_kdk_device(gravis_ctx* ctx, const gravis_solver* solver, int nsys,
+                                         const uint64_t* offsets, double* x, double* y,
+                                         double* z, double* vx, double* vy, double* vz,
+                                         const double* m, double* ax, double* ay, double* az,
+                                         double dt, uint64_t nsteps, int use_graph);
+
+/* Reverse mode of nsteps KDK steps from (x0, v0) with masses m (a(x0) is
+ * computed first). The final-state cotangent is either the gradient of
+ * L = sum_i |x_i(T) - target_i|^2 (targets tx/ty/tz non-NULL; per-system
+ * loss written to loss[nsys] if non-NULL) or given directly as gx/gy/gz
+ * (position) and gvx/gvy/gvz (velocity, may be NULL). Outputs the
+ * cotangents of the initial positions, velocities and masses; the
+ * trajectory is recomputed from checkpoints every checkpoint_every steps
+ * (0 = 32). final_state (6 n: x, y, z, vx, vy, vz at T) may be NULL. The
+ * pair VJP runs in FP64; the recomputed forward uses solver->precision.
+ * Host pointers. */
+gravis_status gravis_ensemble_adjoint(gravis_ctx* ctx, const gravis_solver* solver, int nsys,
+                                      const uint64_t* offsets, const double* x0,
+                                      const double* y0, const double* z0, const double* vx0,
+                                      const double* vy0, const double* vz0, const double* m,
+                                      double dt, uint64_t nsteps, uint32_t checkpoint_every,
+                                      const double* tx, const double* ty, const double* tz,
+                                      const double* gx, const double* gy, const double* gz,
+                                      const double* gvx, const double* gvy, const double* gvz,
+                                      double* loss, double* final_state, double* grad_x,
+                                      double* grad_y, double* grad_z, double* grad_vx,
+                                      double* grad_vy, double* grad_vz, double* grad_m);
+gravis_status gravis_ensemble_adjoint_device(
+    gravis_ctx* ctx, const gravis_solver* solver, int nsys, const uint64_t* offsets,
+    const double* x0, const double* y0, const double* z0, const double* vx0, const double* vy0,
+    const double* vz0, const double* m, double dt, uint64_t nsteps, uint32_t checkpoint_every,
+    const double* tx, const double* ty, const double* tz, const double* gx, const double* gy,
+    const double* gz, const double* gvx, const double* gvy, const double* gvz, double* loss,
+    double* final_state, double* grad_x, double* grad_y, double* grad_z, double* grad_vx,
+    double* grad_vy, double* grad_vz, double* grad_m);
+
 /* ---- introspection (no reference counterpart: test and bench hooks) ---- */
 
 /* Kernel timing: when enabled, the dominant force kernel of every call is

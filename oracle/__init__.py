@@ -352,3 +352,59 @@ def sph_step(s: SphState, dt: float, gravity: int = 1, eps: float = 0.0, theta: 
     params = params or SphParams.default()
     st = s.c_struct()
     return _sph_lib().orc_sph_step(C.byref(st), C.byref(params), gravity, eps, theta, dt)
+
+
+# ---- ensemble + adjoint (gravis_oracle_adjoint.c) -------------------------
+
+def _adj_lib():
+    L = lib()
+    if not getattr(L, "_adj_ready", False):
+        spec = {
+            "orc_ens_forward": (None, [_sz, _d, _d, _u64] + [_dp] * 7),
+            "orc_ens_adjoint": (None, [_sz, _d, _d, _u64] + [_dp] * 10 + [C.c_void_p] * 3
+                                + [_dp] * 7),
+            "orc_ens_counts": (None, [_u64, _sz, _u32, _u32, _u32p]),
+            "orc_ens_targets": (None, [_u64, _sz, _dp, _dp, _dp]),
+        }
+        for name, (res, args) in spec.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        L._adj_ready = True
+    return L
+
+
+def ens_forward(s: State, eps: float, dt: float, nsteps: int) -> None:
+    """KDK with gather-form brute force (a(x0) first); state updated in place."""
+    _adj_lib().orc_ens_forward(s.n, eps, dt, nsteps, s.x, s.y, s.z, s.vx, s.vy, s.vz, s.m)
+
+
+def ens_adjoint(s: State, eps: float, dt: float, nsteps: int, gx, gy, gz, gv=None):
+    """Cotangents of (x0, v0, m) for cotangents (gx, gy, gz) of x_T (and gv
+    of v_T). Returns dict of arrays x, y, z, vx, vy, vz, m."""
+    n = s.n
+    out = {k: _z(n) for k in ("x", "y", "z", "vx", "vy", "vz", "m")}
+    gv_ptrs = [None, None, None]
+    keep = []
+    if gv is not None:
+        keep = [np.ascontiguousarray(a, dtype=np.float64) for a in gv]
+        gv_ptrs = [a.ctypes.data_as(C.c_void_p) for a in keep]
+    _adj_lib().orc_ens_adjoint(n, eps, dt, nsteps, s.x, s.y, s.z, s.vx, s.vy, s.vz, s.m,
+                               np.ascontiguousarray(gx, dtype=np.float64),
+                               np.ascontiguousarray(gy, dtype=np.float64),
+                               np.ascontiguousarray(gz, dtype=np.float64), *gv_ptrs,
+                               out["x"], out["y"], out["z"], out["vx"], out["vy"], out["vz"],
+                               out["m"])
+    return out
+
+
+def ens_counts(seed: int, nsys: int, lo: int = 512, hi: int = 8192) -> np.ndarray:
+    c = np.zeros(nsys, dtype=np.uint32)
+    _adj_lib().orc_ens_counts(seed, nsys, lo, hi, c)
+    return c
+
+
+def ens_targets(seed: int, n: int):
+    tx, ty, tz = _z(n), _z(n), _z(n)
+    _adj_lib().orc_ens_targets(seed, n, tx, ty, tz)
+    return tx, ty, tz
