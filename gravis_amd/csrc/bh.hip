@@ -17,7 +17,8 @@
 //   COM     bottom up in FP64: leaves sum their particles in sorted order,
 //           internal nodes sum their children in order (m, m*x; no FMA);
 //           com = sum(m x) / sum(m).
-//   walk    one wavefront per group of `group` consecutive sorted targets.
+//   walk    one wavefront per target group: the first tree node on each
+//           root-to-leaf path holding at most `group` (<= 64) particles.
 //           The frontier is processed breadth first, 64 nodes at a time
 //           (one per lane). A node is accepted for the whole group when
 //           s^2 < theta^2 d_min^2 with d_min the distance from its COM to the
@@ -43,12 +44,14 @@ constexpr uint32_t kLeafFlag = 0x80000000u;
 constexpr int kTreeBlock = 256;
 constexpr int kTreeGrid = 1024;
 
-// meta = {first, count, first child, (nchild << 8) | level}
+// meta = {first, count, first child, flags}; flags = level | nchild << 8 |
+// kGroupRoot (the parent holds more particles than a target group).
+constexpr uint32_t kGroupRoot = 1u << 16;
 
 __global__ void tree_init_kernel(uint4* __restrict__ meta, uint32_t* __restrict__ lvl,
                                  uint32_t n) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    meta[0] = make_uint4(0u, n, 0u, 0u);
+    meta[0] = make_uint4(0u, n, 0u, kGroupRoot);
     lvl[0] = 0u;
     lvl[1] = 1u;
     for (int L = 1; L <= kMaxLevel + 1; ++L) {
@@ -127,7 +130,7 @@ __global__ __launch_bounds__(1024) void tree_scan_kernel(const uint32_t* __restr
 __global__ __launch_bounds__(kTreeBlock) void tree_write_kernel(
     const uint64_t* __restrict__ keys, uint4* __restrict__ meta, const uint32_t* __restrict__ lvl,
     int L, const uint32_t* __restrict__ nch, const uint32_t* __restrict__ off, uint32_t cap,
-    uint32_t* __restrict__ flag) {
+    uint32_t gsize, uint32_t* __restrict__ flag) {
     const uint32_t b = lvl[2 * L], e = lvl[2 * L + 1];
     const int shift = 3 * (kMaxLevel - (L + 1));
     for (uint32_t v = b + blockIdx.x * kTreeBlock + threadIdx.x; v < e;
@@ -150,11 +153,12 @@ __global__ __launch_bounds__(kTreeBlock) void tree_write_kernel(
                 if (digit_of(keys[mid], shift) > d) hi = mid;
                 else lo = mid + 1;
             }
-            meta[c0 + k] = make_uint4(pos, lo - pos, 0u, (uint32_t)(L + 1));
+            meta[c0 + k] = make_uint4(pos, lo - pos, 0u,
+                                      (uint32_t)(L + 1) | (md.y > gsize ? kGroupRoot : 0u));
             ++k;
             pos = lo;
         }
-        meta[v] = make_uint4(md.x, md.y, c0, (nc << 8) | (uint32_t)L);
+        meta[v] = make_uint4(md.x, md.y, c0, (md.w & kGroupRoot) | (nc << 8) | (uint32_t)L);
     }
 }
 
@@ -168,7 +172,7 @@ __global__ __launch_bounds__(kTreeBlock) void tree_sums_kernel(const double4* __
     for (uint32_t v = b + blockIdx.x * kTreeBlock + threadIdx.x; v < e;
          v += gridDim.x * kTreeBlock) {
         const uint4 md = meta[v];
-        const uint32_t nc = md.w >> 8;
+        const uint32_t nc = (md.w >> 8) & 255u;
         double M = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
         if (nc == 0) {
             for (uint32_t k = md.x; k < md.x + md.y; ++k) {
@@ -211,10 +215,34 @@ __global__ __launch_bounds__(kTreeBlock) void tree_com_kernel(double4* __restric
     }
 }
 
+// Target groups: the first node on every root-to-leaf path holding at most
+// gsize particles; a leaf holding more (coincident points at level 21) is
+// cut into runs of gsize. Appended in any order (each group's result depends
+// only on its own particles and the tree).
+__global__ __launch_bounds__(kTreeBlock) void group_emit_kernel(const uint4* __restrict__ meta,
+                                                               const uint32_t* __restrict__ lvl,
+                                                               uint32_t gsize,
+                                                               uint2* __restrict__ groups,
+                                                               uint32_t* __restrict__ ngroups) {
+    const uint32_t total = lvl[2 * (kMaxLevel + 1)];
+    for (uint32_t v = blockIdx.x * kTreeBlock + threadIdx.x; v < total;
+         v += gridDim.x * kTreeBlock) {
+        const uint4 md = meta[v];
+        if (!(md.w & kGroupRoot)) continue;
+        const uint32_t nc = (md.w >> 8) & 255u;
+        if (md.y > gsize && nc != 0) continue;
+        const uint32_t k = (md.y + gsize - 1) / gsize;
+        const uint32_t at = atomicAdd(ngroups, k);
+        for (uint32_t q = 0; q < k; ++q)
+            groups[at + q] = make_uint2(md.x + q * gsize, min(gsize, md.y - q * gsize));
+    }
+}
+
 __global__ void walk_reset_kernel(uint32_t* __restrict__ counter,
                                   unsigned long long* __restrict__ stats) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         counter[0] = 0u;
+        counter[1] = 0u;
         for (int k = 0; k < 4; ++k) stats[k] = 0ull;
     }
 }
@@ -235,7 +263,8 @@ __device__ __forceinline__ void pair_ref(const double4 s, double px, double py, 
 }
 
 __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
-    const double4* __restrict__ src, const uint4* __restrict__ meta, int n, int gs, int ngroups,
+    const double4* __restrict__ src, const uint4* __restrict__ meta, int n,
+    const uint2* __restrict__ groups, const uint32_t* __restrict__ ngroups,
     const double* __restrict__ box, double theta2, double eps2, uint32_t* __restrict__ frontier, uint32_t fcap,
     uint32_t* __restrict__ counter, uint32_t* __restrict__ flag, double* __restrict__ part,
     unsigned long long* __restrict__ stats) {
@@ -255,9 +284,10 @@ __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
         uint32_t g = 0;
         if (lane == 0) g = atomicAdd(counter, 1u);
         g = __shfl(g, 0);
-        if (g >= (uint32_t)ngroups) break;
-        const int i0 = (int)g * gs;
-        const int cnt = min(gs, n - i0);
+        if (g >= ngroups[0]) break;
+        const uint2 gr = groups[g];
+        const int i0 = (int)gr.x;
+        const int cnt = (int)gr.y;
         const bool active = lane < cnt;
         const double4 me = src[i0 + (active ? lane : 0)];
         double lo[3] = {me.x, me.y, me.z}, hi[3] = {me.x, me.y, me.z};
@@ -311,7 +341,7 @@ __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
                 const uint4 md = meta[v];
                 const double4 c = src[(size_t)n + v];
                 const uint32_t level = md.w & 255u;
-                const uint32_t nchild = md.w >> 8;
+                const uint32_t nchild = (md.w >> 8) & 255u;
                 const double s = ldexp(side0, -(int)level);
                 const double ex = fmax(fmax(lo[0] - c.x, c.x - hi[0]), 0.0);
                 const double ey = fmax(fmax(lo[1] - c.y, c.y - hi[1]), 0.0);
@@ -408,7 +438,7 @@ gravis_status bh_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
         tree_count_kernel<<<tg, kTreeBlock, 0, st>>>(keys, meta, lvl, L, leaf, nch);
         tree_scan_kernel<<<1, 1024, 0, st>>>(nch, off, lvl, L);
         tree_write_kernel<<<tg, kTreeBlock, 0, st>>>(keys, meta, lvl, L, nch, off, (uint32_t)cap,
-                                                    flags + 1);
+                                                    (uint32_t)gs, flags + 1);
     }
     // level 21 holds leaves only (zero children); its scan closes the node range
     tree_count_kernel<<<tg, kTreeBlock, 0, st>>>(keys, meta, lvl, kMaxLevel, leaf, nch);
@@ -419,14 +449,15 @@ gravis_status bh_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
     tree_com_kernel<<<tg, kTreeBlock, 0, st>>>(src, ni, meta, lvl, sums);
     GRAVIS_TRY(check_launch(c, "tree com"));
 
-    // walk
-    const int ngroups = (ni + gs - 1) / gs;
-    const int blocks = (int)std::min<long>((ngroups + kWalkWaves - 1) / kWalkWaves,
+    // target groups, then the walk
+    const int blocks = (int)std::min<long>((long)(n / 64 / kWalkWaves) + 1,
                                            (long)c->num_cus * 4);
-    const uint32_t fcap = (uint32_t)std::min<size_t>(cap, 32768);
+    const uint32_t fcap = (uint32_t)std::min<size_t>(cap, 65536);
     uint32_t *front, *counter;
+    uint2* groups;
     GRAVIS_TRY(buf(c, "bh_front", (size_t)blocks * kWalkWaves * 2 * fcap, &front));
     GRAVIS_TRY(buf(c, "bh_counter", 4, &counter));
+    GRAVIS_TRY(buf(c, "bh_groups", n, &groups));
     double* part;
     GRAVIS_TRY(buf(c, "bh_part", 3 * n, &part));
     const double theta2 = s->theta * s->theta;
@@ -434,8 +465,9 @@ gravis_status bh_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
     unsigned long long* stats;
     GRAVIS_TRY(buf(c, "bh_stats", 4, &stats));
     walk_reset_kernel<<<1, 64, 0, st>>>(counter, stats);
+    group_emit_kernel<<<tg, kTreeBlock, 0, st>>>(meta, lvl, (uint32_t)gs, groups, counter + 1);
     timer_begin(c);
-    bh_walk_kernel<<<blocks, kWalkWaves * kWave, 0, st>>>(src, meta, ni, gs, ngroups,
+    bh_walk_kernel<<<blocks, kWalkWaves * kWave, 0, st>>>(src, meta, ni, groups, counter + 1,
                                                           box, theta2, eps2, front, fcap,
                                                           counter, flags, part, stats);
     timer_end(c, "bh_walk");

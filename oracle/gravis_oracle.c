@@ -29,9 +29,9 @@
  *              axis bit order x:1 y:2 z:4 as octree.rs:67-79
  *   sort       stable sort of the 63-bit keys, ties by original index
  *   ltree      leaf-16 linear octree, breadth-first node order, FP64 COM sums
- *   gwalk      group walk: 64 consecutive sorted particles share one
- *              interaction list built breadth first with a box-distance
- *              opening test s^2 < theta^2 * d_min^2
+ *   gwalk      group walk: the particles of one compact tree cell (at most
+ *              64) share one interaction list built breadth first with a
+ *              box-distance opening test s^2 < theta^2 * d_min^2
  *
  * Parity status of each function is stated in tests/test_oracle_*.py.
  */
@@ -764,6 +764,42 @@ static void gwalk(const ltree* t, double side0, double theta2, double eps2, cons
     if (ninter) *ninter += cnt_inter;
 }
 
+/* Target groups: the first node on every root-to-leaf path holding at most
+ * `group` particles (compact by construction); a leaf holding more than
+ * `group` particles (coincident points at level 21) is cut into runs of
+ * `group`. Every particle lies in exactly one group. Group order does not
+ * affect any result. Returns the group count; gf/gc receive (first, count). */
+static size_t lt_groups(const ltree* t, uint32_t group, uint32_t** gf, uint32_t** gc) {
+    size_t cap = 64, ng = 0;
+    uint32_t* f = (uint32_t*)malloc(cap * 4);
+    uint32_t* c = (uint32_t*)malloc(cap * 4);
+    uint32_t* stack = (uint32_t*)malloc((t->len + 1) * 4);
+    size_t sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        uint32_t v = stack[--sp];
+        if (t->cnt[v] <= group || t->nchild[v] == 0) {
+            for (uint32_t b = 0; b < t->cnt[v]; b += group) {
+                if (ng == cap) {
+                    cap *= 2;
+                    f = (uint32_t*)realloc(f, cap * 4);
+                    c = (uint32_t*)realloc(c, cap * 4);
+                }
+                uint32_t k = t->cnt[v] - b < group ? t->cnt[v] - b : group;
+                f[ng] = t->first[v] + b;
+                c[ng] = k;
+                ng++;
+            }
+            continue;
+        }
+        for (uint32_t q = 0; q < t->nchild[v]; ++q) stack[sp++] = t->child[v] + q;
+    }
+    free(stack);
+    *gf = f;
+    *gc = c;
+    return ng;
+}
+
 /* Full linear-octree Barnes-Hut: box, keys, sort, tree, group walk.
  * Accumulates into ax/ay/az in the caller's order. Returns the total number
  * of pair interactions evaluated (targets x sources). */
@@ -789,7 +825,8 @@ uint64_t orc_bh_group_accel(size_t n, double eps, double theta, uint32_t leaf, u
     lt_finish_com(&t, sx, sy, sz);
     double side0 = 2.0 * box[3];
     double eps2 = eps * eps, theta2 = theta * theta;
-    size_t ng = (n + group - 1) / group;
+    uint32_t *gfirst, *gcnt;
+    size_t ng = lt_groups(&t, group, &gfirst, &gcnt);
     uint64_t total = 0;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -802,8 +839,7 @@ uint64_t orc_bh_group_accel(size_t n, double eps, double theta, uint32_t leaf, u
 #pragma omp for schedule(dynamic, 1)
 #endif
         for (long long g = 0; g < (long long)ng; ++g) {
-            size_t g0 = (size_t)g * group, g1 = g0 + group;
-            if (g1 > n) g1 = n;
+            size_t g0 = gfirst[g], g1 = g0 + gcnt[g];
             gwalk(&t, side0, theta2, eps2, sx, sy, sz, sm, g0, g1, bx, by, bz, fa, fb, &total);
         }
         free(fa); free(fb);
@@ -811,6 +847,8 @@ uint64_t orc_bh_group_accel(size_t n, double eps, double theta, uint32_t leaf, u
     for (size_t k = 0; k < n; ++k) {
         ax[perm[k]] += bx[k]; ay[perm[k]] += by[k]; az[perm[k]] += bz[k];
     }
+    free(gfirst);
+    free(gcnt);
     lt_free(&t);
     free(keys); free(sk); free(perm); free(s);
     return total;
