@@ -57,6 +57,15 @@ def load() -> C.CDLL:
     global _LIB
     if _LIB is not None:
         return _LIB
+    # PyTorch-ROCm ships its own libamdhip64.so.7 with the same SONAME as
+    # /opt/rocm's. Whichever is loaded first serves the whole process, and
+    # torch cannot initialise on a newer runtime loaded before it, so torch
+    # (when installed) is imported first and this library binds to its
+    # runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built; run make -C gravis_amd/csrc "
                           "(this package has no CPU fallback)")
