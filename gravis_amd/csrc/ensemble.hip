@@ -47,44 +47,27 @@ __global__ __launch_bounds__(kEnsBlock) void ens_force_kernel(
     const int ii = active ? i : b.i0;
     const double px = x[ii], py = y[ii], pz = z[ii];
     if (MIXED) {
-        __shared__ float4 tile[kEnsBlock];
-        // block-relative FP32 coordinates (systems are centred on their COM)
-        double lo[3] = {px, py, pz}, hi[3] = {px, py, pz};
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1)
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                lo[d] = fmin(lo[d], __shfl_xor(lo[d], o));
-                hi[d] = fmax(hi[d], __shfl_xor(hi[d], o));
-            }
-        __shared__ double sb[kEnsBlock / kWave][6];
-        if ((threadIdx.x & (kWave - 1)) == 0)
-            for (int d = 0; d < 3; ++d) { sb[threadIdx.x / kWave][d] = lo[d]; sb[threadIdx.x / kWave][3 + d] = hi[d]; }
-        __syncthreads();
-        for (int d = 0; d < 3; ++d) { lo[d] = sb[0][d]; hi[d] = sb[0][3 + d]; }
-        for (int w = 1; w < kEnsBlock / kWave; ++w)
-            for (int d = 0; d < 3; ++d) { lo[d] = fmin(lo[d], sb[w][d]); hi[d] = fmax(hi[d], sb[w][3 + d]); }
-        const double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]),
-                     cz = 0.5 * (lo[2] + hi[2]);
-        const float xi = (float)(px - cx), yi = (float)(py - cy), zi = (float)(pz - cz);
+        // Systems are not spatially sorted, so every pair separation is formed
+        // in FP64 and rounded once to FP32; r^2, v_rsq_f32, m r^-3 and the
+        // accumulation are FP32 (tile sums Kahan-added).
+        __shared__ double4 tile[kEnsBlock];
         const float e2 = (float)eps2;
         float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
         for (int jb = b.j0; jb < b.j1; jb += kEnsBlock) {
             const int j = jb + threadIdx.x;
             __syncthreads();
-            tile[threadIdx.x] = j < b.j1 ? make_float4((float)(x[j] - cx), (float)(y[j] - cy),
-                                                       (float)(z[j] - cz), (float)m[j])
-                                         : make_float4(0.f, 0.f, 0.f, 0.f);
+            tile[threadIdx.x] = j < b.j1 ? make_double4(x[j], y[j], z[j], m[j])
+                                         : make_double4(px, py, pz, 0.0);
             __syncthreads();
             const int cnt = min(kEnsBlock, b.j1 - jb);
             float tx = 0.f, ty = 0.f, tz = 0.f;
             for (int k = 0; k < cnt; ++k) {
-                const float4 s = tile[k];
-                const float dx = s.x - xi, dy = s.y - yi, dz = s.z - zi;
+                const double4 s = tile[k];
+                const float dx = (float)(s.x - px), dy = (float)(s.y - py), dz = (float)(s.z - pz);
                 const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, e2)));
                 float ri = __builtin_amdgcn_rsqf(r2);
                 ri = r2 > 0.f ? ri : 0.f;
-                const float f = (s.w * ri) * (ri * ri);
+                const float f = ((float)s.w * ri) * (ri * ri);
                 tx = __builtin_fmaf(f, dx, tx);
                 ty = __builtin_fmaf(f, dy, ty);
                 tz = __builtin_fmaf(f, dz, tz);
