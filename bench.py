@@ -97,10 +97,13 @@ def probe_copy():
     print(json.dumps({"probe": "copy", "bytes_read": a.numel() * 8, "bytes_written": b.numel() * 8}))
 
 
-def run_bh(args, g, torch, ctx):
+def run_bh(args, g, torch, ctx, precision="mixed", steps=None):
     """Barnes-Hut KDK steps on two Plummer spheres of n_each particles (seeds
     42/43, a = 1, M = 1 each) 10 a apart on a parabolic approach, eps 0.01,
-    dt 5e-4, theta 0.5, leaf 16, groups of 64."""
+    dt 5e-4, theta 0.5, leaf 16, groups of at most 64. precision 'mixed':
+    FP32 pair forces on FP64 group-relative separations (FP64 tree, state and
+    opening test); 'fp64': the reference arithmetic."""
+    steps = steps or args.bh_steps
     import ctypes as C
     ne = args.bh_n_each
     n = 2 * ne
@@ -112,7 +115,7 @@ def run_bh(args, g, torch, ctx):
     for f in ("ax", "ay", "az"):
         T[f] = torch.zeros(n, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
-    bh = g.BarnesHut(0.01, 0.5, 16, 64, ctx=ctx)
+    bh = g.BarnesHut(0.01, 0.5, 16, 64, precision=precision, ctx=ctx)
     L = ctx._L
     keys = ("x", "y", "z", "vx", "vy", "vz", "mass", "ax", "ay", "az")
     d = bh.desc()
@@ -126,12 +129,12 @@ def run_bh(args, g, torch, ctx):
                                       ("x", "y", "z", "mass", "ax", "ay", "az")]))
     kdk(2)  # warmup
     torch.cuda.synchronize()
-    log(f"bench: timing {args.bh_steps} BH KDK steps")
+    log(f"bench: timing {steps} BH KDK steps ({precision})")
     t0 = time.perf_counter()
-    for _ in range(args.bh_steps):
+    for _ in range(steps):
         kdk(1)
     torch.cuda.synchronize()
-    t_step = (time.perf_counter() - t0) / args.bh_steps
+    t_step = (time.perf_counter() - t0) / steps
     ctx.set_profiling(True)
     kdk(1)
     ctx.set_profiling(False)
@@ -143,8 +146,8 @@ def run_bh(args, g, torch, ctx):
     # source a 32 B record read by the wave; per target 32 B in + 24 B out.
     walk_bytes = st["node_tests"] * 48 + (st["accepted_nodes"] + st["leaf_particles"]) * 32 \
         + n * 56
-    return {"n": n, "steps_per_s": round(1.0 / t_step, 3), "ms_per_step": round(t_step * 1e3, 3),
-            "steps": args.bh_steps, "theta": 0.5, "leaf_size": 16, "group_size": 64,
+    return {"n": n, "precision": precision, "steps_per_s": round(1.0 / t_step, 3),
+            "ms_per_step": round(t_step * 1e3, 3), "steps": steps, "theta": 0.5, "leaf_size": 16, "group_size": 64,
             "softening": 0.01, "dt": 5e-4,
             "walk_kernel_ms": round(walk_s * 1e3, 3),
             "walk_stats": st,
@@ -153,7 +156,7 @@ def run_bh(args, g, torch, ctx):
             "walk_algorithmic_GBps": round(walk_bytes / walk_s / 1e9, 1),
             "walk_frac_hbm_peak": round(walk_bytes / walk_s / 8.0e12, 4),
             "workload": "two Plummer spheres (seeds 42/43, a=1, M=1 each) 10a apart, parabolic "
-                        "approach, KDK with Barnes-Hut theta=0.5, leaf 16, FP64"}
+                        "approach, KDK with Barnes-Hut theta=0.5, leaf 16"}
 
 
 def main():
@@ -267,9 +270,10 @@ def main():
         log(f"bench: CPU baseline on {args.cpu_sample} particles")
         cpu = cpu_baseline(args.cpu_sample, eps)
 
-    bh = None
+    bh = bh64 = None
     if not args.no_bh:
-        bh = run_bh(args, g, torch, ctx)
+        bh = run_bh(args, g, torch, ctx, "mixed")
+        bh64 = run_bh(args, g, torch, ctx, "fp64", max(2, args.bh_steps // 4))
 
     traffic = traffic_from_profiles(f"bf_mixed_n{n}")
     out = {
@@ -303,6 +307,7 @@ def main():
         "mixed_over_fp64": round(t_fp64 / t_mixed, 3),
         "force_error_vs_fp64": {"rms": err_rms, "max": err_max},
         "barnes_hut": bh,
+        "barnes_hut_fp64": bh64,
     }
     print(json.dumps(out), flush=True)
 

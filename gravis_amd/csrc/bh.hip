@@ -25,9 +25,10 @@
 //           group's bounding box (so never less accurate than the
 //           reference's per-particle test); a rejected leaf contributes its
 //           particles directly; a rejected internal node pushes its children.
-//           Accepted nodes and opened leaves are appended, in frontier
-//           order, to a per-wave list in LDS that all 64 lanes then sweep
-//           with wave-uniform (scalar) loads. Pair arithmetic is
+//           Accepted nodes and the particles of opened leaves are appended,
+//           in frontier order, as source indices to a per-wave LDS list;
+//           each flush round stages 64 source records in LDS (one load per
+//           lane) that all 64 lanes then read back as broadcasts. Pair arithmetic is
 //           barnes_hut.rs:114-135 in FP64 (sqrt and divide correctly
 //           rounded, no contraction); a source at r^2 = 0 is skipped.
 // The accumulation order is the list order, so results are bit-for-bit the
@@ -215,26 +216,54 @@ __global__ __launch_bounds__(kTreeBlock) void tree_com_kernel(double4* __restric
     }
 }
 
-// Target groups: the first node on every root-to-leaf path holding at most
-// gsize particles; a leaf holding more (coincident points at level 21) is
-// cut into runs of gsize. Appended in any order (each group's result depends
-// only on its own particles and the tree).
+// Target groups of at most gsize particles, contiguous in sorted order: the
+// children of every internal node holding more than gsize particles are
+// visited in order and consecutive children holding at most gsize are packed
+// greedily while the total stays within gsize; a child leaf holding more
+// (coincident points at level 21) is cut into runs of gsize; a root holding
+// at most gsize (or a root leaf) is cut the same way. Appended in any order:
+// each group's result depends only on its own particles and the tree.
+__device__ __forceinline__ void emit_runs(uint2* groups, uint32_t* ngroups, uint32_t first,
+                                          uint32_t cnt, uint32_t gsize) {
+    const uint32_t k = (cnt + gsize - 1) / gsize;
+    const uint32_t at = atomicAdd(ngroups, k);
+    for (uint32_t q = 0; q < k; ++q)
+        groups[at + q] = make_uint2(first + q * gsize, min(gsize, cnt - q * gsize));
+}
+
 __global__ __launch_bounds__(kTreeBlock) void group_emit_kernel(const uint4* __restrict__ meta,
                                                                const uint32_t* __restrict__ lvl,
                                                                uint32_t gsize,
                                                                uint2* __restrict__ groups,
                                                                uint32_t* __restrict__ ngroups) {
     const uint32_t total = lvl[2 * (kMaxLevel + 1)];
-    for (uint32_t v = blockIdx.x * kTreeBlock + threadIdx.x; v < total;
-         v += gridDim.x * kTreeBlock) {
-        const uint4 md = meta[v];
-        if (!(md.w & kGroupRoot)) continue;
+    for (uint32_t p = blockIdx.x * kTreeBlock + threadIdx.x; p < total;
+         p += gridDim.x * kTreeBlock) {
+        const uint4 md = meta[p];
         const uint32_t nc = (md.w >> 8) & 255u;
-        if (md.y > gsize && nc != 0) continue;
-        const uint32_t k = (md.y + gsize - 1) / gsize;
-        const uint32_t at = atomicAdd(ngroups, k);
-        for (uint32_t q = 0; q < k; ++q)
-            groups[at + q] = make_uint2(md.x + q * gsize, min(gsize, md.y - q * gsize));
+        if (p == 0 && (md.y <= gsize || nc == 0)) {
+            emit_runs(groups, ngroups, md.x, md.y, gsize);
+            continue;
+        }
+        if (md.y <= gsize || nc == 0) continue;
+        uint32_t rf = 0, rc = 0;
+        for (uint32_t q = 0; q < nc; ++q) {
+            const uint4 cm = meta[md.z + q];
+            const uint32_t cc = (cm.w >> 8) & 255u;
+            if (cm.y <= gsize) {
+                if (rc > 0 && rc + cm.y > gsize) {
+                    emit_runs(groups, ngroups, rf, rc, gsize);
+                    rc = 0;
+                }
+                if (rc == 0) rf = cm.x;
+                rc += cm.y;
+            } else {
+                if (rc > 0) emit_runs(groups, ngroups, rf, rc, gsize);
+                rc = 0;
+                if (cc == 0) emit_runs(groups, ngroups, cm.x, cm.y, gsize);
+            }
+        }
+        if (rc > 0) emit_runs(groups, ngroups, rf, rc, gsize);
     }
 }
 
@@ -248,7 +277,7 @@ __global__ void walk_reset_kernel(uint32_t* __restrict__ counter,
 }
 
 constexpr int kWalkWaves = 4;
-constexpr int kListCap = 1024;
+constexpr uint32_t kListCap = 1024;  // source indices per wave (LDS)
 
 __device__ __forceinline__ void pair_ref(const double4 s, double px, double py, double pz,
                                          double eps2, double& ax, double& ay, double& az) {
@@ -262,18 +291,29 @@ __device__ __forceinline__ void pair_ref(const double4 s, double px, double py, 
     az = use ? az + f * dz : az;
 }
 
+// The interaction list holds indices into src: a particle k < n (a source of
+// an opened leaf) or n + v (accepted node v), appended in frontier order.
+// A flush sweeps it in rounds of 64: every lane loads one record into LDS,
+// then all lanes read the 64 records back as broadcasts.
+// MIXED: each staged record is rounded to FP32 relative to the group's box
+// centre (separations formed in FP64 first), pair forces are FP32 (v_rsq_f32)
+// and every round's FP32 sum is Kahan-added; the opening test, tree sums and
+// output stay FP64.
+template <bool MIXED>
 __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
     const double4* __restrict__ src, const uint4* __restrict__ meta, int n,
     const uint2* __restrict__ groups, const uint32_t* __restrict__ ngroups,
-    const double* __restrict__ box, double theta2, double eps2, uint32_t* __restrict__ frontier, uint32_t fcap,
-    uint32_t* __restrict__ counter, uint32_t* __restrict__ flag, double* __restrict__ part,
-    unsigned long long* __restrict__ stats) {
+    const double* __restrict__ box, double theta2, double eps2, uint32_t* __restrict__ frontier,
+    uint32_t fcap, uint32_t* __restrict__ counter, uint32_t* __restrict__ flag,
+    double* __restrict__ part, unsigned long long* __restrict__ stats) {
     __shared__ uint32_t lists[kWalkWaves][kListCap];
-    // per-wave tallies: node tests, accepted nodes, leaf particles, pair interactions
+    __shared__ double4 stages[kWalkWaves][kWave];
     unsigned long long t_nodes = 0, t_acc = 0, t_leafp = 0, t_pairs = 0;
+    const float eps2f = (float)eps2;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
     uint32_t* list = lists[w];
+    double4* stage = stages[w];
     const uint64_t lt = (1ull << lane) - 1ull;
     const size_t slot = (size_t)blockIdx.x * kWalkWaves + w;
     uint32_t* fa0 = frontier + slot * 2 * (size_t)fcap;
@@ -299,29 +339,65 @@ __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
                 hi[d] = fmax(hi[d], __shfl_xor(hi[d], o));
             }
         double ax = 0.0, ay = 0.0, az = 0.0;
+        // MIXED: box-relative FP32 target and Kahan-compensated FP32 sums
+        const double gcx = 0.5 * (lo[0] + hi[0]), gcy = 0.5 * (lo[1] + hi[1]),
+                     gcz = 0.5 * (lo[2] + hi[2]);
+        const float fx = (float)(me.x - gcx), fy = (float)(me.y - gcy), fz = (float)(me.z - gcz);
+        float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
+        float4* stage32 = reinterpret_cast<float4*>(stage);
         uint32_t nl = 0;  // list length (wave uniform)
 
         auto flush = [&]() {
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            __builtin_amdgcn_wave_barrier();
-            for (uint32_t q = 0; q < nl; ++q) {
-                const uint32_t e = __builtin_amdgcn_readfirstlane(list[q]);
-                if (e & kLeafFlag) {
-                    const uint4 md = meta[e & ~kLeafFlag];
-                    const uint32_t kb = __builtin_amdgcn_readfirstlane(md.x);
-                    const uint32_t ke = kb + __builtin_amdgcn_readfirstlane(md.y);
-                    for (uint32_t k = kb; k < ke; ++k) pair_ref(src[k], me.x, me.y, me.z, eps2, ax, ay, az);
-                    t_leafp += ke - kb;
-                    t_pairs += (unsigned long long)(ke - kb) * (unsigned long long)cnt;
+            for (uint32_t q0 = 0; q0 < nl; q0 += kWave) {
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                const uint32_t q = q0 + lane;
+                if (q < nl) {
+                    const double4 r = src[list[q]];
+                    if (MIXED)
+                        stage32[lane] = make_float4((float)(r.x - gcx), (float)(r.y - gcy),
+                                                    (float)(r.z - gcz), (float)r.w);
+                    else
+                        stage[lane] = r;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+                __builtin_amdgcn_wave_barrier();
+                const uint32_t m = min(nl - q0, (uint32_t)kWave);
+                if (MIXED) {
+                    float tx = 0.f, ty = 0.f, tz = 0.f;
+                    for (uint32_t k = 0; k < m; ++k) {
+                        const float4 r = stage32[k];
+                        const float dx = r.x - fx, dy = r.y - fy, dz = r.z - fz;
+                        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2f)));
+                        float ri = __builtin_amdgcn_rsqf(r2);
+                        ri = r2 > 0.f ? ri : 0.f;
+                        const float f = (r.w * ri) * (ri * ri);
+                        tx = __builtin_fmaf(f, dx, tx);
+                        ty = __builtin_fmaf(f, dy, ty);
+                        tz = __builtin_fmaf(f, dz, tz);
+                    }
+                    float y = tx - kx, t = sx + y; kx = (t - sx) - y; sx = t;
+                    y = ty - ky; t = sy + y; ky = (t - sy) - y; sy = t;
+                    y = tz - kz; t = sz + y; kz = (t - sz) - y; sz = t;
                 } else {
-                    pair_ref(src[(size_t)n + e], me.x, me.y, me.z, eps2, ax, ay, az);
-                    t_acc += 1;
-                    t_pairs += (unsigned long long)cnt;
+                    for (uint32_t k = 0; k < m; ++k)
+                        pair_ref(stage[k], me.x, me.y, me.z, eps2, ax, ay, az);
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
             __builtin_amdgcn_wave_barrier();
             nl = 0;
+        };
+        // appends the uniform index range [first, first + count) in order
+        auto append_range = [&](uint32_t first, uint32_t count) {
+            while (count > 0) {
+                if (nl == kListCap) flush();
+                const uint32_t k = min(count, kListCap - nl);
+                for (uint32_t r = lane; r < k; r += kWave) list[nl + r] = first + r;
+                nl += k;
+                first += k;
+                count -= k;
+            }
         };
 
         uint32_t* fa = fa0;
@@ -333,7 +409,6 @@ __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
         while (nf > 0 && !overflow) {
             uint32_t nn = 0;
             for (uint32_t base = 0; base < nf; base += kWave) {
-                if (nl > (uint32_t)(kListCap - kWave)) flush();
                 const uint32_t idx = base + lane;
                 const bool valid = idx < nf;
                 t_nodes += min(nf - base, (uint32_t)kWave);
@@ -350,26 +425,60 @@ __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
                 const bool accept = valid && (s * s < theta2 * d2);
                 const bool open = valid && !accept && nchild > 0;
                 const bool direct = valid && !accept && nchild == 0;
-                const uint64_t bl = __ballot(accept || direct);
-                if (accept || direct)
-                    list[nl + (uint32_t)__popcll(bl & lt)] = direct ? (v | kLeafFlag) : v;
-                nl += (uint32_t)__popcll(bl);
-                // children of opened internal nodes -> next frontier, lane order
-                const uint32_t k = open ? nchild : 0u;
-                uint32_t sc = k;
+                // sources of this chunk, in lane order: 1 per accepted node,
+                // count per opened leaf
+                const uint32_t add = accept ? 1u : (direct ? md.y : 0u);
+                uint32_t sc = add;
 #pragma unroll
                 for (int o = 1; o < kWave; o <<= 1) {
                     const uint32_t t = __shfl_up(sc, o);
                     if (lane >= o) sc += t;
                 }
                 const uint32_t total = __shfl(sc, kWave - 1);
-                if (nn + total > fcap) {
+                const uint64_t accb = __ballot(accept);
+                t_acc += (unsigned long long)__popcll(accb);
+                t_leafp += total - (uint32_t)__popcll(accb);
+                t_pairs += (unsigned long long)total * (unsigned long long)cnt;
+                if (total > 0) {
+                    if (nl + total > kListCap) flush();
+                    if (total <= kListCap) {
+                        const uint32_t pos = nl + sc - add;
+                        if (accept) list[pos] = (uint32_t)n + v;
+                        else if (direct)
+                            for (uint32_t q = 0; q < add; ++q) list[pos + q] = md.x + q;
+                        nl += total;
+                    } else {
+                        // a leaf larger than the list (coincident points):
+                        // lane by lane, same order
+                        uint64_t todo = __ballot(add > 0);
+                        while (todo) {
+                            const int l = __ffsll((long long)todo) - 1;
+                            todo &= todo - 1;
+                            const uint32_t lacc = __shfl((uint32_t)accept, l);
+                            const uint32_t lv = __shfl(v, l);
+                            const uint32_t lf = __shfl(md.x, l);
+                            const uint32_t la = __shfl(add, l);
+                            if (lacc) append_range((uint32_t)n + lv, 1u);
+                            else append_range(lf, la);
+                        }
+                    }
+                }
+                // children of opened internal nodes -> next frontier, lane order
+                const uint32_t k = open ? nchild : 0u;
+                uint32_t kc = k;
+#pragma unroll
+                for (int o = 1; o < kWave; o <<= 1) {
+                    const uint32_t t = __shfl_up(kc, o);
+                    if (lane >= o) kc += t;
+                }
+                const uint32_t ktot = __shfl(kc, kWave - 1);
+                if (nn + ktot > fcap) {
                     overflow = true;
                     break;
                 }
-                const uint32_t ex0 = nn + sc - k;
+                const uint32_t ex0 = nn + kc - k;
                 for (uint32_t q = 0; q < k; ++q) fb[ex0 + q] = md.z + q;
-                nn += total;
+                nn += ktot;
             }
             __threadfence_block();
             __builtin_amdgcn_wave_barrier();
@@ -379,6 +488,11 @@ __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
             nf = nn;
         }
         flush();
+        if (MIXED) {
+            ax = (double)sx - (double)kx;
+            ay = (double)sy - (double)ky;
+            az = (double)sz - (double)kz;
+        }
         if (overflow && lane == 0) atomicOr(flag, 1u);
         if (active) {
             const size_t i = (size_t)i0 + lane;
@@ -467,9 +581,14 @@ gravis_status bh_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
     walk_reset_kernel<<<1, 64, 0, st>>>(counter, stats);
     group_emit_kernel<<<tg, kTreeBlock, 0, st>>>(meta, lvl, (uint32_t)gs, groups, counter + 1);
     timer_begin(c);
-    bh_walk_kernel<<<blocks, kWalkWaves * kWave, 0, st>>>(src, meta, ni, groups, counter + 1,
-                                                          box, theta2, eps2, front, fcap,
-                                                          counter, flags, part, stats);
+    if (s->precision == GRAVIS_MIXED)
+        bh_walk_kernel<true><<<blocks, kWalkWaves * kWave, 0, st>>>(
+            src, meta, ni, groups, counter + 1, box, theta2, eps2, front, fcap, counter, flags,
+            part, stats);
+    else
+        bh_walk_kernel<false><<<blocks, kWalkWaves * kWave, 0, st>>>(
+            src, meta, ni, groups, counter + 1, box, theta2, eps2, front, fcap, counter, flags,
+            part, stats);
     timer_end(c, "bh_walk");
     GRAVIS_TRY(check_launch(c, "bh_walk"));
     uint32_t hflags[2] = {0, 0};

@@ -29,9 +29,10 @@
  *              axis bit order x:1 y:2 z:4 as octree.rs:67-79
  *   sort       stable sort of the 63-bit keys, ties by original index
  *   ltree      leaf-16 linear octree, breadth-first node order, FP64 COM sums
- *   gwalk      group walk: the particles of one compact tree cell (at most
- *              64) share one interaction list built breadth first with a
- *              box-distance opening test s^2 < theta^2 * d_min^2
+ *   gwalk      group walk: the particles of one group (consecutive sibling
+ *              cells packed to at most 64) share one interaction list built
+ *              breadth first with a box-distance opening test
+ *              s^2 < theta^2 * d_min^2
  *
  * Parity status of each function is stated in tests/test_oracle_*.py.
  */
@@ -764,40 +765,64 @@ static void gwalk(const ltree* t, double side0, double theta2, double eps2, cons
     if (ninter) *ninter += cnt_inter;
 }
 
-/* Target groups: the first node on every root-to-leaf path holding at most
- * `group` particles (compact by construction); a leaf holding more than
- * `group` particles (coincident points at level 21) is cut into runs of
- * `group`. Every particle lies in exactly one group. Group order does not
- * affect any result. Returns the group count; gf/gc receive (first, count). */
-static size_t lt_groups(const ltree* t, uint32_t group, uint32_t** gf, uint32_t** gc) {
-    size_t cap = 64, ng = 0;
-    uint32_t* f = (uint32_t*)malloc(cap * 4);
-    uint32_t* c = (uint32_t*)malloc(cap * 4);
-    uint32_t* stack = (uint32_t*)malloc((t->len + 1) * 4);
-    size_t sp = 0;
-    stack[sp++] = 0;
-    while (sp > 0) {
-        uint32_t v = stack[--sp];
-        if (t->cnt[v] <= group || t->nchild[v] == 0) {
-            for (uint32_t b = 0; b < t->cnt[v]; b += group) {
-                if (ng == cap) {
-                    cap *= 2;
-                    f = (uint32_t*)realloc(f, cap * 4);
-                    c = (uint32_t*)realloc(c, cap * 4);
-                }
-                uint32_t k = t->cnt[v] - b < group ? t->cnt[v] - b : group;
-                f[ng] = t->first[v] + b;
-                c[ng] = k;
-                ng++;
-            }
-            continue;
-        }
-        for (uint32_t q = 0; q < t->nchild[v]; ++q) stack[sp++] = t->child[v] + q;
+/* Target groups (at most `group` particles each, contiguous in sorted order):
+ * for every internal node holding more than `group` particles, its children
+ * are visited in order and consecutive children holding at most `group`
+ * particles are packed greedily into one group while the total stays within
+ * `group`; a child leaf holding more than `group` particles (coincident
+ * points at level 21) is cut into runs of `group`; a root holding at most
+ * `group` particles is one group. Every particle lies in exactly one group.
+ * Group order does not affect any result. Returns the group count; gf/gc
+ * receive (first, count). */
+typedef struct {
+    uint32_t *f, *c;
+    size_t n, cap;
+} glist;
+
+static void gl_push(glist* g, uint32_t f, uint32_t c) {
+    if (g->n == g->cap) {
+        g->cap = g->cap ? 2 * g->cap : 64;
+        g->f = (uint32_t*)realloc(g->f, g->cap * 4);
+        g->c = (uint32_t*)realloc(g->c, g->cap * 4);
     }
-    free(stack);
-    *gf = f;
-    *gc = c;
-    return ng;
+    g->f[g->n] = f;
+    g->c[g->n] = c;
+    g->n++;
+}
+
+static void gl_runs(glist* g, uint32_t first, uint32_t cnt, uint32_t group) {
+    for (uint32_t b = 0; b < cnt; b += group) gl_push(g, first + b, cnt - b < group ? cnt - b : group);
+}
+
+static size_t lt_groups(const ltree* t, uint32_t group, uint32_t** gf, uint32_t** gc) {
+    glist g = {0};
+    if (t->cnt[0] <= group || t->nchild[0] == 0) {
+        gl_runs(&g, t->first[0], t->cnt[0], group);
+    } else {
+        for (size_t p = 0; p < t->len; ++p) {
+            if (t->cnt[p] <= group || t->nchild[p] == 0) continue;
+            uint32_t rf = 0, rc = 0;
+            for (uint32_t q = 0; q < t->nchild[p]; ++q) {
+                uint32_t c = t->child[p] + q;
+                if (t->cnt[c] <= group) {
+                    if (rc > 0 && rc + t->cnt[c] > group) {
+                        gl_push(&g, rf, rc);
+                        rc = 0;
+                    }
+                    if (rc == 0) rf = t->first[c];
+                    rc += t->cnt[c];
+                } else {
+                    if (rc > 0) gl_push(&g, rf, rc);
+                    rc = 0;
+                    if (t->nchild[c] == 0) gl_runs(&g, t->first[c], t->cnt[c], group);
+                }
+            }
+            if (rc > 0) gl_push(&g, rf, rc);
+        }
+    }
+    *gf = g.f;
+    *gc = g.c;
+    return g.n;
 }
 
 /* Full linear-octree Barnes-Hut: box, keys, sort, tree, group walk.

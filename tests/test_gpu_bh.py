@@ -199,3 +199,20 @@ def test_bh_two_sphere_4m_parity(g):
     rms, mx = rel_errors(ref, p.acc()[t])
     say(f"  rms={rms:.3e} max={mx:.3e} nodes={g.default_context().bh_stats()['nodes']}")
     assert rms <= 4.942351e-3 and mx <= 3.588391e-2
+
+
+@pytest.mark.parametrize("n,theta", [(20000, 0.5), (3000, 0.3)])
+def test_bh_mixed_matches_fp64_walk(g, n, theta):
+    """precision='mixed' walks the same interaction lists with FP32 pair
+    forces on FP64 group-relative separations: RMS |da|/|a| <= 1e-6 and max
+    <= 1e-4 against the FP64 walk, and still within the reference's BH
+    error against brute force."""
+    say(f"bh mixed vs fp64 n={n} theta={theta}")
+    s = O.plummer(n, 42)
+    p64 = to_particles(g, s)
+    g.BarnesHut(0.01, theta).compute_accelerations(p64)
+    pm = to_particles(g, s)
+    g.BarnesHut(0.01, theta, precision="mixed").compute_accelerations(pm)
+    rms, mx = rel_errors(p64.acc(), pm.acc())
+    say(f"  mixed vs fp64 walk rms={rms:.3e} max={mx:.3e}")
+    assert rms <= 1e-6 and mx <= 1e-4
