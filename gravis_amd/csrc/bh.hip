@@ -163,18 +163,23 @@ __global__ __launch_bounds__(kTreeBlock) void tree_write_kernel(
     }
 }
 
-// Raw FP64 sums (sum m x, sum m y, sum m z, sum m) of the nodes of level L.
+// Raw FP64 sums (sum m x, sum m y, sum m z, sum m) and the particle bounding
+// box of the nodes of level L.
 __global__ __launch_bounds__(kTreeBlock) void tree_sums_kernel(const double4* __restrict__ src,
                                                               const uint4* __restrict__ meta,
                                                               const uint32_t* __restrict__ lvl,
                                                               int L,
-                                                              double4* __restrict__ sums) {
+                                                              double4* __restrict__ sums,
+                                                              double4* __restrict__ bmin,
+                                                              double4* __restrict__ bmax) {
     const uint32_t b = lvl[2 * L], e = lvl[2 * L + 1];
     for (uint32_t v = b + blockIdx.x * kTreeBlock + threadIdx.x; v < e;
          v += gridDim.x * kTreeBlock) {
         const uint4 md = meta[v];
         const uint32_t nc = (md.w >> 8) & 255u;
         double M = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
+        double lx = INFINITY, ly = INFINITY, lz = INFINITY;
+        double hx = -INFINITY, hy = -INFINITY, hz = -INFINITY;
         if (nc == 0) {
             for (uint32_t k = md.x; k < md.x + md.y; ++k) {
                 const double4 p = src[k];
@@ -182,6 +187,9 @@ __global__ __launch_bounds__(kTreeBlock) void tree_sums_kernel(const double4* __
                 X = X + p.w * p.x;
                 Y = Y + p.w * p.y;
                 Z = Z + p.w * p.z;
+                lx = fmin(lx, p.x); hx = fmax(hx, p.x);
+                ly = fmin(ly, p.y); hy = fmax(hy, p.y);
+                lz = fmin(lz, p.z); hz = fmax(hz, p.z);
             }
         } else {
             for (uint32_t c = md.z; c < md.z + nc; ++c) {
@@ -190,9 +198,15 @@ __global__ __launch_bounds__(kTreeBlock) void tree_sums_kernel(const double4* __
                 X = X + q.x;
                 Y = Y + q.y;
                 Z = Z + q.z;
+                const double4 a = bmin[c], b = bmax[c];
+                lx = fmin(lx, a.x); hx = fmax(hx, b.x);
+                ly = fmin(ly, a.y); hy = fmax(hy, b.y);
+                lz = fmin(lz, a.z); hz = fmax(hz, b.z);
             }
         }
         sums[v] = make_double4(X, Y, Z, M);
+        bmin[v] = make_double4(lx, ly, lz, 0.0);
+        bmax[v] = make_double4(hx, hy, hz, 0.0);
     }
 }
 
@@ -219,10 +233,12 @@ __global__ __launch_bounds__(kTreeBlock) void tree_com_kernel(double4* __restric
 // Target groups of at most gsize particles, contiguous in sorted order: the
 // children of every internal node holding more than gsize particles are
 // visited in order and consecutive children holding at most gsize are packed
-// greedily while the total stays within gsize; a child leaf holding more
-// (coincident points at level 21) is cut into runs of gsize; a root holding
-// at most gsize (or a root leaf) is cut the same way. Appended in any order:
-// each group's result depends only on its own particles and the tree.
+// greedily while the total stays within gsize and the packed particles'
+// bounding box is no wider (largest axis extent) than twice the widest
+// packed child's; a child leaf holding more (coincident points at level 21)
+// is cut into runs of gsize; a root holding at most gsize (or a root leaf) is
+// cut the same way. Appended in any order: each group's result depends only
+// on its own particles and the tree.
 __device__ __forceinline__ void emit_runs(uint2* groups, uint32_t* ngroups, uint32_t first,
                                           uint32_t cnt, uint32_t gsize) {
     const uint32_t k = (cnt + gsize - 1) / gsize;
@@ -231,8 +247,14 @@ __device__ __forceinline__ void emit_runs(uint2* groups, uint32_t* ngroups, uint
         groups[at + q] = make_uint2(first + q * gsize, min(gsize, cnt - q * gsize));
 }
 
+__device__ __forceinline__ double extent(double4 lo, double4 hi) {
+    return fmax(fmax(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z);
+}
+
 __global__ __launch_bounds__(kTreeBlock) void group_emit_kernel(const uint4* __restrict__ meta,
                                                                const uint32_t* __restrict__ lvl,
+                                                               const double4* __restrict__ bmin,
+                                                               const double4* __restrict__ bmax,
                                                                uint32_t gsize,
                                                                uint2* __restrict__ groups,
                                                                uint32_t* __restrict__ ngroups) {
@@ -247,16 +269,34 @@ __global__ __launch_bounds__(kTreeBlock) void group_emit_kernel(const uint4* __r
         }
         if (md.y <= gsize || nc == 0) continue;
         uint32_t rf = 0, rc = 0;
+        double4 rlo = make_double4(0, 0, 0, 0), rhi = rlo;
+        double rmax = 0.0;
         for (uint32_t q = 0; q < nc; ++q) {
-            const uint4 cm = meta[md.z + q];
+            const uint32_t cv = md.z + q;
+            const uint4 cm = meta[cv];
             const uint32_t cc = (cm.w >> 8) & 255u;
             if (cm.y <= gsize) {
-                if (rc > 0 && rc + cm.y > gsize) {
+                const double4 clo = bmin[cv], chi = bmax[cv];
+                const double cext = extent(clo, chi);
+                if (rc > 0) {
+                    const double4 ulo = make_double4(fmin(rlo.x, clo.x), fmin(rlo.y, clo.y),
+                                                     fmin(rlo.z, clo.z), 0.0);
+                    const double4 uhi = make_double4(fmax(rhi.x, chi.x), fmax(rhi.y, chi.y),
+                                                     fmax(rhi.z, chi.z), 0.0);
+                    if (rc + cm.y <= gsize && extent(ulo, uhi) <= 2.0 * fmax(rmax, cext)) {
+                        rc += cm.y;
+                        rlo = ulo;
+                        rhi = uhi;
+                        rmax = fmax(rmax, cext);
+                        continue;
+                    }
                     emit_runs(groups, ngroups, rf, rc, gsize);
-                    rc = 0;
                 }
-                if (rc == 0) rf = cm.x;
-                rc += cm.y;
+                rf = cm.x;
+                rc = cm.y;
+                rlo = clo;
+                rhi = chi;
+                rmax = cext;
             } else {
                 if (rc > 0) emit_runs(groups, ngroups, rf, rc, gsize);
                 rc = 0;
@@ -532,11 +572,13 @@ gravis_status bh_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
     // particles and have at most 8 children each.
     const size_t cap = 1 + (size_t)kMaxLevel * 8 * (n / (leaf + 1) + 1);
     if (cap > 0x7FFFFFFFull) return fail(c, GRAVIS_INVALID_ARG, "tree too large");
-    double4 *src, *sums;
+    double4 *src, *sums, *bmin, *bmax;
     uint4* meta;
     uint32_t *lvl, *nch, *off, *flags;
     GRAVIS_TRY(buf(c, "bh_src", n + cap, &src));
     GRAVIS_TRY(buf(c, "bh_sums", cap, &sums));
+    GRAVIS_TRY(buf(c, "bh_bmin", cap, &bmin));
+    GRAVIS_TRY(buf(c, "bh_bmax", cap, &bmax));
     GRAVIS_TRY(buf(c, "bh_meta", cap, &meta));
     GRAVIS_TRY(buf(c, "bh_lvl", 2 * (kMaxLevel + 2), &lvl));
     GRAVIS_TRY(buf(c, "bh_nch", cap, &nch));
@@ -559,7 +601,7 @@ gravis_status bh_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
     tree_scan_kernel<<<1, 1024, 0, st>>>(nch, off, lvl, kMaxLevel);
     GRAVIS_TRY(check_launch(c, "tree build"));
     for (int L = kMaxLevel; L >= 0; --L)
-        tree_sums_kernel<<<tg, kTreeBlock, 0, st>>>(src, meta, lvl, L, sums);
+        tree_sums_kernel<<<tg, kTreeBlock, 0, st>>>(src, meta, lvl, L, sums, bmin, bmax);
     tree_com_kernel<<<tg, kTreeBlock, 0, st>>>(src, ni, meta, lvl, sums);
     GRAVIS_TRY(check_launch(c, "tree com"));
 
@@ -579,7 +621,8 @@ gravis_status bh_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
     unsigned long long* stats;
     GRAVIS_TRY(buf(c, "bh_stats", 4, &stats));
     walk_reset_kernel<<<1, 64, 0, st>>>(counter, stats);
-    group_emit_kernel<<<tg, kTreeBlock, 0, st>>>(meta, lvl, (uint32_t)gs, groups, counter + 1);
+    group_emit_kernel<<<tg, kTreeBlock, 0, st>>>(meta, lvl, bmin, bmax, (uint32_t)gs, groups,
+                                                 counter + 1);
     timer_begin(c);
     if (s->precision == GRAVIS_MIXED)
         bh_walk_kernel<true><<<blocks, kWalkWaves * kWave, 0, st>>>(

@@ -569,6 +569,7 @@ void orc_sort_keys(size_t n, const uint64_t* keys, uint64_t* skeys, uint32_t* pe
 typedef struct {
     uint32_t *level, *first, *cnt, *child, *nchild;
     double *mass, *comx, *comy, *comz;
+    double *blo, *bhi; /* particle bounding box, 3 per node */
     size_t len, cap;
 } ltree;
 
@@ -585,12 +586,15 @@ static void lt_reserve(ltree* t, size_t need) {
     t->comx = (double*)realloc(t->comx, c * 8);
     t->comy = (double*)realloc(t->comy, c * 8);
     t->comz = (double*)realloc(t->comz, c * 8);
+    t->blo = (double*)realloc(t->blo, 3 * c * 8);
+    t->bhi = (double*)realloc(t->bhi, 3 * c * 8);
     t->cap = c;
 }
 
 static void lt_free(ltree* t) {
     free(t->level); free(t->first); free(t->cnt); free(t->child); free(t->nchild);
     free(t->mass); free(t->comx); free(t->comy); free(t->comz);
+    free(t->blo); free(t->bhi);
 }
 
 static inline uint32_t digit_at(uint64_t key, uint32_t child_level) {
@@ -635,18 +639,27 @@ static void lt_build(ltree* t, size_t n, const uint64_t* sk, uint32_t leaf, cons
     /* bottom-up sums: nodes in reverse BFS order have children done first */
     for (size_t vi = t->len; vi-- > 0;) {
         double M = 0, X = 0, Y = 0, Z = 0;
+        double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
         if (t->nchild[vi] == 0) {
             for (uint32_t k = t->first[vi]; k < t->first[vi] + t->cnt[vi]; ++k) {
                 M = M + sm[k];
                 X = X + sm[k] * sx[k]; Y = Y + sm[k] * sy[k]; Z = Z + sm[k] * sz[k];
+                lo[0] = fmin(lo[0], sx[k]); hi[0] = fmax(hi[0], sx[k]);
+                lo[1] = fmin(lo[1], sy[k]); hi[1] = fmax(hi[1], sy[k]);
+                lo[2] = fmin(lo[2], sz[k]); hi[2] = fmax(hi[2], sz[k]);
             }
         } else {
             for (uint32_t c = t->child[vi]; c < t->child[vi] + t->nchild[vi]; ++c) {
                 M = M + t->mass[c];
                 X = X + t->comx[c]; Y = Y + t->comy[c]; Z = Z + t->comz[c];
+                for (int d = 0; d < 3; ++d) {
+                    lo[d] = fmin(lo[d], t->blo[3 * c + d]);
+                    hi[d] = fmax(hi[d], t->bhi[3 * c + d]);
+                }
             }
         }
         t->mass[vi] = M; t->comx[vi] = X; t->comy[vi] = Y; t->comz[vi] = Z;
+        for (int d = 0; d < 3; ++d) { t->blo[3 * vi + d] = lo[d]; t->bhi[3 * vi + d] = hi[d]; }
     }
 }
 
@@ -768,12 +781,13 @@ static void gwalk(const ltree* t, double side0, double theta2, double eps2, cons
 /* Target groups (at most `group` particles each, contiguous in sorted order):
  * for every internal node holding more than `group` particles, its children
  * are visited in order and consecutive children holding at most `group`
- * particles are packed greedily into one group while the total stays within
- * `group`; a child leaf holding more than `group` particles (coincident
- * points at level 21) is cut into runs of `group`; a root holding at most
- * `group` particles is one group. Every particle lies in exactly one group.
- * Group order does not affect any result. Returns the group count; gf/gc
- * receive (first, count). */
+ * particles are packed greedily while the total stays within `group` and
+ * the packed particles' bounding box is no wider (largest axis extent) than
+ * twice the widest packed child's; a child leaf holding more than `group`
+ * particles (coincident points at level 21) is cut into runs of `group`; a
+ * root holding at most `group` particles is one group. Every particle lies
+ * in exactly one group. Group order does not affect any result. Returns the
+ * group count; gf/gc receive (first, count). */
 typedef struct {
     uint32_t *f, *c;
     size_t n, cap;
@@ -802,15 +816,29 @@ static size_t lt_groups(const ltree* t, uint32_t group, uint32_t** gf, uint32_t*
         for (size_t p = 0; p < t->len; ++p) {
             if (t->cnt[p] <= group || t->nchild[p] == 0) continue;
             uint32_t rf = 0, rc = 0;
+            double rlo[3], rhi[3], rmax = 0.0;
             for (uint32_t q = 0; q < t->nchild[p]; ++q) {
                 uint32_t c = t->child[p] + q;
                 if (t->cnt[c] <= group) {
-                    if (rc > 0 && rc + t->cnt[c] > group) {
+                    const double* clo = t->blo + 3 * c;
+                    const double* chi = t->bhi + 3 * c;
+                    double cext = fmax(fmax(chi[0] - clo[0], chi[1] - clo[1]), chi[2] - clo[2]);
+                    if (rc > 0) {
+                        double ulo[3], uhi[3];
+                        for (int d = 0; d < 3; ++d) { ulo[d] = fmin(rlo[d], clo[d]); uhi[d] = fmax(rhi[d], chi[d]); }
+                        double uext = fmax(fmax(uhi[0] - ulo[0], uhi[1] - ulo[1]), uhi[2] - ulo[2]);
+                        if (rc + t->cnt[c] <= group && uext <= 2.0 * fmax(rmax, cext)) {
+                            rc += t->cnt[c];
+                            for (int d = 0; d < 3; ++d) { rlo[d] = ulo[d]; rhi[d] = uhi[d]; }
+                            rmax = fmax(rmax, cext);
+                            continue;
+                        }
                         gl_push(&g, rf, rc);
-                        rc = 0;
                     }
-                    if (rc == 0) rf = t->first[c];
-                    rc += t->cnt[c];
+                    rf = t->first[c];
+                    rc = t->cnt[c];
+                    for (int d = 0; d < 3; ++d) { rlo[d] = clo[d]; rhi[d] = chi[d]; }
+                    rmax = cext;
                 } else {
                     if (rc > 0) gl_push(&g, rf, rc);
                     rc = 0;
