@@ -63,6 +63,9 @@ class Context:
     def set_profiling(self, on: bool):
         self.check(self._L.gravis_set_profiling(self.handle, 1 if on else 0))
 
+    def set_tuning(self, key: str, value: int):
+        self.check(self._L.gravis_set_tuning(self.handle, key.encode(), int(value)))
+
     def bh_stats(self) -> dict:
         """Work counters of the last Barnes-Hut call."""
         out = (C.c_uint64 * 5)()

@@ -525,6 +525,18 @@ gravis_status gravis_ensemble_adjoint(
     return GRAVIS_OK;
 }
 
+gravis_status gravis_set_tuning(gravis_ctx* g, const char* key, int64_t value) {
+    if (!g || !key) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    if (std::strcmp(key, "bh_frontier_cap") == 0) {
+        if (value < 64 || value > (1ll << 26))
+            return fail(c, GRAVIS_INVALID_ARG, "bh_frontier_cap must be in [64, 2^26]");
+        c->bh_frontier_cap = (uint32_t)value;
+        return GRAVIS_OK;
+    }
+    return fail(c, GRAVIS_INVALID_ARG, "unknown tuning key '%s'", key);
+}
+
 gravis_status gravis_bh_stats(gravis_ctx* g, uint64_t out[5]) {
     if (!g || !out) return GRAVIS_INVALID_ARG;
     for (int k = 0; k < 5; ++k) out[k] = g->c.bh_stats[k];

@@ -25,6 +25,8 @@
 //           group's bounding box (so never less accurate than the
 //           reference's per-particle test); a rejected leaf contributes its
 //           particles directly; a rejected internal node pushes its children.
+//           If a group's frontier would exceed the per-wave capacity, the
+//           group is walked again one target at a time (point boxes).
 //           Accepted nodes and the particles of opened leaves are appended,
 //           in frontier order, as source indices to a per-wave LDS list;
 //           each flush round stages 64 source records in LDS (one load per
@@ -354,10 +356,11 @@ __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
     const int w = threadIdx.x / kWave;
     uint32_t* list = lists[w];
     double4* stage = stages[w];
+    float4* stage32 = reinterpret_cast<float4*>(stage);
     const uint64_t lt = (1ull << lane) - 1ull;
     const size_t slot = (size_t)blockIdx.x * kWalkWaves + w;
-    uint32_t* fa0 = frontier + slot * 2 * (size_t)fcap;
-    uint32_t* fb0 = fa0 + fcap;
+    uint32_t* const fa0 = frontier + slot * 2 * (size_t)fcap;
+    uint32_t* const fb0 = fa0 + fcap;
     const double side0 = 2.0 * box[3];  // root side, octree.rs:81-85
 
     for (;;) {
@@ -370,22 +373,23 @@ __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
         const int cnt = (int)gr.y;
         const bool active = lane < cnt;
         const double4 me = src[i0 + (active ? lane : 0)];
-        double lo[3] = {me.x, me.y, me.z}, hi[3] = {me.x, me.y, me.z};
+        double glo[3] = {me.x, me.y, me.z}, ghi[3] = {me.x, me.y, me.z};
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1)
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                lo[d] = fmin(lo[d], __shfl_xor(lo[d], o));
-                hi[d] = fmax(hi[d], __shfl_xor(hi[d], o));
+                glo[d] = fmin(glo[d], __shfl_xor(glo[d], o));
+                ghi[d] = fmax(ghi[d], __shfl_xor(ghi[d], o));
             }
-        double ax = 0.0, ay = 0.0, az = 0.0;
-        // MIXED: box-relative FP32 target and Kahan-compensated FP32 sums
-        const double gcx = 0.5 * (lo[0] + hi[0]), gcy = 0.5 * (lo[1] + hi[1]),
-                     gcz = 0.5 * (lo[2] + hi[2]);
+        // MIXED: records and target are rounded to FP32 relative to the
+        // group's box centre
+        const double gcx = 0.5 * (glo[0] + ghi[0]), gcy = 0.5 * (glo[1] + ghi[1]),
+                     gcz = 0.5 * (glo[2] + ghi[2]);
         const float fx = (float)(me.x - gcx), fy = (float)(me.y - gcy), fz = (float)(me.z - gcz);
+        double ax = 0.0, ay = 0.0, az = 0.0;
         float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
-        float4* stage32 = reinterpret_cast<float4*>(stage);
-        uint32_t nl = 0;  // list length (wave uniform)
+        uint32_t nl = 0;    // list length (wave uniform)
+        bool keep = true;   // this lane's target is being walked
 
         auto flush = [&]() {
             for (uint32_t q0 = 0; q0 < nl; q0 += kWave) {
@@ -416,12 +420,20 @@ __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
                         ty = __builtin_fmaf(f, dy, ty);
                         tz = __builtin_fmaf(f, dz, tz);
                     }
-                    float y = tx - kx, t = sx + y; kx = (t - sx) - y; sx = t;
-                    y = ty - ky; t = sy + y; ky = (t - sy) - y; sy = t;
-                    y = tz - kz; t = sz + y; kz = (t - sz) - y; sz = t;
+                    if (keep) {
+                        float y = tx - kx, t = sx + y; kx = (t - sx) - y; sx = t;
+                        y = ty - ky; t = sy + y; ky = (t - sy) - y; sy = t;
+                        y = tz - kz; t = sz + y; kz = (t - sz) - y; sz = t;
+                    }
                 } else {
+                    double tx = ax, ty = ay, tz = az;
                     for (uint32_t k = 0; k < m; ++k)
-                        pair_ref(stage[k], me.x, me.y, me.z, eps2, ax, ay, az);
+                        pair_ref(stage[k], me.x, me.y, me.z, eps2, tx, ty, tz);
+                    if (keep) {
+                        ax = tx;
+                        ay = ty;
+                        az = tz;
+                    }
                 }
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -439,101 +451,117 @@ __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
                 count -= k;
             }
         };
-
-        uint32_t* fa = fa0;
-        uint32_t* fb = fb0;
-        if (lane == 0) fa[0] = 0u;
-        uint32_t nf = 1;
-        bool overflow = false;
-        __threadfence_block();
-        while (nf > 0 && !overflow) {
-            uint32_t nn = 0;
-            for (uint32_t base = 0; base < nf; base += kWave) {
-                const uint32_t idx = base + lane;
-                const bool valid = idx < nf;
-                t_nodes += min(nf - base, (uint32_t)kWave);
-                const uint32_t v = valid ? fa[idx] : 0u;
-                const uint4 md = meta[v];
-                const double4 c = src[(size_t)n + v];
-                const uint32_t level = md.w & 255u;
-                const uint32_t nchild = (md.w >> 8) & 255u;
-                const double s = ldexp(side0, -(int)level);
-                const double ex = fmax(fmax(lo[0] - c.x, c.x - hi[0]), 0.0);
-                const double ey = fmax(fmax(lo[1] - c.y, c.y - hi[1]), 0.0);
-                const double ez = fmax(fmax(lo[2] - c.z, c.z - hi[2]), 0.0);
-                const double d2 = ex * ex + ey * ey + ez * ez;
-                const bool accept = valid && (s * s < theta2 * d2);
-                const bool open = valid && !accept && nchild > 0;
-                const bool direct = valid && !accept && nchild == 0;
-                // sources of this chunk, in lane order: 1 per accepted node,
-                // count per opened leaf
-                const uint32_t add = accept ? 1u : (direct ? md.y : 0u);
-                uint32_t sc = add;
+        // Breadth-first walk with opening box (lo, hi) for `ntargets`
+        // targets; returns true if a next frontier would exceed fcap.
+        auto walk = [&](const double (&lo)[3], const double (&hi)[3], int ntargets) -> bool {
+            uint32_t* fa = fa0;
+            uint32_t* fb = fb0;
+            if (lane == 0) fa[0] = 0u;
+            uint32_t nf = 1;
+            bool overflow = false;
+            __threadfence_block();
+            while (nf > 0 && !overflow) {
+                uint32_t nn = 0;
+                for (uint32_t base = 0; base < nf; base += kWave) {
+                    const uint32_t idx = base + lane;
+                    const bool valid = idx < nf;
+                    t_nodes += min(nf - base, (uint32_t)kWave);
+                    const uint32_t v = valid ? fa[idx] : 0u;
+                    const uint4 md = meta[v];
+                    const double4 c = src[(size_t)n + v];
+                    const uint32_t level = md.w & 255u;
+                    const uint32_t nchild = (md.w >> 8) & 255u;
+                    const double s = ldexp(side0, -(int)level);
+                    const double ex = fmax(fmax(lo[0] - c.x, c.x - hi[0]), 0.0);
+                    const double ey = fmax(fmax(lo[1] - c.y, c.y - hi[1]), 0.0);
+                    const double ez = fmax(fmax(lo[2] - c.z, c.z - hi[2]), 0.0);
+                    const double d2 = ex * ex + ey * ey + ez * ez;
+                    const bool accept = valid && (s * s < theta2 * d2);
+                    const bool open = valid && !accept && nchild > 0;
+                    const bool direct = valid && !accept && nchild == 0;
+                    // sources of this chunk in lane order: 1 per accepted
+                    // node, count per opened leaf
+                    const uint32_t add = accept ? 1u : (direct ? md.y : 0u);
+                    uint32_t sc = add;
 #pragma unroll
-                for (int o = 1; o < kWave; o <<= 1) {
-                    const uint32_t t = __shfl_up(sc, o);
-                    if (lane >= o) sc += t;
-                }
-                const uint32_t total = __shfl(sc, kWave - 1);
-                const uint64_t accb = __ballot(accept);
-                t_acc += (unsigned long long)__popcll(accb);
-                t_leafp += total - (uint32_t)__popcll(accb);
-                t_pairs += (unsigned long long)total * (unsigned long long)cnt;
-                if (total > 0) {
-                    if (nl + total > kListCap) flush();
-                    if (total <= kListCap) {
-                        const uint32_t pos = nl + sc - add;
-                        if (accept) list[pos] = (uint32_t)n + v;
-                        else if (direct)
-                            for (uint32_t q = 0; q < add; ++q) list[pos + q] = md.x + q;
-                        nl += total;
-                    } else {
-                        // a leaf larger than the list (coincident points):
-                        // lane by lane, same order
-                        uint64_t todo = __ballot(add > 0);
-                        while (todo) {
-                            const int l = __ffsll((long long)todo) - 1;
-                            todo &= todo - 1;
-                            const uint32_t lacc = __shfl((uint32_t)accept, l);
-                            const uint32_t lv = __shfl(v, l);
-                            const uint32_t lf = __shfl(md.x, l);
-                            const uint32_t la = __shfl(add, l);
-                            if (lacc) append_range((uint32_t)n + lv, 1u);
-                            else append_range(lf, la);
+                    for (int o = 1; o < kWave; o <<= 1) {
+                        const uint32_t t = __shfl_up(sc, o);
+                        if (lane >= o) sc += t;
+                    }
+                    const uint32_t total = __shfl(sc, kWave - 1);
+                    const uint64_t accb = __ballot(accept);
+                    t_acc += (unsigned long long)__popcll(accb);
+                    t_leafp += total - (uint32_t)__popcll(accb);
+                    t_pairs += (unsigned long long)total * (unsigned long long)ntargets;
+                    if (total > 0) {
+                        if (nl + total > kListCap) flush();
+                        if (total <= kListCap) {
+                            const uint32_t pos = nl + sc - add;
+                            if (accept) list[pos] = (uint32_t)n + v;
+                            else if (direct)
+                                for (uint32_t q = 0; q < add; ++q) list[pos + q] = md.x + q;
+                            nl += total;
+                        } else {
+                            // a leaf larger than the list (coincident points):
+                            // lane by lane, same order
+                            uint64_t todo = __ballot(add > 0);
+                            while (todo) {
+                                const int l = __ffsll((long long)todo) - 1;
+                                todo &= todo - 1;
+                                const uint32_t lacc = __shfl((uint32_t)accept, l);
+                                const uint32_t lv = __shfl(v, l);
+                                const uint32_t lf = __shfl(md.x, l);
+                                const uint32_t la = __shfl(add, l);
+                                if (lacc) append_range((uint32_t)n + lv, 1u);
+                                else append_range(lf, la);
+                            }
                         }
                     }
-                }
-                // children of opened internal nodes -> next frontier, lane order
-                const uint32_t k = open ? nchild : 0u;
-                uint32_t kc = k;
+                    // children of opened internal nodes -> next frontier
+                    const uint32_t k = open ? nchild : 0u;
+                    uint32_t kc = k;
 #pragma unroll
-                for (int o = 1; o < kWave; o <<= 1) {
-                    const uint32_t t = __shfl_up(kc, o);
-                    if (lane >= o) kc += t;
+                    for (int o = 1; o < kWave; o <<= 1) {
+                        const uint32_t t = __shfl_up(kc, o);
+                        if (lane >= o) kc += t;
+                    }
+                    const uint32_t ktot = __shfl(kc, kWave - 1);
+                    if (nn + ktot > fcap) {
+                        overflow = true;
+                        break;
+                    }
+                    const uint32_t ex0 = nn + kc - k;
+                    for (uint32_t q = 0; q < k; ++q) fb[ex0 + q] = md.z + q;
+                    nn += ktot;
                 }
-                const uint32_t ktot = __shfl(kc, kWave - 1);
-                if (nn + ktot > fcap) {
-                    overflow = true;
-                    break;
-                }
-                const uint32_t ex0 = nn + kc - k;
-                for (uint32_t q = 0; q < k; ++q) fb[ex0 + q] = md.z + q;
-                nn += ktot;
+                __threadfence_block();
+                __builtin_amdgcn_wave_barrier();
+                uint32_t* t = fa;
+                fa = fb;
+                fb = t;
+                nf = nn;
             }
-            __threadfence_block();
-            __builtin_amdgcn_wave_barrier();
-            uint32_t* t = fa;
-            fa = fb;
-            fb = t;
-            nf = nn;
+            flush();
+            return overflow;
+        };
+
+        if (walk(glo, ghi, cnt)) {
+            // the group box opens too much: walk every target alone
+            ax = ay = az = 0.0;
+            sx = sy = sz = kx = ky = kz = 0.f;
+            for (int l = 0; l < cnt; ++l) {
+                const double px = __shfl(me.x, l), py = __shfl(me.y, l), pz = __shfl(me.z, l);
+                const double p[3] = {px, py, pz};
+                keep = lane == l;
+                if (walk(p, p, 1) && lane == 0) atomicOr(flag, 1u);
+            }
+            keep = true;
         }
-        flush();
         if (MIXED) {
             ax = (double)sx - (double)kx;
             ay = (double)sy - (double)ky;
             az = (double)sz - (double)kz;
         }
-        if (overflow && lane == 0) atomicOr(flag, 1u);
         if (active) {
             const size_t i = (size_t)i0 + lane;
             part[i] = ax;
@@ -608,7 +636,7 @@ gravis_status bh_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
     // target groups, then the walk
     const int blocks = (int)std::min<long>((long)(n / 64 / kWalkWaves) + 1,
                                            (long)c->num_cus * 4);
-    const uint32_t fcap = (uint32_t)std::min<size_t>(cap, 65536);
+    const uint32_t fcap = (uint32_t)std::min<size_t>(cap, c->bh_frontier_cap);
     uint32_t *front, *counter;
     uint2* groups;
     GRAVIS_TRY(buf(c, "bh_front", (size_t)blocks * kWalkWaves * 2 * fcap, &front));
@@ -645,8 +673,8 @@ gravis_status bh_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
         return fail(c, GRAVIS_INTERNAL, "octree node capacity exceeded (%zu nodes)", cap);
     if (hflags[0])
         return fail(c, GRAVIS_INTERNAL,
-                    "Barnes-Hut walk frontier exceeded %u nodes (theta %g too small for n=%zu)",
-                    fcap, s->theta, n);
+                    "Barnes-Hut walk frontier of a single target exceeded %u nodes (theta %g)",
+                    fcap, s->theta);
     return finish_forces(c, part, 1, ni, perm, fa);
 }
 

@@ -45,6 +45,8 @@ struct Ctx {
     // last Barnes-Hut call: node tests, accepted nodes, leaf particles,
     // pair interactions, tree nodes
     unsigned long long bh_stats[5] = {0, 0, 0, 0, 0};
+    // Barnes-Hut walk: breadth-first frontier capacity per wave (nodes)
+    uint32_t bh_frontier_cap = 16384;
 };
 
 // Status helpers ----------------------------------------------------------

@@ -206,6 +206,12 @@ gravis_status gravis_set_profiling(gravis_ctx* ctx, int enable);
 gravis_status gravis_kernel_time(gravis_ctx* ctx, const char* name, double* ms_total,
                                  uint64_t* launches);
 
+/* Tuning knobs (no reference counterpart): "bh_frontier_cap" = per-wave
+ * breadth-first frontier capacity of the Barnes-Hut walk in nodes (default
+ * 16384; a group whose frontier would exceed it is walked one target at a
+ * time). */
+gravis_status gravis_set_tuning(gravis_ctx* ctx, const char* key, int64_t value);
+
 /* Work counters of the last Barnes-Hut force call: out[0] frontier node
  * tests, out[1] accepted (monopole) nodes, out[2] leaf particles used as
  * sources, out[3] target-source pair interactions, out[4] tree nodes. */

@@ -725,17 +725,14 @@ size_t orc_linear_tree(size_t n, const double* x, const double* y, const double*
  *   internal, not accepted    -> children appended to the next frontier
  * Accelerations are accumulated in list order with the reference pair
  * arithmetic (barnes_hut.rs:114-135); a source at r2 == 0 is skipped. */
-static void gwalk(const ltree* t, double side0, double theta2, double eps2, const double* sx,
-                  const double* sy, const double* sz, const double* sm, size_t g0, size_t g1,
-                  double* ax, double* ay, double* az, uint32_t* fa, uint32_t* fb,
-                  uint64_t* ninter) {
-    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (size_t i = g0; i < g1; ++i) {
-        lo[0] = fmin(lo[0], sx[i]); hi[0] = fmax(hi[0], sx[i]);
-        lo[1] = fmin(lo[1], sy[i]); hi[1] = fmax(hi[1], sy[i]);
-        lo[2] = fmin(lo[2], sz[i]); hi[2] = fmax(hi[2], sz[i]);
-    }
-    for (size_t i = g0; i < g1; ++i) { ax[i] = 0; ay[i] = 0; az[i] = 0; }
+static size_t g_max_frontier = 0;
+
+/* One breadth-first walk for targets [g0, g1) with opening box (lo, hi).
+ * Returns 1 (and leaves partial sums) if a next frontier exceeds fcap. */
+static int walk_box(const ltree* t, double side0, double theta2, double eps2, const double* sx,
+                    const double* sy, const double* sz, const double* sm, size_t g0, size_t g1,
+                    const double lo[3], const double hi[3], double* ax, double* ay, double* az,
+                    uint32_t* fa, uint32_t* fb, uint32_t fcap, uint64_t* ninter) {
     size_t nf = 1;
     fa[0] = 0;
     uint64_t cnt_inter = 0;
@@ -774,8 +771,53 @@ static void gwalk(const ltree* t, double side0, double theta2, double eps2, cons
         }
         uint32_t* tmp = fa; fa = fb; fb = tmp;
         nf = nn;
+        if (nf > g_max_frontier) {
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+            if (nf > g_max_frontier) g_max_frontier = nf;
+        }
+        if (nf > fcap) {
+            if (ninter) *ninter += cnt_inter;
+            return 1;
+        }
     }
     if (ninter) *ninter += cnt_inter;
+    return 0;
+}
+
+/* Group walk for targets [g0, g1): the group's bounding box opens the tree;
+ * if a breadth-first frontier would exceed fcap nodes, the group's sums are
+ * discarded and every target is walked alone with its own position as the
+ * box (whose frontier the opening geometry bounds). */
+static void gwalk(const ltree* t, double side0, double theta2, double eps2, const double* sx,
+                  const double* sy, const double* sz, const double* sm, size_t g0, size_t g1,
+                  double* ax, double* ay, double* az, uint32_t* fa, uint32_t* fb, uint32_t fcap,
+                  uint64_t* ninter) {
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (size_t i = g0; i < g1; ++i) {
+        lo[0] = fmin(lo[0], sx[i]); hi[0] = fmax(hi[0], sx[i]);
+        lo[1] = fmin(lo[1], sy[i]); hi[1] = fmax(hi[1], sy[i]);
+        lo[2] = fmin(lo[2], sz[i]); hi[2] = fmax(hi[2], sz[i]);
+    }
+    for (size_t i = g0; i < g1; ++i) { ax[i] = 0; ay[i] = 0; az[i] = 0; }
+    if (!walk_box(t, side0, theta2, eps2, sx, sy, sz, sm, g0, g1, lo, hi, ax, ay, az, fa, fb, fcap,
+                  ninter))
+        return;
+    for (size_t i = g0; i < g1; ++i) {
+        ax[i] = 0; ay[i] = 0; az[i] = 0;
+        double p[3] = {sx[i], sy[i], sz[i]};
+        walk_box(t, side0, theta2, eps2, sx, sy, sz, sm, i, i + 1, p, p, ax, ay, az, fa, fb,
+                 0xFFFFFFFFu, ninter);
+    }
+}
+
+/* Largest breadth-first frontier seen by the group walk since the last call
+ * (test hook for the device walk's frontier capacity). */
+size_t orc_bh_max_frontier(void) {
+    size_t v = g_max_frontier;
+    g_max_frontier = 0;
+    return v;
 }
 
 /* Target groups (at most `group` particles each, contiguous in sorted order):
@@ -857,8 +899,8 @@ static size_t lt_groups(const ltree* t, uint32_t group, uint32_t** gf, uint32_t*
  * Accumulates into ax/ay/az in the caller's order. Returns the total number
  * of pair interactions evaluated (targets x sources). */
 uint64_t orc_bh_group_accel(size_t n, double eps, double theta, uint32_t leaf, uint32_t group,
-                            const double* x, const double* y, const double* z, const double* m,
-                            double* ax, double* ay, double* az, int nthreads) {
+                            uint32_t fcap, const double* x, const double* y, const double* z,
+                            const double* m, double* ax, double* ay, double* az, int nthreads) {
     if (n == 0) return 0;
     double box[4];
     orc_root_box(n, x, y, z, box);
@@ -893,7 +935,7 @@ uint64_t orc_bh_group_accel(size_t n, double eps, double theta, uint32_t leaf, u
 #endif
         for (long long g = 0; g < (long long)ng; ++g) {
             size_t g0 = gfirst[g], g1 = g0 + gcnt[g];
-            gwalk(&t, side0, theta2, eps2, sx, sy, sz, sm, g0, g1, bx, by, bz, fa, fb, &total);
+            gwalk(&t, side0, theta2, eps2, sx, sy, sz, sm, g0, g1, bx, by, bz, fa, fb, fcap, &total);
         }
         free(fa); free(fb);
     }
@@ -918,7 +960,7 @@ static void orc_force(int algo, size_t n, double eps, double theta, const double
     for (size_t i = 0; i < n; ++i) { ax[i] = 0.0; ay[i] = 0.0; az[i] = 0.0; }
     if (algo == ORC_BF) orc_bf_accel(n, eps, x, y, z, m, ax, ay, az);
     else if (algo == ORC_BH_REF) orc_bh_accel(n, eps, theta, x, y, z, m, ax, ay, az, nthreads);
-    else orc_bh_group_accel(n, eps, theta, 16, 64, x, y, z, m, ax, ay, az, nthreads);
+    else orc_bh_group_accel(n, eps, theta, 16, 64, 16384, x, y, z, m, ax, ay, az, nthreads);
 }
 
 void orc_kdk_steps(size_t n, int algo, double eps, double theta, double* x, double* y,

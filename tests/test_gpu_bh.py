@@ -216,3 +216,23 @@ def test_bh_mixed_matches_fp64_walk(g, n, theta):
     rms, mx = rel_errors(p64.acc(), pm.acc())
     say(f"  mixed vs fp64 walk rms={rms:.3e} max={mx:.3e}")
     assert rms <= 1e-6 and mx <= 1e-4
+
+
+def test_bh_frontier_fallback_bit_exact(g):
+    """With a tiny frontier capacity (forced from the host) most groups fall
+    back to per-target walks; the GPU still matches the oracle, which applies
+    the same rule, bit for bit."""
+    say("bh frontier fallback (cap 64)")
+    s = O.plummer(5000, 12)
+    ref = s.copy()
+    O.bh_group_accel(ref, 0.05, 0.5, 16, 64, frontier_cap=64)
+    ctx = g.Context(0)
+    ctx.set_tuning("bh_frontier_cap", 64)
+    p = to_particles(g, s)
+    g.BarnesHut(0.05, 0.5, ctx=ctx).compute_accelerations(p)
+    assert np.array_equal(p.acc(), ref.acc())
+    st = ctx.bh_stats()
+    normal = s.copy()
+    O.bh_group_accel(normal, 0.05, 0.5)
+    assert not np.array_equal(normal.acc(), ref.acc())  # the fallback did run
+    ctx.close()
