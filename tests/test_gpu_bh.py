@@ -219,15 +219,16 @@ def test_bh_mixed_matches_fp64_walk(g, n, theta):
 
 
 def test_bh_frontier_fallback_bit_exact(g):
-    """With a tiny frontier capacity (forced from the host) most groups fall
-    back to per-target walks; the GPU still matches the oracle, which applies
-    the same rule, bit for bit."""
-    say("bh frontier fallback (cap 64)")
+    """With a small frontier capacity (forced from the host: 350 nodes, above
+    the largest single-target frontier of this input, 327, below the largest
+    group frontier, 402) some groups fall back to per-target walks; the GPU
+    still matches the oracle, which applies the same rule, bit for bit."""
+    say("bh frontier fallback (cap 350)")
     s = O.plummer(5000, 12)
     ref = s.copy()
-    O.bh_group_accel(ref, 0.05, 0.5, 16, 64, frontier_cap=64)
+    O.bh_group_accel(ref, 0.05, 0.5, 16, 64, frontier_cap=350)
     ctx = g.Context(0)
-    ctx.set_tuning("bh_frontier_cap", 64)
+    ctx.set_tuning("bh_frontier_cap", 350)
     p = to_particles(g, s)
     g.BarnesHut(0.05, 0.5, ctx=ctx).compute_accelerations(p)
     assert np.array_equal(p.acc(), ref.acc())
