@@ -22,6 +22,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -49,6 +51,9 @@ def parse():
     ap.add_argument("--bh-n-each", type=int, default=2097152)
     ap.add_argument("--bh-steps", type=int, default=20)
     ap.add_argument("--no-bh", action="store_true")
+    ap.add_argument("--ens-steps", type=int, default=200)
+    ap.add_argument("--ens-adj-steps", type=int, default=64)
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--probe-copy", action="store_true",
                     help="only run a 1 GiB device copy (PMC byte-counter calibration)")
     return ap.parse_args()
@@ -159,6 +164,67 @@ def run_bh(args, g, torch, ctx, precision="mixed", steps=None):
                         "approach, KDK with Barnes-Hut theta=0.5, leaf 16"}
 
 
+def run_ensemble(args, g, torch, ctx):
+    """256 Plummer spheres (seed 42 + s), N_s drawn in [512, 8192] (seed 42),
+    eps 0.05, dt 1e-3, KDK with mixed brute force; one force launch covers
+    every system and each step replays a captured hipGraph. Then the
+    checkpointed adjoint (every 32 steps) of L = sum |x(T) - target|^2
+    (targets seeded 10000 + s)."""
+    import ctypes as C
+    import gravis_amd.ensemble as E
+    cnts = E.counts(42, 256, 512, 8192)
+    n = int(cnts.sum())
+    inter = float((cnts.astype(np.float64) * (cnts.astype(np.float64) - 1)).sum())
+    log(f"bench: ensemble of {len(cnts)} systems, N_total={n}")
+    st = E.plummer_ensemble(cnts)
+    tg = [np.concatenate(a) for a in zip(*[E.targets(10000 + s, int(c))
+                                            for s, c in enumerate(cnts)])]
+    dev = torch.device("cuda:0")
+    T = {k: torch.from_numpy(st[k]).to(dev) for k in st}
+    TT = [torch.from_numpy(a).to(dev) for a in tg]
+    torch.cuda.synchronize()
+    off = E._offsets(cnts)
+    solver = E._solver(0.05, "mixed")
+    L = ctx._L
+    p = lambda t: C.c_void_p(t.data_ptr())
+    ctx.check(L.gravis_ensemble_accel_device(ctx.handle, C.byref(solver), len(cnts), E._p(off),
+                                             *[p(T[k]) for k in ("x", "y", "z", "m", "ax", "ay",
+                                                                  "az")]))
+    keys = ("x", "y", "z", "vx", "vy", "vz", "m", "ax", "ay", "az")
+
+    def kdk(steps):
+        ctx.check(L.gravis_ensemble_kdk_device(ctx.handle, C.byref(solver), len(cnts), E._p(off),
+                                               *[p(T[k]) for k in keys], 1e-3, steps, 1))
+    kdk(2)
+    torch.cuda.synchronize()
+    log(f"bench: timing {args.ens_steps} ensemble KDK steps (hipGraph)")
+    t0 = time.perf_counter()
+    kdk(args.ens_steps)
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / args.ens_steps
+    out = torch.zeros(7, n, dtype=torch.float64, device=dev)
+    loss = torch.zeros(len(cnts), dtype=torch.float64, device=dev)
+    log(f"bench: timing the adjoint over {args.ens_adj_steps} steps")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.check(L.gravis_ensemble_adjoint_device(
+        ctx.handle, C.byref(solver), len(cnts), E._p(off),
+        *[p(T[k]) for k in ("x", "y", "z", "vx", "vy", "vz", "m")], 1e-3, args.ens_adj_steps, 32,
+        *[p(a) for a in TT], None, None, None, None, None, None, p(loss), None,
+        *[p(out[r]) for r in range(7)]))
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    return {"systems": len(cnts), "n_total": n, "interactions_per_step": inter,
+            "steps": args.ens_steps, "ms_per_step": round(t_step * 1e3, 4),
+            "G_interactions_per_s": round(inter / t_step / 1e9, 3),
+            "adjoint": {"steps": args.ens_adj_steps, "checkpoint_every": 32,
+                        "seconds": round(t_adj, 4),
+                        "ms_per_step": round(t_adj / args.ens_adj_steps * 1e3, 4),
+                        "note": "forward with checkpoints + segment recompute + FP64 pair-VJP"},
+            "workload": "256 Plummer spheres, N_s in [512, 8192] (seeded), eps 0.05, dt 1e-3, "
+                        "mixed brute force, hipGraph step"}
+
+
 def main():
     args = parse()
     if args.probe_copy:
@@ -169,7 +235,6 @@ def main():
               "(the multi-GPU layout is a design in DESIGN.md, not built)", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
     import gravis_amd as g
 
@@ -275,6 +340,10 @@ def main():
         bh = run_bh(args, g, torch, ctx, "mixed")
         bh64 = run_bh(args, g, torch, ctx, "fp64", max(2, args.bh_steps // 4))
 
+    ens = None
+    if not args.no_ensemble:
+        ens = run_ensemble(args, g, torch, ctx)
+
     traffic = traffic_from_profiles(f"bf_mixed_n{n}")
     out = {
         "metric": METRIC,
@@ -308,6 +377,7 @@ def main():
         "force_error_vs_fp64": {"rms": err_rms, "max": err_max},
         "barnes_hut": bh,
         "barnes_hut_fp64": bh64,
+        "ensemble": ens,
     }
     print(json.dumps(out), flush=True)
 

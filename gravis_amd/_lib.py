@@ -99,6 +99,8 @@ def load() -> C.CDLL:
         "gravis_scenario_plummer_pair": (st, [sz, d, d, C.c_uint64, C.c_uint64, d, d] + [vp] * 7),
         "gravis_bh_stats": (st, [vp, C.POINTER(C.c_uint64 * 5)]),
         "gravis_set_tuning": (st, [vp, C.c_char_p, C.c_int64]),
+        "gravis_scenario_ensemble_counts": (st, [C.c_uint64, i32, C.c_uint32, C.c_uint32, vp]),
+        "gravis_scenario_targets": (st, [C.c_uint64, sz, vp, vp, vp]),
         "gravis_ensemble_kdk": (st, [vp, C.POINTER(Solver), i32, vp] + [vp] * 10
                                 + [d, C.c_uint64, i32]),
         "gravis_ensemble_accel_device": (st, [vp, C.POINTER(Solver), i32, vp] + [vp] * 7),

@@ -225,3 +225,28 @@ extern "C" gravis_status gravis_scenario_plummer_pair(size_t n_each, double scal
     com_shift(2 * n_each, x, y, z, vx, vy, vz, m);
     return GRAVIS_OK;
 }
+
+// Ensemble layout (no reference counterpart): counts[s] = lo + u64 % (hi -
+// lo + 1) from one ChaCha20 stream seeded with `seed`, in system order.
+extern "C" gravis_status gravis_scenario_ensemble_counts(uint64_t seed, int nsys, uint32_t lo,
+                                                         uint32_t hi, uint32_t* counts) {
+    if (nsys <= 0 || hi < lo || !counts) return GRAVIS_INVALID_ARG;
+    ChaChaStream rng(seed);
+    const uint64_t span = (uint64_t)(hi - lo) + 1;
+    for (int s = 0; s < nsys; ++s) counts[s] = lo + (uint32_t)(rng.next_u64() % span);
+    return GRAVIS_OK;
+}
+
+// Loss targets U[-1, 1)^3, drawn x, y, z per particle from a stream seeded
+// with `seed`.
+extern "C" gravis_status gravis_scenario_targets(uint64_t seed, size_t n, double* tx, double* ty,
+                                                 double* tz) {
+    if (!tx || !ty || !tz) return GRAVIS_INVALID_ARG;
+    ChaChaStream rng(seed);
+    for (size_t i = 0; i < n; ++i) {
+        tx[i] = rng.range(-1.0, 1.0);
+        ty[i] = rng.range(-1.0, 1.0);
+        tz[i] = rng.range(-1.0, 1.0);
+    }
+    return GRAVIS_OK;
+}

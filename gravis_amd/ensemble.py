@@ -34,6 +34,32 @@ def _p(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def counts(seed: int = 42, nsys: int = 256, lo: int = 512, hi: int = 8192) -> np.ndarray:
+    """Seeded system sizes N_s in [lo, hi] (gravis_scenario_ensemble_counts)."""
+    c = np.zeros(nsys, dtype=np.uint32)
+    _lib.check(None, _lib.load().gravis_scenario_ensemble_counts(seed, nsys, lo, hi, _p(c)))
+    return c
+
+
+def targets(seed: int, n: int):
+    """Seeded loss targets U[-1, 1)^3 (gravis_scenario_targets)."""
+    t = [np.zeros(n) for _ in range(3)]
+    _lib.check(None, _lib.load().gravis_scenario_targets(seed, n, *[_p(a) for a in t]))
+    return t
+
+
+def plummer_ensemble(cnts, seed0: int = 42) -> dict:
+    """Concatenated Plummer spheres (a = 1, M = 1), system s seeded seed0 + s."""
+    from . import scenarios
+    parts = [scenarios.plummer(int(n), seed0 + s) for s, n in enumerate(cnts)]
+    st = {k: np.concatenate([getattr(p, k) for p in parts]) for k in FIELDS}
+    st["m"] = np.concatenate([p.mass for p in parts])
+    n = len(st["m"])
+    for k in ("ax", "ay", "az"):
+        st[k] = np.zeros(n)
+    return st
+
+
 def run(state: dict, counts, eps: float, dt: float, nsteps: int, precision: str = "fp64",
         use_graph: bool = True, ctx: Context | None = None) -> None:
     """nsteps of KDK for every system; state holds numpy arrays x..vz, m,

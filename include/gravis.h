@@ -249,6 +249,13 @@ double gravis_two_body_period(double semi_major_axis);
  * Sphere A fills indices [0, n_each), B [n_each, 2 n_each). The layout
  * follows GalaxyCollision (crates/sim-core/src/scenarios/
  * galaxy_collision.rs:134-154), with Plummer spheres in place of disks. */
+/* Batched-ensemble layout (no reference counterpart): counts[s] =
+ * lo + (u64 draw) % (hi - lo + 1) from ChaCha20Rng::seed_from_u64(seed),
+ * in system order; and loss targets U[-1, 1)^3 drawn x, y, z per particle. */
+gravis_status gravis_scenario_ensemble_counts(uint64_t seed, int nsys, uint32_t lo, uint32_t hi,
+                                              uint32_t* counts);
+gravis_status gravis_scenario_targets(uint64_t seed, size_t n, double* tx, double* ty,
+                                      double* tz);
 gravis_status gravis_scenario_plummer_pair(size_t n_each, double scale_radius,
                                            double total_mass, uint64_t seed_a, uint64_t seed_b,
                                            double separation, double impact, double* x,

@@ -90,3 +90,14 @@ def test_scenario_plummer_bit_exact_with_oracle():
     o = O.two_body(0.7)
     assert np.array_equal(t.x, o.x) and np.array_equal(t.vy, o.vy)
     assert g.scenarios.two_body_period() == O.two_body_period()
+
+
+def test_ensemble_layout_matches_oracle():
+    import numpy as np
+    import gravis_amd.ensemble as E
+    import oracle as O
+    assert (E.counts(42, 256) == O.ens_counts(42, 256)).all()
+    t = E.targets(10007, 999)
+    o = O.ens_targets(10007, 999)
+    for a, b in zip(t, o):
+        assert np.array_equal(a, b)
