@@ -95,14 +95,23 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def _u8(a: np.ndarray):
+    assert a.dtype == np.uint8 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.c_void_p)
+
+
 class Particles:
-    """Structure-of-arrays particle storage (particle.rs:21-64), f64."""
+    """Structure-of-arrays particle storage (particle.rs:21-64), f64, with
+    the SPH fields and particle types (particle.rs:2-10; 4 = gas)."""
 
     FIELDS = ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "mass")
+    SPH_FIELDS = ("density", "pressure", "internal_energy", "du_dt", "smoothing_length",
+                  "sound_speed", "alpha_visc")
+    GAS = 4
 
     def __init__(self, capacity: int = 0):
         self.count = 0
-        for f in self.FIELDS:
+        for f in self.FIELDS + self.SPH_FIELDS:
             setattr(self, f, np.zeros(0, dtype=np.float64))
         self.particle_type = np.zeros(0, dtype=np.uint8)
 
@@ -113,19 +122,42 @@ class Particles:
         p.count = n
         for f, v in zip(("x", "y", "z", "vx", "vy", "vz", "mass"), (x, y, z, vx, vy, vz, mass)):
             setattr(p, f, np.ascontiguousarray(v, dtype=np.float64).copy())
-        p.ax, p.ay, p.az = np.zeros(n), np.zeros(n), np.zeros(n)
+        for f in ("ax", "ay", "az") + cls.SPH_FIELDS:
+            setattr(p, f, np.zeros(n))
         p.particle_type = np.zeros(n, dtype=np.uint8)
         return p
 
-    def add(self, x, y, z, vx, vy, vz, mass) -> int:
-        """particle.rs:94-96: append one particle, a = 0; returns its index."""
-        for f, v in zip(("x", "y", "z", "vx", "vy", "vz", "mass"), (x, y, z, vx, vy, vz, mass)):
-            setattr(self, f, np.append(getattr(self, f), float(v)))
-        for f in ("ax", "ay", "az"):
-            setattr(self, f, np.append(getattr(self, f), 0.0))
-        self.particle_type = np.append(self.particle_type, np.uint8(0))
+    @classmethod
+    def zeros(cls, n: int):
+        p = cls()
+        p.count = n
+        for f in cls.FIELDS + cls.SPH_FIELDS:
+            setattr(p, f, np.zeros(n))
+        p.particle_type = np.zeros(n, dtype=np.uint8)
+        return p
+
+    def _append(self, values: dict, ptype: int) -> int:
+        for f in self.FIELDS + self.SPH_FIELDS:
+            setattr(self, f, np.append(getattr(self, f), float(values.get(f, 0.0))))
+        self.particle_type = np.append(self.particle_type, np.uint8(ptype))
         self.count += 1
         return self.count - 1
+
+    def add(self, x, y, z, vx, vy, vz, mass) -> int:
+        """particle.rs:94-96: append one particle, a = 0; returns its index."""
+        return self._append(dict(x=x, y=y, z=z, vx=vx, vy=vy, vz=vz, mass=mass), 0)
+
+    def add_gas(self, x, y, z, vx, vy, vz, mass, internal_energy, smoothing_length) -> int:
+        """particle.rs:136-154: a gas particle with alpha_visc = 1."""
+        return self._append(dict(x=x, y=y, z=z, vx=vx, vy=vy, vz=vz, mass=mass,
+                                 internal_energy=internal_energy,
+                                 smoothing_length=smoothing_length, alpha_visc=1.0), self.GAS)
+
+    def is_gas(self, i: int) -> bool:
+        return int(self.particle_type[i]) == self.GAS
+
+    def has_gas(self) -> bool:
+        return bool((self.particle_type == self.GAS).any())
 
     def clear_accelerations(self):
         """particle.rs:173-179."""
@@ -133,10 +165,14 @@ class Particles:
         self.ay[:] = 0.0
         self.az[:] = 0.0
 
+    def clear_sph_rates(self):
+        """particle.rs:182-186."""
+        self.du_dt[:] = 0.0
+
     def copy(self) -> "Particles":
         p = Particles()
         p.count = self.count
-        for f in self.FIELDS:
+        for f in self.FIELDS + self.SPH_FIELDS:
             setattr(p, f, getattr(self, f).copy())
         p.particle_type = self.particle_type.copy()
         return p
@@ -237,11 +273,14 @@ class _Diagnostics:
              ctx: Context | None):
         ctx = ctx or default_context()
         out = _lib.Diag()
+        gas = p.has_gas()
         ctx.check(ctx._L.gravis_diagnostics(ctx.handle, p.count, float(softening),
                                             1 if with_pot else 0,
                                             *[_ptr(getattr(p, f)) for f in ("x", "y", "z", "vx",
                                                                             "vy", "vz", "mass")],
-                                            None, None, float(time), int(step), C.byref(out)))
+                                            _ptr(p.internal_energy) if gas else None,
+                                            _u8(p.particle_type) if gas else None,
+                                            float(time), int(step), C.byref(out)))
         return out
 
     def compute(self, p: Particles, softening: float, time: float = 0.0, step: int = 0,
@@ -254,6 +293,21 @@ class _Diagnostics:
 
 
 diagnostics = _Diagnostics()
+
+# Particles field -> gravis_sph_state member
+_SPH_MAP = dict(x="x", y="y", z="z", vx="vx", vy="vy", vz="vz", ax="ax", ay="ay", az="az",
+                m="mass", rho="density", p="pressure", u="internal_energy", dudt="du_dt",
+                h="smoothing_length", c="sound_speed", alpha="alpha_visc")
+
+
+def sph_state(p: Particles) -> _lib.SphState:
+    """gravis_sph_state over the host arrays of p (borrowed, not copied)."""
+    st = _lib.SphState()
+    st.n = p.count
+    for k in _lib.SPH_FIELDS:
+        setattr(st, k, _ptr(getattr(p, _SPH_MAP[k])).value)
+    st.type = _u8(p.particle_type).value
+    return st
 
 
 def morton_sort(x, y, z, ctx: Context | None = None):
@@ -318,6 +372,22 @@ class _Scenarios:
     @staticmethod
     def two_body_period(semi_major_axis: float = 1.0) -> float:
         return _lib.load().gravis_two_body_period(semi_major_axis)
+
+    @staticmethod
+    def evrard(n: int, seed: int = 42, total_mass: float = 1.0, radius: float = 1.0,
+               u0: float = 0.05) -> Particles:
+        """EvrardCollapse::generate (evrard_collapse.rs:45-57): n gas
+        particles uniform in a sphere, h = 1.5 (V/n)^(1/3), at rest."""
+        L = _lib.load()
+        p = Particles.zeros(n)
+        st = sph_state(p)
+        _lib.check(None, L.gravis_scenario_evrard(n, total_mass, radius, u0, seed, C.byref(st)))
+        return p
+
+    @staticmethod
+    def evrard_softening(n: int, radius: float = 1.0) -> float:
+        """evrard_collapse.rs:67-70: 0.3 (V/n)^(1/3)."""
+        return _lib.load().gravis_evrard_softening(n, radius)
 
 
 scenarios = _Scenarios()

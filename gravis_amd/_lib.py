@@ -26,7 +26,11 @@ EXPORTS = (
     "gravis_morton_sort", "gravis_scenario_plummer", "gravis_plummer_suggested_dt",
     "gravis_plummer_suggested_softening", "gravis_scenario_two_body", "gravis_two_body_period",
     "gravis_scenario_plummer_pair", "gravis_bh_stats", "gravis_ensemble_kdk",
-    "gravis_ensemble_kdk_device", "gravis_ensemble_accel_device", "gravis_ensemble_adjoint", "gravis_ensemble_adjoint_device",
+    "gravis_ensemble_kdk_device", "gravis_ensemble_accel_device", "gravis_ensemble_adjoint",
+    "gravis_ensemble_adjoint_device", "gravis_set_tuning", "gravis_scenario_ensemble_counts",
+    "gravis_scenario_targets", "gravis_sph_default_params", "gravis_sph_compute",
+    "gravis_sph_compute_device", "gravis_sph_step", "gravis_sph_run", "gravis_sph_run_device",
+    "gravis_sph_stats", "gravis_scenario_evrard", "gravis_evrard_softening",
 )
 
 
@@ -41,6 +45,21 @@ class Diag(C.Structure):
                 ("momentum", C.c_double * 3), ("angular_momentum", C.c_double * 3),
                 ("center_of_mass", C.c_double * 3), ("center_of_mass_velocity", C.c_double * 3),
                 ("virial_ratio", C.c_double), ("thermal_energy", C.c_double)]
+
+
+class SphParams(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("gamma", "eta", "h_max", "cfl", "force_factor",
+                                           "beta_visc", "alpha_min", "alpha_max", "alpha_cond",
+                                           "sigma_visc")]
+
+
+SPH_FIELDS = ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "m", "rho", "p", "u", "dudt",
+              "h", "c", "alpha")
+
+
+class SphState(C.Structure):
+    _fields_ = ([("n", C.c_size_t)] + [(k, C.c_void_p) for k in SPH_FIELDS]
+                + [("type", C.c_void_p)])
 
 
 _LIB = None
@@ -111,6 +130,18 @@ def load() -> C.CDLL:
         "gravis_ensemble_adjoint_device": (st, [vp, C.POINTER(Solver), i32, vp] + [vp] * 7
                                            + [d, C.c_uint64, C.c_uint32] + [vp] * 9
                                            + [vp] * 9),
+        "gravis_sph_default_params": (None, [C.POINTER(SphParams)]),
+        "gravis_sph_compute": (st, [vp, C.POINTER(SphParams), C.POINTER(SphState)] + [vp] * 5),
+        "gravis_sph_compute_device": (st, [vp, C.POINTER(SphParams), C.POINTER(SphState)]
+                                      + [vp] * 5),
+        "gravis_sph_step": (st, [vp, C.POINTER(SphParams), vp, C.POINTER(SphState), d, dp]),
+        "gravis_sph_run": (st, [vp, C.POINTER(SphParams), vp, C.POINTER(SphState), C.c_uint64,
+                                dp, dp]),
+        "gravis_sph_run_device": (st, [vp, C.POINTER(SphParams), vp, C.POINTER(SphState),
+                                       C.c_uint64, dp, dp]),
+        "gravis_sph_stats": (st, [vp, C.POINTER(C.c_uint64 * 4)]),
+        "gravis_scenario_evrard": (st, [sz, d, d, d, C.c_uint64, C.POINTER(SphState)]),
+        "gravis_evrard_softening": (d, [sz, d]),
     }
     for name, (res, args) in spec.items():
         f = getattr(L, name)

@@ -2,6 +2,7 @@
 // kernels. Validates arguments the way the reference's CLI and asserts do
 // (crates/headless/src/main.rs:73-95, octree.rs:16), converts every failure
 // into a gravis_status plus a message, and never throws across the ABI.
+#include <cmath>
 #include <cstdarg>
 #include <cstring>
 #include <new>
@@ -160,6 +161,52 @@ void fill_diag(const double r[19], double time, uint64_t step, gravis_diag* out)
         out->center_of_mass_velocity[d] = r[13 + d];
     }
     out->virial_ratio = r[16];
+}
+
+gravis_status check_sph(Ctx* c, const gravis_sph_params* P, const gravis_sph_state* S) {
+    if (!P) return fail(c, GRAVIS_INVALID_ARG, "SPH params is NULL");
+    if (!S) return fail(c, GRAVIS_INVALID_ARG, "SPH state is NULL");
+    if (S->n > 0 && any_null({S->x, S->y, S->z, S->vx, S->vy, S->vz, S->ax, S->ay, S->az, S->m,
+                              S->rho, S->p, S->u, S->dudt, S->h, S->c, S->alpha, S->type}))
+        return fail(c, GRAVIS_INVALID_ARG, "null array in SPH state");
+    if (S->n > 0x7FFFFFFFull) return fail(c, GRAVIS_INVALID_ARG, "n too large");
+    return GRAVIS_OK;
+}
+
+// The 17 f64 arrays of a gravis_sph_state in declaration order.
+inline double** sph_fields(gravis_sph_state* S, int k) {
+    double** f[17] = {&S->x, &S->y, &S->z, &S->vx, &S->vy, &S->vz, &S->ax, &S->ay, &S->az,
+                      &S->m, &S->rho, &S->p, &S->u, &S->dudt, &S->h, &S->c, &S->alpha};
+    return f[k];
+}
+
+// Device copy of a host SPH state (context buffer); copies it in.
+gravis_status sph_stage_in(Ctx* c, const gravis_sph_state* H, gravis_sph_state* D) {
+    const size_t n = H->n;
+    double* base;
+    uint8_t* ty;
+    GRAVIS_TRY(gravis::buf(c, "sph_host_state", 17 * n, &base));
+    GRAVIS_TRY(gravis::buf(c, "sph_host_type", n, &ty));
+    gravis_sph_state h = *H;
+    D->n = n;
+    for (int k = 0; k < 17; ++k) {
+        *sph_fields(D, k) = base + (size_t)k * n;
+        GRAVIS_TRY(h2d(c, *sph_fields(D, k), *sph_fields(&h, k), n));
+    }
+    D->type = ty;
+    GRAVIS_HIP(c, hipMemcpyAsync(ty, H->type, n, hipMemcpyHostToDevice, c->stream));
+    return GRAVIS_OK;
+}
+
+// Copies the fields an SPH call changes back to the host state.
+gravis_status sph_stage_out(Ctx* c, gravis_sph_state* D, gravis_sph_state* H, bool all) {
+    const size_t n = H->n;
+    for (int k = 0; k < 17; ++k) {
+        const bool changed = all || k >= 10 || (k >= 6 && k <= 8);
+        if (k == 9 || !changed) continue;  // masses never change
+        GRAVIS_TRY(d2h(c, *sph_fields(H, k), *sph_fields(D, k), n));
+    }
+    return GRAVIS_OK;
 }
 
 }  // namespace
@@ -534,6 +581,12 @@ gravis_status gravis_set_tuning(gravis_ctx* g, const char* key, int64_t value) {
         c->bh_frontier_cap = (uint32_t)value;
         return GRAVIS_OK;
     }
+    if (std::strcmp(key, "sph_query_leaves") == 0) {
+        if (value < 0 || value > 512)
+            return fail(c, GRAVIS_INVALID_ARG, "sph_query_leaves must be in [0, 512]");
+        c->sph_range_cap = (uint32_t)value;
+        return GRAVIS_OK;
+    }
     return fail(c, GRAVIS_INVALID_ARG, "unknown tuning key '%s'", key);
 }
 
@@ -565,6 +618,125 @@ gravis_status gravis_morton_sort(gravis_ctx* g, size_t n, const double* x, const
     GRAVIS_HIP(c, hipMemcpyAsync(perm, p, n * 4, hipMemcpyDeviceToHost, c->stream));
     GRAVIS_HIP(c, hipMemcpyAsync(box_out, box, 32, hipMemcpyDeviceToHost, c->stream));
     GRAVIS_HIP(c, hipStreamSynchronize(c->stream));
+    return GRAVIS_OK;
+}
+
+void gravis_sph_default_params(gravis_sph_params* o) {
+    if (!o) return;
+    o->gamma = 5.0 / 3.0;     // solver.rs:42
+    o->eta = 1.2;             // solver.rs:43
+    o->h_max = INFINITY;      // solver.rs:47
+    o->cfl = 0.3;             // solver.rs:45
+    o->force_factor = 0.25;   // solver.rs:46
+    o->beta_visc = 2.0;       // forces.rs:34
+    o->alpha_min = 0.1;       // forces.rs:35
+    o->alpha_max = 2.0;       // forces.rs:36
+    o->alpha_cond = 1.0;      // forces.rs:37
+    o->sigma_visc = 0.1;      // forces.rs:38
+}
+
+gravis_status gravis_sph_compute_device(gravis_ctx* g, const gravis_sph_params* P,
+                                        const gravis_sph_state* S, double* div_v,
+                                        double* max_mu, double* dalpha_dt, double* omega,
+                                        uint32_t* nneigh) {
+    if (!g) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    GRAVIS_TRY(check_sph(c, P, S));
+    if (S->n == 0) return GRAVIS_OK;
+    if (any_null({div_v, max_mu, dalpha_dt}))
+        return fail(c, GRAVIS_INVALID_ARG, "null intermediate array");
+    GRAVIS_HIP(c, hipSetDevice(c->device));
+    return gravis::sph_compute(c, P, S, nullptr, div_v, max_mu, dalpha_dt, omega, nneigh);
+}
+
+gravis_status gravis_sph_compute(gravis_ctx* g, const gravis_sph_params* P,
+                                 const gravis_sph_state* S, double* div_v, double* max_mu,
+                                 double* dalpha_dt, double* omega, uint32_t* nneigh) {
+    if (!g) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    GRAVIS_TRY(check_sph(c, P, S));
+    const size_t n = S->n;
+    if (n == 0) return GRAVIS_OK;
+    if (any_null({div_v, max_mu, dalpha_dt}))
+        return fail(c, GRAVIS_INVALID_ARG, "null intermediate array");
+    GRAVIS_HIP(c, hipSetDevice(c->device));
+    gravis_sph_state D;
+    GRAVIS_TRY(sph_stage_in(c, S, &D));
+    double* iv;
+    uint32_t* nn;
+    GRAVIS_TRY(gravis::buf(c, "sph_host_interm", 4 * n, &iv));
+    GRAVIS_TRY(gravis::buf(c, "sph_host_nneigh", n, &nn));
+    GRAVIS_TRY(gravis::sph_compute(c, P, &D, nullptr, iv, iv + n, iv + 2 * n,
+                                   omega ? iv + 3 * n : nullptr, nneigh ? nn : nullptr));
+    gravis_sph_state H = *S;
+    GRAVIS_TRY(sph_stage_out(c, &D, &H, false));
+    GRAVIS_TRY(d2h(c, div_v, iv, n));
+    GRAVIS_TRY(d2h(c, max_mu, iv + n, n));
+    GRAVIS_TRY(d2h(c, dalpha_dt, iv + 2 * n, n));
+    if (omega) GRAVIS_TRY(d2h(c, omega, iv + 3 * n, n));
+    if (nneigh)
+        GRAVIS_HIP(c, hipMemcpyAsync(nneigh, nn, n * 4, hipMemcpyDeviceToHost, c->stream));
+    GRAVIS_HIP(c, hipStreamSynchronize(c->stream));
+    return GRAVIS_OK;
+}
+
+static gravis_status sph_run_checked(Ctx* c, const gravis_sph_params* P, const gravis_solver* s,
+                                     const gravis_sph_state* S, double dt) {
+    GRAVIS_TRY(check_sph(c, P, S));
+    if (s && s->algorithm != GRAVIS_NO_GRAVITY) GRAVIS_TRY(check_solver(c, s));
+    if (!(dt > 0.0)) return fail(c, GRAVIS_INVALID_ARG, "dt must be positive, got %g", dt);
+    return GRAVIS_OK;
+}
+
+gravis_status gravis_sph_run_device(gravis_ctx* g, const gravis_sph_params* P,
+                                    const gravis_solver* s, gravis_sph_state* S, uint64_t nsteps,
+                                    double* dt_io, double* time_io) {
+    if (!g || !dt_io) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    GRAVIS_TRY(sph_run_checked(c, P, s, S, *dt_io));
+    if (S->n == 0 || nsteps == 0) return GRAVIS_OK;
+    GRAVIS_HIP(c, hipSetDevice(c->device));
+    return gravis::sph_run(c, P, s, S, nsteps, dt_io, time_io, nullptr);
+}
+
+gravis_status gravis_sph_run(gravis_ctx* g, const gravis_sph_params* P, const gravis_solver* s,
+                             gravis_sph_state* S, uint64_t nsteps, double* dt_io,
+                             double* time_io) {
+    if (!g || !dt_io) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    GRAVIS_TRY(sph_run_checked(c, P, s, S, *dt_io));
+    if (S->n == 0 || nsteps == 0) return GRAVIS_OK;
+    GRAVIS_HIP(c, hipSetDevice(c->device));
+    gravis_sph_state D;
+    GRAVIS_TRY(sph_stage_in(c, S, &D));
+    GRAVIS_TRY(gravis::sph_run(c, P, s, &D, nsteps, dt_io, time_io, nullptr));
+    GRAVIS_TRY(sph_stage_out(c, &D, S, true));
+    GRAVIS_HIP(c, hipStreamSynchronize(c->stream));
+    return GRAVIS_OK;
+}
+
+gravis_status gravis_sph_step(gravis_ctx* g, const gravis_sph_params* P, const gravis_solver* s,
+                              gravis_sph_state* S, double dt, double* dt_next) {
+    if (!g || !dt_next) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    GRAVIS_TRY(sph_run_checked(c, P, s, S, dt));
+    if (S->n == 0) {
+        *dt_next = INFINITY;  // no gas particle: dt_min stays +inf (solver.rs:96,130)
+        return GRAVIS_OK;
+    }
+    GRAVIS_HIP(c, hipSetDevice(c->device));
+    gravis_sph_state D;
+    GRAVIS_TRY(sph_stage_in(c, S, &D));
+    double dtc = dt;
+    GRAVIS_TRY(gravis::sph_run(c, P, s, &D, 1, &dtc, nullptr, dt_next));
+    GRAVIS_TRY(sph_stage_out(c, &D, S, true));
+    GRAVIS_HIP(c, hipStreamSynchronize(c->stream));
+    return GRAVIS_OK;
+}
+
+gravis_status gravis_sph_stats(gravis_ctx* g, uint64_t out[4]) {
+    if (!g || !out) return GRAVIS_INVALID_ARG;
+    for (int k = 0; k < 4; ++k) out[k] = g->c.sph_stats[k];
     return GRAVIS_OK;
 }
 

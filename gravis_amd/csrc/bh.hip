@@ -1,40 +1,28 @@
-// bh.hip -- Barnes-Hut gravity on gfx950: Morton-ordered linear octree and
-// a wavefront-coherent group walk.
+// bh.hip -- Barnes-Hut gravity on gfx950: a wavefront-coherent group walk
+// over the Morton-ordered linear octree of tree.hip.
 //
 // Reference: BarnesHut::compute_accelerations (crates/sim-core/src/
 // barnes_hut.rs:139-192) over the arena octree of octree.rs:145-239. The
-// reference inserts particles one by one (one particle per leaf) and walks
-// the tree per particle with a 512-entry stack, accepting a node when
-// s^2 < theta^2 d^2 (s = node side, d = distance to the node's COM).
+// reference walks the tree per particle with a 512-entry stack, accepting a
+// node when s^2 < theta^2 d^2 (s = node side, d = distance to the node's
+// COM).
 //
-// Here the same root cube (octree.rs:14-44) is cut by 63-bit Morton keys, so
-// every node is a run of the key-sorted particles and its cell is the
-// reference's cell at that depth:
-//   build   level by level: each internal node (more than `leaf` particles,
-//           level < 21) finds its child runs by binary search on the next
-//           3-bit digit; nodes are numbered breadth first, children of a
-//           node contiguous in digit order.
-//   COM     bottom up in FP64: leaves sum their particles in sorted order,
-//           internal nodes sum their children in order (m, m*x; no FMA);
-//           com = sum(m x) / sum(m).
-//   walk    one wavefront per target group: the first tree node on each
-//           root-to-leaf path holding at most `group` (<= 64) particles.
-//           The frontier is processed breadth first, 64 nodes at a time
-//           (one per lane). A node is accepted for the whole group when
-//           s^2 < theta^2 d_min^2 with d_min the distance from its COM to the
-//           group's bounding box (so never less accurate than the
-//           reference's per-particle test); a rejected leaf contributes its
-//           particles directly; a rejected internal node pushes its children.
-//           If a group's frontier would exceed the per-wave capacity, the
-//           group is walked again one target at a time (point boxes).
-//           Accepted nodes and the particles of opened leaves are appended,
-//           in frontier order, as source indices to a per-wave LDS list;
-//           each flush round stages 64 source records in LDS (one load per
-//           lane) that all 64 lanes then read back as broadcasts. Pair arithmetic is
-//           barnes_hut.rs:114-135 in FP64 (sqrt and divide correctly
-//           rounded, no contraction); a source at r^2 = 0 is skipped.
-// The accumulation order is the list order, so results are bit-for-bit the
-// CPU oracle's group walk with the same groups.
+// Here one wavefront walks one target group (the first tree node on each
+// root-to-leaf path holding at most `group` (<= 64) particles, siblings
+// packed). The frontier is processed breadth first, 64 nodes at a time (one
+// per lane). A node is accepted for the whole group when
+// s^2 < theta^2 d_min^2 with d_min the distance from its COM to the group's
+// bounding box (so never less accurate than the reference's per-particle
+// test); a rejected leaf contributes its particles directly; a rejected
+// internal node pushes its children. If a group's frontier would exceed the
+// per-wave capacity, the group is walked again one target at a time (point
+// boxes). Accepted nodes and the particles of opened leaves are appended, in
+// frontier order, as source indices to a per-wave LDS list; each flush round
+// stages 64 source records in LDS (one load per lane) that all 64 lanes then
+// read back as broadcasts. Pair arithmetic is barnes_hut.rs:114-135 in FP64
+// (sqrt and divide correctly rounded, no contraction); a source at r^2 = 0 is
+// skipped. The accumulation order is the list order, so results are
+// bit-for-bit the CPU oracle's group walk with the same groups.
 #include <algorithm>
 
 #include "internal.h"
@@ -42,278 +30,10 @@
 namespace gravis {
 namespace {
 
-constexpr int kMaxLevel = 21;
-constexpr uint32_t kLeafFlag = 0x80000000u;
-constexpr int kTreeBlock = 256;
-constexpr int kTreeGrid = 1024;
-
-// meta = {first, count, first child, flags}; flags = level | nchild << 8 |
-// kGroupRoot (the parent holds more particles than a target group).
-constexpr uint32_t kGroupRoot = 1u << 16;
-
-__global__ void tree_init_kernel(uint4* __restrict__ meta, uint32_t* __restrict__ lvl,
-                                 uint32_t n) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    meta[0] = make_uint4(0u, n, 0u, kGroupRoot);
-    lvl[0] = 0u;
-    lvl[1] = 1u;
-    for (int L = 1; L <= kMaxLevel + 1; ++L) {
-        lvl[2 * L] = 1u;
-        lvl[2 * L + 1] = 1u;
-    }
-}
-
-__device__ __forceinline__ uint32_t digit_of(uint64_t key, int shift) {
-    return static_cast<uint32_t>((key >> shift) & 7ull);
-}
-
-// Number of non-empty children of each node of level L.
-__global__ __launch_bounds__(kTreeBlock) void tree_count_kernel(const uint64_t* __restrict__ keys,
-                                                               const uint4* __restrict__ meta,
-                                                               const uint32_t* __restrict__ lvl,
-                                                               int L, uint32_t leaf,
-                                                               uint32_t* __restrict__ nch) {
-    const uint32_t b = lvl[2 * L], e = lvl[2 * L + 1];
-    const int shift = 3 * (kMaxLevel - (L + 1));
-    for (uint32_t v = b + blockIdx.x * kTreeBlock + threadIdx.x; v < e;
-         v += gridDim.x * kTreeBlock) {
-        const uint4 md = meta[v];
-        uint32_t c = 0;
-        if (md.y > leaf && L < kMaxLevel) {
-            uint32_t pos = md.x;
-            const uint32_t end = md.x + md.y;
-            while (pos < end) {
-                const uint32_t d = digit_of(keys[pos], shift);
-                uint32_t lo = pos + 1, hi = end;  // first index with a larger digit
-                while (lo < hi) {
-                    const uint32_t mid = lo + (hi - lo) / 2;
-                    if (digit_of(keys[mid], shift) > d) hi = mid;
-                    else lo = mid + 1;
-                }
-                ++c;
-                pos = lo;
-            }
-        }
-        nch[v - b] = c;
-    }
-}
-
-// Exclusive scan of the level's child counts (one block) and the range of
-// level L+1.
-__global__ __launch_bounds__(1024) void tree_scan_kernel(const uint32_t* __restrict__ nch,
-                                                         uint32_t* __restrict__ off,
-                                                         uint32_t* __restrict__ lvl, int L) {
-    __shared__ uint32_t s[1024];
-    const uint32_t b = lvl[2 * L], e = lvl[2 * L + 1];
-    const uint32_t m = e - b;
-    const uint32_t per = (m + 1023) / 1024;
-    const uint32_t lo = threadIdx.x * per;
-    const uint32_t hi = min(m, lo + per);
-    uint32_t sum = 0;
-    for (uint32_t i = lo; i < hi; ++i) sum += nch[i];
-    s[threadIdx.x] = sum;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        const uint32_t t = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0u;
-        __syncthreads();
-        s[threadIdx.x] += t;
-        __syncthreads();
-    }
-    uint32_t run = s[threadIdx.x] - sum;
-    for (uint32_t i = lo; i < hi; ++i) {
-        off[i] = run;
-        run += nch[i];
-    }
-    if (threadIdx.x == 1023) {
-        lvl[2 * (L + 1)] = e;
-        lvl[2 * (L + 1) + 1] = e + s[1023];
-    }
-}
-
-__global__ __launch_bounds__(kTreeBlock) void tree_write_kernel(
-    const uint64_t* __restrict__ keys, uint4* __restrict__ meta, const uint32_t* __restrict__ lvl,
-    int L, const uint32_t* __restrict__ nch, const uint32_t* __restrict__ off, uint32_t cap,
-    uint32_t gsize, uint32_t* __restrict__ flag) {
-    const uint32_t b = lvl[2 * L], e = lvl[2 * L + 1];
-    const int shift = 3 * (kMaxLevel - (L + 1));
-    for (uint32_t v = b + blockIdx.x * kTreeBlock + threadIdx.x; v < e;
-         v += gridDim.x * kTreeBlock) {
-        const uint32_t nc = nch[v - b];
-        if (nc == 0) continue;
-        const uint4 md = meta[v];
-        const uint32_t c0 = e + off[v - b];
-        if (c0 + nc > cap) {
-            atomicOr(flag, 2u);
-            continue;
-        }
-        uint32_t pos = md.x, k = 0;
-        const uint32_t end = md.x + md.y;
-        while (pos < end) {
-            const uint32_t d = digit_of(keys[pos], shift);
-            uint32_t lo = pos + 1, hi = end;
-            while (lo < hi) {
-                const uint32_t mid = lo + (hi - lo) / 2;
-                if (digit_of(keys[mid], shift) > d) hi = mid;
-                else lo = mid + 1;
-            }
-            meta[c0 + k] = make_uint4(pos, lo - pos, 0u,
-                                      (uint32_t)(L + 1) | (md.y > gsize ? kGroupRoot : 0u));
-            ++k;
-            pos = lo;
-        }
-        meta[v] = make_uint4(md.x, md.y, c0, (md.w & kGroupRoot) | (nc << 8) | (uint32_t)L);
-    }
-}
-
-// Raw FP64 sums (sum m x, sum m y, sum m z, sum m) and the particle bounding
-// box of the nodes of level L.
-__global__ __launch_bounds__(kTreeBlock) void tree_sums_kernel(const double4* __restrict__ src,
-                                                              const uint4* __restrict__ meta,
-                                                              const uint32_t* __restrict__ lvl,
-                                                              int L,
-                                                              double4* __restrict__ sums,
-                                                              double4* __restrict__ bmin,
-                                                              double4* __restrict__ bmax) {
-    const uint32_t b = lvl[2 * L], e = lvl[2 * L + 1];
-    for (uint32_t v = b + blockIdx.x * kTreeBlock + threadIdx.x; v < e;
-         v += gridDim.x * kTreeBlock) {
-        const uint4 md = meta[v];
-        const uint32_t nc = (md.w >> 8) & 255u;
-        double M = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
-        double lx = INFINITY, ly = INFINITY, lz = INFINITY;
-        double hx = -INFINITY, hy = -INFINITY, hz = -INFINITY;
-        if (nc == 0) {
-            for (uint32_t k = md.x; k < md.x + md.y; ++k) {
-                const double4 p = src[k];
-                M = M + p.w;
-                X = X + p.w * p.x;
-                Y = Y + p.w * p.y;
-                Z = Z + p.w * p.z;
-                lx = fmin(lx, p.x); hx = fmax(hx, p.x);
-                ly = fmin(ly, p.y); hy = fmax(hy, p.y);
-                lz = fmin(lz, p.z); hz = fmax(hz, p.z);
-            }
-        } else {
-            for (uint32_t c = md.z; c < md.z + nc; ++c) {
-                const double4 q = sums[c];
-                M = M + q.w;
-                X = X + q.x;
-                Y = Y + q.y;
-                Z = Z + q.z;
-                const double4 a = bmin[c], b = bmax[c];
-                lx = fmin(lx, a.x); hx = fmax(hx, b.x);
-                ly = fmin(ly, a.y); hy = fmax(hy, b.y);
-                lz = fmin(lz, a.z); hz = fmax(hz, b.z);
-            }
-        }
-        sums[v] = make_double4(X, Y, Z, M);
-        bmin[v] = make_double4(lx, ly, lz, 0.0);
-        bmax[v] = make_double4(hx, hy, hz, 0.0);
-    }
-}
-
-// Node source records src[n + v] = (com, mass).
-__global__ __launch_bounds__(kTreeBlock) void tree_com_kernel(double4* __restrict__ src, int n,
-                                                             const uint4* __restrict__ meta,
-                                                             const uint32_t* __restrict__ lvl,
-                                                             const double4* __restrict__ sums) {
-    const uint32_t total = lvl[2 * (kMaxLevel + 1)];
-    for (uint32_t v = blockIdx.x * kTreeBlock + threadIdx.x; v < total;
-         v += gridDim.x * kTreeBlock) {
-        const double4 q = sums[v];
-        double4 r;
-        if (q.w != 0.0) {
-            r = make_double4(q.x / q.w, q.y / q.w, q.z / q.w, q.w);
-        } else {
-            const double4 p = src[meta[v].x];
-            r = make_double4(p.x, p.y, p.z, 0.0);
-        }
-        src[(size_t)n + v] = r;
-    }
-}
-
-// Target groups of at most gsize particles, contiguous in sorted order: the
-// children of every internal node holding more than gsize particles are
-// visited in order and consecutive children holding at most gsize are packed
-// greedily while the total stays within gsize and the packed particles'
-// bounding box is no wider (largest axis extent) than twice the widest
-// packed child's; a child leaf holding more (coincident points at level 21)
-// is cut into runs of gsize; a root holding at most gsize (or a root leaf) is
-// cut the same way. Appended in any order: each group's result depends only
-// on its own particles and the tree.
-__device__ __forceinline__ void emit_runs(uint2* groups, uint32_t* ngroups, uint32_t first,
-                                          uint32_t cnt, uint32_t gsize) {
-    const uint32_t k = (cnt + gsize - 1) / gsize;
-    const uint32_t at = atomicAdd(ngroups, k);
-    for (uint32_t q = 0; q < k; ++q)
-        groups[at + q] = make_uint2(first + q * gsize, min(gsize, cnt - q * gsize));
-}
-
-__device__ __forceinline__ double extent(double4 lo, double4 hi) {
-    return fmax(fmax(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z);
-}
-
-__global__ __launch_bounds__(kTreeBlock) void group_emit_kernel(const uint4* __restrict__ meta,
-                                                               const uint32_t* __restrict__ lvl,
-                                                               const double4* __restrict__ bmin,
-                                                               const double4* __restrict__ bmax,
-                                                               uint32_t gsize,
-                                                               uint2* __restrict__ groups,
-                                                               uint32_t* __restrict__ ngroups) {
-    const uint32_t total = lvl[2 * (kMaxLevel + 1)];
-    for (uint32_t p = blockIdx.x * kTreeBlock + threadIdx.x; p < total;
-         p += gridDim.x * kTreeBlock) {
-        const uint4 md = meta[p];
-        const uint32_t nc = (md.w >> 8) & 255u;
-        if (p == 0 && (md.y <= gsize || nc == 0)) {
-            emit_runs(groups, ngroups, md.x, md.y, gsize);
-            continue;
-        }
-        if (md.y <= gsize || nc == 0) continue;
-        uint32_t rf = 0, rc = 0;
-        double4 rlo = make_double4(0, 0, 0, 0), rhi = rlo;
-        double rmax = 0.0;
-        for (uint32_t q = 0; q < nc; ++q) {
-            const uint32_t cv = md.z + q;
-            const uint4 cm = meta[cv];
-            const uint32_t cc = (cm.w >> 8) & 255u;
-            if (cm.y <= gsize) {
-                const double4 clo = bmin[cv], chi = bmax[cv];
-                const double cext = extent(clo, chi);
-                if (rc > 0) {
-                    const double4 ulo = make_double4(fmin(rlo.x, clo.x), fmin(rlo.y, clo.y),
-                                                     fmin(rlo.z, clo.z), 0.0);
-                    const double4 uhi = make_double4(fmax(rhi.x, chi.x), fmax(rhi.y, chi.y),
-                                                     fmax(rhi.z, chi.z), 0.0);
-                    if (rc + cm.y <= gsize && extent(ulo, uhi) <= 2.0 * fmax(rmax, cext)) {
-                        rc += cm.y;
-                        rlo = ulo;
-                        rhi = uhi;
-                        rmax = fmax(rmax, cext);
-                        continue;
-                    }
-                    emit_runs(groups, ngroups, rf, rc, gsize);
-                }
-                rf = cm.x;
-                rc = cm.y;
-                rlo = clo;
-                rhi = chi;
-                rmax = cext;
-            } else {
-                if (rc > 0) emit_runs(groups, ngroups, rf, rc, gsize);
-                rc = 0;
-                if (cc == 0) emit_runs(groups, ngroups, cm.x, cm.y, gsize);
-            }
-        }
-        if (rc > 0) emit_runs(groups, ngroups, rf, rc, gsize);
-    }
-}
-
 __global__ void walk_reset_kernel(uint32_t* __restrict__ counter,
                                   unsigned long long* __restrict__ stats) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         counter[0] = 0u;
-        counter[1] = 0u;
         for (int k = 0; k < 4; ++k) stats[k] = 0ull;
     }
 }
@@ -357,17 +77,18 @@ __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
     uint32_t* list = lists[w];
     double4* stage = stages[w];
     float4* stage32 = reinterpret_cast<float4*>(stage);
-    const uint64_t lt = (1ull << lane) - 1ull;
     const size_t slot = (size_t)blockIdx.x * kWalkWaves + w;
     uint32_t* const fa0 = frontier + slot * 2 * (size_t)fcap;
     uint32_t* const fb0 = fa0 + fcap;
     const double side0 = 2.0 * box[3];  // root side, octree.rs:81-85
 
+    const uint32_t ng = __builtin_amdgcn_readfirstlane(ngroups[0]);
     for (;;) {
-        uint32_t g = 0;
-        if (lane == 0) g = atomicAdd(counter, 1u);
-        g = __shfl(g, 0);
-        if (g >= ngroups[0]) break;
+        // every lane takes part (lane 0 adds 1), so the fetch sits in uniform
+        // control flow and g is a scalar
+        const uint32_t g =
+            __builtin_amdgcn_readfirstlane(atomicAdd(counter, lane == 0 ? 1u : 0u));
+        if (g >= ng) break;
         const uint2 gr = groups[g];
         const int i0 = (int)gr.x;
         const int cnt = (int)gr.y;
@@ -488,7 +209,7 @@ __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
                         const uint32_t t = __shfl_up(sc, o);
                         if (lane >= o) sc += t;
                     }
-                    const uint32_t total = __shfl(sc, kWave - 1);
+                    const uint32_t total = __builtin_amdgcn_readfirstlane(__shfl(sc, kWave - 1));
                     const uint64_t accb = __ballot(accept);
                     t_acc += (unsigned long long)__popcll(accb);
                     t_leafp += total - (uint32_t)__popcll(accb);
@@ -525,7 +246,7 @@ __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
                         const uint32_t t = __shfl_up(kc, o);
                         if (lane >= o) kc += t;
                     }
-                    const uint32_t ktot = __shfl(kc, kWave - 1);
+                    const uint32_t ktot = __builtin_amdgcn_readfirstlane(__shfl(kc, kWave - 1));
                     if (nn + ktot > fcap) {
                         overflow = true;
                         break;
@@ -579,69 +300,16 @@ __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
 
 }  // namespace
 
-gravis_status bh_force(Ctx* c, const gravis_solver* s, size_t n, const double* x,
-                       const double* y, const double* z, const double* m, const FinishArgs& fa) {
-    if (n == 0) return GRAVIS_OK;
-    if (n > 0x7FFFFFFFull) return fail(c, GRAVIS_INVALID_ARG, "n too large");
-    const uint32_t leaf = s->leaf_size ? s->leaf_size : 16u;
-    const int gs = s->group_size ? (int)s->group_size : kWave;
-    if (gs < 1 || gs > kWave) return fail(c, GRAVIS_INVALID_ARG, "group_size must be 1..64");
-    if (leaf < 1 || leaf > 4096) return fail(c, GRAVIS_INVALID_ARG, "leaf_size must be 1..4096");
+gravis_status bh_walk(Ctx* c, const gravis_solver* s, const Octree& t, const FinishArgs& fa) {
+    const size_t n = t.n;
     const int ni = (int)n;
-
-    double* box;
-    GRAVIS_TRY(buf(c, "bh_box", 4, &box));
-    GRAVIS_TRY(root_box(c, n, x, y, z, box));
-    uint64_t* keys;
-    uint32_t* perm;
-    GRAVIS_TRY(morton_sort(c, n, x, y, z, box, &keys, &perm));
-
-    // Internal nodes of one level are disjoint runs of more than `leaf`
-    // particles and have at most 8 children each.
-    const size_t cap = 1 + (size_t)kMaxLevel * 8 * (n / (leaf + 1) + 1);
-    if (cap > 0x7FFFFFFFull) return fail(c, GRAVIS_INVALID_ARG, "tree too large");
-    double4 *src, *sums, *bmin, *bmax;
-    uint4* meta;
-    uint32_t *lvl, *nch, *off, *flags;
-    GRAVIS_TRY(buf(c, "bh_src", n + cap, &src));
-    GRAVIS_TRY(buf(c, "bh_sums", cap, &sums));
-    GRAVIS_TRY(buf(c, "bh_bmin", cap, &bmin));
-    GRAVIS_TRY(buf(c, "bh_bmax", cap, &bmax));
-    GRAVIS_TRY(buf(c, "bh_meta", cap, &meta));
-    GRAVIS_TRY(buf(c, "bh_lvl", 2 * (kMaxLevel + 2), &lvl));
-    GRAVIS_TRY(buf(c, "bh_nch", cap, &nch));
-    GRAVIS_TRY(buf(c, "bh_off", cap, &off));
-    GRAVIS_TRY(buf(c, "bh_flags", 4, &flags));
-    GRAVIS_TRY(gather_sorted(c, n, x, y, z, m, perm, src));
-
     hipStream_t st = c->stream;
-    GRAVIS_HIP(c, hipMemsetAsync(flags, 0, 4 * sizeof(uint32_t), st));
-    tree_init_kernel<<<1, 64, 0, st>>>(meta, lvl, (uint32_t)n);
-    const unsigned tg = grid_for(cap, kTreeBlock, kTreeGrid);
-    for (int L = 0; L < kMaxLevel; ++L) {
-        tree_count_kernel<<<tg, kTreeBlock, 0, st>>>(keys, meta, lvl, L, leaf, nch);
-        tree_scan_kernel<<<1, 1024, 0, st>>>(nch, off, lvl, L);
-        tree_write_kernel<<<tg, kTreeBlock, 0, st>>>(keys, meta, lvl, L, nch, off, (uint32_t)cap,
-                                                    (uint32_t)gs, flags + 1);
-    }
-    // level 21 holds leaves only (zero children); its scan closes the node range
-    tree_count_kernel<<<tg, kTreeBlock, 0, st>>>(keys, meta, lvl, kMaxLevel, leaf, nch);
-    tree_scan_kernel<<<1, 1024, 0, st>>>(nch, off, lvl, kMaxLevel);
-    GRAVIS_TRY(check_launch(c, "tree build"));
-    for (int L = kMaxLevel; L >= 0; --L)
-        tree_sums_kernel<<<tg, kTreeBlock, 0, st>>>(src, meta, lvl, L, sums, bmin, bmax);
-    tree_com_kernel<<<tg, kTreeBlock, 0, st>>>(src, ni, meta, lvl, sums);
-    GRAVIS_TRY(check_launch(c, "tree com"));
-
-    // target groups, then the walk
     const int blocks = (int)std::min<long>((long)(n / 64 / kWalkWaves) + 1,
                                            (long)c->num_cus * 4);
-    const uint32_t fcap = (uint32_t)std::min<size_t>(cap, c->bh_frontier_cap);
+    const uint32_t fcap = (uint32_t)std::min<size_t>(t.cap, c->bh_frontier_cap);
     uint32_t *front, *counter;
-    uint2* groups;
     GRAVIS_TRY(buf(c, "bh_front", (size_t)blocks * kWalkWaves * 2 * fcap, &front));
-    GRAVIS_TRY(buf(c, "bh_counter", 4, &counter));
-    GRAVIS_TRY(buf(c, "bh_groups", n, &groups));
+    GRAVIS_TRY(buf(c, "bh_counter", 1, &counter));
     double* part;
     GRAVIS_TRY(buf(c, "bh_part", 3 * n, &part));
     const double theta2 = s->theta * s->theta;
@@ -649,33 +317,43 @@ gravis_status bh_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
     unsigned long long* stats;
     GRAVIS_TRY(buf(c, "bh_stats", 4, &stats));
     walk_reset_kernel<<<1, 64, 0, st>>>(counter, stats);
-    group_emit_kernel<<<tg, kTreeBlock, 0, st>>>(meta, lvl, bmin, bmax, (uint32_t)gs, groups,
-                                                 counter + 1);
     timer_begin(c);
     if (s->precision == GRAVIS_MIXED)
         bh_walk_kernel<true><<<blocks, kWalkWaves * kWave, 0, st>>>(
-            src, meta, ni, groups, counter + 1, box, theta2, eps2, front, fcap, counter, flags,
-            part, stats);
+            t.src, t.meta, ni, t.groups, t.ngroups, t.box, theta2, eps2, front, fcap, counter,
+            t.flags, part, stats);
     else
         bh_walk_kernel<false><<<blocks, kWalkWaves * kWave, 0, st>>>(
-            src, meta, ni, groups, counter + 1, box, theta2, eps2, front, fcap, counter, flags,
-            part, stats);
+            t.src, t.meta, ni, t.groups, t.ngroups, t.box, theta2, eps2, front, fcap, counter,
+            t.flags, part, stats);
     timer_end(c, "bh_walk");
     GRAVIS_TRY(check_launch(c, "bh_walk"));
     uint32_t hflags[2] = {0, 0};
-    GRAVIS_HIP(c, hipMemcpyAsync(hflags, flags, sizeof hflags, hipMemcpyDeviceToHost, st));
-    GRAVIS_HIP(c, hipMemcpyAsync(c->bh_stats, stats, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
-    uint32_t hlvl[2 * (kMaxLevel + 2)];
-    GRAVIS_HIP(c, hipMemcpyAsync(hlvl, lvl, sizeof hlvl, hipMemcpyDeviceToHost, st));
+    GRAVIS_HIP(c, hipMemcpyAsync(hflags, t.flags, sizeof hflags, hipMemcpyDeviceToHost, st));
+    GRAVIS_HIP(c, hipMemcpyAsync(c->bh_stats, stats, 4 * sizeof(unsigned long long),
+                                 hipMemcpyDeviceToHost, st));
+    uint32_t nodes = 0;
+    GRAVIS_HIP(c, hipMemcpyAsync(&nodes, t.lvl + 2 * (kTreeMaxLevel + 1), sizeof nodes,
+                                 hipMemcpyDeviceToHost, st));
     GRAVIS_HIP(c, hipStreamSynchronize(st));
-    c->bh_stats[4] = hlvl[2 * (kMaxLevel + 1)];  // node count
+    c->bh_stats[4] = nodes;
     if (hflags[1])
-        return fail(c, GRAVIS_INTERNAL, "octree node capacity exceeded (%zu nodes)", cap);
+        return fail(c, GRAVIS_INTERNAL, "octree node capacity exceeded (%zu nodes)", t.cap);
     if (hflags[0])
         return fail(c, GRAVIS_INTERNAL,
                     "Barnes-Hut walk frontier of a single target exceeded %u nodes (theta %g)",
                     fcap, s->theta);
-    return finish_forces(c, part, 1, ni, perm, fa);
+    return finish_forces(c, part, 1, ni, t.perm, fa);
+}
+
+gravis_status bh_force(Ctx* c, const gravis_solver* s, size_t n, const double* x,
+                       const double* y, const double* z, const double* m, const FinishArgs& fa) {
+    if (n == 0) return GRAVIS_OK;
+    const uint32_t leaf = s->leaf_size ? s->leaf_size : 16u;
+    const uint32_t gs = s->group_size ? s->group_size : (uint32_t)kWave;
+    Octree t;
+    GRAVIS_TRY(build_octree(c, n, x, y, z, m, leaf, gs, &t));
+    return bh_walk(c, s, t, fa);
 }
 
 }  // namespace gravis

@@ -47,6 +47,12 @@ struct Ctx {
     unsigned long long bh_stats[5] = {0, 0, 0, 0, 0};
     // Barnes-Hut walk: breadth-first frontier capacity per wave (nodes)
     uint32_t bh_frontier_cap = 16384;
+    // last SPH compute: groups searched depth first, re-queries, query
+    // rounds, neighbour-pool entries used
+    unsigned long long sph_stats[4] = {0, 0, 0, 0};
+    // SPH neighbour search: overlapping leaves one wave may hold in LDS
+    // before its group is searched depth first (at most 512)
+    uint32_t sph_range_cap = 512;
 };
 
 // Status helpers ----------------------------------------------------------
@@ -98,6 +104,40 @@ gravis_status gather_sorted(Ctx* c, size_t n, const double* x, const double* y,
                             const double* z, const double* m, const uint32_t* d_perm,
                             double4* d_out);
 
+// ---- linear octree (tree.hip) -------------------------------------------
+constexpr int kTreeMaxLevel = 21;  // 63-bit keys: 21 levels below the root
+constexpr int kTreeBlock = 256;
+constexpr int kTreeGrid = 1024;
+// meta.w = level | nchild << 8 | kNodeGroupRoot (the node holds more
+// particles than a target group).
+constexpr uint32_t kNodeGroupRoot = 1u << 16;
+
+// Device view of one built tree. Nodes are numbered breadth first; the
+// children of a node are contiguous and in Morton digit order; every node is
+// the run [first, first + count) of the key-sorted particles.
+struct Octree {
+    size_t n = 0;                    // particles
+    size_t cap = 0;                  // node capacity
+    uint32_t leaf = 16, group = 64;  // leaf capacity, target group size
+    const double* box = nullptr;     // root cube (cx, cy, cz, half_width)
+    const uint64_t* keys = nullptr;  // sorted Morton keys
+    const uint32_t* perm = nullptr;  // sorted position -> caller index
+    double4* src = nullptr;          // [n sorted (x, y, z, m) | cap node (com, mass)]
+    const uint4* meta = nullptr;     // {first, count, first child, flags}
+    const double4* bmin = nullptr;   // particle bounding box of each node
+    const double4* bmax = nullptr;
+    const uint32_t* lvl = nullptr;   // level ranges; lvl[2 (kTreeMaxLevel + 1)] = node count
+    uint32_t* flags = nullptr;       // [0] walk frontier overflow, [1] node capacity exceeded
+    const uint2* groups = nullptr;   // target groups {first, count}
+    uint32_t* ngroups = nullptr;     // number of groups (device)
+};
+
+// Root cube, Morton sort and level-by-level build of the tree of n
+// particles (device SoA), asynchronous on the context stream. The buffers
+// are the context's and are reused by the next build.
+gravis_status build_octree(Ctx* c, size_t n, const double* x, const double* y, const double* z,
+                           const double* m, uint32_t leaf, uint32_t group, Octree* t);
+
 // ---- forces (bf.hip, bh.hip) --------------------------------------------
 // How the finishing kernel writes the result in the caller's order.
 enum FinishMode : int {
@@ -129,6 +169,21 @@ gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
 // Barnes-Hut force (linear octree, group walk).
 gravis_status bh_force(Ctx* c, const gravis_solver* s, size_t n, const double* x,
                        const double* y, const double* z, const double* m, const FinishArgs& fa);
+// The walk alone, on a tree built by build_octree (synchronises the stream
+// to check the capacity flags).
+gravis_status bh_walk(Ctx* c, const gravis_solver* s, const Octree& t, const FinishArgs& fa);
+
+// ---- smoothed particle hydrodynamics (sph.hip), device pointers ---------
+// SphSolver::compute on device state S; `tree` (built on S's positions with
+// target groups of 64) is reused when given. div_v/max_mu/dalpha: n each.
+gravis_status sph_compute(Ctx* c, const gravis_sph_params* P, const gravis_sph_state* S,
+                          const Octree* tree, double* div_v, double* max_mu, double* dalpha,
+                          double* omega, uint32_t* nneigh);
+// nsteps of step_with_sph with headless's dt rule; *dt_io in/out, time_io
+// and dt_next_out may be null.
+gravis_status sph_run(Ctx* c, const gravis_sph_params* P, const gravis_solver* g,
+                      const gravis_sph_state* S, uint64_t nsteps, double* dt_io,
+                      double* time_io, double* dt_next_out);
 
 // ---- integrator / diagnostics -------------------------------------------
 gravis_status kick_drift(Ctx* c, size_t n, double* x, double* y, double* z, double* vx,

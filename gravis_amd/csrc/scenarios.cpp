@@ -250,3 +250,49 @@ extern "C" gravis_status gravis_scenario_targets(uint64_t seed, size_t n, double
     }
     return GRAVIS_OK;
 }
+
+// EvrardCollapse::generate (crates/sim-core/src/scenarios/evrard_collapse.rs:
+// 45-57): sphere_spacing (scenarios/mod.rs:17-22) then
+// fill_uniform_gas_sphere (mod.rs:33-51), one rejection draw of x, y, z per
+// trial; Particles::add_gas (particle.rs:136-154) sets alpha 1 and zeroes
+// the rest.
+extern "C" gravis_status gravis_scenario_evrard(size_t n, double total_mass, double radius,
+                                                double u0, uint64_t seed,
+                                                gravis_sph_state* s) {
+    if (!s || s->n != n) return GRAVIS_INVALID_ARG;
+    if (n == 0) return GRAVIS_OK;
+    double* f[17] = {s->x, s->y, s->z, s->vx, s->vy, s->vz, s->ax, s->ay, s->az,
+                     s->m, s->rho, s->p, s->u, s->dudt, s->h, s->c, s->alpha};
+    for (double* a : f)
+        if (!a) return GRAVIS_INVALID_ARG;
+    if (!s->type || !(radius > 0.0)) return GRAVIS_INVALID_ARG;
+    ChaChaStream rng(seed);
+    const double mass = total_mass / static_cast<double>(n);
+    const double vol = 4.0 / 3.0 * kPi * (radius * radius * radius);
+    const double h = 1.5 * std::cbrt(vol / static_cast<double>(n));
+    size_t cnt = 0;
+    while (cnt < n) {
+        const double x = rng.uniform01() * 2.0 * radius - radius;
+        const double y = rng.uniform01() * 2.0 * radius - radius;
+        const double z = rng.uniform01() * 2.0 * radius - radius;
+        if (x * x + y * y + z * z <= radius * radius) {
+            for (double* a : f) a[cnt] = 0.0;
+            s->x[cnt] = x;
+            s->y[cnt] = y;
+            s->z[cnt] = z;
+            s->m[cnt] = mass;
+            s->u[cnt] = u0;
+            s->h[cnt] = h;
+            s->alpha[cnt] = 1.0;
+            s->type[cnt] = 4;
+            ++cnt;
+        }
+    }
+    return GRAVIS_OK;
+}
+
+// EvrardCollapse::suggested_softening (evrard_collapse.rs:67-70).
+extern "C" double gravis_evrard_softening(size_t n, double radius) {
+    const double vol = 4.0 / 3.0 * kPi * (radius * radius * radius);
+    return 0.3 * std::cbrt(vol / static_cast<double>(n));
+}

@@ -1,0 +1,922 @@
+// sph.hip -- grad-h smoothed particle hydrodynamics on gfx950.
+//
+// Reference: SphSolver::compute / compute_timestep and step_with_sph
+// (crates/sim-core/src/sph/solver.rs:61-212), compute_density
+// (sph/density.rs:42-157), compute_forces (sph/forces.rs:69-232), the
+// Wendland C2 kernel (sph/kernel.rs:17-108), the neighbour lists of
+// sph/neighbors.rs:56-119 and the ball query of octree.rs:245-308.
+//
+// Layout. Particles are handled in the Morton order of the linear octree
+// (tree.hip); per sorted particle s the kernels keep
+//   pos[s]  = (x, y, z, m)            (the tree's source records)
+//   vh[s]   = (vx, vy, vz, h)
+//   aux[s]  = (rho, p, c, alpha)
+//   aux2[s] = (omega, u)
+// and a neighbour list {start, count} into one index pool, as the
+// reference's flat NeighborList (neighbors.rs:13-19); a re-query appends a
+// new list to the pool (NeighborList::replace_neighbors, neighbors.rs:36-44).
+//
+// Neighbour search. One wavefront per tree target group: the group's box
+// (over the targets that need a query) grown by the largest query radius is
+// tested against the tree's per-node particle boxes breadth first; the
+// leaves that overlap are sorted by first particle, and their particles are
+// staged in LDS 64 at a time and tested by every lane against its own target
+// with the reference's exact test |x_j - x_i|^2 <= R^2 (octree.rs:275-283).
+// Lists therefore come out in ascending Morton order, the order of the
+// reference's depth-first octant walk. The box test is conservative in
+// floating point (every rounding is monotone), so no neighbour is lost. A
+// group whose frontier or leaf list exceeds its LDS capacity is searched
+// again one target at a time by a depth-first walk with an LDS stack.
+//
+// Density. One lane per gas particle runs the reference's iteration
+// (<= 5 passes, tolerance 1e-3) over its list; when the reference would
+// re-query (2 h_new > 2.2 h) the lane records the new radius and stops; the
+// host then re-queries those particles and resumes them, until none is left.
+//
+// Arithmetic is FP64 in the reference's operation order (no contraction;
+// division and sqrt correctly rounded); only values that depend on h alone
+// (sigma / h^3, -3 / h) are hoisted out of the pair loops.
+#include <algorithm>
+#include <cmath>
+
+#include "internal.h"
+
+namespace gravis {
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kNorm3d = 21.0 / (16.0 * kPi);  // kernel.rs:17
+constexpr double kDensityFloor = 1e-10;          // forces.rs:12
+constexpr double kEnergyFloor = 1e-10;           // solver.rs:13
+constexpr double kDtFloor = 1e-6;                // solver.rs:17
+constexpr int kMaxHIter = 5;                     // density.rs:13
+constexpr double kHTol = 1e-3;                   // density.rs:16
+constexpr double kPad = 1.1;                     // density.rs:52
+constexpr uint8_t kGas = 4;                      // particle.rs:9
+
+// per-particle progress of the density phase
+constexpr uint8_t kQNone = 0;     // not gas
+constexpr uint8_t kQQuery = 1;    // needs a (re-)query at radius[s]
+constexpr uint8_t kQDensity = 2;  // list ready, density iteration pending
+constexpr uint8_t kQDone = 3;
+
+// control words: [0] pool entries used, [1] pool overflow, [2] groups sent to
+// the depth-first search, [3] re-queries requested, [4..5] min dt bits
+constexpr int kCtlWords = 8;
+
+constexpr int kQWaves = 4;
+constexpr uint32_t kQFront = 512;   // breadth-first frontier (nodes) per wave
+constexpr uint32_t kQRanges = 512;  // overlapping leaves per wave
+constexpr uint32_t kQStack = 256;   // depth-first stack (>= 1 + 7 * 22)
+
+__device__ __forceinline__ void lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, int lane, uint32_t* total) {
+    uint32_t s = v;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t t = __shfl_up(s, o);
+        if (lane >= o) s += t;
+    }
+    *total = __builtin_amdgcn_readfirstlane(__shfl(s, kWave - 1));
+    return s - v;
+}
+
+// kernel.rs:44-62
+__device__ __forceinline__ void grad_w(double dx, double dy, double dz, double r, double h,
+                                       double& gx, double& gy, double& gz) {
+    gx = gy = gz = 0.0;
+    if (r < 1e-30) return;
+    const double q = r / h;
+    if (q >= 2.0) return;
+    const double h3 = h * h * h;
+    const double s = 1.0 - 0.5 * q;
+    const double s3 = s * s * s;
+    const double dwdq = kNorm3d / h3 * (-5.0 * q) * s3;
+    const double f = dwdq / (h * r);
+    gx = f * dx;
+    gy = f * dy;
+    gz = f * dz;
+}
+
+// kernel.rs:93-107
+__device__ __forceinline__ double abs_grad_w(double r, double h) {
+    if (r < 1e-30) return 0.0;
+    const double q = r / h;
+    if (q >= 2.0) return 0.0;
+    const double h3 = h * h * h;
+    const double s = 1.0 - 0.5 * q;
+    const double s3 = s * s * s;
+    const double dwdq = kNorm3d / h3 * (-5.0 * q) * s3;
+    return -dwdq / h;
+}
+
+// ---------------------------------------------------------------------------
+// Gather of the caller's arrays into sorted records, and the first query
+// radius 2 h 1.1 of every gas particle (neighbors.rs:67-70 with
+// density.rs:52).
+__global__ __launch_bounds__(256) void sph_prep_kernel(
+    int n, const uint32_t* __restrict__ perm, const double* __restrict__ vx,
+    const double* __restrict__ vy, const double* __restrict__ vz, const double* __restrict__ h,
+    const double* __restrict__ rho, const double* __restrict__ pr, const double* __restrict__ cs,
+    const double* __restrict__ alpha, const double* __restrict__ u,
+    const uint8_t* __restrict__ type, double4* __restrict__ vh, double4* __restrict__ aux,
+    double2* __restrict__ aux2, uint8_t* __restrict__ qstate, uint8_t* __restrict__ iter,
+    double* __restrict__ hcur, double* __restrict__ radius, uint32_t* __restrict__ ctl) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s == 0)
+        for (int k = 0; k < kCtlWords; ++k) ctl[k] = 0u;
+    if (s >= n) return;
+    const uint32_t p = perm[s];
+    const double hp = h[p];
+    vh[s] = make_double4(vx[p], vy[p], vz[p], hp);
+    aux[s] = make_double4(rho[p], pr[p], cs[p], alpha[p]);
+    aux2[s] = make_double2(1.0, u[p]);
+    const bool gas = type[p] == kGas;
+    qstate[s] = gas ? kQQuery : kQNone;
+    iter[s] = 0;
+    hcur[s] = hp;
+    radius[s] = 2.0 * hp * kPad;
+}
+
+// Breadth-first group query of one target group (see the file comment).
+// Targets with qstate == kQQuery get a list {start, count} in nbr and move
+// on to kQDensity (kQDone if their iteration already ended); a group that
+// exceeds the LDS capacities is appended to dfs_groups instead. All control
+// flow is wave uniform: g and every count are scalar values.
+__device__ __forceinline__ void query_group(
+    uint32_t g, int lane, const double4* __restrict__ pos, const uint4* __restrict__ meta,
+    const double4* __restrict__ bmin, const double4* __restrict__ bmax,
+    const uint2* __restrict__ groups, uint8_t* __restrict__ qstate,
+    const uint8_t* __restrict__ iter, const double* __restrict__ radius,
+    uint2* __restrict__ nbr, uint32_t* __restrict__ pool, uint32_t pool_cap,
+    uint32_t* __restrict__ ctl, uint32_t* __restrict__ dfs_groups, uint32_t rcap,
+    uint32_t* fa, uint32_t* fb, uint2* rg, double4* stage) {
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint2 gr = groups[g];
+    const uint32_t s = gr.x + (uint32_t)lane;
+    const bool inside = (uint32_t)lane < gr.y;
+    const bool act = inside && qstate[s] == kQQuery;
+    if (__ballot(act) == 0ull) return;
+    const double4 me = pos[inside ? s : gr.x];
+    const double R = act ? radius[s] : 0.0;
+    const double R2 = R * R;
+    double lo[3], hi[3];
+    lo[0] = act ? me.x : INFINITY; hi[0] = act ? me.x : -INFINITY;
+    lo[1] = act ? me.y : INFINITY; hi[1] = act ? me.y : -INFINITY;
+    lo[2] = act ? me.z : INFINITY; hi[2] = act ? me.z : -INFINITY;
+    double R2g = act ? R2 : 0.0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], __shfl_xor(lo[d], o));
+            hi[d] = fmax(hi[d], __shfl_xor(hi[d], o));
+        }
+        R2g = fmax(R2g, __shfl_xor(R2g, o));
+    }
+
+    // breadth-first search for the overlapping leaves
+    if (lane == 0) fa[0] = 0u;
+    lds_sync();
+    uint32_t nf = 1, nr = 0;
+    bool ovf = rcap == 0;  // 0: every group searched depth first
+    while (nf > 0 && !ovf) {
+        uint32_t nn = 0;
+        for (uint32_t base = 0; base < nf; base += kWave) {
+            const uint32_t idx = base + (uint32_t)lane;
+            const bool valid = idx < nf;
+            const uint32_t v = valid ? fa[idx] : 0u;
+            const uint4 md = meta[v];
+            const double4 a = bmin[v], b = bmax[v];
+            const double ex = fmax(fmax(a.x - hi[0], lo[0] - b.x), 0.0);
+            const double ey = fmax(fmax(a.y - hi[1], lo[1] - b.y), 0.0);
+            const double ez = fmax(fmax(a.z - hi[2], lo[2] - b.z), 0.0);
+            const double d2 = ex * ex + ey * ey + ez * ez;
+            const bool hit = valid && d2 <= R2g;
+            const uint32_t nchild = (md.w >> 8) & 255u;
+            const bool leafhit = hit && nchild == 0;
+            const uint64_t lb = __ballot(leafhit);
+            const uint32_t nl = (uint32_t)__popcll(lb);
+            if (nr + nl > rcap) {
+                ovf = true;
+                break;
+            }
+            if (leafhit) rg[nr + (uint32_t)__popcll(lb & below)] = make_uint2(md.x, md.y);
+            nr += nl;
+            const uint32_t k = (hit && nchild > 0) ? nchild : 0u;
+            uint32_t ktot;
+            const uint32_t kx = wave_excl_scan(k, lane, &ktot);
+            if (nn + ktot > kQFront) {
+                ovf = true;
+                break;
+            }
+            for (uint32_t q = 0; q < k; ++q) fb[nn + kx + q] = md.z + q;
+            nn += ktot;
+        }
+        lds_sync();
+        uint32_t* t = fa;
+        fa = fb;
+        fb = t;
+        nf = nn;
+    }
+    if (ovf) {
+        if (lane == 0) dfs_groups[atomicAdd(&ctl[2], 1u)] = g;
+        return;
+    }
+
+    // sort the leaves by first particle (bitonic, one wave)
+    uint32_t np = 1;
+    while (np < nr) np <<= 1;
+    for (uint32_t i = nr + (uint32_t)lane; i < np; i += kWave) rg[i] = make_uint2(0xFFFFFFFFu, 0u);
+    lds_sync();
+    for (uint32_t k = 2; k <= np; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = (uint32_t)lane; i < np; i += kWave) {
+                const uint32_t ij = i ^ j;
+                if (ij > i) {
+                    const uint2 x = rg[i], y = rg[ij];
+                    const bool up = (i & k) == 0;
+                    if ((x.x > y.x) == up) {
+                        rg[i] = y;
+                        rg[ij] = x;
+                    }
+                }
+            }
+            lds_sync();
+        }
+    }
+    // counts -> exclusive offsets
+    uint32_t total = 0;
+    for (uint32_t base = 0; base < nr; base += kWave) {
+        const uint32_t i = base + (uint32_t)lane;
+        const uint32_t cnt = i < nr ? rg[i].y : 0u;
+        uint32_t t;
+        const uint32_t ex = wave_excl_scan(cnt, lane, &t);
+        if (i < nr) rg[i].y = total + ex;
+        total += t;
+    }
+    lds_sync();
+
+    // two passes over the candidates: count, then write
+    uint32_t mine = 0, wpos = 0;
+    bool failed = false;
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) {
+            uint32_t gtot;
+            const uint32_t ex = wave_excl_scan(act ? mine : 0u, lane, &gtot);
+            uint32_t base = 0;
+            if (lane == 0) {
+                base = atomicAdd(&ctl[0], gtot);
+                if ((uint64_t)base + gtot > (uint64_t)pool_cap) atomicOr(&ctl[1], 1u);
+            }
+            base = __builtin_amdgcn_readfirstlane(base);
+            if ((uint64_t)base + gtot > (uint64_t)pool_cap) {
+                failed = true;
+                break;
+            }
+            wpos = base + ex;
+            if (act) nbr[s] = make_uint2(wpos, mine);
+        }
+        for (uint32_t q0 = 0; q0 < total; q0 += kWave) {
+            const uint32_t q = q0 + (uint32_t)lane;
+            if (q < total) {
+                uint32_t a = 0, b = nr;  // last leaf whose offset <= q
+                while (b - a > 1) {
+                    const uint32_t mid = (a + b) >> 1;
+                    if (rg[mid].y <= q) a = mid;
+                    else b = mid;
+                }
+                const uint32_t j = rg[a].x + (q - rg[a].y);
+                const double4 pj = pos[j];
+                stage[lane] = make_double4(pj.x, pj.y, pj.z, (double)j);
+            }
+            lds_sync();
+            const uint32_t m = min(total - q0, (uint32_t)kWave);
+            if (act) {
+                for (uint32_t k = 0; k < m; ++k) {
+                    const double4 c = stage[k];
+                    const double dx = c.x - me.x, dy = c.y - me.y, dz = c.z - me.z;
+                    const double dsq = dx * dx + dy * dy + dz * dz;
+                    if (dsq <= R2) {
+                        if (pass == 0) ++mine;
+                        else pool[wpos++] = (uint32_t)c.w;
+                    }
+                }
+            }
+            lds_sync();
+        }
+    }
+    if (act && !failed) qstate[s] = iter[s] >= kMaxHIter ? kQDone : kQDensity;
+}
+
+__global__ __launch_bounds__(kQWaves* kWave) void sph_query_kernel(
+    const double4* __restrict__ pos, const uint4* __restrict__ meta,
+    const double4* __restrict__ bmin, const double4* __restrict__ bmax,
+    const uint2* __restrict__ groups, const uint32_t* __restrict__ ngroups,
+    uint32_t* __restrict__ counter, uint8_t* __restrict__ qstate,
+    const uint8_t* __restrict__ iter, const double* __restrict__ radius,
+    uint2* __restrict__ nbr, uint32_t* __restrict__ pool, uint32_t pool_cap,
+    uint32_t* __restrict__ ctl, uint32_t* __restrict__ dfs_groups, uint32_t rcap) {
+    __shared__ uint32_t fronts[kQWaves][2][kQFront];
+    __shared__ uint2 leaves[kQWaves][kQRanges];
+    __shared__ double4 stages[kQWaves][kWave];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = threadIdx.x / kWave;
+    const uint32_t ng = __builtin_amdgcn_readfirstlane(ngroups[0]);
+    for (;;) {
+        // every lane takes part (lane 0 adds 1), so the fetch sits in uniform
+        // control flow and g is a scalar
+        const uint32_t g =
+            __builtin_amdgcn_readfirstlane(atomicAdd(counter, lane == 0 ? 1u : 0u));
+        if (g >= ng) break;
+        query_group(g, lane, pos, meta, bmin, bmax, groups, qstate, iter, radius, nbr, pool,
+                    pool_cap, ctl, dfs_groups, rcap, fronts[w][0], fronts[w][1], leaves[w],
+                    stages[w]);
+    }
+}
+
+// Depth-first search, one target at a time, for the groups the
+// breadth-first kernel could not hold in LDS. Children are pushed in reverse
+// digit order, so leaves are reached in ascending Morton order.
+__global__ __launch_bounds__(kWave) void sph_query_dfs_kernel(
+    const double4* __restrict__ pos, const uint4* __restrict__ meta,
+    const double4* __restrict__ bmin, const double4* __restrict__ bmax,
+    const uint2* __restrict__ groups, const uint32_t* __restrict__ dfs_groups,
+    uint8_t* __restrict__ qstate, const uint8_t* __restrict__ iter,
+    const double* __restrict__ radius, uint2* __restrict__ nbr, uint32_t* __restrict__ pool,
+    uint32_t pool_cap, uint32_t* __restrict__ ctl) {
+    __shared__ uint32_t stack[kQStack];
+    const int lane = threadIdx.x;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t ndfs = ctl[2];
+    for (uint32_t gi = blockIdx.x; gi < ndfs; gi += gridDim.x) {
+        const uint2 gr = groups[dfs_groups[gi]];
+        for (uint32_t l = 0; l < gr.y; ++l) {
+            const uint32_t s = gr.x + l;
+            if (qstate[s] != kQQuery) continue;
+            const double4 me = pos[s];
+            const double R = radius[s];
+            const double R2 = R * R;
+            uint32_t cnt = 0, wpos = 0;
+            bool failed = false;
+            for (int pass = 0; pass < 2 && !failed; ++pass) {
+                if (pass == 1) {
+                    uint32_t base = 0;
+                    if (lane == 0) {
+                        base = atomicAdd(&ctl[0], cnt);
+                        if ((uint64_t)base + cnt > (uint64_t)pool_cap) atomicOr(&ctl[1], 1u);
+                    }
+                    base = __builtin_amdgcn_readfirstlane(base);
+                    if ((uint64_t)base + cnt > (uint64_t)pool_cap) {
+                        failed = true;
+                        break;
+                    }
+                    wpos = base;
+                    if (lane == 0) nbr[s] = make_uint2(base, cnt);
+                }
+                uint32_t sp = 0;
+                {
+                    const double4 a = bmin[0], b = bmax[0];
+                    const double ex = fmax(fmax(a.x - me.x, me.x - b.x), 0.0);
+                    const double ey = fmax(fmax(a.y - me.y, me.y - b.y), 0.0);
+                    const double ez = fmax(fmax(a.z - me.z, me.z - b.z), 0.0);
+                    if (ex * ex + ey * ey + ez * ez <= R2) {
+                        if (lane == 0) stack[0] = 0u;
+                        sp = 1;
+                    }
+                }
+                lds_sync();
+                while (sp > 0) {
+                    const uint32_t v = stack[sp - 1];
+                    --sp;
+                    lds_sync();
+                    const uint4 md = meta[v];
+                    const uint32_t nchild = (md.w >> 8) & 255u;
+                    if (nchild == 0) {
+                        for (uint32_t k0 = 0; k0 < md.y; k0 += kWave) {
+                            const uint32_t k = k0 + (uint32_t)lane;
+                            bool hit = false;
+                            if (k < md.y) {
+                                const double4 c = pos[md.x + k];
+                                const double dx = c.x - me.x, dy = c.y - me.y, dz = c.z - me.z;
+                                hit = dx * dx + dy * dy + dz * dz <= R2;
+                            }
+                            const uint64_t hb = __ballot(hit);
+                            if (pass == 1 && hit)
+                                pool[wpos + (uint32_t)__popcll(hb & below)] = md.x + k;
+                            if (pass == 0) cnt += (uint32_t)__popcll(hb);
+                            else wpos += (uint32_t)__popcll(hb);
+                        }
+                    } else {
+                        bool acc = false;
+                        const uint32_t cv = md.z + (uint32_t)lane;
+                        if ((uint32_t)lane < nchild) {
+                            const double4 a = bmin[cv], b = bmax[cv];
+                            const double ex = fmax(fmax(a.x - me.x, me.x - b.x), 0.0);
+                            const double ey = fmax(fmax(a.y - me.y, me.y - b.y), 0.0);
+                            const double ez = fmax(fmax(a.z - me.z, me.z - b.z), 0.0);
+                            acc = ex * ex + ey * ey + ez * ez <= R2;
+                        }
+                        const uint64_t ab = __ballot(acc);
+                        const uint32_t na = (uint32_t)__popcll(ab);
+                        if (acc) stack[sp + na - 1 - (uint32_t)__popcll(ab & below)] = cv;
+                        sp += na;
+                        lds_sync();
+                    }
+                }
+            }
+            if (!failed && lane == 0) qstate[s] = iter[s] >= kMaxHIter ? kQDone : kQDensity;
+        }
+    }
+}
+
+// The density / smoothing-length iteration of density.rs:58-107 for every
+// particle in kQDensity, then the equation of state (density.rs:111-119).
+__global__ __launch_bounds__(256) void sph_density_kernel(
+    int n, const double4* __restrict__ pos, const uint32_t* __restrict__ pool,
+    const uint2* __restrict__ nbr, uint8_t* __restrict__ qstate, uint8_t* __restrict__ iter,
+    double* __restrict__ hcur, double* __restrict__ radius, double4* __restrict__ vh,
+    double4* __restrict__ aux, double2* __restrict__ aux2, double gamma, double eta,
+    double h_max, uint32_t* __restrict__ ctl) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    bool paused = false;
+    if (s < n && qstate[s] == kQDensity) {
+        const double4 me = pos[s];
+        const double mi = me.w;
+        const uint2 nb = nbr[s];
+        double h = hcur[s];
+        int it = iter[s];
+        double omega = aux2[s].x;
+        double rho = 0.0, hn = h;
+        while (it < kMaxHIter) {
+            const double h2 = 2.0 * h;
+            const double nh3 = kNorm3d / (h * h * h);
+            const double m3h = -3.0 / h;
+            double rs = 0.0, ds = 0.0;
+            for (uint32_t k = 0; k < nb.y; ++k) {
+                const double4 q4 = pos[pool[nb.x + k]];
+                const double dx = me.x - q4.x, dy = me.y - q4.y, dz = me.z - q4.z;
+                const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                if (r < h2) {
+                    const double q = r / h;
+                    double wv = 0.0, dwh = 0.0;
+                    if (q < 2.0) {
+                        const double sq = 1.0 - 0.5 * q;
+                        const double s3 = sq * sq * sq;
+                        const double s4 = s3 * sq;
+                        wv = nh3 * s4 * (1.0 + 2.0 * q);
+                        const double dwdq = nh3 * (-5.0 * q) * s3;
+                        dwh = m3h * wv - q / h * dwdq;
+                    }
+                    rs += q4.w * wv;
+                    ds += q4.w * dwh;
+                }
+            }
+            rho = rs;
+            hn = rho > 0.0 ? fmin(eta * cbrt(mi / rho), h_max) : fmin(h, h_max);
+            if (rho > 0.0) {
+                const double dh = -hn / (3.0 * rho);
+                omega = 1.0 - dh * ds;
+                if (omega < 0.1) omega = 0.1;
+            }
+            ++it;
+            if (fabs(hn - h) / fmax(h, 1e-30) < kHTol) break;
+            if (2.0 * hn > 2.0 * h * kPad) {
+                radius[s] = 2.0 * hn * kPad;
+                h = hn;
+                paused = true;
+                break;
+            }
+            h = hn;
+        }
+        const double4 v = vh[s];
+        vh[s] = make_double4(v.x, v.y, v.z, hn);
+        const double u = aux2[s].y;
+        aux2[s] = make_double2(omega, u);
+        const double4 a = aux[s];
+        const bool ended = !paused || it >= kMaxHIter;
+        const double p = ended ? (gamma - 1.0) * rho * u : a.y;
+        const double c = ended ? sqrt(gamma * (gamma - 1.0) * fmax(u, 0.0)) : a.z;
+        aux[s] = make_double4(rho, p, c, a.w);
+        hcur[s] = h;
+        iter[s] = (uint8_t)it;
+        qstate[s] = paused ? kQQuery : kQDone;
+    }
+    const uint64_t pb = __ballot(paused);
+    if (pb != 0ull && (threadIdx.x & (kWave - 1)) == 0) atomicAdd(&ctl[3], (uint32_t)__popcll(pb));
+}
+
+struct ForceOut {
+    double *ax, *ay, *az, *dudt;           // += (caller order)
+    double *rho, *h, *p, *c;               // density phase results (gas)
+    double *div_v, *max_mu, *dalpha;       // ForceIntermediates (assigned)
+    double* omega;                         // may be null
+    uint32_t* nneigh;                      // may be null
+};
+
+// compute_forces (forces.rs:80-230), one lane per particle over its list.
+__global__ __launch_bounds__(256) void sph_force_kernel(
+    int n, const uint32_t* __restrict__ perm, const double4* __restrict__ pos,
+    const double4* __restrict__ vh, const double4* __restrict__ aux,
+    const double2* __restrict__ aux2, const uint8_t* __restrict__ qstate,
+    const uint32_t* __restrict__ pool, const uint2* __restrict__ nbr, gravis_sph_params P,
+    ForceOut o) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const uint32_t pi = perm[s];
+    if (qstate[s] == kQNone) {
+        o.div_v[pi] = 0.0;
+        o.max_mu[pi] = 0.0;
+        o.dalpha[pi] = 0.0;
+        if (o.omega) o.omega[pi] = 1.0;
+        if (o.nneigh) o.nneigh[pi] = 0u;
+        return;
+    }
+    const double4 me = pos[s];
+    const double4 vi = vh[s];
+    const double4 ai = aux[s];
+    const double2 bi = aux2[s];
+    const double hi = vi.w;
+    const double rho_i = fmax(ai.x, kDensityFloor);
+    const double p_i = ai.y, c_i = ai.z, alpha_i = ai.w;
+    const double omega_i = bi.x, u_i = bi.y;
+    const double pr_i = rho_i > 0.0 ? p_i / (omega_i * rho_i * rho_i) : 0.0;
+    double axs = 0.0, ays = 0.0, azs = 0.0, du = 0.0, dv = 0.0, mmu = 0.0;
+    const uint2 nb = nbr[s];
+    for (uint32_t k = 0; k < nb.y; ++k) {
+        const uint32_t j = pool[nb.x + k];
+        if (j == (uint32_t)s) continue;
+        const double4 pj = pos[j];
+        const double dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        const double4 vj = vh[j];
+        const double hj = vj.w;
+        const double havg = 0.5 * (hi + hj);
+        if (r >= 2.0 * hi && r >= 2.0 * hj) continue;
+        const double mj = pj.w;
+        const double4 aj = aux[j];
+        const double2 bj = aux2[j];
+        const double rho_j = fmax(aj.x, kDensityFloor);
+        const double p_j = aj.y, c_j = aj.z, alpha_j = aj.w, omega_j = bj.x;
+        const double dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
+        const double vr = dvx * dx + dvy * dy + dvz * dz;
+        double gi0, gi1, gi2, gj0, gj1, gj2, ga0, ga1, ga2;
+        grad_w(dx, dy, dz, r, hi, gi0, gi1, gi2);
+        grad_w(dx, dy, dz, r, hj, gj0, gj1, gj2);
+        grad_w(dx, dy, dz, r, havg, ga0, ga1, ga2);
+        const double pr_j = rho_j > 0.0 ? p_j / (omega_j * rho_j * rho_j) : 0.0;
+        double pij = 0.0;
+        if (vr < 0.0) {
+            const double aavg = 0.5 * (alpha_i + alpha_j);
+            const double cavg = 0.5 * (c_i + c_j);
+            const double ravg = 0.5 * (rho_i + rho_j);
+            const double eta2 = 0.01 * havg * havg;
+            const double mu = havg * vr / (r * r + eta2);
+            mmu = fmax(mmu, fabs(mu));
+            pij = ravg > 0.0 ? (-aavg * cavg * mu + P.beta_visc * mu * mu) / ravg : 0.0;
+        }
+        axs -= mj * (pr_i * gi0 + pr_j * gj0 + pij * ga0);
+        ays -= mj * (pr_i * gi1 + pr_j * gj1 + pij * ga1);
+        azs -= mj * (pr_i * gi2 + pr_j * gj2 + pij * ga2);
+        const double vgi = dvx * gi0 + dvy * gi1 + dvz * gi2;
+        const double vga = dvx * ga0 + dvy * ga1 + dvz * ga2;
+        du += mj * (pr_i * vgi + 0.5 * pij * vga);
+        dv -= mj * vgi;
+        if (P.alpha_cond > 0.0) {
+            const double u_j = bj.y;
+            const double rsum = rho_i + rho_j;
+            if (rsum > 0.0) {
+                const double ravg = 0.5 * rsum;
+                const double dp = fabs(p_i - p_j);
+                const double vsig = sqrt(dp / ravg);
+                const double agw = abs_grad_w(r, havg);
+                du += mj * P.alpha_cond * vsig * (u_i - u_j) / rsum * agw;
+            }
+        }
+    }
+    o.ax[pi] += axs;
+    o.ay[pi] += ays;
+    o.az[pi] += azs;
+    o.dudt[pi] += du;
+    const double divv = rho_i > 0.0 ? dv / rho_i : 0.0;
+    o.div_v[pi] = divv;
+    o.max_mu[pi] = mmu;
+    const double tau = c_i > 0.0 ? hi / (2.0 * P.sigma_visc * c_i) : 1e10;
+    const double src = fmax(-divv, 0.0) * (P.alpha_max - alpha_i);
+    o.dalpha[pi] = -(alpha_i - P.alpha_min) / tau + src;
+    o.rho[pi] = ai.x;
+    o.h[pi] = hi;
+    o.p[pi] = p_i;
+    o.c[pi] = c_i;
+    if (o.omega) o.omega[pi] = omega_i;
+    if (o.nneigh) o.nneigh[pi] = nb.y;
+}
+
+// ---- step_with_sph (solver.rs:145-212) ----------------------------------
+
+// First half kicks of v and u (gas, floored), drift, and the clears of
+// a and du/dt (solver.rs:151-177). dt from device memory.
+__global__ __launch_bounds__(256) void sph_kick_drift_kernel(
+    int n, const double* __restrict__ dtp, double* __restrict__ x, double* __restrict__ y,
+    double* __restrict__ z, double* __restrict__ vx, double* __restrict__ vy,
+    double* __restrict__ vz, double* __restrict__ ax, double* __restrict__ ay,
+    double* __restrict__ az, double* __restrict__ u, double* __restrict__ dudt,
+    const uint8_t* __restrict__ type) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double dt = *dtp;
+    const double hdt = 0.5 * dt;
+    const double vxi = vx[i] + ax[i] * hdt;
+    const double vyi = vy[i] + ay[i] * hdt;
+    const double vzi = vz[i] + az[i] * hdt;
+    vx[i] = vxi;
+    vy[i] = vyi;
+    vz[i] = vzi;
+    if (type[i] == kGas) {
+        double ui = u[i] + dudt[i] * hdt;
+        if (ui < kEnergyFloor) ui = kEnergyFloor;
+        u[i] = ui;
+    }
+    x[i] = x[i] + vxi * dt;
+    y[i] = y[i] + vyi * dt;
+    z[i] = z[i] + vzi * dt;
+    ax[i] = 0.0;
+    ay[i] = 0.0;
+    az[i] = 0.0;
+    dudt[i] = 0.0;
+}
+
+// Second half kicks, the viscosity switch update (solver.rs:191-208) and
+// the per-particle timestep of compute_timestep (solver.rs:90-131), reduced
+// to the minimum (non-negative doubles order as their bit patterns; NaN
+// candidates are ignored as f64::min ignores them).
+__global__ __launch_bounds__(256) void sph_kick_kernel(
+    int n, const double* __restrict__ dtp, double* __restrict__ vx, double* __restrict__ vy,
+    double* __restrict__ vz, const double* __restrict__ ax, const double* __restrict__ ay,
+    const double* __restrict__ az, double* __restrict__ u, const double* __restrict__ dudt,
+    double* __restrict__ alpha, const double* __restrict__ h, const double* __restrict__ cs,
+    const double* __restrict__ div_v, const double* __restrict__ max_mu,
+    const double* __restrict__ dalpha, const uint8_t* __restrict__ type, gravis_sph_params P,
+    unsigned long long* __restrict__ dtmin) {
+    __shared__ unsigned long long red[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double cand = INFINITY;
+    if (i < n) {
+        const double dt = *dtp;
+        const double hdt = 0.5 * dt;
+        const double axi = ax[i], ayi = ay[i], azi = az[i];
+        vx[i] = vx[i] + axi * hdt;
+        vy[i] = vy[i] + ayi * hdt;
+        vz[i] = vz[i] + azi * hdt;
+        if (type[i] == kGas) {
+            double ui = u[i] + dudt[i] * hdt;
+            if (ui < kEnergyFloor) ui = kEnergyFloor;
+            u[i] = ui;
+            double al = alpha[i] + dalpha[i] * dt;
+            al = al < P.alpha_min ? P.alpha_min : (al > P.alpha_max ? P.alpha_max : al);
+            alpha[i] = al;
+            const double hi = h[i];
+            const double am = sqrt(axi * axi + ayi * ayi + azi * azi);
+            if (am > 0.0) cand = fmin(cand, P.force_factor * sqrt(hi / am));
+            const double c = cs[i];
+            const double vsig = c + hi * fabs(div_v[i]) + 1.2 * (al * c + P.beta_visc * max_mu[i]);
+            if (vsig > 0.0) cand = fmin(cand, P.cfl * hi / vsig);
+        }
+    }
+    unsigned long long bits = (unsigned long long)__double_as_longlong(cand);
+    if (!(cand >= 0.0)) bits = 0x7FF0000000000000ull;  // NaN: ignored
+    red[threadIdx.x] = bits;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            const unsigned long long a = red[threadIdx.x], b = red[threadIdx.x + o];
+            red[threadIdx.x] = a < b ? a : b;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMin(dtmin, red[0]);
+}
+
+__global__ void sph_query_reset_kernel(uint32_t* __restrict__ counter, uint32_t* __restrict__ ctl) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    counter[0] = 0u;
+    ctl[1] = 0u;
+    ctl[2] = 0u;
+    ctl[3] = 0u;
+}
+
+__global__ void sph_pool_rewind_kernel(uint32_t* __restrict__ ctl, uint32_t used) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) ctl[0] = used;
+}
+
+// dt_next = max(dt_min, 1e-6) (solver.rs:130); the headless loop's
+// dt = min(dt_next, 1.5 dt) and sim_time += dt (main.rs:178-187).
+__global__ void sph_dt_kernel(unsigned long long* __restrict__ dtmin, double* __restrict__ dtp,
+                              double* __restrict__ dt_next, double* __restrict__ time) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double dm = __longlong_as_double(*dtmin);
+    const double nx = dm < kDtFloor ? kDtFloor : dm;
+    const double cur = *dtp;
+    *dt_next = nx;
+    *time = *time + cur;
+    *dtp = fmin(nx, cur * 1.5);
+    *dtmin = 0x7FF0000000000000ull;
+}
+
+__global__ void sph_dt_init_kernel(double* __restrict__ dtp, double dt, double* __restrict__ time,
+                                   double t0, unsigned long long* __restrict__ dtmin) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    *dtp = dt;
+    *time = t0;
+    *dtmin = 0x7FF0000000000000ull;
+}
+
+// Device buffer that keeps its first keep_bytes when it grows.
+gravis_status grow_keep(Ctx* c, const char* name, size_t bytes, size_t keep_bytes, void** out) {
+    DevBuf& b = c->bufs[name];
+    if (b.bytes >= bytes) {
+        *out = b.p;
+        return GRAVIS_OK;
+    }
+    void* np = nullptr;
+    const size_t want = bytes + bytes / 4;
+    GRAVIS_HIP(c, hipStreamSynchronize(c->stream));
+    hipError_t e = hipMalloc(&np, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(c, GRAVIS_OUT_OF_MEMORY, "hipMalloc(%zu) for %s: %s", want, name,
+                    hipGetErrorString(e));
+    }
+    if (b.p && keep_bytes > 0)
+        GRAVIS_HIP(c, hipMemcpyAsync(np, b.p, std::min(keep_bytes, b.bytes),
+                                     hipMemcpyDeviceToDevice, c->stream));
+    GRAVIS_HIP(c, hipStreamSynchronize(c->stream));
+    if (b.p) (void)hipFree(b.p);
+    b.p = np;
+    b.bytes = want;
+    *out = np;
+    return GRAVIS_OK;
+}
+
+}  // namespace
+
+gravis_status sph_compute(Ctx* c, const gravis_sph_params* P, const gravis_sph_state* S,
+                          const Octree* tree, double* div_v, double* max_mu, double* dalpha,
+                          double* omega, uint32_t* nneigh) {
+    const size_t n = S->n;
+    if (n == 0) return GRAVIS_OK;
+    if (n > 0x7FFFFFFFull) return fail(c, GRAVIS_INVALID_ARG, "n too large");
+    const int ni = (int)n;
+    if (!P || !(P->gamma > 0.0) || !(P->eta > 0.0) || !(P->h_max > 0.0))
+        return fail(c, GRAVIS_INVALID_ARG, "SPH parameters: gamma, eta, h_max must be > 0");
+    for (int k = 0; k < 4; ++k) c->sph_stats[k] = 0;
+    Octree own;
+    if (!tree) {
+        GRAVIS_TRY(build_octree(c, n, S->x, S->y, S->z, S->m, 16u, (uint32_t)kWave, &own));
+        tree = &own;
+    }
+    const Octree& t = *tree;
+    hipStream_t st = c->stream;
+    double4 *vh, *aux;
+    double2* aux2;
+    uint8_t *qstate, *iter;
+    double *hcur, *radius;
+    uint2* nbr;
+    uint32_t *ctl, *dfs_groups, *counter;
+    GRAVIS_TRY(buf(c, "sph_vh", n, &vh));
+    GRAVIS_TRY(buf(c, "sph_aux", n, &aux));
+    GRAVIS_TRY(buf(c, "sph_aux2", n, &aux2));
+    GRAVIS_TRY(buf(c, "sph_qstate", n, &qstate));
+    GRAVIS_TRY(buf(c, "sph_iter", n, &iter));
+    GRAVIS_TRY(buf(c, "sph_hcur", n, &hcur));
+    GRAVIS_TRY(buf(c, "sph_radius", n, &radius));
+    GRAVIS_TRY(buf(c, "sph_nbr", n, &nbr));
+    GRAVIS_TRY(buf(c, "sph_ctl", kCtlWords, &ctl));
+    GRAVIS_TRY(buf(c, "sph_dfs", n, &dfs_groups));
+    GRAVIS_TRY(buf(c, "sph_counter", 2, &counter));
+    // pool: grows (keeping what is written) when a round runs out
+    size_t pool_cap = std::max<size_t>(c->bufs["sph_pool"].bytes / sizeof(uint32_t), n * 96);
+    pool_cap = std::min<size_t>(pool_cap, 0xFFFFFFFFull);
+    void* pv = nullptr;
+    GRAVIS_TRY(grow_keep(c, "sph_pool", pool_cap * sizeof(uint32_t), 0, &pv));
+    pool_cap = std::min<size_t>(c->bufs["sph_pool"].bytes / sizeof(uint32_t), 0xFFFFFFFFull);
+    uint32_t* pool = static_cast<uint32_t*>(pv);
+
+    const unsigned g256 = grid_for(n, 256);
+    sph_prep_kernel<<<g256, 256, 0, st>>>(ni, t.perm, S->vx, S->vy, S->vz, S->h, S->rho, S->p,
+                                          S->c, S->alpha, S->u, S->type, vh, aux, aux2, qstate,
+                                          iter, hcur, radius, ctl);
+    GRAVIS_TRY(check_launch(c, "sph_prep"));
+    const int qblocks = (int)std::min<long>((long)(n / 64 / kQWaves) + 1, (long)c->num_cus * 8);
+    const int dblocks = c->num_cus * 4;
+    for (int round = 0;; ++round) {
+        if (round > 64) return fail(c, GRAVIS_INTERNAL, "SPH neighbour rounds did not settle");
+        sph_query_reset_kernel<<<1, 64, 0, st>>>(counter, ctl);
+        timer_begin(c);
+        sph_query_kernel<<<qblocks, kQWaves * kWave, 0, st>>>(
+            t.src, t.meta, t.bmin, t.bmax, t.groups, t.ngroups, counter, qstate, iter, radius,
+            nbr, pool, (uint32_t)pool_cap, ctl, dfs_groups,
+            std::min<uint32_t>(c->sph_range_cap, kQRanges));
+        sph_query_dfs_kernel<<<dblocks, kWave, 0, st>>>(t.src, t.meta, t.bmin, t.bmax, t.groups,
+                                                        dfs_groups, qstate, iter, radius, nbr,
+                                                        pool, (uint32_t)pool_cap, ctl);
+        timer_end(c, "sph_query");
+        GRAVIS_TRY(check_launch(c, "sph_query"));
+        timer_begin(c);
+        sph_density_kernel<<<g256, 256, 0, st>>>(ni, t.src, pool, nbr, qstate, iter, hcur,
+                                                 radius, vh, aux, aux2, P->gamma, P->eta,
+                                                 P->h_max, ctl);
+        timer_end(c, "sph_density");
+        GRAVIS_TRY(check_launch(c, "sph_density"));
+        uint32_t h_ctl[kCtlWords];
+        GRAVIS_HIP(c, hipMemcpyAsync(h_ctl, ctl, sizeof h_ctl, hipMemcpyDeviceToHost, st));
+        GRAVIS_HIP(c, hipStreamSynchronize(st));
+        c->sph_stats[0] += h_ctl[2];
+        c->sph_stats[1] += h_ctl[3];
+        if (h_ctl[1]) {
+            // some lists did not fit: keep the written prefix, grow, query again
+            const size_t used = std::min<size_t>(h_ctl[0], pool_cap);
+            const size_t want = std::max<size_t>(2 * pool_cap, (size_t)h_ctl[0] + n * 8);
+            if (want > 0xFFFFFFFFull)
+                return fail(c, GRAVIS_OUT_OF_MEMORY, "SPH neighbour pool exceeds 2^32 entries");
+            GRAVIS_TRY(grow_keep(c, "sph_pool", want * sizeof(uint32_t), used * sizeof(uint32_t),
+                                 &pv));
+            pool = static_cast<uint32_t*>(pv);
+            pool_cap = std::min<size_t>(c->bufs["sph_pool"].bytes / sizeof(uint32_t),
+                                        0xFFFFFFFFull);
+            sph_pool_rewind_kernel<<<1, 64, 0, st>>>(ctl, (uint32_t)used);
+            continue;
+        }
+        c->sph_stats[2] += 1;
+        if (h_ctl[3] == 0) {
+            c->sph_stats[3] = h_ctl[0];
+            break;
+        }
+    }
+    ForceOut o{S->ax, S->ay, S->az, S->dudt, S->rho, S->h, S->p, S->c,
+               div_v, max_mu, dalpha, omega, nneigh};
+    timer_begin(c);
+    sph_force_kernel<<<g256, 256, 0, st>>>(ni, t.perm, t.src, vh, aux, aux2, qstate, pool, nbr,
+                                           *P, o);
+    timer_end(c, "sph_force");
+    return check_launch(c, "sph_force");
+}
+
+gravis_status sph_run(Ctx* c, const gravis_sph_params* P, const gravis_solver* g,
+                      const gravis_sph_state* S, uint64_t nsteps, double* dt_io,
+                      double* time_io, double* dt_next_out) {
+    const size_t n = S->n;
+    if (n == 0 || nsteps == 0) return GRAVIS_OK;
+    const int ni = (int)n;
+    hipStream_t st = c->stream;
+    double *dts, *iv;
+    unsigned long long* dtmin;
+    GRAVIS_TRY(buf(c, "sph_dts", 4, &dts));  // dt, time, dt_next
+    GRAVIS_TRY(buf(c, "sph_dtmin", 1, &dtmin));
+    GRAVIS_TRY(buf(c, "sph_interm", 3 * n, &iv));
+    double *div_v = iv, *max_mu = iv + n, *dalpha = iv + 2 * n;
+    sph_dt_init_kernel<<<1, 64, 0, st>>>(dts, *dt_io, dts + 1, time_io ? *time_io : 0.0, dtmin);
+    const unsigned g256 = grid_for(n, 256);
+    const bool grav = g && g->algorithm != GRAVIS_NO_GRAVITY;
+    FinishArgs fa;
+    fa.mode = kAccumulate;
+    fa.ax = S->ax;
+    fa.ay = S->ay;
+    fa.az = S->az;
+    for (uint64_t k = 0; k < nsteps; ++k) {
+        sph_kick_drift_kernel<<<g256, 256, 0, st>>>(ni, dts, S->x, S->y, S->z, S->vx, S->vy,
+                                                    S->vz, S->ax, S->ay, S->az, S->u, S->dudt,
+                                                    S->type);
+        GRAVIS_TRY(check_launch(c, "sph_kick_drift"));
+        const Octree* shared = nullptr;
+        Octree t;
+        if (grav && g->algorithm == GRAVIS_BARNES_HUT) {
+            const uint32_t leaf = g->leaf_size ? g->leaf_size : 16u;
+            const uint32_t gs = g->group_size ? g->group_size : (uint32_t)kWave;
+            GRAVIS_TRY(build_octree(c, n, S->x, S->y, S->z, S->m, leaf, gs, &t));
+            GRAVIS_TRY(bh_walk(c, g, t, fa));
+            if (gs == (uint32_t)kWave) shared = &t;  // the same tree serves the SPH search
+        } else if (grav) {
+            GRAVIS_TRY(bf_force(c, g, n, S->x, S->y, S->z, S->m, fa));
+        }
+        GRAVIS_TRY(sph_compute(c, P, S, shared, div_v, max_mu, dalpha, nullptr, nullptr));
+        sph_kick_kernel<<<g256, 256, 0, st>>>(ni, dts, S->vx, S->vy, S->vz, S->ax, S->ay, S->az,
+                                              S->u, S->dudt, S->alpha, S->h, S->c, div_v,
+                                              max_mu, dalpha, S->type, *P, dtmin);
+        sph_dt_kernel<<<1, 64, 0, st>>>(dtmin, dts, dts + 2, dts + 1);
+        GRAVIS_TRY(check_launch(c, "sph_kick"));
+    }
+    double h[3];
+    GRAVIS_HIP(c, hipMemcpyAsync(h, dts, sizeof h, hipMemcpyDeviceToHost, st));
+    GRAVIS_HIP(c, hipStreamSynchronize(st));
+    *dt_io = h[0];
+    if (time_io) *time_io = h[1];
+    if (dt_next_out) *dt_next_out = h[2];
+    return GRAVIS_OK;
+}
+
+}  // namespace gravis

@@ -1,0 +1,364 @@
+// tree.hip -- Morton-ordered linear octree on gfx950 (shared by the
+// Barnes-Hut walk and the SPH neighbour search).
+//
+// Reference: the arena octree of crates/sim-core/src/octree.rs:145-239,
+// which inserts particles one by one (one particle per leaf) into the root
+// cube of octree.rs:14-44. Here the same root cube is cut by 63-bit Morton
+// keys, so every node is a run of the key-sorted particles and its cell is
+// the reference's cell at that depth:
+//   build   level by level: each internal node (more than `leaf` particles,
+//           level < 21) finds its child runs by binary search on the next
+//           3-bit digit; nodes are numbered breadth first, children of a
+//           node contiguous in digit order.
+//   sums    bottom up in FP64: leaves sum their particles in sorted order,
+//           internal nodes sum their children in order (m, m*x; no FMA);
+//           com = sum(m x) / sum(m). The particle bounding box of every node
+//           is kept beside it.
+//   groups  target groups of at most `group` (<= 64) particles, one per
+//           wavefront, cut from the tree (group_emit_kernel).
+#include <algorithm>
+
+#include "internal.h"
+
+namespace gravis {
+namespace {
+
+constexpr int kMaxLevel = kTreeMaxLevel;
+constexpr uint32_t kGroupRoot = kNodeGroupRoot;
+
+__global__ void tree_init_kernel(uint4* __restrict__ meta, uint32_t* __restrict__ lvl,
+                                 uint32_t n, uint32_t* __restrict__ flags,
+                                 uint32_t* __restrict__ ngroups) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (int k = 0; k < 4; ++k) flags[k] = 0u;
+    *ngroups = 0u;
+    meta[0] = make_uint4(0u, n, 0u, kGroupRoot);
+    lvl[0] = 0u;
+    lvl[1] = 1u;
+    for (int L = 1; L <= kMaxLevel + 1; ++L) {
+        lvl[2 * L] = 1u;
+        lvl[2 * L + 1] = 1u;
+    }
+}
+
+__device__ __forceinline__ uint32_t digit_of(uint64_t key, int shift) {
+    return static_cast<uint32_t>((key >> shift) & 7ull);
+}
+
+// Number of non-empty children of each node of level L.
+__global__ __launch_bounds__(kTreeBlock) void tree_count_kernel(const uint64_t* __restrict__ keys,
+                                                               const uint4* __restrict__ meta,
+                                                               const uint32_t* __restrict__ lvl,
+                                                               int L, uint32_t leaf,
+                                                               uint32_t* __restrict__ nch) {
+    const uint32_t b = lvl[2 * L], e = lvl[2 * L + 1];
+    const int shift = 3 * (kMaxLevel - (L + 1));
+    for (uint32_t v = b + blockIdx.x * kTreeBlock + threadIdx.x; v < e;
+         v += gridDim.x * kTreeBlock) {
+        const uint4 md = meta[v];
+        uint32_t c = 0;
+        if (md.y > leaf && L < kMaxLevel) {
+            uint32_t pos = md.x;
+            const uint32_t end = md.x + md.y;
+            while (pos < end) {
+                const uint32_t d = digit_of(keys[pos], shift);
+                uint32_t lo = pos + 1, hi = end;  // first index with a larger digit
+                while (lo < hi) {
+                    const uint32_t mid = lo + (hi - lo) / 2;
+                    if (digit_of(keys[mid], shift) > d) hi = mid;
+                    else lo = mid + 1;
+                }
+                ++c;
+                pos = lo;
+            }
+        }
+        nch[v - b] = c;
+    }
+}
+
+// Exclusive scan of the level's child counts (one block) and the range of
+// level L+1.
+__global__ __launch_bounds__(1024) void tree_scan_kernel(const uint32_t* __restrict__ nch,
+                                                         uint32_t* __restrict__ off,
+                                                         uint32_t* __restrict__ lvl, int L) {
+    __shared__ uint32_t s[1024];
+    const uint32_t b = lvl[2 * L], e = lvl[2 * L + 1];
+    const uint32_t m = e - b;
+    const uint32_t per = (m + 1023) / 1024;
+    const uint32_t lo = threadIdx.x * per;
+    const uint32_t hi = min(m, lo + per);
+    uint32_t sum = 0;
+    for (uint32_t i = lo; i < hi; ++i) sum += nch[i];
+    s[threadIdx.x] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const uint32_t t = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0u;
+        __syncthreads();
+        s[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t run = s[threadIdx.x] - sum;
+    for (uint32_t i = lo; i < hi; ++i) {
+        off[i] = run;
+        run += nch[i];
+    }
+    if (threadIdx.x == 1023) {
+        lvl[2 * (L + 1)] = e;
+        lvl[2 * (L + 1) + 1] = e + s[1023];
+    }
+}
+
+__global__ __launch_bounds__(kTreeBlock) void tree_write_kernel(
+    const uint64_t* __restrict__ keys, uint4* __restrict__ meta, const uint32_t* __restrict__ lvl,
+    int L, const uint32_t* __restrict__ nch, const uint32_t* __restrict__ off, uint32_t cap,
+    uint32_t gsize, uint32_t* __restrict__ flag) {
+    const uint32_t b = lvl[2 * L], e = lvl[2 * L + 1];
+    const int shift = 3 * (kMaxLevel - (L + 1));
+    for (uint32_t v = b + blockIdx.x * kTreeBlock + threadIdx.x; v < e;
+         v += gridDim.x * kTreeBlock) {
+        const uint32_t nc = nch[v - b];
+        if (nc == 0) continue;
+        const uint4 md = meta[v];
+        const uint32_t c0 = e + off[v - b];
+        if (c0 + nc > cap) {
+            atomicOr(flag, 2u);
+            continue;
+        }
+        uint32_t pos = md.x, k = 0;
+        const uint32_t end = md.x + md.y;
+        while (pos < end) {
+            const uint32_t d = digit_of(keys[pos], shift);
+            uint32_t lo = pos + 1, hi = end;
+            while (lo < hi) {
+                const uint32_t mid = lo + (hi - lo) / 2;
+                if (digit_of(keys[mid], shift) > d) hi = mid;
+                else lo = mid + 1;
+            }
+            meta[c0 + k] = make_uint4(pos, lo - pos, 0u,
+                                      (uint32_t)(L + 1) | (md.y > gsize ? kGroupRoot : 0u));
+            ++k;
+            pos = lo;
+        }
+        meta[v] = make_uint4(md.x, md.y, c0, (md.w & kGroupRoot) | (nc << 8) | (uint32_t)L);
+    }
+}
+
+// Raw FP64 sums (sum m x, sum m y, sum m z, sum m) and the particle bounding
+// box of the nodes of level L.
+__global__ __launch_bounds__(kTreeBlock) void tree_sums_kernel(const double4* __restrict__ src,
+                                                              const uint4* __restrict__ meta,
+                                                              const uint32_t* __restrict__ lvl,
+                                                              int L,
+                                                              double4* __restrict__ sums,
+                                                              double4* __restrict__ bmin,
+                                                              double4* __restrict__ bmax) {
+    const uint32_t b = lvl[2 * L], e = lvl[2 * L + 1];
+    for (uint32_t v = b + blockIdx.x * kTreeBlock + threadIdx.x; v < e;
+         v += gridDim.x * kTreeBlock) {
+        const uint4 md = meta[v];
+        const uint32_t nc = (md.w >> 8) & 255u;
+        double M = 0.0, X = 0.0, Y = 0.0, Z = 0.0;
+        double lx = INFINITY, ly = INFINITY, lz = INFINITY;
+        double hx = -INFINITY, hy = -INFINITY, hz = -INFINITY;
+        if (nc == 0) {
+            for (uint32_t k = md.x; k < md.x + md.y; ++k) {
+                const double4 p = src[k];
+                M = M + p.w;
+                X = X + p.w * p.x;
+                Y = Y + p.w * p.y;
+                Z = Z + p.w * p.z;
+                lx = fmin(lx, p.x); hx = fmax(hx, p.x);
+                ly = fmin(ly, p.y); hy = fmax(hy, p.y);
+                lz = fmin(lz, p.z); hz = fmax(hz, p.z);
+            }
+        } else {
+            for (uint32_t c = md.z; c < md.z + nc; ++c) {
+                const double4 q = sums[c];
+                M = M + q.w;
+                X = X + q.x;
+                Y = Y + q.y;
+                Z = Z + q.z;
+                const double4 a = bmin[c], b = bmax[c];
+                lx = fmin(lx, a.x); hx = fmax(hx, b.x);
+                ly = fmin(ly, a.y); hy = fmax(hy, b.y);
+                lz = fmin(lz, a.z); hz = fmax(hz, b.z);
+            }
+        }
+        sums[v] = make_double4(X, Y, Z, M);
+        bmin[v] = make_double4(lx, ly, lz, 0.0);
+        bmax[v] = make_double4(hx, hy, hz, 0.0);
+    }
+}
+
+// Node source records src[n + v] = (com, mass).
+__global__ __launch_bounds__(kTreeBlock) void tree_com_kernel(double4* __restrict__ src, int n,
+                                                             const uint4* __restrict__ meta,
+                                                             const uint32_t* __restrict__ lvl,
+                                                             const double4* __restrict__ sums) {
+    const uint32_t total = lvl[2 * (kMaxLevel + 1)];
+    for (uint32_t v = blockIdx.x * kTreeBlock + threadIdx.x; v < total;
+         v += gridDim.x * kTreeBlock) {
+        const double4 q = sums[v];
+        double4 r;
+        if (q.w != 0.0) {
+            r = make_double4(q.x / q.w, q.y / q.w, q.z / q.w, q.w);
+        } else {
+            const double4 p = src[meta[v].x];
+            r = make_double4(p.x, p.y, p.z, 0.0);
+        }
+        src[(size_t)n + v] = r;
+    }
+}
+
+// Target groups of at most gsize particles, contiguous in sorted order: the
+// children of every internal node holding more than gsize particles are
+// visited in order and consecutive children holding at most gsize are packed
+// greedily while the total stays within gsize and the packed particles'
+// bounding box is no wider (largest axis extent) than twice the widest
+// packed child's; a child leaf holding more (coincident points at level 21)
+// is cut into runs of gsize; a root holding at most gsize (or a root leaf) is
+// cut the same way. Appended in any order: each group's result depends only
+// on its own particles and the tree.
+__device__ __forceinline__ void emit_runs(uint2* groups, uint32_t* ngroups, uint32_t first,
+                                          uint32_t cnt, uint32_t gsize) {
+    const uint32_t k = (cnt + gsize - 1) / gsize;
+    const uint32_t at = atomicAdd(ngroups, k);
+    for (uint32_t q = 0; q < k; ++q)
+        groups[at + q] = make_uint2(first + q * gsize, min(gsize, cnt - q * gsize));
+}
+
+__device__ __forceinline__ double extent(double4 lo, double4 hi) {
+    return fmax(fmax(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z);
+}
+
+__global__ __launch_bounds__(kTreeBlock) void group_emit_kernel(const uint4* __restrict__ meta,
+                                                               const uint32_t* __restrict__ lvl,
+                                                               const double4* __restrict__ bmin,
+                                                               const double4* __restrict__ bmax,
+                                                               uint32_t gsize,
+                                                               uint2* __restrict__ groups,
+                                                               uint32_t* __restrict__ ngroups) {
+    const uint32_t total = lvl[2 * (kMaxLevel + 1)];
+    for (uint32_t p = blockIdx.x * kTreeBlock + threadIdx.x; p < total;
+         p += gridDim.x * kTreeBlock) {
+        const uint4 md = meta[p];
+        const uint32_t nc = (md.w >> 8) & 255u;
+        if (p == 0 && (md.y <= gsize || nc == 0)) {
+            emit_runs(groups, ngroups, md.x, md.y, gsize);
+            continue;
+        }
+        if (md.y <= gsize || nc == 0) continue;
+        uint32_t rf = 0, rc = 0;
+        double4 rlo = make_double4(0, 0, 0, 0), rhi = rlo;
+        double rmax = 0.0;
+        for (uint32_t q = 0; q < nc; ++q) {
+            const uint32_t cv = md.z + q;
+            const uint4 cm = meta[cv];
+            const uint32_t cc = (cm.w >> 8) & 255u;
+            if (cm.y <= gsize) {
+                const double4 clo = bmin[cv], chi = bmax[cv];
+                const double cext = extent(clo, chi);
+                if (rc > 0) {
+                    const double4 ulo = make_double4(fmin(rlo.x, clo.x), fmin(rlo.y, clo.y),
+                                                     fmin(rlo.z, clo.z), 0.0);
+                    const double4 uhi = make_double4(fmax(rhi.x, chi.x), fmax(rhi.y, chi.y),
+                                                     fmax(rhi.z, chi.z), 0.0);
+                    if (rc + cm.y <= gsize && extent(ulo, uhi) <= 2.0 * fmax(rmax, cext)) {
+                        rc += cm.y;
+                        rlo = ulo;
+                        rhi = uhi;
+                        rmax = fmax(rmax, cext);
+                        continue;
+                    }
+                    emit_runs(groups, ngroups, rf, rc, gsize);
+                }
+                rf = cm.x;
+                rc = cm.y;
+                rlo = clo;
+                rhi = chi;
+                rmax = cext;
+            } else {
+                if (rc > 0) emit_runs(groups, ngroups, rf, rc, gsize);
+                rc = 0;
+                if (cc == 0) emit_runs(groups, ngroups, cm.x, cm.y, gsize);
+            }
+        }
+        if (rc > 0) emit_runs(groups, ngroups, rf, rc, gsize);
+    }
+}
+
+}  // namespace
+
+gravis_status build_octree(Ctx* c, size_t n, const double* x, const double* y, const double* z,
+                           const double* m, uint32_t leaf, uint32_t group, Octree* t) {
+    if (n == 0) return fail(c, GRAVIS_INVALID_ARG, "octree of zero particles");
+    if (n > 0x7FFFFFFFull) return fail(c, GRAVIS_INVALID_ARG, "n too large");
+    if (leaf < 1 || leaf > 4096) return fail(c, GRAVIS_INVALID_ARG, "leaf_size must be 1..4096");
+    if (group < 1 || group > (uint32_t)kWave)
+        return fail(c, GRAVIS_INVALID_ARG, "group_size must be 1..64");
+    double* box;
+    GRAVIS_TRY(buf(c, "tree_box", 4, &box));
+    GRAVIS_TRY(root_box(c, n, x, y, z, box));
+    uint64_t* keys;
+    uint32_t* perm;
+    GRAVIS_TRY(morton_sort(c, n, x, y, z, box, &keys, &perm));
+
+    // Internal nodes of one level are disjoint runs of more than `leaf`
+    // particles and have at most 8 children each.
+    const size_t cap = 1 + (size_t)kMaxLevel * 8 * (n / (leaf + 1) + 1);
+    if (cap > 0x7FFFFFFFull) return fail(c, GRAVIS_INVALID_ARG, "tree too large");
+    double4 *src, *sums, *bmin, *bmax;
+    uint4* meta;
+    uint32_t *lvl, *nch, *off, *flags, *ngroups;
+    uint2* groups;
+    GRAVIS_TRY(buf(c, "tree_src", n + cap, &src));
+    GRAVIS_TRY(buf(c, "tree_sums", cap, &sums));
+    GRAVIS_TRY(buf(c, "tree_bmin", cap, &bmin));
+    GRAVIS_TRY(buf(c, "tree_bmax", cap, &bmax));
+    GRAVIS_TRY(buf(c, "tree_meta", cap, &meta));
+    GRAVIS_TRY(buf(c, "tree_lvl", 2 * (kMaxLevel + 2), &lvl));
+    GRAVIS_TRY(buf(c, "tree_nch", cap, &nch));
+    GRAVIS_TRY(buf(c, "tree_off", cap, &off));
+    GRAVIS_TRY(buf(c, "tree_flags", 4, &flags));
+    GRAVIS_TRY(buf(c, "tree_ngroups", 1, &ngroups));
+    GRAVIS_TRY(buf(c, "tree_groups", n, &groups));
+    GRAVIS_TRY(gather_sorted(c, n, x, y, z, m, perm, src));
+
+    hipStream_t st = c->stream;
+    tree_init_kernel<<<1, 64, 0, st>>>(meta, lvl, (uint32_t)n, flags, ngroups);
+    const unsigned tg = grid_for(cap, kTreeBlock, kTreeGrid);
+    for (int L = 0; L < kMaxLevel; ++L) {
+        tree_count_kernel<<<tg, kTreeBlock, 0, st>>>(keys, meta, lvl, L, leaf, nch);
+        tree_scan_kernel<<<1, 1024, 0, st>>>(nch, off, lvl, L);
+        tree_write_kernel<<<tg, kTreeBlock, 0, st>>>(keys, meta, lvl, L, nch, off, (uint32_t)cap,
+                                                    group, flags + 1);
+    }
+    // level 21 holds leaves only (zero children); its scan closes the node range
+    tree_count_kernel<<<tg, kTreeBlock, 0, st>>>(keys, meta, lvl, kMaxLevel, leaf, nch);
+    tree_scan_kernel<<<1, 1024, 0, st>>>(nch, off, lvl, kMaxLevel);
+    GRAVIS_TRY(check_launch(c, "tree build"));
+    for (int L = kMaxLevel; L >= 0; --L)
+        tree_sums_kernel<<<tg, kTreeBlock, 0, st>>>(src, meta, lvl, L, sums, bmin, bmax);
+    tree_com_kernel<<<tg, kTreeBlock, 0, st>>>(src, (int)n, meta, lvl, sums);
+    group_emit_kernel<<<tg, kTreeBlock, 0, st>>>(meta, lvl, bmin, bmax, group, groups, ngroups);
+    GRAVIS_TRY(check_launch(c, "tree sums and groups"));
+
+    t->n = n;
+    t->cap = cap;
+    t->leaf = leaf;
+    t->group = group;
+    t->box = box;
+    t->keys = keys;
+    t->perm = perm;
+    t->src = src;
+    t->meta = meta;
+    t->bmin = bmin;
+    t->bmax = bmax;
+    t->lvl = lvl;
+    t->flags = flags;
+    t->groups = groups;
+    t->ngroups = ngroups;
+    return GRAVIS_OK;
+}
+
+}  // namespace gravis

@@ -196,6 +196,77 @@ gravis_status gravis_ensemble_adjoint_device(
     double* final_state, double* grad_x, double* grad_y, double* grad_z, double* grad_vx,
     double* grad_vy, double* grad_vz, double* grad_m);
 
+/* ---- smoothed particle hydrodynamics ------------------------------------
+ * SphSolver (crates/sim-core/src/sph/solver.rs:20-53) and ForceParams
+ * (sph/forces.rs:15-41). Defaults (gravis_sph_default_params) are
+ * sim-core's: gamma 5/3, eta 1.2, h_max +inf, CFL 0.3, force factor 0.25,
+ * beta 2, alpha in [0.1, 2], conductivity 1, switch decay sigma 0.1. */
+typedef struct gravis_sph_params {
+    double gamma;        /* adiabatic index */
+    double eta;          /* h = eta (m / rho)^(1/3) */
+    double h_max;        /* upper bound on h (+inf disables) */
+    double cfl;          /* CFL safety factor */
+    double force_factor; /* acceleration timestep factor */
+    double beta_visc;    /* Monaghan beta */
+    double alpha_min;    /* Morris-Monaghan switch floor */
+    double alpha_max;    /* switch ceiling */
+    double alpha_cond;   /* Price conductivity coefficient (0 disables) */
+    double sigma_visc;   /* switch decay parameter */
+} gravis_sph_params;
+
+/* Particles (crates/sim-core/src/particle.rs:21-64): all 18 SoA arrays of n
+ * particles; type 4 = gas (particle.rs:4-10). Host or device pointers per
+ * entry point. */
+typedef struct gravis_sph_state {
+    size_t n;
+    double *x, *y, *z, *vx, *vy, *vz, *ax, *ay, *az, *m;
+    double *rho, *p, *u, *dudt, *h, *c, *alpha;
+    uint8_t* type;
+} gravis_sph_state;
+
+void gravis_sph_default_params(gravis_sph_params* out);
+
+/* SphSolver::compute (solver.rs:61-80): ball-query neighbour lists at
+ * 2.2 h (sph/neighbors.rs:56-119), the density / smoothing-length iteration
+ * with re-queries and the equation of state (sph/density.rs:42-157), then
+ * the forces (sph/forces.rs:69-232). Writes rho, h, p, c of gas particles,
+ * ADDS to ax/ay/az and dudt, and writes the ForceIntermediates div_v,
+ * max_mu, dalpha_dt (n each; 0 for non-gas). omega (grad-h factors, 1 for
+ * non-gas) and nneigh (final neighbour-list lengths) may be NULL. Host
+ * pointers. */
+gravis_status gravis_sph_compute(gravis_ctx* ctx, const gravis_sph_params* params,
+                                 const gravis_sph_state* state, double* div_v, double* max_mu,
+                                 double* dalpha_dt, double* omega, uint32_t* nneigh);
+/* Same with device pointers, asynchronous except for the synchronisations
+ * that read back the re-query count. */
+gravis_status gravis_sph_compute_device(gravis_ctx* ctx, const gravis_sph_params* params,
+                                        const gravis_sph_state* state, double* div_v,
+                                        double* max_mu, double* dalpha_dt, double* omega,
+                                        uint32_t* nneigh);
+
+/* step_with_sph (solver.rs:145-212): half kicks of v and u (floor 1e-10),
+ * drift, gravity with `gravity` (NULL or GRAVIS_NO_GRAVITY: none), SPH
+ * compute, half kicks of v and u, viscosity switch update, and the next
+ * timestep of compute_timestep (solver.rs:90-131) in *dt_next. Host
+ * pointers. */
+gravis_status gravis_sph_step(gravis_ctx* ctx, const gravis_sph_params* params,
+                              const gravis_solver* gravity, gravis_sph_state* state, double dt,
+                              double* dt_next);
+
+/* nsteps of headless's SPH loop (crates/headless/src/main.rs:178-187):
+ * each step runs step_with_sph with the current dt, then
+ * dt = min(dt_next, 1.5 dt); *dt_io holds the first dt on entry and the
+ * next step's dt on return; *time_io is advanced by the steps taken
+ * (may be NULL). State stays device resident across the steps. Host
+ * pointers. */
+gravis_status gravis_sph_run(gravis_ctx* ctx, const gravis_sph_params* params,
+                             const gravis_solver* gravity, gravis_sph_state* state,
+                             uint64_t nsteps, double* dt_io, double* time_io);
+/* Same with device state pointers (dt_io, time_io stay host scalars). */
+gravis_status gravis_sph_run_device(gravis_ctx* ctx, const gravis_sph_params* params,
+                                    const gravis_solver* gravity, gravis_sph_state* state,
+                                    uint64_t nsteps, double* dt_io, double* time_io);
+
 /* ---- introspection (no reference counterpart: test and bench hooks) ---- */
 
 /* Kernel timing: when enabled, the dominant force kernel of every call is
@@ -209,13 +280,20 @@ gravis_status gravis_kernel_time(gravis_ctx* ctx, const char* name, double* ms_t
 /* Tuning knobs (no reference counterpart): "bh_frontier_cap" = per-wave
  * breadth-first frontier capacity of the Barnes-Hut walk in nodes (default
  * 16384; a group whose frontier would exceed it is walked one target at a
- * time). */
+ * time); "sph_query_leaves" = overlapping tree leaves one wave of the SPH
+ * neighbour search holds (1..512, default 512; a group needing more is
+ * searched one target at a time, depth first). */
 gravis_status gravis_set_tuning(gravis_ctx* ctx, const char* key, int64_t value);
 
 /* Work counters of the last Barnes-Hut force call: out[0] frontier node
  * tests, out[1] accepted (monopole) nodes, out[2] leaf particles used as
  * sources, out[3] target-source pair interactions, out[4] tree nodes. */
 gravis_status gravis_bh_stats(gravis_ctx* ctx, uint64_t out[5]);
+
+/* Work counters of the last SPH compute: out[0] target groups searched
+ * depth first (LDS capacity exceeded), out[1] re-queries, out[2] query
+ * rounds, out[3] neighbour-pool entries in use. */
+gravis_status gravis_sph_stats(gravis_ctx* ctx, uint64_t out[4]);
 
 /* Morton keys (63-bit, 21 bits per axis over the root cube of
  * octree.rs:14-44) and their stable sort permutation, device computed,
@@ -242,13 +320,17 @@ gravis_status gravis_scenario_two_body(double eccentricity, double mass_ratio,
                                        double semi_major_axis, double* x, double* y, double* z,
                                        double* vx, double* vy, double* vz, double* m);
 double gravis_two_body_period(double semi_major_axis);
-/* Two Plummer spheres (n_each particles, scale radius a, mass M each;
- * seeds seed_a, seed_b) whose centres are `separation` apart along x with
- * offset `impact` along y, on a parabolic approach: relative speed
- * sqrt(2 G 2M / sqrt(separation^2 + impact^2)), centre-of-mass frame.
- * Sphere A fills indices [0, n_each), B [n_each, 2 n_each). The layout
- * follows GalaxyCollision (crates/sim-core/src/scenarios/
- * galaxy_collision.rs:134-154), with Plummer spheres in place of disks. */
+/* EvrardCollapse::generate (crates/sim-core/src/scenarios/
+ * evrard_collapse.rs:45-57) with fill_uniform_gas_sphere and sphere_spacing
+ * (scenarios/mod.rs:17-51): n gas particles (type 4) uniform in a sphere of
+ * radius R by rejection from the cube [-R, R)^3 (ChaCha20Rng seeded with
+ * seed), mass M/n, h = 1.5 (V/n)^(1/3), u = u0, alpha 1, at rest; every
+ * other field 0 (Particles::add_gas, particle.rs:136-154). state->n must be
+ * n; host arrays. The suggested softening is 0.3 (V/n)^(1/3)
+ * (evrard_collapse.rs:67-70). */
+gravis_status gravis_scenario_evrard(size_t n, double total_mass, double radius, double u0,
+                                     uint64_t seed, gravis_sph_state* state);
+double gravis_evrard_softening(size_t n, double radius);
 /* Batched-ensemble layout (no reference counterpart): counts[s] =
  * lo + (u64 draw) % (hi - lo + 1) from ChaCha20Rng::seed_from_u64(seed),
  * in system order; and loss targets U[-1, 1)^3 drawn x, y, z per particle. */
@@ -256,6 +338,13 @@ gravis_status gravis_scenario_ensemble_counts(uint64_t seed, int nsys, uint32_t 
                                               uint32_t* counts);
 gravis_status gravis_scenario_targets(uint64_t seed, size_t n, double* tx, double* ty,
                                       double* tz);
+/* Two Plummer spheres (n_each particles, scale radius a, mass M each;
+ * seeds seed_a, seed_b) whose centres are `separation` apart along x with
+ * offset `impact` along y, on a parabolic approach: relative speed
+ * sqrt(2 G 2M / sqrt(separation^2 + impact^2)), centre-of-mass frame.
+ * Sphere A fills indices [0, n_each), B [n_each, 2 n_each). The layout
+ * follows GalaxyCollision (crates/sim-core/src/scenarios/
+ * galaxy_collision.rs:134-154), with Plummer spheres in place of disks. */
 gravis_status gravis_scenario_plummer_pair(size_t n_each, double scale_radius,
                                            double total_mass, uint64_t seed_a, uint64_t seed_b,
                                            double separation, double impact, double* x,

@@ -302,6 +302,8 @@ def _sph_lib():
             "orc_w": (_d, [_d, _d]),
             "orc_dw_dh": (_d, [_d, _d]),
             "orc_grad_w": (None, [_d, _d, _d, _d, _d, _dp]),
+            "orc_abs_grad_w": (_d, [_d, _d]),
+            "orc_ball_query": (_sz, [sp, _sz, _d, _u32p, _sz]),
             "orc_sph_compute": (None, [sp, pp, _dp, _dp, _dp, _dp, _u32p]),
             "orc_sph_timestep": (_d, [sp, pp, _dp, _dp]),
             "orc_sph_step": (_d, [sp, pp, _i, _d, _d, _d]),
@@ -314,6 +316,47 @@ def _sph_lib():
             f.argtypes = args
         _SPH = L
     return _SPH
+
+
+def sph_diagnostics(s: SphState, eps: float, with_potential: bool = True, time: float = 0.0,
+                    step: int = 0) -> dict:
+    """diagnostics::compute with the gas thermal energy (diagnostics.rs:37-148)."""
+    out = _z(19)
+    lib().orc_diagnostics(s.n, eps, 1 if with_potential else 0, s.x, s.y, s.z, s.vx, s.vy,
+                          s.vz, s.m, s.u.ctypes.data_as(C.c_void_p),
+                          s.type.ctypes.data_as(C.c_void_p), time, step, out)
+    return dict(zip(DIAG_FIELDS, out.tolist()))
+
+
+def sph_bf_accel(s: SphState, eps: float) -> None:
+    """BruteForce::compute_accelerations on an SPH state (accumulates)."""
+    lib().orc_bf_accel(s.n, eps, s.x, s.y, s.z, s.m, s.ax, s.ay, s.az)
+
+
+def sph_w(r: float, h: float) -> float:
+    return _sph_lib().orc_w(r, h)
+
+
+def sph_dw_dh(r: float, h: float) -> float:
+    return _sph_lib().orc_dw_dh(r, h)
+
+
+def sph_grad_w(dx: float, dy: float, dz: float, r: float, h: float) -> np.ndarray:
+    g = np.zeros(3)
+    _sph_lib().orc_grad_w(dx, dy, dz, r, h, g)
+    return g
+
+
+def sph_abs_grad_w(r: float, h: float) -> float:
+    return _sph_lib().orc_abs_grad_w(r, h)
+
+
+def ball_query(s: SphState, i: int, radius: float) -> np.ndarray:
+    """Octree::query_ball (octree.rs:245-308) from particle i, in tree order."""
+    st = s.c_struct()
+    out = np.zeros(s.n, dtype=np.uint32)
+    k = _sph_lib().orc_ball_query(C.byref(st), i, radius, out, s.n)
+    return out[:k].copy()
 
 
 def evrard(n: int, seed: int = 42, total_mass: float = 1.0, radius: float = 1.0,
@@ -351,7 +394,8 @@ def sph_timestep(s: SphState, inter: dict, params: SphParams | None = None) -> f
 
 def sph_step(s: SphState, dt: float, gravity: int = 1, eps: float = 0.0, theta: float = 0.5,
              params: SphParams | None = None) -> float:
-    """step_with_sph; gravity 0 none, 1 brute force, 2 reference Barnes-Hut."""
+    """step_with_sph; gravity 0 none, 1 brute force, 2 reference Barnes-Hut,
+    3 linear-octree group Barnes-Hut (leaf 16, group 64)."""
     params = params or SphParams.default()
     st = s.c_struct()
     return _sph_lib().orc_sph_step(C.byref(st), C.byref(params), gravity, eps, theta, dt)

@@ -106,6 +106,8 @@ void orc_grad_w(double dx, double dy, double dz, double r, double h, double* g) 
     grad_w(dx, dy, dz, r, h, g);
 }
 
+double orc_abs_grad_w(double r, double h) { return abs_grad_w(r, h); }
+
 /* ---- reference octree ball query (octree.rs:245-308) ------------------- */
 #define SNONE 0xFFFFFFFFu
 typedef struct {
@@ -247,6 +249,20 @@ static void ball(const stree* t, const orc_sph_state* s, size_t i, double radius
     out->len = 0;
     double p[3] = {s->x[i], s->y[i], s->z[i]};
     s_query(t, 0, p, radius, radius * radius, s, out);
+}
+
+/* Octree::query_ball from particle i's position over a tree of the whole
+ * state; writes at most cap indices, returns the count found. */
+size_t orc_ball_query(const orc_sph_state* s, size_t i, double radius, uint32_t* out, size_t cap) {
+    stree t = {0};
+    s_build(&t, s);
+    ulist l = {0};
+    ball(&t, s, i, radius, &l);
+    size_t k = l.len;
+    for (size_t q = 0; q < k && q < cap; ++q) out[q] = l.v[q];
+    free(l.v);
+    free(t.v);
+    return k;
 }
 
 /* ---- density.rs:42-157 ------------------------------------------------- */
@@ -415,7 +431,8 @@ double orc_sph_timestep(const orc_sph_state* s, const orc_sph_params* pp, const 
 }
 
 /* step_with_sph (solver.rs:145-212). gravity: 0 none, 1 brute force,
- * 2 reference Barnes-Hut. Returns the next dt. */
+ * 2 reference Barnes-Hut, 3 the linear-octree group walk (leaf 16, group
+ * 64). Returns the next dt. */
 double orc_sph_step(orc_sph_state* s, const orc_sph_params* pp, int gravity, double eps,
                     double theta, double dt) {
     size_t n = s->n;
@@ -430,6 +447,9 @@ double orc_sph_step(orc_sph_state* s, const orc_sph_params* pp, int gravity, dou
     for (size_t i = 0; i < n; ++i) { s->ax[i] = 0.0; s->ay[i] = 0.0; s->az[i] = 0.0; s->dudt[i] = 0.0; }
     if (gravity == 1) orc_bf_accel(n, eps, s->x, s->y, s->z, s->m, s->ax, s->ay, s->az);
     else if (gravity == 2) orc_bh_accel(n, eps, theta, s->x, s->y, s->z, s->m, s->ax, s->ay, s->az, 0);
+    else if (gravity == 3)
+        orc_bh_group_accel(n, eps, theta, 16, 64, 16384, s->x, s->y, s->z, s->m, s->ax, s->ay,
+                           s->az, 0);
     double* tmp = (double*)malloc(3 * n * sizeof(double));
     orc_sph_compute(s, pp, tmp, tmp + n, tmp + 2 * n, NULL, NULL);
     for (size_t i = 0; i < n; ++i) { s->vx[i] += s->ax[i] * h; s->vy[i] += s->ay[i] * h; s->vz[i] += s->az[i] * h; }
@@ -438,7 +458,9 @@ double orc_sph_step(orc_sph_state* s, const orc_sph_params* pp, int gravity, dou
             s->u[i] += s->dudt[i] * h;
             if (s->u[i] < ENERGY_FLOOR) s->u[i] = ENERGY_FLOOR;
             s->alpha[i] += tmp[2 * n + i] * dt;
-            s->alpha[i] = fmin(fmax(s->alpha[i], pp->alpha_min), pp->alpha_max);
+            /* f64::clamp (solver.rs:205-207): NaN stays NaN */
+            double al = s->alpha[i];
+            s->alpha[i] = al < pp->alpha_min ? pp->alpha_min : (al > pp->alpha_max ? pp->alpha_max : al);
         }
     double dtn = orc_sph_timestep(s, pp, tmp, tmp + n);
     free(tmp);

@@ -101,3 +101,20 @@ def test_ensemble_layout_matches_oracle():
     o = O.ens_targets(10007, 999)
     for a, b in zip(t, o):
         assert np.array_equal(a, b)
+
+
+def test_scenario_evrard_bit_exact_with_oracle():
+    """gravis_scenario_evrard (host code in the library) reproduces the
+    oracle's EvrardCollapse::generate bit for bit (same ChaCha20 stream)."""
+    import numpy as np
+    import gravis_amd as g
+    import oracle as O
+    for n, seed in ((3000, 11), (257, 42)):
+        p = g.scenarios.evrard(n, seed=seed)
+        s = O.evrard(n, seed=seed)
+        for a, b in (("x", "x"), ("y", "y"), ("z", "z"), ("mass", "m"),
+                     ("smoothing_length", "h"), ("internal_energy", "u"), ("alpha_visc", "alpha")):
+            assert np.array_equal(getattr(p, a), getattr(s, b)), (n, a)
+        assert np.array_equal(p.particle_type, s.type)
+        assert not p.vx.any() and not p.density.any()
+        assert g.scenarios.evrard_softening(n) == O.evrard_softening(n)
