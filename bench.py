@@ -54,6 +54,11 @@ def parse():
     ap.add_argument("--ens-steps", type=int, default=200)
     ap.add_argument("--ens-adj-steps", type=int, default=64)
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--sph-n", type=int, default=1048576)
+    ap.add_argument("--sph-steps", type=int, default=20)
+    ap.add_argument("--sph-warmup", type=int, default=3)
+    ap.add_argument("--sph-cpu-sample", type=int, default=65536)
+    ap.add_argument("--no-sph", action="store_true")
     ap.add_argument("--probe-copy", action="store_true",
                     help="only run a 1 GiB device copy (PMC byte-counter calibration)")
     return ap.parse_args()
@@ -225,6 +230,94 @@ def run_ensemble(args, g, torch, ctx):
                         "mixed brute force, hipGraph step"}
 
 
+def run_sph(args, g, torch, ctx):
+    """Evrard collapse (EvrardCollapse::generate, seed 42: n gas particles
+    uniform in R = 1, M = 1, u0 = 0.05) with sim-core's SPH (Wendland C2,
+    fixed-point h, Monaghan viscosity with the Morris-Monaghan switch, Price
+    conductivity) and FP64 Barnes-Hut self-gravity (theta 0.5, leaf 16,
+    eps = suggested_softening), stepped with headless's rule
+    dt = min(dt_next, 1.5 dt) from dt = 1e-3. State stays in HBM."""
+    import ctypes as C
+    from gravis_amd import _lib
+    from gravis_amd.sph import SphSolver
+    n = args.sph_n
+    dev = torch.device("cuda:0")
+    log(f"bench: SPH Evrard N={n}; generating ICs")
+    p = g.scenarios.evrard(n)
+    eps = g.scenarios.evrard_softening(n)
+    fields = dict(x="x", y="y", z="z", vx="vx", vy="vy", vz="vz", ax="ax", ay="ay", az="az",
+                  m="mass", rho="density", p="pressure", u="internal_energy", dudt="du_dt",
+                  h="smoothing_length", c="sound_speed", alpha="alpha_visc")
+    T = {k: torch.from_numpy(getattr(p, v)).to(dev) for k, v in fields.items()}
+    T["type"] = torch.from_numpy(p.particle_type).to(dev)
+    inter = [torch.zeros(n, dtype=torch.float64, device=dev) for _ in range(3)]
+    torch.cuda.synchronize()
+    st = _lib.SphState()
+    st.n = n
+    for k in _lib.SPH_FIELDS:
+        setattr(st, k, T[k].data_ptr())
+    st.type = T["type"].data_ptr()
+    solver = SphSolver(ctx)
+    prm = solver.params()
+    bh = g.BarnesHut(eps, 0.5, 16, 64, precision="fp64", ctx=ctx)
+    d = bh.desc()
+    L = ctx._L
+    ctx.check(L.gravis_accel_device(ctx.handle, C.byref(d), n,
+                                    *[C.c_void_p(T[k].data_ptr()) for k in
+                                      ("x", "y", "z", "m", "ax", "ay", "az")]))
+    ctx.check(L.gravis_sph_compute_device(ctx.handle, C.byref(prm), C.byref(st),
+                                          *[C.c_void_p(t.data_ptr()) for t in inter],
+                                          None, None))
+    dt = C.c_double(1e-3)
+    tsim = C.c_double(0.0)
+
+    def run(steps):
+        ctx.check(L.gravis_sph_run_device(ctx.handle, C.byref(prm), C.byref(d), C.byref(st),
+                                          steps, C.byref(dt), C.byref(tsim)))
+
+    log(f"bench: {args.sph_warmup} SPH warmup steps")
+    run(args.sph_warmup)
+    torch.cuda.synchronize()
+    log(f"bench: timing {args.sph_steps} SPH steps")
+    t_start = tsim.value
+    t0 = time.perf_counter()
+    run(args.sph_steps)
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / args.sph_steps
+    sim_span = tsim.value - t_start
+    ctx.set_profiling(True)
+    run(1)
+    ctx.set_profiling(False)
+    kms = {}
+    for k in ("sph_query", "sph_density", "sph_force", "bh_walk"):
+        ms, launches = ctx.kernel_time(k)
+        kms[k] = round(ms, 3)
+    stats = solver.stats()
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle as O
+        ns = args.sph_cpu_sample
+        s = O.evrard(ns)
+        O.sph_compute(s)          # converge h first (untimed)
+        O.sph_timestep(s, {"div_v": np.zeros(ns), "max_mu": np.zeros(ns)})
+        t0 = time.perf_counter()
+        O.sph_step(s, 1e-4, gravity=0)
+        t_cpu = time.perf_counter() - t0
+        cpu = {"value": round(ns / t_cpu, 1), "unit": "particle-steps/s (SPH only)",
+               "cores": 1, "kind": "port",
+               "sample": f"one step_with_sph without gravity on a {ns}-particle Evrard sphere "
+                         f"(sim-core algorithm, f64, 1 thread, lists by the reference octree "
+                         f"ball query): {t_cpu:.2f} s"}
+    return {"n": n, "steps_per_s": round(1.0 / t_step, 3), "ms_per_step": round(t_step * 1e3, 3),
+            "particle_steps_per_s": round(n / t_step, 1),
+            "steps": args.sph_steps, "warmup": args.sph_warmup, "sim_time_span": sim_span,
+            "dt_last": dt.value, "softening": eps, "theta": 0.5,
+            "kernel_ms_one_step": kms, "last_compute_stats": stats, "cpu_baseline": cpu,
+            "workload": "Evrard collapse (uniform sphere R=1, M=1, u0=0.05, seed 42), sim-core "
+                        "SPH (Wendland C2, fixed-point h, Monaghan AV + switch, conductivity), "
+                        "FP64 Barnes-Hut gravity theta=0.5, headless dt rule"}
+
+
 def main():
     args = parse()
     if args.probe_copy:
@@ -344,6 +437,10 @@ def main():
     if not args.no_ensemble:
         ens = run_ensemble(args, g, torch, ctx)
 
+    sph = None
+    if not args.no_sph:
+        sph = run_sph(args, g, torch, ctx)
+
     traffic = traffic_from_profiles(f"bf_mixed_n{n}")
     out = {
         "metric": METRIC,
@@ -378,6 +475,7 @@ def main():
         "barnes_hut": bh,
         "barnes_hut_fp64": bh64,
         "ensemble": ens,
+        "sph": sph,
     }
     print(json.dumps(out), flush=True)
 
