@@ -385,6 +385,18 @@ class _Scenarios:
         return p
 
     @staticmethod
+    def evrard_1r(n: int, seed: int = 42, total_mass: float = 1.0, radius: float = 1.0,
+                  u0: float = 0.05, eta: float = 1.2) -> Particles:
+        """Gas sphere with rho ~ 1/r (no reference counterpart): r = R sqrt(U),
+        isotropic, at rest, h = eta (m / rho(r))^(1/3)."""
+        L = _lib.load()
+        p = Particles.zeros(n)
+        st = sph_state(p)
+        _lib.check(None, L.gravis_scenario_evrard_1r(n, total_mass, radius, u0, eta, seed,
+                                                     C.byref(st)))
+        return p
+
+    @staticmethod
     def evrard_softening(n: int, radius: float = 1.0) -> float:
         """evrard_collapse.rs:67-70: 0.3 (V/n)^(1/3)."""
         return _lib.load().gravis_evrard_softening(n, radius)

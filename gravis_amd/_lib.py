@@ -31,6 +31,7 @@ EXPORTS = (
     "gravis_scenario_targets", "gravis_sph_default_params", "gravis_sph_compute",
     "gravis_sph_compute_device", "gravis_sph_step", "gravis_sph_run", "gravis_sph_run_device",
     "gravis_sph_stats", "gravis_scenario_evrard", "gravis_evrard_softening",
+    "gravis_scenario_evrard_1r",
 )
 
 
@@ -142,6 +143,7 @@ def load() -> C.CDLL:
         "gravis_sph_stats": (st, [vp, C.POINTER(C.c_uint64 * 4)]),
         "gravis_scenario_evrard": (st, [sz, d, d, d, C.c_uint64, C.POINTER(SphState)]),
         "gravis_evrard_softening": (d, [sz, d]),
+        "gravis_scenario_evrard_1r": (st, [sz, d, d, d, d, C.c_uint64, C.POINTER(SphState)]),
     }
     for name, (res, args) in spec.items():
         f = getattr(L, name)

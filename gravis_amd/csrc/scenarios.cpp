@@ -296,3 +296,40 @@ extern "C" double gravis_evrard_softening(size_t n, double radius) {
     const double vol = 4.0 / 3.0 * kPi * (radius * radius * radius);
     return 0.3 * std::cbrt(vol / static_cast<double>(n));
 }
+
+// Variant initial conditions (no reference counterpart): rho(r) =
+// M / (2 pi R^2 r), so M(<r) = M (r/R)^2 and r = R sqrt(U), U =
+// random::<f64>(); cos(theta) in U[-1, 1) and phi in U[0, 2 pi) by
+// random_range, drawn per particle in that order; at rest, u = u0,
+// alpha 1, m = M/n, h = eta (m / rho(r))^(1/3) (r floored at 1e-6 R).
+extern "C" gravis_status gravis_scenario_evrard_1r(size_t n, double total_mass, double radius,
+                                                   double u0, double eta, uint64_t seed,
+                                                   gravis_sph_state* s) {
+    if (!s || s->n != n) return GRAVIS_INVALID_ARG;
+    if (n == 0) return GRAVIS_OK;
+    double* f[17] = {s->x, s->y, s->z, s->vx, s->vy, s->vz, s->ax, s->ay, s->az,
+                     s->m, s->rho, s->p, s->u, s->dudt, s->h, s->c, s->alpha};
+    for (double* a : f)
+        if (!a) return GRAVIS_INVALID_ARG;
+    if (!s->type || !(radius > 0.0) || !(eta > 0.0)) return GRAVIS_INVALID_ARG;
+    ChaChaStream rng(seed);
+    const double mass = total_mass / static_cast<double>(n);
+    for (size_t i = 0; i < n; ++i) {
+        const double r = radius * std::sqrt(rng.uniform01());
+        const double ct = rng.range(-1.0, 1.0);
+        const double ph = rng.range(0.0, 2.0 * kPi);
+        const double st = std::sqrt(std::fmax(0.0, 1.0 - ct * ct));
+        for (double* a : f) a[i] = 0.0;
+        s->x[i] = r * st * std::cos(ph);
+        s->y[i] = r * st * std::sin(ph);
+        s->z[i] = r * ct;
+        s->m[i] = mass;
+        s->u[i] = u0;
+        const double rr = std::fmax(r, 1e-6 * radius);
+        const double rho = total_mass / (2.0 * kPi * radius * radius * rr);
+        s->h[i] = eta * std::cbrt(mass / rho);
+        s->alpha[i] = 1.0;
+        s->type[i] = 4;
+    }
+    return GRAVIS_OK;
+}

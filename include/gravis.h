@@ -331,6 +331,14 @@ double gravis_two_body_period(double semi_major_axis);
 gravis_status gravis_scenario_evrard(size_t n, double total_mass, double radius, double u0,
                                      uint64_t seed, gravis_sph_state* state);
 double gravis_evrard_softening(size_t n, double radius);
+/* Variant initial conditions (no reference counterpart; the configuration
+ * with rho ~ 1/r): n gas particles in a sphere of radius R and mass M with
+ * rho(r) = M / (2 pi R^2 r), i.e. r = R sqrt(U) with U = random::<f64>(),
+ * cos(theta) in U[-1, 1), phi in U[0, 2 pi), drawn per particle in that
+ * order from ChaCha20Rng::seed_from_u64(seed); at rest, u = u0, alpha 1,
+ * m = M/n, h = eta (m / rho(r))^(1/3). Host arrays, state->n = n. */
+gravis_status gravis_scenario_evrard_1r(size_t n, double total_mass, double radius, double u0,
+                                        double eta, uint64_t seed, gravis_sph_state* state);
 /* Batched-ensemble layout (no reference counterpart): counts[s] =
  * lo + (u64 draw) % (hi - lo + 1) from ChaCha20Rng::seed_from_u64(seed),
  * in system order; and loss targets U[-1, 1)^3 drawn x, y, z per particle. */

@@ -249,15 +249,26 @@ class _SphStateC(C.Structure):
 
 
 class SphParams(C.Structure):
-    """SphSolver::new + ForceParams::default (solver.rs:40-53, forces.rs:30-41)."""
+    """SphSolver::new + ForceParams::default (solver.rs:40-53, forces.rs:30-41)
+    plus the build's variant knobs (0 = sim-core's choice; see
+    gravis_oracle_sph.c)."""
     _fields_ = [(k, C.c_double) for k in ("gamma", "eta", "h_max", "cfl", "force_factor",
                                           "beta_visc", "alpha_min", "alpha_max", "alpha_cond",
-                                          "sigma_visc")]
+                                          "sigma_visc", "h_tol")] + [
+        (k, C.c_int32) for k in ("kernel", "h_solver", "max_h_iter", "alpha_switch",
+                                 "pair_fp32", "soft_h")]
 
     @classmethod
     def default(cls):
         p = cls()
         _sph_lib().orc_sph_default_params(C.byref(p))
+        return p
+
+    @classmethod
+    def variant(cls):
+        """Cubic spline, Newton h to 1e-4, fixed alpha, no conduction, eps = h."""
+        p = cls()
+        _sph_lib().orc_sph_variant_params(C.byref(p))
         return p
 
 
@@ -299,6 +310,13 @@ def _sph_lib():
         pp = C.POINTER(SphParams)
         spec = {
             "orc_sph_default_params": (None, [pp]),
+            "orc_sph_variant_params": (None, [pp]),
+            "orc_cs_w": (_d, [_d, _d]),
+            "orc_cs_dw_dh": (_d, [_d, _d]),
+            "orc_cs_grad_w": (None, [_d, _d, _d, _d, _d, _dp]),
+            "orc_evrard_1r": (None, [_sz, _d, _d, _d, _d, _u64, sp]),
+            "orc_bf_accel_eps": (None, [_sz] + [_dp] * 8),
+            "orc_bh_group_accel_eps": (_u64, [_sz, _dp, _d, _u32, _u32, _u32] + [_dp] * 7 + [_i]),
             "orc_w": (_d, [_d, _d]),
             "orc_dw_dh": (_d, [_d, _d]),
             "orc_grad_w": (None, [_d, _d, _d, _d, _d, _dp]),
@@ -365,6 +383,42 @@ def evrard(n: int, seed: int = 42, total_mass: float = 1.0, radius: float = 1.0,
     st = s.c_struct()
     _sph_lib().orc_evrard(n, total_mass, radius, u0, seed, C.byref(st))
     return s
+
+
+def evrard_1r(n: int, seed: int = 42, total_mass: float = 1.0, radius: float = 1.0,
+              u0: float = 0.05, eta: float = 1.2) -> SphState:
+    """Variant ICs: rho ~ 1/r gas sphere (see gravis_oracle_sph.c)."""
+    s = SphState(n)
+    st = s.c_struct()
+    _sph_lib().orc_evrard_1r(n, total_mass, radius, u0, eta, seed, C.byref(st))
+    return s
+
+
+def cs_w(r: float, h: float) -> float:
+    return _sph_lib().orc_cs_w(r, h)
+
+
+def cs_dw_dh(r: float, h: float) -> float:
+    return _sph_lib().orc_cs_dw_dh(r, h)
+
+
+def cs_grad_w(dx, dy, dz, r, h) -> np.ndarray:
+    g = np.zeros(3)
+    _sph_lib().orc_cs_grad_w(dx, dy, dz, r, h, g)
+    return g
+
+
+def bf_accel_eps(s: SphState, eps_i: np.ndarray) -> None:
+    """Brute force with per-target softening eps_i (accumulates)."""
+    _sph_lib().orc_bf_accel_eps(s.n, np.ascontiguousarray(eps_i, dtype=np.float64), s.x, s.y,
+                                s.z, s.m, s.ax, s.ay, s.az)
+
+
+def bh_group_accel_eps(s: SphState, eps_i: np.ndarray, theta: float = 0.5) -> None:
+    """Linear-octree group walk with per-target softening (accumulates)."""
+    _sph_lib().orc_bh_group_accel_eps(s.n, np.ascontiguousarray(eps_i, dtype=np.float64),
+                                      theta, 16, 64, FRONTIER_CAP, s.x, s.y, s.z, s.m, s.ax,
+                                      s.ay, s.az, 0)
 
 
 def evrard_softening(n: int, radius: float = 1.0) -> float:

@@ -729,7 +729,8 @@ static size_t g_max_frontier = 0;
 
 /* One breadth-first walk for targets [g0, g1) with opening box (lo, hi).
  * Returns 1 (and leaves partial sums) if a next frontier exceeds fcap. */
-static int walk_box(const ltree* t, double side0, double theta2, double eps2, const double* sx,
+static int walk_box(const ltree* t, double side0, double theta2, double eps2, const double* e2s,
+                    const double* sx,
                     const double* sy, const double* sz, const double* sm, size_t g0, size_t g1,
                     const double lo[3], const double hi[3], double* ax, double* ay, double* az,
                     uint32_t* fa, uint32_t* fb, uint32_t fcap, uint64_t* ninter) {
@@ -760,7 +761,7 @@ static int walk_box(const ltree* t, double side0, double theta2, double eps2, co
                 else { srcx = sx[k]; srcy = sy[k]; srcz = sz[k]; srcm = sm[k]; }
                 for (size_t i = g0; i < g1; ++i) {
                     double dx = srcx - sx[i], dy = srcy - sy[i], dz = srcz - sz[i];
-                    double r2 = dx * dx + dy * dy + dz * dz + eps2;
+                    double r2 = dx * dx + dy * dy + dz * dz + (e2s ? e2s[i] : eps2);
                     if (r2 == 0.0) continue;
                     double inv_r3 = 1.0 / (r2 * sqrt(r2));
                     double fm = 1.0 * srcm * inv_r3;
@@ -790,7 +791,8 @@ static int walk_box(const ltree* t, double side0, double theta2, double eps2, co
  * if a breadth-first frontier would exceed fcap nodes, the group's sums are
  * discarded and every target is walked alone with its own position as the
  * box (whose frontier the opening geometry bounds). */
-static void gwalk(const ltree* t, double side0, double theta2, double eps2, const double* sx,
+static void gwalk(const ltree* t, double side0, double theta2, double eps2, const double* e2s,
+                  const double* sx,
                   const double* sy, const double* sz, const double* sm, size_t g0, size_t g1,
                   double* ax, double* ay, double* az, uint32_t* fa, uint32_t* fb, uint32_t fcap,
                   uint64_t* ninter) {
@@ -801,13 +803,13 @@ static void gwalk(const ltree* t, double side0, double theta2, double eps2, cons
         lo[2] = fmin(lo[2], sz[i]); hi[2] = fmax(hi[2], sz[i]);
     }
     for (size_t i = g0; i < g1; ++i) { ax[i] = 0; ay[i] = 0; az[i] = 0; }
-    if (!walk_box(t, side0, theta2, eps2, sx, sy, sz, sm, g0, g1, lo, hi, ax, ay, az, fa, fb, fcap,
-                  ninter))
+    if (!walk_box(t, side0, theta2, eps2, e2s, sx, sy, sz, sm, g0, g1, lo, hi, ax, ay, az, fa, fb,
+                  fcap, ninter))
         return;
     for (size_t i = g0; i < g1; ++i) {
         ax[i] = 0; ay[i] = 0; az[i] = 0;
         double p[3] = {sx[i], sy[i], sz[i]};
-        walk_box(t, side0, theta2, eps2, sx, sy, sz, sm, i, i + 1, p, p, ax, ay, az, fa, fb,
+        walk_box(t, side0, theta2, eps2, e2s, sx, sy, sz, sm, i, i + 1, p, p, ax, ay, az, fa, fb,
                  0xFFFFFFFFu, ninter);
     }
 }
@@ -898,9 +900,10 @@ static size_t lt_groups(const ltree* t, uint32_t group, uint32_t** gf, uint32_t*
 /* Full linear-octree Barnes-Hut: box, keys, sort, tree, group walk.
  * Accumulates into ax/ay/az in the caller's order. Returns the total number
  * of pair interactions evaluated (targets x sources). */
-uint64_t orc_bh_group_accel(size_t n, double eps, double theta, uint32_t leaf, uint32_t group,
-                            uint32_t fcap, const double* x, const double* y, const double* z,
-                            const double* m, double* ax, double* ay, double* az, int nthreads) {
+static uint64_t bh_group(size_t n, double eps, const double* eps_i, double theta, uint32_t leaf,
+                         uint32_t group, uint32_t fcap, const double* x, const double* y,
+                         const double* z, const double* m, double* ax, double* ay, double* az,
+                         int nthreads) {
     if (n == 0) return 0;
     double box[4];
     orc_root_box(n, x, y, z, box);
@@ -909,11 +912,12 @@ uint64_t orc_bh_group_accel(size_t n, double eps, double theta, uint32_t leaf, u
     uint32_t* perm = (uint32_t*)malloc(n * 4);
     orc_morton63(n, x, y, z, box, keys);
     orc_sort_keys(n, keys, sk, perm);
-    double* s = (double*)malloc(7 * n * 8);
+    double* s = (double*)malloc(8 * n * 8);
     double *sx = s, *sy = s + n, *sz = s + 2 * n, *sm = s + 3 * n;
-    double *bx = s + 4 * n, *by = s + 5 * n, *bz = s + 6 * n;
+    double *bx = s + 4 * n, *by = s + 5 * n, *bz = s + 6 * n, *e2 = s + 7 * n;
     for (size_t k = 0; k < n; ++k) {
         sx[k] = x[perm[k]]; sy[k] = y[perm[k]]; sz[k] = z[perm[k]]; sm[k] = m[perm[k]];
+        if (eps_i) e2[k] = eps_i[perm[k]] * eps_i[perm[k]];
     }
     ltree t = {0};
     lt_build(&t, n, sk, leaf, sx, sy, sz, sm);
@@ -935,7 +939,8 @@ uint64_t orc_bh_group_accel(size_t n, double eps, double theta, uint32_t leaf, u
 #endif
         for (long long g = 0; g < (long long)ng; ++g) {
             size_t g0 = gfirst[g], g1 = g0 + gcnt[g];
-            gwalk(&t, side0, theta2, eps2, sx, sy, sz, sm, g0, g1, bx, by, bz, fa, fb, fcap, &total);
+            gwalk(&t, side0, theta2, eps2, eps_i ? e2 : NULL, sx, sy, sz, sm, g0, g1, bx, by, bz,
+                  fa, fb, fcap, &total);
         }
         free(fa); free(fb);
     }
@@ -947,6 +952,21 @@ uint64_t orc_bh_group_accel(size_t n, double eps, double theta, uint32_t leaf, u
     lt_free(&t);
     free(keys); free(sk); free(perm); free(s);
     return total;
+}
+
+uint64_t orc_bh_group_accel(size_t n, double eps, double theta, uint32_t leaf, uint32_t group,
+                            uint32_t fcap, const double* x, const double* y, const double* z,
+                            const double* m, double* ax, double* ay, double* az, int nthreads) {
+    return bh_group(n, eps, NULL, theta, leaf, group, fcap, x, y, z, m, ax, ay, az, nthreads);
+}
+
+/* The same walk with Plummer softening eps_i[i] of each TARGET i
+ * (a_i = sum m_j d / (r^2 + eps_i^2)^{3/2}, sources as above). */
+uint64_t orc_bh_group_accel_eps(size_t n, const double* eps_i, double theta, uint32_t leaf,
+                                uint32_t group, uint32_t fcap, const double* x, const double* y,
+                                const double* z, const double* m, double* ax, double* ay,
+                                double* az, int nthreads) {
+    return bh_group(n, 0.0, eps_i, theta, leaf, group, fcap, x, y, z, m, ax, ay, az, nthreads);
 }
 
 /* ------------------------------------------------------------------------ */

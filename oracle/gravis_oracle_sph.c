@@ -14,6 +14,21 @@
  *              compute_timestep, step_with_sph)
  *   ICs        crates/sim-core/src/scenarios/evrard_collapse.rs:45-71,
  *              scenarios/mod.rs:17-51
+ *
+ * The same functions carry a build-defined physics variant (no reference
+ * counterpart), selected by the params' knobs; the defaults reproduce
+ * sim-core exactly:
+ *   kernel 1      cubic-spline M4, W = (1/pi h^3) f(q), f = 1 - 1.5q^2 +
+ *                 0.75q^3 (q<1), 0.25(2-q)^3 (1<=q<2)
+ *   h_solver 1    Newton-Raphson on rho(h) - m (eta/h)^3 = 0 to |dh|/h <
+ *                 h_tol, falling back to the fixed-point step when the
+ *                 derivative is not positive, steps clamped to [h/2, 2h];
+ *                 re-query when 2 h_new exceeds the list's radius
+ *   alpha_switch 0  alpha fixed (no Morris-Monaghan switch)
+ *   soft_h 1      gravity softening eps_i = h_i of the target
+ *   ICs           orc_evrard_1r: rho ~ 1/r (r = R sqrt(U)), h from the
+ *                 analytic density
+ * pair_fp32 only concerns the device (FP32 pair loops); the oracle is FP64.
  */
 #include <math.h>
 #include <stdint.h>
@@ -31,6 +46,9 @@
 
 #include "gravis_oracle.h"
 
+void orc_bf_accel_eps(size_t n, const double* eps_i, const double* x, const double* y,
+                      const double* z, const double* m, double* ax, double* ay, double* az);
+
 typedef struct {
     size_t n;
     double *x, *y, *z, *vx, *vy, *vz, *ax, *ay, *az, *m;
@@ -41,6 +59,8 @@ typedef struct {
 typedef struct {
     double gamma, eta, h_max, cfl, force_factor;
     double beta_visc, alpha_min, alpha_max, alpha_cond, sigma_visc;
+    double h_tol;
+    int32_t kernel, h_solver, max_h_iter, alpha_switch, pair_fp32, soft_h;
 } orc_sph_params;
 
 void orc_sph_default_params(orc_sph_params* p) {
@@ -54,6 +74,26 @@ void orc_sph_default_params(orc_sph_params* p) {
     p->alpha_max = 2.0;
     p->alpha_cond = 1.0;
     p->sigma_visc = 0.1;
+    p->h_tol = H_TOL;
+    p->kernel = 0;
+    p->h_solver = 0;
+    p->max_h_iter = MAX_H_ITER;
+    p->alpha_switch = 1;
+    p->pair_fp32 = 0;
+    p->soft_h = 0;
+}
+
+/* The physics variant: cubic spline, Newton h, fixed alpha, no conduction,
+ * eps = h (see the header). */
+void orc_sph_variant_params(orc_sph_params* p) {
+    orc_sph_default_params(p);
+    p->kernel = 1;
+    p->h_solver = 1;
+    p->h_tol = 1e-4;
+    p->max_h_iter = 30;
+    p->alpha_switch = 0;
+    p->alpha_cond = 0.0;
+    p->soft_h = 1;
 }
 
 /* ---- kernel.rs ---------------------------------------------------------- */
@@ -107,6 +147,51 @@ void orc_grad_w(double dx, double dy, double dz, double r, double h, double* g) 
 }
 
 double orc_abs_grad_w(double r, double h) { return abs_grad_w(r, h); }
+
+/* ---- cubic spline M4 (variant), support 2h, sigma = 1/pi --------------- */
+#define CS_NORM (1.0 / SPH_PI)
+static double cs_f(double q) {
+    if (q < 1.0) return 1.0 - 1.5 * q * q + 0.75 * q * q * q;
+    if (q < 2.0) { double t = 2.0 - q; return 0.25 * t * t * t; }
+    return 0.0;
+}
+static double cs_df(double q) {
+    if (q < 1.0) return -3.0 * q + 2.25 * q * q;
+    if (q < 2.0) { double t = 2.0 - q; return -0.75 * t * t; }
+    return 0.0;
+}
+double orc_cs_w(double r, double h) { return CS_NORM / (h * h * h) * cs_f(r / h); }
+double orc_cs_dw_dh(double r, double h) {
+    double q = r / h;
+    return -CS_NORM / (h * h * h * h) * (3.0 * cs_f(q) + q * cs_df(q));
+}
+static void cs_grad(double dx, double dy, double dz, double r, double h, double g[3]) {
+    g[0] = g[1] = g[2] = 0.0;
+    if (r < 1e-30) return;
+    double q = r / h;
+    if (q >= 2.0) return;
+    double f = CS_NORM / (h * h * h * h) * cs_df(q) / r;
+    g[0] = f * dx; g[1] = f * dy; g[2] = f * dz;
+}
+static double cs_abs_grad(double r, double h) {
+    if (r < 1e-30) return 0.0;
+    double q = r / h;
+    if (q >= 2.0) return 0.0;
+    return -CS_NORM / (h * h * h * h) * cs_df(q);
+}
+void orc_cs_grad_w(double dx, double dy, double dz, double r, double h, double* g) {
+    cs_grad(dx, dy, dz, r, h, g);
+}
+
+static double k_w(int k, double r, double h) { return k ? orc_cs_w(r, h) : orc_w(r, h); }
+static double k_dwdh(int k, double r, double h) { return k ? orc_cs_dw_dh(r, h) : orc_dw_dh(r, h); }
+static void k_grad(int k, double dx, double dy, double dz, double r, double h, double g[3]) {
+    if (k) cs_grad(dx, dy, dz, r, h, g);
+    else grad_w(dx, dy, dz, r, h, g);
+}
+static double k_absgrad(int k, double r, double h) {
+    return k ? cs_abs_grad(r, h) : abs_grad_w(r, h);
+}
 
 /* ---- reference octree ball query (octree.rs:245-308) ------------------- */
 #define SNONE 0xFFFFFFFFu
@@ -266,7 +351,7 @@ size_t orc_ball_query(const orc_sph_state* s, size_t i, double radius, uint32_t*
 }
 
 /* ---- density.rs:42-157 ------------------------------------------------- */
-static void density_dwdh(const orc_sph_state* s, const ulist* nb, size_t i, double h,
+static void density_dwdh(const orc_sph_state* s, const ulist* nb, size_t i, double h, int kern,
                          double* rho, double* dsum) {
     double xi = s->x[i], yi = s->y[i], zi = s->z[i];
     double r_ = 0.0, d_ = 0.0;
@@ -276,8 +361,8 @@ static void density_dwdh(const orc_sph_state* s, const ulist* nb, size_t i, doub
         double r = sqrt(dx * dx + dy * dy + dz * dz);
         if (r < 2.0 * h) {
             double mj = s->m[j];
-            r_ += mj * orc_w(r, h);
-            d_ += mj * orc_dw_dh(r, h);
+            r_ += mj * k_w(kern, r, h);
+            d_ += mj * k_dwdh(kern, r, h);
         }
     }
     *rho = r_;
@@ -289,18 +374,32 @@ static ulist* compute_density(orc_sph_state* s, const stree* t, const orc_sph_pa
                               double* omega) {
     size_t n = s->n;
     ulist* nl = (ulist*)calloc(n, sizeof(ulist));
+    double* lr = (double*)calloc(n, sizeof(double)); /* radius of each list */
     for (size_t i = 0; i < n; ++i) {
         omega[i] = 1.0;
         if (s->type[i] != GAS) continue;
-        ball(t, s, i, 2.0 * s->h[i] * 1.1, &nl[i]);
+        lr[i] = 2.0 * s->h[i] * 1.1;
+        ball(t, s, i, lr[i], &nl[i]);
     }
     for (size_t i = 0; i < n; ++i) {
         if (s->type[i] != GAS) continue;
         double h = s->h[i], mi = s->m[i];
-        for (int it = 0; it < MAX_H_ITER; ++it) {
+        for (int it = 0; it < pp->max_h_iter; ++it) {
             double rho, dsum;
-            density_dwdh(s, &nl[i], i, h, &rho, &dsum);
-            double h_new = rho > 0.0 ? fmin(pp->eta * cbrt(mi / rho), pp->h_max) : fmin(h, pp->h_max);
+            density_dwdh(s, &nl[i], i, h, pp->kernel, &rho, &dsum);
+            double h_new;
+            if (pp->h_solver == 0) {
+                h_new = rho > 0.0 ? fmin(pp->eta * cbrt(mi / rho), pp->h_max) : fmin(h, pp->h_max);
+            } else {
+                double e = pp->eta / h;
+                double rt = mi * e * e * e;
+                double gp = dsum + 3.0 * rt / h;
+                double hn = h - (rho - rt) / gp;
+                if (!(gp > 0.0) || !(rho > 0.0)) hn = rho > 0.0 ? pp->eta * cbrt(mi / rho) : 2.0 * h;
+                if (hn < 0.5 * h) hn = 0.5 * h;
+                if (hn > 2.0 * h) hn = 2.0 * h;
+                h_new = fmin(hn, pp->h_max);
+            }
             if (rho > 0.0) {
                 double dh = -h_new / (3.0 * rho);
                 omega[i] = 1.0 - dh * dsum;
@@ -308,9 +407,14 @@ static ulist* compute_density(orc_sph_state* s, const stree* t, const orc_sph_pa
             }
             s->rho[i] = rho;
             s->h[i] = h_new;
-            if (fabs(h_new - h) / fmax(h, 1e-30) < H_TOL) break;
-            double sr = 2.0 * h * 1.1;
-            if (2.0 * h_new > sr) ball(t, s, i, 2.0 * h_new * 1.1, &nl[i]);
+            if (fabs(h_new - h) / fmax(h, 1e-30) < pp->h_tol) break;
+            /* sim-core compares with the current h (density.rs:96-97); the
+             * variant with the radius the list was built with */
+            double sr = pp->h_solver == 0 ? 2.0 * h * 1.1 : lr[i];
+            if (2.0 * h_new > sr) {
+                lr[i] = 2.0 * h_new * 1.1;
+                ball(t, s, i, lr[i], &nl[i]);
+            }
             h = h_new;
         }
     }
@@ -320,6 +424,7 @@ static ulist* compute_density(orc_sph_state* s, const stree* t, const orc_sph_pa
         s->p[i] = (pp->gamma - 1.0) * rho * u;
         s->c[i] = sqrt(pp->gamma * (pp->gamma - 1.0) * fmax(u, 0.0));
     }
+    free(lr);
     return nl;
 }
 
@@ -353,9 +458,9 @@ static void compute_forces(orc_sph_state* s, const ulist* nl, const double* omeg
             double dvx = vxi - s->vx[j], dvy = vyi - s->vy[j], dvz = vzi - s->vz[j];
             double vr = dvx * dx + dvy * dy + dvz * dz;
             double gi[3], gj[3], ga[3];
-            grad_w(dx, dy, dz, r, hi, gi);
-            grad_w(dx, dy, dz, r, hj, gj);
-            grad_w(dx, dy, dz, r, havg, ga);
+            k_grad(pp->kernel, dx, dy, dz, r, hi, gi);
+            k_grad(pp->kernel, dx, dy, dz, r, hj, gj);
+            k_grad(pp->kernel, dx, dy, dz, r, havg, ga);
             double pr_j = rho_j > 0.0 ? p_j / (om_j * rho_j * rho_j) : 0.0;
             double pij = 0.0;
             if (vr < 0.0) {
@@ -380,7 +485,7 @@ static void compute_forces(orc_sph_state* s, const ulist* nl, const double* omeg
                     double ravg = 0.5 * rs;
                     double dp = fabs(p_i - p_j);
                     double vsig = sqrt(dp / ravg);
-                    double agw = abs_grad_w(r, havg);
+                    double agw = k_absgrad(pp->kernel, r, havg);
                     du += mj * pp->alpha_cond * vsig * (u_i - u_j) / rs * agw;
                 }
             }
@@ -391,7 +496,7 @@ static void compute_forces(orc_sph_state* s, const ulist* nl, const double* omeg
         max_mu[i] = mmu;
         double tau = c_i > 0.0 ? hi / (2.0 * pp->sigma_visc * c_i) : 1e10;
         double src = fmax(-div_v[i], 0.0) * (pp->alpha_max - a_i);
-        dalpha[i] = -(a_i - pp->alpha_min) / tau + src;
+        dalpha[i] = pp->alpha_switch ? -(a_i - pp->alpha_min) / tau + src : 0.0;
     }
 }
 
@@ -445,7 +550,11 @@ double orc_sph_step(orc_sph_state* s, const orc_sph_params* pp, int gravity, dou
         }
     for (size_t i = 0; i < n; ++i) { s->x[i] += s->vx[i] * dt; s->y[i] += s->vy[i] * dt; s->z[i] += s->vz[i] * dt; }
     for (size_t i = 0; i < n; ++i) { s->ax[i] = 0.0; s->ay[i] = 0.0; s->az[i] = 0.0; s->dudt[i] = 0.0; }
-    if (gravity == 1) orc_bf_accel(n, eps, s->x, s->y, s->z, s->m, s->ax, s->ay, s->az);
+    if (pp->soft_h && gravity == 1) orc_bf_accel_eps(n, s->h, s->x, s->y, s->z, s->m, s->ax, s->ay, s->az);
+    else if (pp->soft_h && gravity == 3)
+        orc_bh_group_accel_eps(n, s->h, theta, 16, 64, 16384, s->x, s->y, s->z, s->m, s->ax,
+                               s->ay, s->az, 0);
+    else if (gravity == 1) orc_bf_accel(n, eps, s->x, s->y, s->z, s->m, s->ax, s->ay, s->az);
     else if (gravity == 2) orc_bh_accel(n, eps, theta, s->x, s->y, s->z, s->m, s->ax, s->ay, s->az, 0);
     else if (gravity == 3)
         orc_bh_group_accel(n, eps, theta, 16, 64, 16384, s->x, s->y, s->z, s->m, s->ax, s->ay,
@@ -457,10 +566,13 @@ double orc_sph_step(orc_sph_state* s, const orc_sph_params* pp, int gravity, dou
         if (s->type[i] == GAS) {
             s->u[i] += s->dudt[i] * h;
             if (s->u[i] < ENERGY_FLOOR) s->u[i] = ENERGY_FLOOR;
-            s->alpha[i] += tmp[2 * n + i] * dt;
-            /* f64::clamp (solver.rs:205-207): NaN stays NaN */
-            double al = s->alpha[i];
-            s->alpha[i] = al < pp->alpha_min ? pp->alpha_min : (al > pp->alpha_max ? pp->alpha_max : al);
+            if (pp->alpha_switch) {
+                s->alpha[i] += tmp[2 * n + i] * dt;
+                /* f64::clamp (solver.rs:205-207): NaN stays NaN */
+                double al = s->alpha[i];
+                s->alpha[i] = al < pp->alpha_min ? pp->alpha_min
+                                                 : (al > pp->alpha_max ? pp->alpha_max : al);
+            }
         }
     double dtn = orc_sph_timestep(s, pp, tmp, tmp + n);
     free(tmp);
@@ -496,6 +608,59 @@ void orc_evrard(size_t n, double total_mass, double radius, double u0, uint64_t 
             s->alpha[cnt] = 1.0;
             cnt++;
         }
+    }
+}
+
+/* Brute force with Plummer softening eps_i of each TARGET (the variant's
+ * eps = h): a_i += sum_{j != i} m_j d / (r^2 + eps_i^2)^{3/2}, gather form,
+ * sources in index order. */
+void orc_bf_accel_eps(size_t n, const double* eps_i, const double* x, const double* y,
+                      const double* z, const double* m, double* ax, double* ay, double* az) {
+    for (size_t i = 0; i < n; ++i) {
+        double e2 = eps_i[i] * eps_i[i];
+        double a0 = 0, a1 = 0, a2 = 0;
+        for (size_t j = 0; j < n; ++j) {
+            if (j == i) continue;
+            double dx = x[j] - x[i], dy = y[j] - y[i], dz = z[j] - z[i];
+            double r2 = dx * dx + dy * dy + dz * dz + e2;
+            double w = 1.0 / (r2 * sqrt(r2));
+            double f = m[j] * w;
+            a0 += f * dx; a1 += f * dy; a2 += f * dz;
+        }
+        ax[i] += a0; ay[i] += a1; az[i] += a2;
+    }
+}
+
+/* Variant initial conditions: n gas particles in a sphere of radius R and
+ * mass M with rho(r) = M / (2 pi R^2 r), i.e. M(<r) = M (r/R)^2, sampled as
+ * r = R sqrt(U) with U = random::<f64>(), cos(theta) in U[-1, 1),
+ * phi in U[0, 2 pi) (rand's random_range), drawn in that order per particle
+ * from ChaCha20Rng::seed_from_u64(seed); at rest, u = u0, alpha 1, m = M/n,
+ * h = eta (m / rho(r))^(1/3) from the analytic density (r floored at
+ * 1e-6 R). */
+void orc_evrard_1r(size_t n, double total_mass, double radius, double u0, double eta,
+                   uint64_t seed, orc_sph_state* s) {
+    orc_rng rng;
+    orc_rng_seed_u64(&rng, seed);
+    double mass = total_mass / (double)n;
+    for (size_t i = 0; i < n; ++i) {
+        double r = radius * sqrt(orc_rng_f64(&rng));
+        double ct = orc_rng_range(&rng, -1.0, 1.0);
+        double ph = orc_rng_range(&rng, 0.0, 2.0 * SPH_PI);
+        double st = sqrt(fmax(0.0, 1.0 - ct * ct));
+        s->x[i] = r * st * cos(ph);
+        s->y[i] = r * st * sin(ph);
+        s->z[i] = r * ct;
+        s->vx[i] = s->vy[i] = s->vz[i] = 0.0;
+        s->ax[i] = s->ay[i] = s->az[i] = 0.0;
+        s->m[i] = mass;
+        s->type[i] = GAS;
+        s->rho[i] = s->p[i] = s->dudt[i] = s->c[i] = 0.0;
+        s->u[i] = u0;
+        double rr = fmax(r, 1e-6 * radius);
+        double rho = total_mass / (2.0 * SPH_PI * radius * radius * rr);
+        s->h[i] = eta * cbrt(mass / rho);
+        s->alpha[i] = 1.0;
     }
 }
 

@@ -118,3 +118,17 @@ def test_scenario_evrard_bit_exact_with_oracle():
         assert np.array_equal(p.particle_type, s.type)
         assert not p.vx.any() and not p.density.any()
         assert g.scenarios.evrard_softening(n) == O.evrard_softening(n)
+
+
+def test_scenario_evrard_1r_bit_exact_with_oracle():
+    """The rho ~ 1/r generator in the library equals the oracle's bit for bit."""
+    import numpy as np
+    import gravis_amd as g
+    import oracle as O
+    for n, seed in ((2000, 42), (513, 5)):
+        p = g.scenarios.evrard_1r(n, seed=seed)
+        s = O.evrard_1r(n, seed=seed)
+        for a, b in (("x", "x"), ("y", "y"), ("z", "z"), ("mass", "m"),
+                     ("smoothing_length", "h"), ("internal_energy", "u"), ("alpha_visc", "alpha")):
+            assert np.array_equal(getattr(p, a), getattr(s, b)), (n, a)
+        assert np.array_equal(p.particle_type, s.type)
