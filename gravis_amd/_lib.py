@@ -31,7 +31,8 @@ EXPORTS = (
     "gravis_scenario_targets", "gravis_sph_default_params", "gravis_sph_compute",
     "gravis_sph_compute_device", "gravis_sph_step", "gravis_sph_run", "gravis_sph_run_device",
     "gravis_sph_stats", "gravis_scenario_evrard", "gravis_evrard_softening",
-    "gravis_scenario_evrard_1r",
+    "gravis_scenario_evrard_1r", "gravis_sph_variant_params", "gravis_sph_accel",
+    "gravis_sph_accel_device",
 )
 
 
@@ -51,7 +52,9 @@ class Diag(C.Structure):
 class SphParams(C.Structure):
     _fields_ = [(k, C.c_double) for k in ("gamma", "eta", "h_max", "cfl", "force_factor",
                                            "beta_visc", "alpha_min", "alpha_max", "alpha_cond",
-                                           "sigma_visc")]
+                                           "sigma_visc", "h_tol")] + [
+        (k, C.c_int32) for k in ("kernel", "h_solver", "max_h_iter", "alpha_switch",
+                                 "pair_fp32", "soft_h")]
 
 
 SPH_FIELDS = ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "m", "rho", "p", "u", "dudt",
@@ -132,6 +135,7 @@ def load() -> C.CDLL:
                                            + [d, C.c_uint64, C.c_uint32] + [vp] * 9
                                            + [vp] * 9),
         "gravis_sph_default_params": (None, [C.POINTER(SphParams)]),
+        "gravis_sph_variant_params": (None, [C.POINTER(SphParams)]),
         "gravis_sph_compute": (st, [vp, C.POINTER(SphParams), C.POINTER(SphState)] + [vp] * 5),
         "gravis_sph_compute_device": (st, [vp, C.POINTER(SphParams), C.POINTER(SphState)]
                                       + [vp] * 5),
@@ -141,6 +145,8 @@ def load() -> C.CDLL:
         "gravis_sph_run_device": (st, [vp, C.POINTER(SphParams), vp, C.POINTER(SphState),
                                        C.c_uint64, dp, dp]),
         "gravis_sph_stats": (st, [vp, C.POINTER(C.c_uint64 * 4)]),
+        "gravis_sph_accel": (st, [vp, C.POINTER(SphParams), vp, C.POINTER(SphState)]),
+        "gravis_sph_accel_device": (st, [vp, C.POINTER(SphParams), vp, C.POINTER(SphState)]),
         "gravis_scenario_evrard": (st, [sz, d, d, d, C.c_uint64, C.POINTER(SphState)]),
         "gravis_evrard_softening": (d, [sz, d]),
         "gravis_scenario_evrard_1r": (st, [sz, d, d, d, d, C.c_uint64, C.POINTER(SphState)]),

@@ -633,6 +633,26 @@ void gravis_sph_default_params(gravis_sph_params* o) {
     o->alpha_max = 2.0;       // forces.rs:36
     o->alpha_cond = 1.0;      // forces.rs:37
     o->sigma_visc = 0.1;      // forces.rs:38
+    o->h_tol = 1e-3;          // density.rs:16
+    o->kernel = 0;
+    o->h_solver = 0;
+    o->max_h_iter = 5;        // density.rs:13
+    o->alpha_switch = 1;
+    o->pair_fp32 = 0;
+    o->soft_h = 0;
+}
+
+void gravis_sph_variant_params(gravis_sph_params* o) {
+    if (!o) return;
+    gravis_sph_default_params(o);
+    o->kernel = 1;
+    o->h_solver = 1;
+    o->h_tol = 1e-4;
+    o->max_h_iter = 30;
+    o->alpha_switch = 0;
+    o->alpha_cond = 0.0;
+    o->soft_h = 1;
+    o->pair_fp32 = 1;
 }
 
 gravis_status gravis_sph_compute_device(gravis_ctx* g, const gravis_sph_params* P,
@@ -730,6 +750,37 @@ gravis_status gravis_sph_step(gravis_ctx* g, const gravis_sph_params* P, const g
     double dtc = dt;
     GRAVIS_TRY(gravis::sph_run(c, P, s, &D, 1, &dtc, nullptr, dt_next));
     GRAVIS_TRY(sph_stage_out(c, &D, S, true));
+    GRAVIS_HIP(c, hipStreamSynchronize(c->stream));
+    return GRAVIS_OK;
+}
+
+gravis_status gravis_sph_accel_device(gravis_ctx* g, const gravis_sph_params* P,
+                                      const gravis_solver* s, const gravis_sph_state* S) {
+    if (!g) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    GRAVIS_TRY(check_sph(c, P, S));
+    if (S->n == 0 || !s || s->algorithm == GRAVIS_NO_GRAVITY) return GRAVIS_OK;
+    GRAVIS_TRY(check_solver(c, s));
+    GRAVIS_HIP(c, hipSetDevice(c->device));
+    gravis::Octree t;
+    return gravis::sph_gravity(c, P, s, S, &t, nullptr);
+}
+
+gravis_status gravis_sph_accel(gravis_ctx* g, const gravis_sph_params* P, const gravis_solver* s,
+                               gravis_sph_state* S) {
+    if (!g) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    GRAVIS_TRY(check_sph(c, P, S));
+    if (S->n == 0 || !s || s->algorithm == GRAVIS_NO_GRAVITY) return GRAVIS_OK;
+    GRAVIS_TRY(check_solver(c, s));
+    GRAVIS_HIP(c, hipSetDevice(c->device));
+    gravis_sph_state D;
+    GRAVIS_TRY(sph_stage_in(c, S, &D));
+    gravis::Octree t;
+    GRAVIS_TRY(gravis::sph_gravity(c, P, s, &D, &t, nullptr));
+    GRAVIS_TRY(d2h(c, S->ax, D.ax, S->n));
+    GRAVIS_TRY(d2h(c, S->ay, D.ay, S->n));
+    GRAVIS_TRY(d2h(c, S->az, D.az, S->n));
     GRAVIS_HIP(c, hipStreamSynchronize(c->stream));
     return GRAVIS_OK;
 }

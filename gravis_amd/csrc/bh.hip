@@ -65,13 +65,13 @@ template <bool MIXED>
 __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
     const double4* __restrict__ src, const uint4* __restrict__ meta, int n,
     const uint2* __restrict__ groups, const uint32_t* __restrict__ ngroups,
-    const double* __restrict__ box, double theta2, double eps2, uint32_t* __restrict__ frontier,
+    const double* __restrict__ box, double theta2, double eps2_all,
+    const double* __restrict__ eps2_t, uint32_t* __restrict__ frontier,
     uint32_t fcap, uint32_t* __restrict__ counter, uint32_t* __restrict__ flag,
     double* __restrict__ part, unsigned long long* __restrict__ stats) {
     __shared__ uint32_t lists[kWalkWaves][kListCap];
     __shared__ double4 stages[kWalkWaves][kWave];
     unsigned long long t_nodes = 0, t_acc = 0, t_leafp = 0, t_pairs = 0;
-    const float eps2f = (float)eps2;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
     uint32_t* list = lists[w];
@@ -94,6 +94,9 @@ __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
         const int cnt = (int)gr.y;
         const bool active = lane < cnt;
         const double4 me = src[i0 + (active ? lane : 0)];
+        // softening of this lane's target: one value, or one per target
+        const double eps2 = eps2_t ? eps2_t[i0 + (active ? lane : 0)] : eps2_all;
+        const float eps2f = (float)eps2;
         double glo[3] = {me.x, me.y, me.z}, ghi[3] = {me.x, me.y, me.z};
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1)
@@ -300,7 +303,8 @@ __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
 
 }  // namespace
 
-gravis_status bh_walk(Ctx* c, const gravis_solver* s, const Octree& t, const FinishArgs& fa) {
+gravis_status bh_walk(Ctx* c, const gravis_solver* s, const Octree& t, const FinishArgs& fa,
+                      const double* eps2_sorted) {
     const size_t n = t.n;
     const int ni = (int)n;
     hipStream_t st = c->stream;
@@ -320,12 +324,12 @@ gravis_status bh_walk(Ctx* c, const gravis_solver* s, const Octree& t, const Fin
     timer_begin(c);
     if (s->precision == GRAVIS_MIXED)
         bh_walk_kernel<true><<<blocks, kWalkWaves * kWave, 0, st>>>(
-            t.src, t.meta, ni, t.groups, t.ngroups, t.box, theta2, eps2, front, fcap, counter,
-            t.flags, part, stats);
+            t.src, t.meta, ni, t.groups, t.ngroups, t.box, theta2, eps2, eps2_sorted, front, fcap,
+            counter, t.flags, part, stats);
     else
         bh_walk_kernel<false><<<blocks, kWalkWaves * kWave, 0, st>>>(
-            t.src, t.meta, ni, t.groups, t.ngroups, t.box, theta2, eps2, front, fcap, counter,
-            t.flags, part, stats);
+            t.src, t.meta, ni, t.groups, t.ngroups, t.box, theta2, eps2, eps2_sorted, front, fcap,
+            counter, t.flags, part, stats);
     timer_end(c, "bh_walk");
     GRAVIS_TRY(check_launch(c, "bh_walk"));
     uint32_t hflags[2] = {0, 0};

@@ -170,8 +170,10 @@ gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
 gravis_status bh_force(Ctx* c, const gravis_solver* s, size_t n, const double* x,
                        const double* y, const double* z, const double* m, const FinishArgs& fa);
 // The walk alone, on a tree built by build_octree (synchronises the stream
-// to check the capacity flags).
-gravis_status bh_walk(Ctx* c, const gravis_solver* s, const Octree& t, const FinishArgs& fa);
+// to check the capacity flags). eps2_sorted (tree order, may be null): a
+// squared Plummer softening per target instead of the solver's.
+gravis_status bh_walk(Ctx* c, const gravis_solver* s, const Octree& t, const FinishArgs& fa,
+                      const double* eps2_sorted = nullptr);
 
 // ---- smoothed particle hydrodynamics (sph.hip), device pointers ---------
 // SphSolver::compute on device state S; `tree` (built on S's positions with
@@ -179,6 +181,12 @@ gravis_status bh_walk(Ctx* c, const gravis_solver* s, const Octree& t, const Fin
 gravis_status sph_compute(Ctx* c, const gravis_sph_params* P, const gravis_sph_state* S,
                           const Octree* tree, double* div_v, double* max_mu, double* dalpha,
                           double* omega, uint32_t* nneigh);
+// Gravity for an SPH state (accumulates into S's a): the solver's softening,
+// or eps_i = h_i when P->soft_h (Barnes-Hut only). With Barnes-Hut the tree
+// is built into *t and, if shared is not null and the groups have 64
+// targets, *shared = t for the neighbour search.
+gravis_status sph_gravity(Ctx* c, const gravis_sph_params* P, const gravis_solver* g,
+                          const gravis_sph_state* S, Octree* t, const Octree** shared);
 // nsteps of step_with_sph with headless's dt rule; *dt_io in/out, time_io
 // and dt_next_out may be null.
 gravis_status sph_run(Ctx* c, const gravis_sph_params* P, const gravis_solver* g,

@@ -49,9 +49,8 @@ constexpr double kNorm3d = 21.0 / (16.0 * kPi);  // kernel.rs:17
 constexpr double kDensityFloor = 1e-10;          // forces.rs:12
 constexpr double kEnergyFloor = 1e-10;           // solver.rs:13
 constexpr double kDtFloor = 1e-6;                // solver.rs:17
-constexpr int kMaxHIter = 5;                     // density.rs:13
-constexpr double kHTol = 1e-3;                   // density.rs:16
 constexpr double kPad = 1.1;                     // density.rs:52
+constexpr double kCsNorm = 1.0 / kPi;            // cubic spline M4, support 2h
 constexpr uint8_t kGas = 4;                      // particle.rs:9
 
 // per-particle progress of the density phase
@@ -114,6 +113,23 @@ __device__ __forceinline__ double abs_grad_w(double r, double h) {
     return -dwdq / h;
 }
 
+// Cubic spline M4 shape f(q) and f'(q) (support 2h): 1 - 1.5q^2 + 0.75q^3
+// for q < 1, 0.25 (2 - q)^3 for 1 <= q < 2.
+template <typename T>
+__device__ __forceinline__ void cs_f_df(T q, T& f, T& df) {
+    if (q < T(1)) {
+        f = T(1) - T(1.5) * q * q + T(0.75) * q * q * q;
+        df = T(-3) * q + T(2.25) * q * q;
+    } else if (q < T(2)) {
+        const T t = T(2) - q;
+        f = T(0.25) * t * t * t;
+        df = T(-0.75) * t * t;
+    } else {
+        f = T(0);
+        df = T(0);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Gather of the caller's arrays into sorted records, and the first query
 // radius 2 h 1.1 of every gas particle (neighbors.rs:67-70 with
@@ -153,7 +169,7 @@ __device__ __forceinline__ void query_group(
     const uint2* __restrict__ groups, uint8_t* __restrict__ qstate,
     const uint8_t* __restrict__ iter, const double* __restrict__ radius,
     uint2* __restrict__ nbr, uint32_t* __restrict__ pool, uint32_t pool_cap,
-    uint32_t* __restrict__ ctl, uint32_t* __restrict__ dfs_groups, uint32_t rcap,
+    uint32_t* __restrict__ ctl, uint32_t* __restrict__ dfs_groups, uint32_t rcap, int maxit,
     uint32_t* fa, uint32_t* fb, uint2* rg, double4* stage) {
     const uint64_t below = (1ull << lane) - 1ull;
     const uint2 gr = groups[g];
@@ -310,7 +326,7 @@ __device__ __forceinline__ void query_group(
             lds_sync();
         }
     }
-    if (act && !failed) qstate[s] = iter[s] >= kMaxHIter ? kQDone : kQDensity;
+    if (act && !failed) qstate[s] = iter[s] >= maxit ? kQDone : kQDensity;
 }
 
 __global__ __launch_bounds__(kQWaves* kWave) void sph_query_kernel(
@@ -320,7 +336,7 @@ __global__ __launch_bounds__(kQWaves* kWave) void sph_query_kernel(
     uint32_t* __restrict__ counter, uint8_t* __restrict__ qstate,
     const uint8_t* __restrict__ iter, const double* __restrict__ radius,
     uint2* __restrict__ nbr, uint32_t* __restrict__ pool, uint32_t pool_cap,
-    uint32_t* __restrict__ ctl, uint32_t* __restrict__ dfs_groups, uint32_t rcap) {
+    uint32_t* __restrict__ ctl, uint32_t* __restrict__ dfs_groups, uint32_t rcap, int maxit) {
     __shared__ uint32_t fronts[kQWaves][2][kQFront];
     __shared__ uint2 leaves[kQWaves][kQRanges];
     __shared__ double4 stages[kQWaves][kWave];
@@ -334,7 +350,7 @@ __global__ __launch_bounds__(kQWaves* kWave) void sph_query_kernel(
             __builtin_amdgcn_readfirstlane(atomicAdd(counter, lane == 0 ? 1u : 0u));
         if (g >= ng) break;
         query_group(g, lane, pos, meta, bmin, bmax, groups, qstate, iter, radius, nbr, pool,
-                    pool_cap, ctl, dfs_groups, rcap, fronts[w][0], fronts[w][1], leaves[w],
+                    pool_cap, ctl, dfs_groups, rcap, maxit, fronts[w][0], fronts[w][1], leaves[w],
                     stages[w]);
     }
 }
@@ -348,7 +364,7 @@ __global__ __launch_bounds__(kWave) void sph_query_dfs_kernel(
     const uint2* __restrict__ groups, const uint32_t* __restrict__ dfs_groups,
     uint8_t* __restrict__ qstate, const uint8_t* __restrict__ iter,
     const double* __restrict__ radius, uint2* __restrict__ nbr, uint32_t* __restrict__ pool,
-    uint32_t pool_cap, uint32_t* __restrict__ ctl) {
+    uint32_t pool_cap, uint32_t* __restrict__ ctl, int maxit) {
     __shared__ uint32_t stack[kQStack];
     const int lane = threadIdx.x;
     const uint64_t below = (1ull << lane) - 1ull;
@@ -429,19 +445,105 @@ __global__ __launch_bounds__(kWave) void sph_query_dfs_kernel(
                     }
                 }
             }
-            if (!failed && lane == 0) qstate[s] = iter[s] >= kMaxHIter ? kQDone : kQDensity;
+            if (!failed && lane == 0) qstate[s] = iter[s] >= maxit ? kQDone : kQDensity;
         }
     }
 }
 
-// The density / smoothing-length iteration of density.rs:58-107 for every
-// particle in kQDensity, then the equation of state (density.rs:111-119).
+// Density sums over a list at smoothing length h: rho = sum m_j W(r, h) and
+// S = sum m_j dW/dh(r, h) over list members with r < 2h (density.rs:130-157).
+// K = 0: Wendland C2 in sim-core's operation order (only sigma / h^3 and
+// -3 / h hoisted); K = 1: cubic spline M4 in the oracle's order. F32: the
+// separation is formed in FP64, rounded, and the pair terms are FP32 with
+// reciprocals; the sums stay FP64.
+template <int K, bool F32>
+__device__ __forceinline__ void density_sums(const double4 me, double h,
+                                             const double4* __restrict__ pos,
+                                             const uint32_t* __restrict__ pool, uint2 nb,
+                                             double& rho, double& dsum) {
+    double rs = 0.0, ds = 0.0;
+    if (!F32) {
+        const double h2 = 2.0 * h;
+        const double nh3 = (K == 0 ? kNorm3d : kCsNorm) / (h * h * h);
+        const double nh4 = kCsNorm / (h * h * h * h);
+        const double m3h = -3.0 / h;
+        for (uint32_t k = 0; k < nb.y; ++k) {
+            const double4 q4 = pos[pool[nb.x + k]];
+            const double dx = me.x - q4.x, dy = me.y - q4.y, dz = me.z - q4.z;
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            if (r < h2) {
+                const double q = r / h;
+                double wv = 0.0, dwh = 0.0;
+                if (K == 0) {
+                    if (q < 2.0) {
+                        const double sq = 1.0 - 0.5 * q;
+                        const double s3 = sq * sq * sq;
+                        const double s4 = s3 * sq;
+                        wv = nh3 * s4 * (1.0 + 2.0 * q);
+                        const double dwdq = nh3 * (-5.0 * q) * s3;
+                        dwh = m3h * wv - q / h * dwdq;
+                    }
+                } else {
+                    double f, df;
+                    cs_f_df(q, f, df);
+                    wv = nh3 * f;
+                    dwh = -nh4 * (3.0 * f + q * df);
+                }
+                rs += q4.w * wv;
+                ds += q4.w * dwh;
+            }
+        }
+    } else {
+        const float hf = (float)h;
+        const float ih = 1.0f / hf;
+        const float h2sq = 4.0f * hf * hf;
+        const float nh3 = (float)(K == 0 ? kNorm3d : kCsNorm) * ih * ih * ih;
+        const float nh4 = nh3 * ih;
+        for (uint32_t k = 0; k < nb.y; ++k) {
+            const double4 q4 = pos[pool[nb.x + k]];
+            const float dx = (float)(me.x - q4.x), dy = (float)(me.y - q4.y),
+                        dz = (float)(me.z - q4.z);
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            if (r2 < h2sq) {
+                const float r = __builtin_sqrtf(r2);
+                const float q = r * ih;
+                float wv, dwh;
+                if (K == 0) {
+                    const float sq = fmaxf(1.0f - 0.5f * q, 0.0f);
+                    const float s3 = sq * sq * sq;
+                    wv = nh3 * s3 * sq * (1.0f + 2.0f * q);
+                    dwh = -nh4 * (3.0f * s3 * sq * (1.0f + 2.0f * q) - 5.0f * q * q * s3);
+                } else {
+                    float f, df;
+                    cs_f_df(q, f, df);
+                    wv = nh3 * f;
+                    dwh = -nh4 * (3.0f * f + q * df);
+                }
+                const float mj = (float)q4.w;
+                rs += (double)(mj * wv);
+                ds += (double)(mj * dwh);
+            }
+        }
+    }
+    rho = rs;
+    dsum = ds;
+}
+
+struct DensityArgs {
+    double gamma, eta, h_max, h_tol;
+    int h_solver, max_it;
+};
+
+// The density / smoothing-length iteration of density.rs:58-107 (h_solver 0)
+// or the Newton iteration of the variant (h_solver 1) for every particle in
+// kQDensity, then the equation of state (density.rs:111-119).
+template <int K, bool F32>
 __global__ __launch_bounds__(256) void sph_density_kernel(
     int n, const double4* __restrict__ pos, const uint32_t* __restrict__ pool,
     const uint2* __restrict__ nbr, uint8_t* __restrict__ qstate, uint8_t* __restrict__ iter,
     double* __restrict__ hcur, double* __restrict__ radius, double4* __restrict__ vh,
-    double4* __restrict__ aux, double2* __restrict__ aux2, double gamma, double eta,
-    double h_max, uint32_t* __restrict__ ctl) {
+    double4* __restrict__ aux, double2* __restrict__ aux2, DensityArgs A,
+    uint32_t* __restrict__ ctl) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     bool paused = false;
     if (s < n && qstate[s] == kQDensity) {
@@ -452,40 +554,32 @@ __global__ __launch_bounds__(256) void sph_density_kernel(
         int it = iter[s];
         double omega = aux2[s].x;
         double rho = 0.0, hn = h;
-        while (it < kMaxHIter) {
-            const double h2 = 2.0 * h;
-            const double nh3 = kNorm3d / (h * h * h);
-            const double m3h = -3.0 / h;
-            double rs = 0.0, ds = 0.0;
-            for (uint32_t k = 0; k < nb.y; ++k) {
-                const double4 q4 = pos[pool[nb.x + k]];
-                const double dx = me.x - q4.x, dy = me.y - q4.y, dz = me.z - q4.z;
-                const double r = sqrt(dx * dx + dy * dy + dz * dz);
-                if (r < h2) {
-                    const double q = r / h;
-                    double wv = 0.0, dwh = 0.0;
-                    if (q < 2.0) {
-                        const double sq = 1.0 - 0.5 * q;
-                        const double s3 = sq * sq * sq;
-                        const double s4 = s3 * sq;
-                        wv = nh3 * s4 * (1.0 + 2.0 * q);
-                        const double dwdq = nh3 * (-5.0 * q) * s3;
-                        dwh = m3h * wv - q / h * dwdq;
-                    }
-                    rs += q4.w * wv;
-                    ds += q4.w * dwh;
-                }
+        while (it < A.max_it) {
+            double ds;
+            density_sums<K, F32>(me, h, pos, pool, nb, rho, ds);
+            if (A.h_solver == 0) {
+                hn = rho > 0.0 ? fmin(A.eta * cbrt(mi / rho), A.h_max) : fmin(h, A.h_max);
+            } else {
+                const double e = A.eta / h;
+                const double rt = mi * e * e * e;
+                const double gp = ds + 3.0 * rt / h;
+                double hx = h - (rho - rt) / gp;
+                if (!(gp > 0.0) || !(rho > 0.0)) hx = rho > 0.0 ? A.eta * cbrt(mi / rho) : 2.0 * h;
+                if (hx < 0.5 * h) hx = 0.5 * h;
+                if (hx > 2.0 * h) hx = 2.0 * h;
+                hn = fmin(hx, A.h_max);
             }
-            rho = rs;
-            hn = rho > 0.0 ? fmin(eta * cbrt(mi / rho), h_max) : fmin(h, h_max);
             if (rho > 0.0) {
                 const double dh = -hn / (3.0 * rho);
                 omega = 1.0 - dh * ds;
                 if (omega < 0.1) omega = 0.1;
             }
             ++it;
-            if (fabs(hn - h) / fmax(h, 1e-30) < kHTol) break;
-            if (2.0 * hn > 2.0 * h * kPad) {
+            if (fabs(hn - h) / fmax(h, 1e-30) < A.h_tol) break;
+            // sim-core compares with the current h (density.rs:96-97); the
+            // variant with the radius the list was built with
+            const double sr = A.h_solver == 0 ? 2.0 * h * kPad : radius[s];
+            if (2.0 * hn > sr) {
                 radius[s] = 2.0 * hn * kPad;
                 h = hn;
                 paused = true;
@@ -498,9 +592,9 @@ __global__ __launch_bounds__(256) void sph_density_kernel(
         const double u = aux2[s].y;
         aux2[s] = make_double2(omega, u);
         const double4 a = aux[s];
-        const bool ended = !paused || it >= kMaxHIter;
-        const double p = ended ? (gamma - 1.0) * rho * u : a.y;
-        const double c = ended ? sqrt(gamma * (gamma - 1.0) * fmax(u, 0.0)) : a.z;
+        const bool ended = !paused || it >= A.max_it;
+        const double p = ended ? (A.gamma - 1.0) * rho * u : a.y;
+        const double c = ended ? sqrt(A.gamma * (A.gamma - 1.0) * fmax(u, 0.0)) : a.z;
         aux[s] = make_double4(rho, p, c, a.w);
         hcur[s] = h;
         iter[s] = (uint8_t)it;
@@ -518,7 +612,61 @@ struct ForceOut {
     uint32_t* nneigh;                      // may be null
 };
 
+// Kernel gradient factor F with grad W = F d (d = x_i - x_j) in FP64:
+// K = 0 sim-core's grad_w (kernel.rs:44-62), K = 1 the cubic spline in the
+// oracle's order.
+template <int K>
+__device__ __forceinline__ double grad_factor(double r, double h) {
+    if (K == 0) {
+        if (r < 1e-30) return 0.0;
+        const double q = r / h;
+        if (q >= 2.0) return 0.0;
+        const double h3 = h * h * h;
+        const double sq = 1.0 - 0.5 * q;
+        const double s3 = sq * sq * sq;
+        const double dwdq = kNorm3d / h3 * (-5.0 * q) * s3;
+        return dwdq / (h * r);
+    } else {
+        if (r < 1e-30) return 0.0;
+        const double q = r / h;
+        if (q >= 2.0) return 0.0;
+        double f, df;
+        cs_f_df(q, f, df);
+        return kCsNorm / (h * h * h * h) * df / r;
+    }
+}
+
+template <int K>
+__device__ __forceinline__ double abs_grad(double r, double h) {
+    if (K == 0) return abs_grad_w(r, h);
+    if (r < 1e-30) return 0.0;
+    const double q = r / h;
+    if (q >= 2.0) return 0.0;
+    double f, df;
+    cs_f_df(q, f, df);
+    return -kCsNorm / (h * h * h * h) * df;
+}
+
+// FP32 gradient factor from r, 1/r and 1/h.
+template <int K>
+__device__ __forceinline__ float grad_factor_f(float r, float ir, float ih) {
+    const float q = r * ih;
+    if (!(q < 2.0f) || r < 1e-30f) return 0.0f;
+    const float ih2 = ih * ih;
+    if (K == 0) {
+        const float sq = 1.0f - 0.5f * q;
+        return (float)kNorm3d * ih2 * ih2 * (-5.0f * q) * sq * sq * sq * ir;
+    } else {
+        float f, df;
+        cs_f_df(q, f, df);
+        return (float)kCsNorm * ih2 * ih2 * df * ir;
+    }
+}
+
 // compute_forces (forces.rs:80-230), one lane per particle over its list.
+// F32: separations formed in FP64 and rounded, pair terms in FP32, sums in
+// FP64. alpha_switch 0: dalpha/dt = 0.
+template <int K, bool F32>
 __global__ __launch_bounds__(256) void sph_force_kernel(
     int n, const uint32_t* __restrict__ perm, const double4* __restrict__ pos,
     const double4* __restrict__ vh, const double4* __restrict__ aux,
@@ -547,56 +695,114 @@ __global__ __launch_bounds__(256) void sph_force_kernel(
     const double pr_i = rho_i > 0.0 ? p_i / (omega_i * rho_i * rho_i) : 0.0;
     double axs = 0.0, ays = 0.0, azs = 0.0, du = 0.0, dv = 0.0, mmu = 0.0;
     const uint2 nb = nbr[s];
-    for (uint32_t k = 0; k < nb.y; ++k) {
-        const uint32_t j = pool[nb.x + k];
-        if (j == (uint32_t)s) continue;
-        const double4 pj = pos[j];
-        const double dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;
-        const double r = sqrt(dx * dx + dy * dy + dz * dz);
-        const double4 vj = vh[j];
-        const double hj = vj.w;
-        const double havg = 0.5 * (hi + hj);
-        if (r >= 2.0 * hi && r >= 2.0 * hj) continue;
-        const double mj = pj.w;
-        const double4 aj = aux[j];
-        const double2 bj = aux2[j];
-        const double rho_j = fmax(aj.x, kDensityFloor);
-        const double p_j = aj.y, c_j = aj.z, alpha_j = aj.w, omega_j = bj.x;
-        const double dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
-        const double vr = dvx * dx + dvy * dy + dvz * dz;
-        double gi0, gi1, gi2, gj0, gj1, gj2, ga0, ga1, ga2;
-        grad_w(dx, dy, dz, r, hi, gi0, gi1, gi2);
-        grad_w(dx, dy, dz, r, hj, gj0, gj1, gj2);
-        grad_w(dx, dy, dz, r, havg, ga0, ga1, ga2);
-        const double pr_j = rho_j > 0.0 ? p_j / (omega_j * rho_j * rho_j) : 0.0;
-        double pij = 0.0;
-        if (vr < 0.0) {
-            const double aavg = 0.5 * (alpha_i + alpha_j);
-            const double cavg = 0.5 * (c_i + c_j);
-            const double ravg = 0.5 * (rho_i + rho_j);
-            const double eta2 = 0.01 * havg * havg;
-            const double mu = havg * vr / (r * r + eta2);
-            mmu = fmax(mmu, fabs(mu));
-            pij = ravg > 0.0 ? (-aavg * cavg * mu + P.beta_visc * mu * mu) / ravg : 0.0;
-        }
-        axs -= mj * (pr_i * gi0 + pr_j * gj0 + pij * ga0);
-        ays -= mj * (pr_i * gi1 + pr_j * gj1 + pij * ga1);
-        azs -= mj * (pr_i * gi2 + pr_j * gj2 + pij * ga2);
-        const double vgi = dvx * gi0 + dvy * gi1 + dvz * gi2;
-        const double vga = dvx * ga0 + dvy * ga1 + dvz * ga2;
-        du += mj * (pr_i * vgi + 0.5 * pij * vga);
-        dv -= mj * vgi;
-        if (P.alpha_cond > 0.0) {
-            const double u_j = bj.y;
-            const double rsum = rho_i + rho_j;
-            if (rsum > 0.0) {
-                const double ravg = 0.5 * rsum;
-                const double dp = fabs(p_i - p_j);
-                const double vsig = sqrt(dp / ravg);
-                const double agw = abs_grad_w(r, havg);
-                du += mj * P.alpha_cond * vsig * (u_i - u_j) / rsum * agw;
+    if (!F32) {
+        for (uint32_t k = 0; k < nb.y; ++k) {
+            const uint32_t j = pool[nb.x + k];
+            if (j == (uint32_t)s) continue;
+            const double4 pj = pos[j];
+            const double dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            const double4 vj = vh[j];
+            const double hj = vj.w;
+            const double havg = 0.5 * (hi + hj);
+            if (r >= 2.0 * hi && r >= 2.0 * hj) continue;
+            const double mj = pj.w;
+            const double4 aj = aux[j];
+            const double2 bj = aux2[j];
+            const double rho_j = fmax(aj.x, kDensityFloor);
+            const double p_j = aj.y, c_j = aj.z, alpha_j = aj.w, omega_j = bj.x;
+            const double dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
+            const double vr = dvx * dx + dvy * dy + dvz * dz;
+            const double fi = grad_factor<K>(r, hi), fj = grad_factor<K>(r, hj),
+                         fa = grad_factor<K>(r, havg);
+            const double gi0 = fi * dx, gi1 = fi * dy, gi2 = fi * dz;
+            const double gj0 = fj * dx, gj1 = fj * dy, gj2 = fj * dz;
+            const double ga0 = fa * dx, ga1 = fa * dy, ga2 = fa * dz;
+            const double pr_j = rho_j > 0.0 ? p_j / (omega_j * rho_j * rho_j) : 0.0;
+            double pij = 0.0;
+            if (vr < 0.0) {
+                const double aavg = 0.5 * (alpha_i + alpha_j);
+                const double cavg = 0.5 * (c_i + c_j);
+                const double ravg = 0.5 * (rho_i + rho_j);
+                const double eta2 = 0.01 * havg * havg;
+                const double mu = havg * vr / (r * r + eta2);
+                mmu = fmax(mmu, fabs(mu));
+                pij = ravg > 0.0 ? (-aavg * cavg * mu + P.beta_visc * mu * mu) / ravg : 0.0;
+            }
+            axs -= mj * (pr_i * gi0 + pr_j * gj0 + pij * ga0);
+            ays -= mj * (pr_i * gi1 + pr_j * gj1 + pij * ga1);
+            azs -= mj * (pr_i * gi2 + pr_j * gj2 + pij * ga2);
+            const double vgi = dvx * gi0 + dvy * gi1 + dvz * gi2;
+            const double vga = dvx * ga0 + dvy * ga1 + dvz * ga2;
+            du += mj * (pr_i * vgi + 0.5 * pij * vga);
+            dv -= mj * vgi;
+            if (P.alpha_cond > 0.0) {
+                const double u_j = bj.y;
+                const double rsum = rho_i + rho_j;
+                if (rsum > 0.0) {
+                    const double ravg = 0.5 * rsum;
+                    const double dp = fabs(p_i - p_j);
+                    const double vsig = sqrt(dp / ravg);
+                    const double agw = abs_grad<K>(r, havg);
+                    du += mj * P.alpha_cond * vsig * (u_i - u_j) / rsum * agw;
+                }
             }
         }
+    } else {
+        const float hif = (float)hi, ihi = 1.0f / hif;
+        const float pri = (float)pr_i, rhoi = (float)rho_i, ci = (float)c_i;
+        const float ali = (float)alpha_i, ui = (float)u_i, pif = (float)p_i;
+        const float beta = (float)P.beta_visc, acond = (float)P.alpha_cond;
+        const float vxi = (float)vi.x, vyi = (float)vi.y, vzi = (float)vi.z;
+        float mmuf = 0.0f;
+        for (uint32_t k = 0; k < nb.y; ++k) {
+            const uint32_t j = pool[nb.x + k];
+            if (j == (uint32_t)s) continue;
+            const double4 pj = pos[j];
+            const float dx = (float)(me.x - pj.x), dy = (float)(me.y - pj.y),
+                        dz = (float)(me.z - pj.z);
+            const float r2 = dx * dx + dy * dy + dz * dz;
+            const float r = __builtin_sqrtf(r2);
+            const double4 vj = vh[j];
+            const float hj = (float)vj.w;
+            if (r >= 2.0f * hif && r >= 2.0f * hj) continue;
+            const float havg = 0.5f * (hif + hj);
+            const float mj = (float)pj.w;
+            const double4 aj = aux[j];
+            const double2 bj = aux2[j];
+            const float rhoj = fmaxf((float)aj.x, (float)kDensityFloor);
+            const float pj_ = (float)aj.y, cj = (float)aj.z, alj = (float)aj.w;
+            const float omj = (float)bj.x;
+            const float dvx = vxi - (float)vj.x, dvy = vyi - (float)vj.y, dvz = vzi - (float)vj.z;
+            const float vr = dvx * dx + dvy * dy + dvz * dz;
+            const float ir = r > 0.0f ? 1.0f / r : 0.0f;
+            const float iha = 1.0f / havg;
+            const float fi = grad_factor_f<K>(r, ir, ihi);
+            const float fj = grad_factor_f<K>(r, ir, 1.0f / hj);
+            const float fa = grad_factor_f<K>(r, ir, iha);
+            const float prj = pj_ / (omj * rhoj * rhoj);
+            float pij = 0.0f;
+            if (vr < 0.0f) {
+                const float mu = havg * vr / (r2 + 0.01f * havg * havg);
+                mmuf = fmaxf(mmuf, fabsf(mu));
+                pij = (-0.5f * (ali + alj) * 0.5f * (ci + cj) * mu + beta * mu * mu) /
+                      (0.5f * (rhoi + rhoj));
+            }
+            const float fsum = pri * fi + prj * fj + pij * fa;
+            axs -= (double)(mj * fsum * dx);
+            ays -= (double)(mj * fsum * dy);
+            azs -= (double)(mj * fsum * dz);
+            const float vgi = fi * vr;  // v_ij . grad W(h_i)
+            du += (double)(mj * (pri * vgi + 0.5f * pij * fa * vr));
+            dv -= (double)(mj * vgi);
+            if (acond > 0.0f) {
+                const float rsum = rhoi + rhoj;
+                const float vsig = __builtin_sqrtf(fabsf(pif - pj_) / (0.5f * rsum));
+                const float agw = -fa * r;  // |grad W(h_avg)|
+                du += (double)(mj * acond * vsig * (ui - (float)bj.y) / rsum * agw);
+            }
+        }
+        mmu = (double)mmuf;
     }
     o.ax[pi] += axs;
     o.ay[pi] += ays;
@@ -607,7 +813,7 @@ __global__ __launch_bounds__(256) void sph_force_kernel(
     o.max_mu[pi] = mmu;
     const double tau = c_i > 0.0 ? hi / (2.0 * P.sigma_visc * c_i) : 1e10;
     const double src = fmax(-divv, 0.0) * (P.alpha_max - alpha_i);
-    o.dalpha[pi] = -(alpha_i - P.alpha_min) / tau + src;
+    o.dalpha[pi] = P.alpha_switch ? -(alpha_i - P.alpha_min) / tau + src : 0.0;
     o.rho[pi] = ai.x;
     o.h[pi] = hi;
     o.p[pi] = p_i;
@@ -676,9 +882,12 @@ __global__ __launch_bounds__(256) void sph_kick_kernel(
             double ui = u[i] + dudt[i] * hdt;
             if (ui < kEnergyFloor) ui = kEnergyFloor;
             u[i] = ui;
-            double al = alpha[i] + dalpha[i] * dt;
-            al = al < P.alpha_min ? P.alpha_min : (al > P.alpha_max ? P.alpha_max : al);
-            alpha[i] = al;
+            double al = alpha[i];
+            if (P.alpha_switch) {
+                al = al + dalpha[i] * dt;
+                al = al < P.alpha_min ? P.alpha_min : (al > P.alpha_max ? P.alpha_max : al);
+                alpha[i] = al;
+            }
             const double hi = h[i];
             const double am = sqrt(axi * axi + ayi * ayi + azi * azi);
             if (am > 0.0) cand = fmin(cand, P.force_factor * sqrt(hi / am));
@@ -699,6 +908,17 @@ __global__ __launch_bounds__(256) void sph_kick_kernel(
         __syncthreads();
     }
     if (threadIdx.x == 0) atomicMin(dtmin, red[0]);
+}
+
+// Squared softening h^2 of every particle in the tree's order (eps = h).
+__global__ __launch_bounds__(256) void sph_eps2_kernel(int n, const uint32_t* __restrict__ perm,
+                                                       const double* __restrict__ h,
+                                                       double* __restrict__ e2) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) {
+        const double hs = h[perm[s]];
+        e2[s] = hs * hs;
+    }
 }
 
 __global__ void sph_query_reset_kernel(uint32_t* __restrict__ counter, uint32_t* __restrict__ ctl) {
@@ -771,8 +991,12 @@ gravis_status sph_compute(Ctx* c, const gravis_sph_params* P, const gravis_sph_s
     if (n == 0) return GRAVIS_OK;
     if (n > 0x7FFFFFFFull) return fail(c, GRAVIS_INVALID_ARG, "n too large");
     const int ni = (int)n;
-    if (!P || !(P->gamma > 0.0) || !(P->eta > 0.0) || !(P->h_max > 0.0))
-        return fail(c, GRAVIS_INVALID_ARG, "SPH parameters: gamma, eta, h_max must be > 0");
+    if (!P || !(P->gamma > 0.0) || !(P->eta > 0.0) || !(P->h_max > 0.0) || !(P->h_tol > 0.0))
+        return fail(c, GRAVIS_INVALID_ARG, "SPH parameters: gamma, eta, h_max, h_tol must be > 0");
+    if (P->kernel < 0 || P->kernel > 1 || P->h_solver < 0 || P->h_solver > 1 ||
+        P->max_h_iter < 1 || P->max_h_iter > 200)
+        return fail(c, GRAVIS_INVALID_ARG,
+                    "SPH parameters: kernel and h_solver must be 0 or 1, max_h_iter 1..200");
     for (int k = 0; k < 4; ++k) c->sph_stats[k] = 0;
     Octree own;
     if (!tree) {
@@ -820,16 +1044,23 @@ gravis_status sph_compute(Ctx* c, const gravis_sph_params* P, const gravis_sph_s
         sph_query_kernel<<<qblocks, kQWaves * kWave, 0, st>>>(
             t.src, t.meta, t.bmin, t.bmax, t.groups, t.ngroups, counter, qstate, iter, radius,
             nbr, pool, (uint32_t)pool_cap, ctl, dfs_groups,
-            std::min<uint32_t>(c->sph_range_cap, kQRanges));
+            std::min<uint32_t>(c->sph_range_cap, kQRanges), P->max_h_iter);
         sph_query_dfs_kernel<<<dblocks, kWave, 0, st>>>(t.src, t.meta, t.bmin, t.bmax, t.groups,
                                                         dfs_groups, qstate, iter, radius, nbr,
-                                                        pool, (uint32_t)pool_cap, ctl);
+                                                        pool, (uint32_t)pool_cap, ctl,
+                                                        P->max_h_iter);
         timer_end(c, "sph_query");
         GRAVIS_TRY(check_launch(c, "sph_query"));
         timer_begin(c);
-        sph_density_kernel<<<g256, 256, 0, st>>>(ni, t.src, pool, nbr, qstate, iter, hcur,
-                                                 radius, vh, aux, aux2, P->gamma, P->eta,
-                                                 P->h_max, ctl);
+        const DensityArgs da{P->gamma, P->eta, P->h_max, P->h_tol, P->h_solver, P->max_h_iter};
+#define GRAVIS_DENSITY(K, F) \
+    sph_density_kernel<K, F><<<g256, 256, 0, st>>>(ni, t.src, pool, nbr, qstate, iter, hcur, \
+                                                   radius, vh, aux, aux2, da, ctl)
+        if (P->kernel == 0 && !P->pair_fp32) GRAVIS_DENSITY(0, false);
+        else if (P->kernel == 0) GRAVIS_DENSITY(0, true);
+        else if (!P->pair_fp32) GRAVIS_DENSITY(1, false);
+        else GRAVIS_DENSITY(1, true);
+#undef GRAVIS_DENSITY
         timer_end(c, "sph_density");
         GRAVIS_TRY(check_launch(c, "sph_density"));
         uint32_t h_ctl[kCtlWords];
@@ -860,10 +1091,45 @@ gravis_status sph_compute(Ctx* c, const gravis_sph_params* P, const gravis_sph_s
     ForceOut o{S->ax, S->ay, S->az, S->dudt, S->rho, S->h, S->p, S->c,
                div_v, max_mu, dalpha, omega, nneigh};
     timer_begin(c);
-    sph_force_kernel<<<g256, 256, 0, st>>>(ni, t.perm, t.src, vh, aux, aux2, qstate, pool, nbr,
-                                           *P, o);
+#define GRAVIS_FORCE(K, F)                                                                \
+    sph_force_kernel<K, F><<<g256, 256, 0, st>>>(ni, t.perm, t.src, vh, aux, aux2, qstate, pool, \
+                                                 nbr, *P, o)
+    if (P->kernel == 0 && !P->pair_fp32) GRAVIS_FORCE(0, false);
+    else if (P->kernel == 0) GRAVIS_FORCE(0, true);
+    else if (!P->pair_fp32) GRAVIS_FORCE(1, false);
+    else GRAVIS_FORCE(1, true);
+#undef GRAVIS_FORCE
     timer_end(c, "sph_force");
     return check_launch(c, "sph_force");
+}
+
+gravis_status sph_gravity(Ctx* c, const gravis_sph_params* P, const gravis_solver* g,
+                          const gravis_sph_state* S, Octree* t, const Octree** shared) {
+    const size_t n = S->n;
+    if (n == 0 || !g || g->algorithm == GRAVIS_NO_GRAVITY) return GRAVIS_OK;
+    FinishArgs fa;
+    fa.mode = kAccumulate;
+    fa.ax = S->ax;
+    fa.ay = S->ay;
+    fa.az = S->az;
+    if (g->algorithm == GRAVIS_BARNES_HUT) {
+        const uint32_t leaf = g->leaf_size ? g->leaf_size : 16u;
+        const uint32_t gs = g->group_size ? g->group_size : (uint32_t)kWave;
+        GRAVIS_TRY(build_octree(c, n, S->x, S->y, S->z, S->m, leaf, gs, t));
+        double* e2 = nullptr;
+        if (P->soft_h) {
+            GRAVIS_TRY(buf(c, "sph_eps2", n, &e2));
+            sph_eps2_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>((int)n, t->perm, S->h, e2);
+            GRAVIS_TRY(check_launch(c, "sph_eps2"));
+        }
+        GRAVIS_TRY(bh_walk(c, g, *t, fa, e2));
+        if (shared && gs == (uint32_t)kWave) *shared = t;  // the same tree serves the search
+        return GRAVIS_OK;
+    }
+    if (P->soft_h)
+        return fail(c, GRAVIS_UNSUPPORTED,
+                    "gravity softening eps = h is implemented for Barnes-Hut only");
+    return bf_force(c, g, n, S->x, S->y, S->z, S->m, fa);
 }
 
 gravis_status sph_run(Ctx* c, const gravis_sph_params* P, const gravis_solver* g,
@@ -882,11 +1148,6 @@ gravis_status sph_run(Ctx* c, const gravis_sph_params* P, const gravis_solver* g
     sph_dt_init_kernel<<<1, 64, 0, st>>>(dts, *dt_io, dts + 1, time_io ? *time_io : 0.0, dtmin);
     const unsigned g256 = grid_for(n, 256);
     const bool grav = g && g->algorithm != GRAVIS_NO_GRAVITY;
-    FinishArgs fa;
-    fa.mode = kAccumulate;
-    fa.ax = S->ax;
-    fa.ay = S->ay;
-    fa.az = S->az;
     for (uint64_t k = 0; k < nsteps; ++k) {
         sph_kick_drift_kernel<<<g256, 256, 0, st>>>(ni, dts, S->x, S->y, S->z, S->vx, S->vy,
                                                     S->vz, S->ax, S->ay, S->az, S->u, S->dudt,
@@ -894,15 +1155,7 @@ gravis_status sph_run(Ctx* c, const gravis_sph_params* P, const gravis_solver* g
         GRAVIS_TRY(check_launch(c, "sph_kick_drift"));
         const Octree* shared = nullptr;
         Octree t;
-        if (grav && g->algorithm == GRAVIS_BARNES_HUT) {
-            const uint32_t leaf = g->leaf_size ? g->leaf_size : 16u;
-            const uint32_t gs = g->group_size ? g->group_size : (uint32_t)kWave;
-            GRAVIS_TRY(build_octree(c, n, S->x, S->y, S->z, S->m, leaf, gs, &t));
-            GRAVIS_TRY(bh_walk(c, g, t, fa));
-            if (gs == (uint32_t)kWave) shared = &t;  // the same tree serves the SPH search
-        } else if (grav) {
-            GRAVIS_TRY(bf_force(c, g, n, S->x, S->y, S->z, S->m, fa));
-        }
+        if (grav) GRAVIS_TRY(sph_gravity(c, P, g, S, &t, &shared));
         GRAVIS_TRY(sph_compute(c, P, S, shared, div_v, max_mu, dalpha, nullptr, nullptr));
         sph_kick_kernel<<<g256, 256, 0, st>>>(ni, dts, S->vx, S->vy, S->vz, S->ax, S->ay, S->az,
                                               S->u, S->dudt, S->alpha, S->h, S->c, div_v,

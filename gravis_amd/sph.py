@@ -19,7 +19,7 @@ import numpy as np
 from . import _lib
 from . import Context, GravitySolver, Particles, default_context, sph_state
 
-__all__ = ["SphSolver", "ForceIntermediates", "step_with_sph", "run"]
+__all__ = ["SphSolver", "ForceIntermediates", "accelerations", "step_with_sph", "run"]
 
 
 @dataclass
@@ -35,14 +35,20 @@ class ForceIntermediates:
 
 class SphSolver:
     """SphSolver::new (solver.rs:38-53) with ForceParams::default
-    (forces.rs:30-41); attributes may be changed before a call."""
+    (forces.rs:30-41); attributes may be changed before a call.
+    variant=True selects the build's physics variant (cubic spline, Newton h
+    to 1e-4, fixed alpha, no conduction, gravity eps = h, FP32 pair loops)."""
 
-    def __init__(self, ctx: Context | None = None):
+    def __init__(self, ctx: Context | None = None, variant: bool = False):
         self.ctx = ctx or default_context()
         d = _lib.SphParams()
-        self.ctx._L.gravis_sph_default_params(C.byref(d))
-        for k, _ in _lib.SphParams._fields_:
-            setattr(self, k, float(getattr(d, k)))
+        if variant:
+            self.ctx._L.gravis_sph_variant_params(C.byref(d))
+        else:
+            self.ctx._L.gravis_sph_default_params(C.byref(d))
+        for k, t in _lib.SphParams._fields_:
+            v = getattr(d, k)
+            setattr(self, k, float(v) if t is C.c_double else int(v))
 
     def params(self) -> _lib.SphParams:
         return _lib.SphParams(*[getattr(self, k) for k, _ in _lib.SphParams._fields_])
@@ -76,6 +82,17 @@ def _gravity_desc(gravity: GravitySolver | None):
     if gravity is None:
         return None
     return C.byref(gravity.desc())
+
+
+def accelerations(p: Particles, gravity: GravitySolver | None, sph: SphSolver) -> None:
+    """The gravity call of step_with_sph (solver.rs:180-181), accumulating
+    into p.ax/ay/az; with sph.soft_h the softening of target i is h_i."""
+    if gravity is None:
+        return
+    prm = sph.params()
+    st = sph_state(p)
+    sph.ctx.check(sph.ctx._L.gravis_sph_accel(sph.ctx.handle, C.byref(prm),
+                                              C.byref(gravity.desc()), C.byref(st)))
 
 
 def step_with_sph(p: Particles, gravity: GravitySolver | None, sph: SphSolver,

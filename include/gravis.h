@@ -200,7 +200,15 @@ gravis_status gravis_ensemble_adjoint_device(
  * SphSolver (crates/sim-core/src/sph/solver.rs:20-53) and ForceParams
  * (sph/forces.rs:15-41). Defaults (gravis_sph_default_params) are
  * sim-core's: gamma 5/3, eta 1.2, h_max +inf, CFL 0.3, force factor 0.25,
- * beta 2, alpha in [0.1, 2], conductivity 1, switch decay sigma 0.1. */
+ * beta 2, alpha in [0.1, 2], conductivity 1, switch decay sigma 0.1, and the
+ * fixed-point h iteration (<= 5 passes, tolerance 1e-3) with the Wendland C2
+ * kernel in FP64.
+ * The remaining knobs select a physics variant with no reference
+ * counterpart (gravis_sph_variant_params): cubic-spline M4 kernel,
+ * Newton-Raphson h on rho(h) = m (eta/h)^3 to 1e-4 (re-query when 2 h
+ * leaves the list's radius), alpha fixed (no switch), no conductivity,
+ * gravity softening eps_i = h_i of the target (Barnes-Hut only), and FP32
+ * pair arithmetic with FP64 per-particle sums. */
 typedef struct gravis_sph_params {
     double gamma;        /* adiabatic index */
     double eta;          /* h = eta (m / rho)^(1/3) */
@@ -212,6 +220,13 @@ typedef struct gravis_sph_params {
     double alpha_max;    /* switch ceiling */
     double alpha_cond;   /* Price conductivity coefficient (0 disables) */
     double sigma_visc;   /* switch decay parameter */
+    double h_tol;        /* h iteration tolerance |dh|/h (1e-3) */
+    int32_t kernel;      /* 0 Wendland C2 (sim-core), 1 cubic spline M4 */
+    int32_t h_solver;    /* 0 fixed point (sim-core), 1 Newton-Raphson */
+    int32_t max_h_iter;  /* h iterations (5) */
+    int32_t alpha_switch;/* 1 Morris-Monaghan switch (sim-core), 0 alpha fixed */
+    int32_t pair_fp32;   /* 0 FP64 pair loops (sim-core), 1 FP32 pairs, FP64 sums */
+    int32_t soft_h;      /* 0 the gravity solver's softening, 1 eps_i = h_i */
 } gravis_sph_params;
 
 /* Particles (crates/sim-core/src/particle.rs:21-64): all 18 SoA arrays of n
@@ -225,6 +240,9 @@ typedef struct gravis_sph_state {
 } gravis_sph_state;
 
 void gravis_sph_default_params(gravis_sph_params* out);
+/* The variant above: kernel 1, h_solver 1, h_tol 1e-4, max_h_iter 30,
+ * alpha_switch 0, alpha_cond 0, soft_h 1, pair_fp32 1. */
+void gravis_sph_variant_params(gravis_sph_params* out);
 
 /* SphSolver::compute (solver.rs:61-80): ball-query neighbour lists at
  * 2.2 h (sph/neighbors.rs:56-119), the density / smoothing-length iteration
@@ -243,6 +261,17 @@ gravis_status gravis_sph_compute_device(gravis_ctx* ctx, const gravis_sph_params
                                         const gravis_sph_state* state, double* div_v,
                                         double* max_mu, double* dalpha_dt, double* omega,
                                         uint32_t* nneigh);
+
+/* GravitySolver::compute_accelerations for an SPH state, as step_with_sph
+ * calls it (solver.rs:180-181): accumulates into ax/ay/az with the solver's
+ * softening, or with eps_i = h_i of each target when params->soft_h
+ * (Barnes-Hut only; GRAVIS_UNSUPPORTED otherwise). Host pointers. */
+gravis_status gravis_sph_accel(gravis_ctx* ctx, const gravis_sph_params* params,
+                               const gravis_solver* gravity, gravis_sph_state* state);
+/* Same with device pointers, asynchronous. */
+gravis_status gravis_sph_accel_device(gravis_ctx* ctx, const gravis_sph_params* params,
+                                      const gravis_solver* gravity,
+                                      const gravis_sph_state* state);
 
 /* step_with_sph (solver.rs:145-212): half kicks of v and u (floor 1e-10),
  * drift, gravity with `gravity` (NULL or GRAVIS_NO_GRAVITY: none), SPH
