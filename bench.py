@@ -59,6 +59,9 @@ def parse():
     ap.add_argument("--sph-warmup", type=int, default=3)
     ap.add_argument("--sph-cpu-sample", type=int, default=65536)
     ap.add_argument("--no-sph", action="store_true")
+    ap.add_argument("--c4-n", type=int, default=1048576)
+    ap.add_argument("--c4-steps", type=int, default=200)
+    ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--probe-copy", action="store_true",
                     help="only run a 1 GiB device copy (PMC byte-counter calibration)")
     return ap.parse_args()
@@ -318,6 +321,86 @@ def run_sph(args, g, torch, ctx):
                         "FP64 Barnes-Hut gravity theta=0.5, headless dt rule"}
 
 
+def run_sph_c4(args, g, torch, ctx):
+    """The grad-h SPH configuration with Barnes-Hut self-gravity: n gas
+    particles in R = 1, M = 1 with rho ~ 1/r (seed 42), u = 0.05, at rest,
+    gamma 5/3, cubic-spline kernel, eta 1.2, Newton h to 1e-4, Monaghan
+    viscosity alpha 1 / beta 2 (fixed), no conduction, gravity softening
+    eps = h, FP32 pair loops with FP64 sums (density, forces, gravity walk),
+    FP64 state; c4_steps KDK steps with the global Courant dt (CFL 0.3,
+    headless's dt rule from dt = 1e-4). State stays in HBM."""
+    import ctypes as C
+    from gravis_amd import _lib
+    from gravis_amd.sph import SphSolver
+    n = args.c4_n
+    dev = torch.device("cuda:0")
+    log(f"bench: SPH (cubic spline, Newton h, eps = h) rho~1/r N={n}; generating ICs")
+    p = g.scenarios.evrard_1r(n)
+    fields = dict(x="x", y="y", z="z", vx="vx", vy="vy", vz="vz", ax="ax", ay="ay", az="az",
+                  m="mass", rho="density", p="pressure", u="internal_energy", dudt="du_dt",
+                  h="smoothing_length", c="sound_speed", alpha="alpha_visc")
+    T = {k: torch.from_numpy(getattr(p, v)).to(dev) for k, v in fields.items()}
+    T["type"] = torch.from_numpy(p.particle_type).to(dev)
+    inter = [torch.zeros(n, dtype=torch.float64, device=dev) for _ in range(3)]
+    torch.cuda.synchronize()
+    st = _lib.SphState()
+    st.n = n
+    for k in _lib.SPH_FIELDS:
+        setattr(st, k, T[k].data_ptr())
+    st.type = T["type"].data_ptr()
+    solver = SphSolver(ctx, variant=True)
+    prm = solver.params()
+    bh = g.BarnesHut(0.0, 0.5, 16, 64, precision="mixed", ctx=ctx)
+    d = bh.desc()
+    L = ctx._L
+    ctx.check(L.gravis_sph_accel_device(ctx.handle, C.byref(prm), C.byref(d), C.byref(st)))
+    ctx.check(L.gravis_sph_compute_device(ctx.handle, C.byref(prm), C.byref(st),
+                                          *[C.c_void_p(t.data_ptr()) for t in inter],
+                                          None, None))
+    dt = C.c_double(1e-4)
+    tsim = C.c_double(0.0)
+    torch.cuda.synchronize()
+    log(f"bench: timing {args.c4_steps} SPH steps (variant)")
+    t0 = time.perf_counter()
+    ctx.check(L.gravis_sph_run_device(ctx.handle, C.byref(prm), C.byref(d), C.byref(st),
+                                      args.c4_steps, C.byref(dt), C.byref(tsim)))
+    torch.cuda.synchronize()
+    t_step = (time.perf_counter() - t0) / args.c4_steps
+    ctx.set_profiling(True)
+    ctx.check(L.gravis_sph_run_device(ctx.handle, C.byref(prm), C.byref(d), C.byref(st), 1,
+                                      C.byref(dt), C.byref(tsim)))
+    ctx.set_profiling(False)
+    kms = {}
+    for k in ("sph_query", "sph_density", "sph_force", "bh_walk"):
+        ms, _ = ctx.kernel_time(k)
+        kms[k] = round(ms, 3)
+    stats = solver.stats()
+    ke = float(0.5 * (T["m"] * (T["vx"] ** 2 + T["vy"] ** 2 + T["vz"] ** 2)).sum())
+    eth = float((T["m"] * T["u"]).sum())
+    cpu = None
+    if not args.no_cpu_baseline:
+        import oracle as O
+        ns = args.sph_cpu_sample
+        s = O.evrard_1r(ns)
+        vp = O.SphParams.variant()
+        O.sph_compute(s, vp)  # converge h first (untimed)
+        t0 = time.perf_counter()
+        O.sph_step(s, 1e-4, gravity=0, params=vp)
+        t_cpu = time.perf_counter() - t0
+        cpu = {"value": round(ns / t_cpu, 1), "unit": "particle-steps/s (SPH only)",
+               "cores": 1, "kind": "port",
+               "sample": f"one SPH step of the same physics without gravity on a {ns}-particle "
+                         f"rho~1/r sphere (oracle, f64, 1 thread): {t_cpu:.2f} s"}
+    return {"n": n, "steps": args.c4_steps, "steps_per_s": round(1.0 / t_step, 3),
+            "ms_per_step": round(t_step * 1e3, 3), "particle_steps_per_s": round(n / t_step, 1),
+            "sim_time": tsim.value, "dt_last": dt.value, "kinetic_energy": ke,
+            "thermal_energy": eth, "kernel_ms_one_step": kms, "last_compute_stats": stats,
+            "cpu_baseline": cpu,
+            "workload": "rho~1/r gas sphere (R=1, M=1, u=0.05, seed 42), cubic spline, "
+                        "eta=1.2, Newton h to 1e-4, alpha=1 beta=2, eps=h Barnes-Hut "
+                        "theta=0.5 (mixed walk), FP32 pairs / FP64 sums, CFL 0.3"}
+
+
 def main():
     args = parse()
     if args.probe_copy:
@@ -440,6 +523,9 @@ def main():
     sph = None
     if not args.no_sph:
         sph = run_sph(args, g, torch, ctx)
+    c4 = None
+    if not args.no_c4:
+        c4 = run_sph_c4(args, g, torch, ctx)
 
     traffic = traffic_from_profiles(f"bf_mixed_n{n}")
     out = {
@@ -476,6 +562,7 @@ def main():
         "barnes_hut_fp64": bh64,
         "ensemble": ens,
         "sph": sph,
+        "sph_c4": c4,
     }
     print(json.dumps(out), flush=True)
 
