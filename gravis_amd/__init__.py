@@ -300,6 +300,35 @@ _SPH_MAP = dict(x="x", y="y", z="z", vx="vx", vy="vy", vz="vz", ax="ax", ay="ay"
                 h="smoothing_length", c="sound_speed", alpha="alpha_visc")
 
 
+class _Snapshot:
+    """sim-core snapshots (crates/sim-core/src/snapshot.rs:5-157)."""
+
+    @staticmethod
+    def write(path: str, p: Particles, time: float, step: int, softening: float,
+              dt: float) -> None:
+        """Snapshot::capture + write_to (NBODY003)."""
+        _lib.check(None, _lib.load().gravis_snapshot_write(path.encode(), time, step, softening,
+                                                            dt, C.byref(sph_state(p))))
+
+    @staticmethod
+    def read(path: str):
+        """Snapshot::read_from: returns (particles, time, step, softening, dt,
+        version); accelerations are zero (recompute before stepping)."""
+        L = _lib.load()
+        v = C.c_int32()
+        t, so, dt = C.c_double(), C.c_double(), C.c_double()
+        st_, cnt = C.c_uint64(), C.c_uint64()
+        _lib.check(None, L.gravis_snapshot_info(path.encode(), C.byref(v), C.byref(t),
+                                                C.byref(st_), C.byref(so), C.byref(dt),
+                                                C.byref(cnt)))
+        p = Particles.zeros(cnt.value)
+        _lib.check(None, L.gravis_snapshot_read(path.encode(), C.byref(sph_state(p))))
+        return p, t.value, st_.value, so.value, dt.value, v.value
+
+
+snapshot = _Snapshot()
+
+
 def sph_state(p: Particles) -> _lib.SphState:
     """gravis_sph_state over the host arrays of p (borrowed, not copied)."""
     st = _lib.SphState()
@@ -394,6 +423,17 @@ class _Scenarios:
         st = sph_state(p)
         _lib.check(None, L.gravis_scenario_evrard_1r(n, total_mass, radius, u0, eta, seed,
                                                      C.byref(st)))
+        return p
+
+    @staticmethod
+    def cold_collapse(n: int, seed: int = 42, radius: float = 1.0, total_mass: float = 1.0,
+                      gas: bool = False) -> Particles:
+        """ColdCollapse::generate (cold_collapse.rs:56-102)."""
+        L = _lib.load()
+        p = Particles.zeros(n)
+        st = sph_state(p)
+        _lib.check(None, L.gravis_scenario_cold_collapse(n, radius, total_mass, seed,
+                                                         1 if gas else 0, C.byref(st)))
         return p
 
     @staticmethod

@@ -32,7 +32,8 @@ EXPORTS = (
     "gravis_sph_compute_device", "gravis_sph_step", "gravis_sph_run", "gravis_sph_run_device",
     "gravis_sph_stats", "gravis_scenario_evrard", "gravis_evrard_softening",
     "gravis_scenario_evrard_1r", "gravis_sph_variant_params", "gravis_sph_accel",
-    "gravis_sph_accel_device",
+    "gravis_sph_accel_device", "gravis_snapshot_write", "gravis_snapshot_info",
+    "gravis_snapshot_read", "gravis_scenario_cold_collapse",
 )
 
 
@@ -150,6 +151,11 @@ def load() -> C.CDLL:
         "gravis_scenario_evrard": (st, [sz, d, d, d, C.c_uint64, C.POINTER(SphState)]),
         "gravis_evrard_softening": (d, [sz, d]),
         "gravis_scenario_evrard_1r": (st, [sz, d, d, d, d, C.c_uint64, C.POINTER(SphState)]),
+        "gravis_scenario_cold_collapse": (st, [sz, d, d, C.c_uint64, i32, C.POINTER(SphState)]),
+        "gravis_snapshot_write": (st, [C.c_char_p, d, C.c_uint64, d, d, C.POINTER(SphState)]),
+        "gravis_snapshot_info": (st, [C.c_char_p, C.POINTER(i32), dp, C.POINTER(C.c_uint64), dp,
+                                      dp, C.POINTER(C.c_uint64)]),
+        "gravis_snapshot_read": (st, [C.c_char_p, C.POINTER(SphState)]),
     }
     for name, (res, args) in spec.items():
         f = getattr(L, name)

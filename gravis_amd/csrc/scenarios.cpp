@@ -333,3 +333,59 @@ extern "C" gravis_status gravis_scenario_evrard_1r(size_t n, double total_mass, 
     }
     return GRAVIS_OK;
 }
+
+// ColdCollapse::generate (crates/sim-core/src/scenarios/cold_collapse.rs:
+// 56-102) with fill_uniform_gas_sphere (scenarios/mod.rs:33-51) for the gas
+// form, then shift_to_com_frame (particle.rs:189-226).
+extern "C" gravis_status gravis_scenario_cold_collapse(size_t n, double radius,
+                                                       double total_mass, uint64_t seed,
+                                                       int gas, gravis_sph_state* s) {
+    if (!s || s->n != n || n == 0) return GRAVIS_INVALID_ARG;
+    double* f[17] = {s->x, s->y, s->z, s->vx, s->vy, s->vz, s->ax, s->ay, s->az,
+                     s->m, s->rho, s->p, s->u, s->dudt, s->h, s->c, s->alpha};
+    for (double* a : f)
+        if (!a) return GRAVIS_INVALID_ARG;
+    if (!s->type || !(radius > 0.0) || !(total_mass > 0.0)) return GRAVIS_INVALID_ARG;
+    ChaChaStream rng(seed);
+    const double m = total_mass / static_cast<double>(n);
+    for (double* a : f)
+        for (size_t i = 0; i < n; ++i) a[i] = 0.0;
+    if (gas) {
+        const double vol = 4.0 / 3.0 * kPi * (radius * radius * radius);
+        const double h = 1.5 * std::cbrt(vol / static_cast<double>(n));
+        size_t cnt = 0;
+        while (cnt < n) {
+            const double x = rng.uniform01() * 2.0 * radius - radius;
+            const double y = rng.uniform01() * 2.0 * radius - radius;
+            const double z = rng.uniform01() * 2.0 * radius - radius;
+            if (x * x + y * y + z * z <= radius * radius) {
+                s->x[cnt] = x;
+                s->y[cnt] = y;
+                s->z[cnt] = z;
+                s->m[cnt] = m;
+                s->u[cnt] = 0.05;
+                s->h[cnt] = h;
+                s->alpha[cnt] = 1.0;
+                s->type[cnt] = 4;
+                ++cnt;
+            }
+        }
+    } else {
+        for (size_t i = 0; i < n; ++i) {
+            double x, y, z;
+            for (;;) {
+                x = rng.range(-1.0, 1.0);
+                y = rng.range(-1.0, 1.0);
+                z = rng.range(-1.0, 1.0);
+                if (x * x + y * y + z * z <= 1.0) break;
+            }
+            s->x[i] = x * radius;
+            s->y[i] = y * radius;
+            s->z[i] = z * radius;
+            s->m[i] = m;
+            s->type[i] = 0;
+        }
+    }
+    com_shift(n, s->x, s->y, s->z, s->vx, s->vy, s->vz, s->m);
+    return GRAVIS_OK;
+}

@@ -332,6 +332,21 @@ gravis_status gravis_morton_sort(gravis_ctx* ctx, size_t n, const double* x, con
                                  double* box_out);
 
 
+/* ---- snapshots (host; crates/sim-core/src/snapshot.rs:5-157) ------------
+ * Binary format NBODY001/002/003, little endian. gravis_snapshot_write is
+ * Snapshot::capture + write_to (always v3: positions, velocities, masses,
+ * particle types, internal energy, smoothing length, density).
+ * gravis_snapshot_info reads the header; gravis_snapshot_read
+ * (Snapshot::read_from) fills a state of that count (state->n must equal
+ * it) and zeroes every field the file does not carry, accelerations
+ * included: recompute them before stepping (Snapshot::initialize). Files
+ * that are not snapshots give GRAVIS_INVALID_ARG. */
+gravis_status gravis_snapshot_write(const char* path, double time, uint64_t step,
+                                    double softening, double dt, const gravis_sph_state* state);
+gravis_status gravis_snapshot_info(const char* path, int* version, double* time, uint64_t* step,
+                                   double* softening, double* dt, uint64_t* count);
+gravis_status gravis_snapshot_read(const char* path, gravis_sph_state* state);
+
 /* ---- initial conditions (host; sim-core's scenario generators) ---------- */
 
 /* PlummerSphere::generate (crates/sim-core/src/scenarios/plummer_sphere.rs:
@@ -368,6 +383,15 @@ double gravis_evrard_softening(size_t n, double radius);
  * m = M/n, h = eta (m / rho(r))^(1/3). Host arrays, state->n = n. */
 gravis_status gravis_scenario_evrard_1r(size_t n, double total_mass, double radius, double u0,
                                         double eta, uint64_t seed, gravis_sph_state* state);
+/* ColdCollapse::generate (crates/sim-core/src/scenarios/cold_collapse.rs:
+ * 56-102): n particles of mass M/n uniform in a sphere of radius R at rest,
+ * then the COM frame. gas = 0: rejection with random_range(-1..1) per axis
+ * scaled by R, type 0; gas = 1: fill_uniform_gas_sphere with u = 0.05 and
+ * h = 1.5 (V/n)^(1/3), type 4. state->n = n; host arrays. Suggested dt
+ * t_ff/200 (t_ff/500 for gas), softening 0.3 (V/n)^(1/3)
+ * (cold_collapse.rs:104-123). */
+gravis_status gravis_scenario_cold_collapse(size_t n, double radius, double total_mass,
+                                            uint64_t seed, int gas, gravis_sph_state* state);
 /* Batched-ensemble layout (no reference counterpart): counts[s] =
  * lo + (u64 draw) % (hi - lo + 1) from ChaCha20Rng::seed_from_u64(seed),
  * in system order; and loss targets U[-1, 1)^3 drawn x, y, z per particle. */

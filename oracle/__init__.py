@@ -315,6 +315,7 @@ def _sph_lib():
             "orc_cs_dw_dh": (_d, [_d, _d]),
             "orc_cs_grad_w": (None, [_d, _d, _d, _d, _d, _dp]),
             "orc_evrard_1r": (None, [_sz, _d, _d, _d, _d, _u64, sp]),
+            "orc_cold_collapse": (None, [_sz, _d, _d, _u64, _i, sp]),
             "orc_bf_accel_eps": (None, [_sz] + [_dp] * 8),
             "orc_bh_group_accel_eps": (_u64, [_sz, _dp, _d, _u32, _u32, _u32] + [_dp] * 7 + [_i]),
             "orc_w": (_d, [_d, _d]),
@@ -391,6 +392,15 @@ def evrard_1r(n: int, seed: int = 42, total_mass: float = 1.0, radius: float = 1
     s = SphState(n)
     st = s.c_struct()
     _sph_lib().orc_evrard_1r(n, total_mass, radius, u0, eta, seed, C.byref(st))
+    return s
+
+
+def cold_collapse(n: int, seed: int = 42, radius: float = 1.0, total_mass: float = 1.0,
+                  gas: bool = False) -> SphState:
+    """ColdCollapse::generate (cold_collapse.rs:56-102)."""
+    s = SphState(n)
+    st = s.c_struct()
+    _sph_lib().orc_cold_collapse(n, radius, total_mass, seed, 1 if gas else 0, C.byref(st))
     return s
 
 

@@ -19,6 +19,8 @@ double orc_rng_range(orc_rng* r, double lo, double hi);
 void orc_root_box(size_t n, const double* x, const double* y, const double* z, double out[4]);
 void orc_bf_accel(size_t n, double eps, const double* x, const double* y, const double* z,
                   const double* m, double* ax, double* ay, double* az);
+void orc_shift_to_com(size_t n, double* x, double* y, double* z, double* vx, double* vy,
+                      double* vz, const double* m);
 void orc_bh_accel(size_t n, double eps, double theta, const double* x, const double* y,
                   const double* z, const double* m, double* ax, double* ay, double* az,
                   int nthreads);

@@ -664,6 +664,49 @@ void orc_evrard_1r(size_t n, double total_mass, double radius, double u0, double
     }
 }
 
+/* ColdCollapse::generate (cold_collapse.rs:56-102): gas = 0 rejection with
+ * random_range(-1..1) per axis scaled by R (type 0); gas = 1
+ * fill_uniform_gas_sphere with u = 0.05, h = 1.5 spacing (type 4); then
+ * shift_to_com_frame. Every other field 0. */
+void orc_cold_collapse(size_t n, double radius, double total_mass, uint64_t seed, int gas,
+                       orc_sph_state* s) {
+    orc_rng rng;
+    orc_rng_seed_u64(&rng, seed);
+    double m = total_mass / (double)n;
+    double* f[17] = {s->x, s->y, s->z, s->vx, s->vy, s->vz, s->ax, s->ay, s->az,
+                     s->m, s->rho, s->p, s->u, s->dudt, s->h, s->c, s->alpha};
+    for (int k = 0; k < 17; ++k) memset(f[k], 0, n * sizeof(double));
+    if (gas) {
+        double vol = 4.0 / 3.0 * SPH_PI * (radius * radius * radius);
+        double h = 1.5 * cbrt(vol / (double)n);
+        size_t cnt = 0;
+        while (cnt < n) {
+            double x = orc_rng_f64(&rng) * 2.0 * radius - radius;
+            double y = orc_rng_f64(&rng) * 2.0 * radius - radius;
+            double z = orc_rng_f64(&rng) * 2.0 * radius - radius;
+            if (x * x + y * y + z * z <= radius * radius) {
+                s->x[cnt] = x; s->y[cnt] = y; s->z[cnt] = z;
+                s->m[cnt] = m; s->u[cnt] = 0.05; s->h[cnt] = h; s->alpha[cnt] = 1.0;
+                s->type[cnt] = GAS;
+                cnt++;
+            }
+        }
+    } else {
+        for (size_t i = 0; i < n; ++i) {
+            double x, y, z;
+            do {
+                x = orc_rng_range(&rng, -1.0, 1.0);
+                y = orc_rng_range(&rng, -1.0, 1.0);
+                z = orc_rng_range(&rng, -1.0, 1.0);
+            } while (!(x * x + y * y + z * z <= 1.0));
+            s->x[i] = x * radius; s->y[i] = y * radius; s->z[i] = z * radius;
+            s->m[i] = m;
+            s->type[i] = 0;
+        }
+    }
+    orc_shift_to_com(n, s->x, s->y, s->z, s->vx, s->vy, s->vz, s->m);
+}
+
 double orc_evrard_softening(size_t n, double radius) {
     double vol = 4.0 / 3.0 * SPH_PI * (radius * radius * radius);
     return 0.3 * cbrt(vol / (double)n);
