@@ -437,6 +437,39 @@ class _Scenarios:
         return p
 
     @staticmethod
+    def galaxy_params(**kw) -> _lib.GalaxyParams:
+        """GalaxyCollision::default() (galaxy_collision.rs:51-68) with the
+        given fields replaced."""
+        g = _lib.GalaxyParams()
+        _lib.load().gravis_galaxy_default_params(C.byref(g))
+        for k, v in kw.items():
+            if not hasattr(g, k):
+                raise TypeError(f"GalaxyCollision has no field {k!r}")
+            setattr(g, k, v)
+        return g
+
+    @staticmethod
+    def galaxy_collision(**kw) -> Particles:
+        """GalaxyCollision::generate (galaxy_collision.rs:75-157); keyword
+        arguments are GalaxyCollision fields. 2 * n_per_galaxy particles of
+        types 1 (disk), 2 (bulge), 3 (halo)."""
+        L = _lib.load()
+        g = _Scenarios.galaxy_params(**kw)
+        p = Particles.zeros(2 * g.n_per_galaxy)
+        st = sph_state(p)
+        _lib.check(None, L.gravis_scenario_galaxy_collision(C.byref(g), C.byref(st)))
+        return p
+
+    @staticmethod
+    def galaxy_suggested_dt(**kw) -> float:
+        return _lib.load().gravis_galaxy_suggested_dt(C.byref(_Scenarios.galaxy_params(**kw)))
+
+    @staticmethod
+    def galaxy_suggested_softening(**kw) -> float:
+        return _lib.load().gravis_galaxy_suggested_softening(
+            C.byref(_Scenarios.galaxy_params(**kw)))
+
+    @staticmethod
     def evrard_softening(n: int, radius: float = 1.0) -> float:
         """evrard_collapse.rs:67-70: 0.3 (V/n)^(1/3)."""
         return _lib.load().gravis_evrard_softening(n, radius)

@@ -33,7 +33,9 @@ EXPORTS = (
     "gravis_sph_stats", "gravis_scenario_evrard", "gravis_evrard_softening",
     "gravis_scenario_evrard_1r", "gravis_sph_variant_params", "gravis_sph_accel",
     "gravis_sph_accel_device", "gravis_snapshot_write", "gravis_snapshot_info",
-    "gravis_snapshot_read", "gravis_scenario_cold_collapse",
+    "gravis_snapshot_read", "gravis_scenario_cold_collapse", "gravis_galaxy_default_params",
+    "gravis_scenario_galaxy_collision", "gravis_galaxy_suggested_dt",
+    "gravis_galaxy_suggested_softening",
 )
 
 
@@ -56,6 +58,15 @@ class SphParams(C.Structure):
                                            "sigma_visc", "h_tol")] + [
         (k, C.c_int32) for k in ("kernel", "h_solver", "max_h_iter", "alpha_switch",
                                  "pair_fp32", "soft_h")]
+
+
+class GalaxyParams(C.Structure):
+    _fields_ = [("n_per_galaxy", C.c_uint64), ("seed", C.c_uint64)] + [
+        (k, C.c_double) for k in ("disk_fraction", "halo_mass_ratio", "baryon_mass",
+                                  "disk_scale_radius", "disk_scale_height",
+                                  "bulge_scale_radius", "halo_scale_radius", "disk_max_radius",
+                                  "halo_max_radius", "separation", "impact_parameter",
+                                  "approach_velocity")]
 
 
 SPH_FIELDS = ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "m", "rho", "p", "u", "dudt",
@@ -156,6 +167,11 @@ def load() -> C.CDLL:
         "gravis_snapshot_info": (st, [C.c_char_p, C.POINTER(i32), dp, C.POINTER(C.c_uint64), dp,
                                       dp, C.POINTER(C.c_uint64)]),
         "gravis_snapshot_read": (st, [C.c_char_p, C.POINTER(SphState)]),
+        "gravis_galaxy_default_params": (None, [C.POINTER(GalaxyParams)]),
+        "gravis_scenario_galaxy_collision": (st, [C.POINTER(GalaxyParams),
+                                                  C.POINTER(SphState)]),
+        "gravis_galaxy_suggested_dt": (d, [C.POINTER(GalaxyParams)]),
+        "gravis_galaxy_suggested_softening": (d, [C.POINTER(GalaxyParams)]),
     }
     for name, (res, args) in spec.items():
         f = getattr(L, name)

@@ -254,6 +254,16 @@ bool build(const Cli& c, State* s, Built* b) {
         b->dt = gas ? t_ff / 500.0 : t_ff / 200.0;  // cold_collapse.rs:104-115
         b->softening =
             std::pow(4.0 / 3.0 * kPi * (1.0 * 1.0 * 1.0) / (double)n, 1.0 / 3.0) * 0.3;
+    } else if (sc == "galaxy-collision") {
+        gravis_galaxy_params g;
+        gravis_galaxy_default_params(&g);
+        g.n_per_galaxy = (c.particles > 0 ? (uint64_t)c.particles : 10000) / 2;
+        g.seed = c.seed;
+        s->resize(2 * g.n_per_galaxy);
+        check(nullptr, gravis_scenario_galaxy_collision(&g, &s->view), "galaxy-collision");
+        b->name = "Galaxy Collision";
+        b->dt = gravis_galaxy_suggested_dt(&g);
+        b->softening = gravis_galaxy_suggested_softening(&g);
     } else if (sc == "evrard-collapse") {
         const size_t n = c.particles > 0 ? (size_t)c.particles : 5000;
         s->resize(n);
@@ -265,8 +275,9 @@ bool build(const Cli& c, State* s, Built* b) {
     } else {
         std::fprintf(stderr,
                      "Unknown scenario: %s. Available on this backend: plummer, two-body, "
-                     "cold-collapse, cold-collapse-gas, evrard-collapse (the reference's "
-                     "galaxy-collision and pure-hydro scenarios run on sim-core's CPU path)\n",
+                     "cold-collapse, cold-collapse-gas, galaxy-collision, evrard-collapse (the "
+                     "reference's pure-hydro and protoplanetary scenarios run on sim-core's "
+                     "CPU path)\n",
                      sc.c_str());
         return false;
     }

@@ -118,6 +118,36 @@ void com_shift(size_t n, double* x, double* y, double* z, double* vx, double* vy
     }
 }
 
+// sample_plummer_particle (plummer_sphere.rs:13-58): one position and
+// velocity of a Plummer sphere of mass M and scale radius a.
+void plummer_particle(ChaChaStream& rng, double total_mass, double a, double pos[3],
+                      double vel[3]) {
+    // position: inverse CDF of M(<r) = r^3 / (r^2 + a^2)^(3/2)
+    const double xm = rng.range(0.001, 1.0);
+    const double r = a / std::sqrt(std::pow(xm, -2.0 / 3.0) - 1.0);
+    const double ct = rng.range(-1.0, 1.0);
+    const double st = std::sqrt(1.0 - ct * ct);
+    const double ph = rng.range(0.0, 2.0 * kPi);
+    pos[0] = r * st * std::cos(ph);
+    pos[1] = r * st * std::sin(ph);
+    pos[2] = r * ct;
+    // speed: rejection from q^2 (1 - q^2)^(7/2) with bound 0.1
+    const double vesc = std::sqrt(2.0 * 1.0 * total_mass / std::sqrt(r * r + a * a));
+    double q;
+    for (;;) {
+        q = rng.range(0.0, 1.0);
+        const double gq = q * q * std::pow(1.0 - q * q, 3.5);
+        if (rng.uniform01() < gq / 0.1) break;
+    }
+    const double v = q * vesc;
+    const double cv = rng.range(-1.0, 1.0);
+    const double sv = std::sqrt(1.0 - cv * cv);
+    const double pv = rng.range(0.0, 2.0 * kPi);
+    vel[0] = v * sv * std::cos(pv);
+    vel[1] = v * sv * std::sin(pv);
+    vel[2] = v * cv;
+}
+
 }  // namespace
 
 extern "C" {
@@ -128,33 +158,12 @@ gravis_status gravis_scenario_plummer(size_t n, double scale_radius, double tota
     if (n == 0 || !(scale_radius > 0.0) || !(total_mass > 0.0)) return GRAVIS_INVALID_ARG;
     if (!x || !y || !z || !vx || !vy || !vz || !m) return GRAVIS_INVALID_ARG;
     ChaChaStream rng(seed);
-    const double a = scale_radius;
     const double mp = total_mass / static_cast<double>(n);
     for (size_t i = 0; i < n; ++i) {
-        // position: inverse CDF of M(<r) = r^3 / (r^2 + a^2)^(3/2)
-        const double xm = rng.range(0.001, 1.0);
-        const double r = a / std::sqrt(std::pow(xm, -2.0 / 3.0) - 1.0);
-        const double ct = rng.range(-1.0, 1.0);
-        const double st = std::sqrt(1.0 - ct * ct);
-        const double ph = rng.range(0.0, 2.0 * kPi);
-        x[i] = r * st * std::cos(ph);
-        y[i] = r * st * std::sin(ph);
-        z[i] = r * ct;
-        // speed: rejection from q^2 (1 - q^2)^(7/2) with bound 0.1
-        const double vesc = std::sqrt(2.0 * 1.0 * total_mass / std::sqrt(r * r + a * a));
-        double q;
-        for (;;) {
-            q = rng.range(0.0, 1.0);
-            const double gq = q * q * std::pow(1.0 - q * q, 3.5);
-            if (rng.uniform01() < gq / 0.1) break;
-        }
-        const double v = q * vesc;
-        const double cv = rng.range(-1.0, 1.0);
-        const double sv = std::sqrt(1.0 - cv * cv);
-        const double pv = rng.range(0.0, 2.0 * kPi);
-        vx[i] = v * sv * std::cos(pv);
-        vy[i] = v * sv * std::sin(pv);
-        vz[i] = v * cv;
+        double p[3], v[3];
+        plummer_particle(rng, total_mass, scale_radius, p, v);
+        x[i] = p[0]; y[i] = p[1]; z[i] = p[2];
+        vx[i] = v[0]; vy[i] = v[1]; vz[i] = v[2];
         m[i] = mp;
     }
     com_shift(n, x, y, z, vx, vy, vz, m);
@@ -388,4 +397,217 @@ extern "C" gravis_status gravis_scenario_cold_collapse(size_t n, double radius,
     }
     com_shift(n, s->x, s->y, s->z, s->vx, s->vy, s->vz, s->m);
     return GRAVIS_OK;
+}
+
+// GalaxyCollision (crates/sim-core/src/scenarios/galaxy_collision.rs:20-450).
+// Each galaxy: exponential disk (rejection against U[0, R_max]), Plummer
+// bulge, truncated Hernquist halo with isotropic Gaussian velocities; both
+// galaxies are drawn from one stream, then offset along x/y with opposite
+// x velocities, then moved to the COM frame.
+namespace {
+
+constexpr uint8_t kDiskStar = 1, kBulgeStar = 2, kDarkMatter = 3;
+
+// disk_enclosed_mass + circular_velocity (galaxy_collision.rs:193-257).
+double disk_enclosed(double r, double m_disk, double rd) {
+    if (r <= 0.0) return 0.0;
+    const double x = r / rd;
+    return m_disk * (1.0 - (1.0 + x) * std::exp(-x));
+}
+
+double circular_velocity(double r, double m_disk, double rd, double m_bulge, double ab,
+                         double m_halo, double ah) {
+    if (r <= 0.0) return 0.0;
+    const double v2_disk = 1.0 * disk_enclosed(r, m_disk, rd) / r;
+    const double v2_bulge = 1.0 * m_bulge * r * r / std::pow(r * r + ab * ab, 1.5);
+    const double v2_halo = 1.0 * m_halo * r / ((r + ah) * (r + ah));
+    return std::sqrt(v2_disk + v2_bulge + v2_halo);
+}
+
+struct Sink {
+    gravis_sph_state* s;
+    size_t k;
+    void add(double x, double y, double z, double vx, double vy, double vz, double m,
+             uint8_t type) {
+        s->x[k] = x; s->y[k] = y; s->z[k] = z;
+        s->vx[k] = vx; s->vy[k] = vy; s->vz[k] = vz;
+        s->m[k] = m;
+        s->type[k] = type;
+        ++k;
+    }
+};
+
+// generate_galaxy = generate_disk, generate_bulge, generate_halo
+// (galaxy_collision.rs:204-450), in that order from one stream.
+void galaxy(const gravis_galaxy_params& g, ChaChaStream& rng, size_t n_disk, size_t n_bulge,
+            size_t n_halo, double m_disk, double m_bulge, double m_halo, Sink& out) {
+    const double rd = g.disk_scale_radius, ab = g.bulge_scale_radius, ah = g.halo_scale_radius;
+    // disk
+    {
+        const double mp = m_disk / static_cast<double>(n_disk);
+        const double max_x = g.disk_max_radius / rd;
+        for (size_t i = 0; i < n_disk; ++i) {
+            double r;
+            for (;;) {
+                const double u = rng.uniform01();
+                const double r_try = u * g.disk_max_radius;
+                const double x = r_try / rd;
+                const double f = x * std::exp(-x);
+                const double f_max = std::exp(-1.0);
+                const double accept = rng.uniform01();
+                if (accept < f / f_max && x < max_x) {
+                    r = r_try;
+                    break;
+                }
+            }
+            const double phi = rng.range(0.0, 2.0 * kPi);
+            const double c = std::cos(phi), s = std::sin(phi);
+            const double px = r * c, py = r * s;
+            const double uz = rng.range(0.01, 0.99);
+            const double pz = g.disk_scale_height * std::atanh(2.0 * uz - 1.0);
+            const double vc = circular_velocity(r, m_disk, rd, m_bulge, ab, m_halo, ah);
+            const double sigma_r = 0.15 * vc;
+            const double v_r = rng.range(-1.0, 1.0) * sigma_r;
+            const double v_phi = vc + rng.range(-1.0, 1.0) * sigma_r * 0.5;
+            const double sigma_z = 0.1 * vc;
+            const double v_z = rng.range(-1.0, 1.0) * sigma_z;
+            out.add(px, py, pz, v_r * c - v_phi * s, v_r * s + v_phi * c, v_z, mp, kDiskStar);
+        }
+    }
+    // bulge
+    {
+        const double mp = m_bulge / static_cast<double>(n_bulge);
+        for (size_t i = 0; i < n_bulge; ++i) {
+            double p[3], v[3];
+            plummer_particle(rng, m_bulge, ab, p, v);
+            out.add(p[0], p[1], p[2], v[0], v[1], v[2], mp, kBulgeStar);
+        }
+    }
+    // halo: r = a sqrt(U) / (1 - sqrt(U)), U in [0.001, (R/(R+a))^2)
+    {
+        const double mp = m_halo / static_cast<double>(n_halo);
+        const double xm = g.halo_max_radius / (g.halo_max_radius + ah);
+        const double u_max = xm * xm;
+        const double sqrt2 = std::sqrt(2.0);
+        for (size_t i = 0; i < n_halo; ++i) {
+            const double u = rng.range(0.001, u_max);
+            const double su = std::sqrt(u);
+            const double r = ah * su / (1.0 - su);
+            const double ct = rng.range(-1.0, 1.0);
+            const double st = std::sqrt(1.0 - ct * ct);
+            const double phi = rng.range(0.0, 2.0 * kPi);
+            const double px = r * st * std::cos(phi);
+            const double py = r * st * std::sin(phi);
+            const double pz = r * ct;
+            const double vc = circular_velocity(r, m_disk, rd, m_bulge, ab, m_halo, ah);
+            const double sigma = vc / sqrt2;
+            // box_muller (scenarios/mod.rs:25-30), twice
+            double gs[4];
+            for (int b = 0; b < 2; ++b) {
+                const double u1 = rng.range(0.001, 1.0);
+                const double u2 = rng.range(0.0, 2.0 * kPi);
+                const double rr = std::sqrt(-2.0 * std::log(u1));
+                gs[2 * b] = rr * std::cos(u2);
+                gs[2 * b + 1] = rr * std::sin(u2);
+            }
+            out.add(px, py, pz, sigma * gs[0], sigma * gs[1], sigma * gs[2], mp, kDarkMatter);
+        }
+    }
+}
+
+// Particle counts proportional to mass (galaxy_collision.rs:80-88).
+void galaxy_counts(const gravis_galaxy_params& g, double* md, double* mb, double* mh,
+                   size_t* nd, size_t* nb, size_t* nh) {
+    *md = g.baryon_mass * g.disk_fraction;
+    *mb = g.baryon_mass * (1.0 - g.disk_fraction);
+    *mh = g.baryon_mass * g.halo_mass_ratio;
+    const double mt = *md + *mb + *mh;
+    const double n = static_cast<double>(g.n_per_galaxy);
+    *nd = static_cast<size_t>(std::round(n * *md / mt));
+    *nb = static_cast<size_t>(std::round(n * *mb / mt));
+    *nh = g.n_per_galaxy - *nd - *nb;
+}
+
+}  // namespace
+
+extern "C" void gravis_galaxy_default_params(gravis_galaxy_params* g) {
+    if (!g) return;
+    g->n_per_galaxy = 50000;
+    g->seed = 42;
+    g->disk_fraction = 0.7;
+    g->halo_mass_ratio = 10.0;
+    g->baryon_mass = 1.0;
+    g->disk_scale_radius = 1.0;
+    g->disk_scale_height = 0.1;
+    g->bulge_scale_radius = 0.3;
+    g->halo_scale_radius = 5.0;
+    g->disk_max_radius = 6.0;
+    g->halo_max_radius = 30.0;
+    g->separation = 15.0;
+    g->impact_parameter = 2.0;
+    g->approach_velocity = 0.5;
+}
+
+extern "C" gravis_status gravis_scenario_galaxy_collision(const gravis_galaxy_params* g,
+                                                          gravis_sph_state* s) {
+    if (!g || !s || g->n_per_galaxy == 0 || s->n != 2 * g->n_per_galaxy)
+        return GRAVIS_INVALID_ARG;
+    if (!(g->disk_scale_radius > 0.0) || !(g->halo_scale_radius > 0.0) ||
+        !(g->disk_max_radius > 0.0) || !(g->halo_max_radius > 0.0) ||
+        !(g->disk_fraction >= 0.0 && g->disk_fraction <= 1.0) || !(g->baryon_mass > 0.0) ||
+        !(g->halo_mass_ratio >= 0.0))
+        return GRAVIS_INVALID_ARG;
+    double* f[17] = {s->x, s->y, s->z, s->vx, s->vy, s->vz, s->ax, s->ay, s->az,
+                     s->m, s->rho, s->p, s->u, s->dudt, s->h, s->c, s->alpha};
+    for (double* a : f)
+        if (!a) return GRAVIS_INVALID_ARG;
+    if (!s->type) return GRAVIS_INVALID_ARG;
+    double md, mb, mh;
+    size_t nd, nb, nh;
+    galaxy_counts(*g, &md, &mb, &mh, &nd, &nb, &nh);
+    if (nd + nb > g->n_per_galaxy) return GRAVIS_INVALID_ARG;
+    const size_t n = s->n;
+    for (double* a : f)
+        for (size_t i = 0; i < n; ++i) a[i] = 0.0;
+    ChaChaStream rng(g->seed);
+    Sink out{s, 0};
+    galaxy(*g, rng, nd, nb, nh, md, mb, mh, out);
+    const size_t half = out.k;
+    galaxy(*g, rng, nd, nb, nh, md, mb, mh, out);
+    const double hs = g->separation / 2.0, hi = g->impact_parameter / 2.0,
+                 hv = g->approach_velocity / 2.0;
+    for (size_t i = 0; i < half; ++i) {
+        s->x[i] -= hs;
+        s->y[i] -= hi;
+        s->vx[i] += hv;
+    }
+    for (size_t i = half; i < n; ++i) {
+        s->x[i] += hs;
+        s->y[i] += hi;
+        s->vx[i] -= hv;
+    }
+    com_shift(n, s->x, s->y, s->z, s->vx, s->vy, s->vz, s->m);
+    return GRAVIS_OK;
+}
+
+// suggested_dt / suggested_softening (galaxy_collision.rs:159-185): t_orb at
+// the disk scale radius over 500; softening 0.05 R_d.
+extern "C" double gravis_galaxy_suggested_dt(const gravis_galaxy_params* g) {
+    const double m_enc_disk = 0.26 * g->baryon_mass * g->disk_fraction;
+    const double r = g->disk_scale_radius;
+    const double ab = g->bulge_scale_radius;
+    const double m_bulge = g->baryon_mass * (1.0 - g->disk_fraction);
+    const double m_enc_bulge = m_bulge * (r * r * r) / std::pow(r * r + ab * ab, 1.5);
+    const double ah = g->halo_scale_radius;
+    const double m_halo = g->baryon_mass * g->halo_mass_ratio;
+    const double x = r / ah;
+    const double m_enc_halo = m_halo * x * x / ((1.0 + x) * (1.0 + x));
+    const double m_enc = m_enc_disk + m_enc_bulge + m_enc_halo;
+    const double vc = std::sqrt(1.0 * m_enc / r);
+    const double t_orb = 2.0 * kPi * r / vc;
+    return t_orb / 500.0;
+}
+
+extern "C" double gravis_galaxy_suggested_softening(const gravis_galaxy_params* g) {
+    return g->disk_scale_radius * 0.05;
 }

@@ -392,6 +392,38 @@ gravis_status gravis_scenario_evrard_1r(size_t n, double total_mass, double radi
  * (cold_collapse.rs:104-123). */
 gravis_status gravis_scenario_cold_collapse(size_t n, double radius, double total_mass,
                                             uint64_t seed, int gas, gravis_sph_state* state);
+
+/* GalaxyCollision's fields (crates/sim-core/src/scenarios/galaxy_collision.rs:
+ * 20-68); gravis_galaxy_default_params gives Default::default(). */
+typedef struct gravis_galaxy_params {
+    uint64_t n_per_galaxy;
+    uint64_t seed;
+    double disk_fraction;
+    double halo_mass_ratio;
+    double baryon_mass;
+    double disk_scale_radius;
+    double disk_scale_height;
+    double bulge_scale_radius;
+    double halo_scale_radius;
+    double disk_max_radius;
+    double halo_max_radius;
+    double separation;
+    double impact_parameter;
+    double approach_velocity;
+} gravis_galaxy_params;
+
+void gravis_galaxy_default_params(gravis_galaxy_params* params);
+/* GalaxyCollision::generate (galaxy_collision.rs:75-157): two galaxies of
+ * disk (type 1), Plummer bulge (type 2) and Hernquist halo (type 3) from one
+ * ChaCha20Rng::seed_from_u64(seed) stream, placed at (-+sep/2, -+b/2, 0)
+ * with x velocities +-v/2, then moved to the COM frame.
+ * state->n = 2 * n_per_galaxy; host arrays; SPH fields zero. */
+gravis_status gravis_scenario_galaxy_collision(const gravis_galaxy_params* params,
+                                               gravis_sph_state* state);
+/* suggested_dt (t_orb(R_d) / 500) and suggested_softening (0.05 R_d),
+ * galaxy_collision.rs:159-185. */
+double gravis_galaxy_suggested_dt(const gravis_galaxy_params* params);
+double gravis_galaxy_suggested_softening(const gravis_galaxy_params* params);
 /* Batched-ensemble layout (no reference counterpart): counts[s] =
  * lo + (u64 draw) % (hi - lo + 1) from ChaCha20Rng::seed_from_u64(seed),
  * in system order; and loss targets U[-1, 1)^3 drawn x, y, z per particle. */

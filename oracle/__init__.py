@@ -64,6 +64,8 @@ def lib():
             "orc_bh_group_accel": (_u64, [_sz, _d, _d, _u32, _u32, _u32] + [_dp] * 7 + [_i]),
             "orc_kdk_steps": (None, [_sz, _i, _d, _d] + [_dp] * 10 + [_d, _u64, _i]),
             "orc_run": (_sz, [_sz, _i, _d, _d] + [_dp] * 7 + [_d, _u64, _u64, _dp, _sz, _i]),
+            "orc_galaxy_collision": (None, [C.c_void_p, _i] + [_dp] * 7 + [C.c_void_p]),
+            "orc_galaxy_suggested_dt": (_d, [C.c_void_p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -132,6 +134,40 @@ def plummer_suggested_dt(a: float = 1.0, total_mass: float = 1.0) -> float:
 
 def plummer_suggested_softening(n: int, a: float = 1.0) -> float:
     return lib().orc_plummer_suggested_softening(n, a)
+
+
+class GalaxyParams(C.Structure):
+    """GalaxyCollision's fields and Default (galaxy_collision.rs:20-68)."""
+    _fields_ = [("n_per_galaxy", C.c_uint64), ("seed", C.c_uint64)] + [
+        (k, C.c_double) for k in ("disk_fraction", "halo_mass_ratio", "baryon_mass",
+                                  "disk_scale_radius", "disk_scale_height",
+                                  "bulge_scale_radius", "halo_scale_radius", "disk_max_radius",
+                                  "halo_max_radius", "separation", "impact_parameter",
+                                  "approach_velocity")]
+
+    @classmethod
+    def default(cls, **kw):
+        p = cls(50000, 42, 0.7, 10.0, 1.0, 1.0, 0.1, 0.3, 5.0, 6.0, 30.0, 15.0, 2.0, 0.5)
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+
+def galaxy_collision(com_shift: bool = True, **kw):
+    """GalaxyCollision::generate (galaxy_collision.rs:75-157). Keyword
+    arguments override Default fields; com_shift=False returns the samples
+    before shift_to_com_frame. Returns (State, particle types)."""
+    p = GalaxyParams.default(**kw)
+    n = 2 * p.n_per_galaxy
+    s = State(n)
+    t = np.zeros(n, dtype=np.uint8)
+    lib().orc_galaxy_collision(C.byref(p), 1 if com_shift else 0, s.x, s.y, s.z, s.vx, s.vy, s.vz, s.m,
+                               t.ctypes.data_as(C.c_void_p))
+    return s, t
+
+
+def galaxy_suggested_dt(**kw) -> float:
+    return lib().orc_galaxy_suggested_dt(C.byref(GalaxyParams.default(**kw)))
 
 
 def two_body(e: float = 0.5, mass_ratio: float = 0.5, a: float = 1.0) -> State:

@@ -14,6 +14,8 @@
  *              crates/sim-core/src/scenarios/plummer_sphere.rs:21-49,119
  *   plummer    crates/sim-core/src/scenarios/plummer_sphere.rs:13-152
  *   two-body   crates/sim-core/src/scenarios/two_body.rs:49-105
+ *   galaxy     crates/sim-core/src/scenarios/galaxy_collision.rs:20-450,
+ *              scenarios/mod.rs:25-30 (box_muller)
  *   com shift  crates/sim-core/src/particle.rs:168-226
  *   brute      crates/sim-core/src/gravity.rs:47-80
  *   kdk        crates/sim-core/src/integrator.rs:41-77
@@ -247,6 +249,181 @@ void orc_two_body(double e, double mass_ratio, double a, double* x, double* y, d
     m[0] = m1; m[1] = m2;
 }
 double orc_two_body_period(double a) { return 2.0 * ORC_PI * sqrt(a * a * a / (1.0 * 1.0)); }
+
+/* ------------------------------------------------------------------------ */
+/* GalaxyCollision (galaxy_collision.rs:20-450). Parameter block in the
+ * field order of the product's gravis_galaxy_params. */
+typedef struct {
+    uint64_t n_per_galaxy, seed;
+    double disk_fraction, halo_mass_ratio, baryon_mass, disk_scale_radius,
+        disk_scale_height, bulge_scale_radius, halo_scale_radius, disk_max_radius,
+        halo_max_radius, separation, impact_parameter, approach_velocity;
+} orc_galaxy_params;
+
+typedef struct {
+    double *x, *y, *z, *vx, *vy, *vz, *m;
+    uint8_t* type;
+    size_t count;
+} gal_out;
+
+static void gal_push(gal_out* o, double x, double y, double z, double vx, double vy, double vz,
+                     double m, uint8_t t) {
+    size_t i = o->count++;
+    o->x[i] = x; o->y[i] = y; o->z[i] = z;
+    o->vx[i] = vx; o->vy[i] = vy; o->vz[i] = vz;
+    o->m[i] = m; o->type[i] = t;
+}
+
+/* galaxy_collision.rs:193-199 */
+static double gal_disk_mass(double r, double m_disk, double rd) {
+    if (r <= 0.0) return 0.0;
+    double x = r / rd;
+    return m_disk * (1.0 - (1.0 + x) * exp(-x));
+}
+
+/* galaxy_collision.rs:236-257 (G = 1) */
+static double gal_vc(double r, double m_disk, double rd, double m_bulge, double b_a,
+                     double m_halo, double h_a) {
+    if (r <= 0.0) return 0.0;
+    double v2_disk = 1.0 * gal_disk_mass(r, m_disk, rd) / r;
+    double v2_bulge = 1.0 * m_bulge * r * r / pow(r * r + b_a * b_a, 1.5);
+    double v2_halo = 1.0 * m_halo * r / ((r + h_a) * (r + h_a));
+    return sqrt(v2_disk + v2_bulge + v2_halo);
+}
+
+/* galaxy_collision.rs:259-346 generate_disk */
+static void gal_disk(const orc_galaxy_params* p, orc_rng* rng, size_t n_disk, double m_disk,
+                     double m_bulge, double m_halo, gal_out* o) {
+    double mpp = m_disk / (double)n_disk;
+    double r_d = p->disk_scale_radius;
+    double max_x = p->disk_max_radius / r_d;
+    for (size_t k = 0; k < n_disk; ++k) {
+        double r;
+        for (;;) {
+            double u = orc_rng_f64(rng);
+            double r_try = u * p->disk_max_radius;
+            double x = r_try / r_d;
+            double f = x * exp(-x);
+            double f_max = exp(-1.0);
+            double accept = orc_rng_f64(rng);
+            if (accept < f / f_max && x < max_x) { r = r_try; break; }
+        }
+        double phi = orc_rng_range(rng, 0.0, 2.0 * ORC_PI);
+        double x = r * cos(phi);
+        double y = r * sin(phi);
+        double u = orc_rng_range(rng, 0.01, 0.99);
+        double z = p->disk_scale_height * atanh(2.0 * u - 1.0);
+        double v_c = gal_vc(r, m_disk, r_d, m_bulge, p->bulge_scale_radius, m_halo,
+                            p->halo_scale_radius);
+        double sigma_r = 0.15 * v_c;
+        double v_r = orc_rng_range(rng, -1.0, 1.0) * sigma_r;
+        double v_phi = v_c + orc_rng_range(rng, -1.0, 1.0) * sigma_r * 0.5;
+        double sigma_z = 0.1 * v_c;
+        double v_z = orc_rng_range(rng, -1.0, 1.0) * sigma_z;
+        double vx = v_r * cos(phi) - v_phi * sin(phi);
+        double vy = v_r * sin(phi) + v_phi * cos(phi);
+        gal_push(o, x, y, z, vx, vy, v_z, mpp, 1);
+    }
+}
+
+/* galaxy_collision.rs:348-366 generate_bulge */
+static void gal_bulge(const orc_galaxy_params* p, orc_rng* rng, size_t n_bulge, double m_bulge,
+                      gal_out* o) {
+    double mpp = m_bulge / (double)n_bulge;
+    for (size_t k = 0; k < n_bulge; ++k) {
+        double pos[3], vel[3];
+        plummer_particle(rng, m_bulge, p->bulge_scale_radius, pos, vel);
+        gal_push(o, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2], mpp, 2);
+    }
+}
+
+/* scenarios/mod.rs:25-30 box_muller */
+static void gal_box_muller(orc_rng* rng, double* g1, double* g2) {
+    double u1 = orc_rng_range(rng, 0.001, 1.0);
+    double u2 = orc_rng_range(rng, 0.0, 2.0 * ORC_PI);
+    double r = sqrt(-2.0 * log(u1));
+    *g1 = r * cos(u2);
+    *g2 = r * sin(u2);
+}
+
+/* galaxy_collision.rs:368-450 generate_halo */
+static void gal_halo(const orc_galaxy_params* p, orc_rng* rng, size_t n_halo, double m_disk,
+                     double m_bulge, double m_halo, gal_out* o) {
+    double mpp = m_halo / (double)n_halo;
+    double a = p->halo_scale_radius;
+    double xt = p->halo_max_radius / (p->halo_max_radius + a);
+    double u_max = xt * xt;
+    for (size_t k = 0; k < n_halo; ++k) {
+        double u = orc_rng_range(rng, 0.001, u_max);
+        double sqrt_u = sqrt(u);
+        double r = a * sqrt_u / (1.0 - sqrt_u);
+        double cos_t = orc_rng_range(rng, -1.0, 1.0);
+        double sin_t = sqrt(1.0 - cos_t * cos_t);
+        double phi = orc_rng_range(rng, 0.0, 2.0 * ORC_PI);
+        double px = r * sin_t * cos(phi);
+        double py = r * sin_t * sin(phi);
+        double pz = r * cos_t;
+        double v_c = gal_vc(r, m_disk, p->disk_scale_radius, m_bulge, p->bulge_scale_radius,
+                            m_halo, a);
+        double sigma = v_c / sqrt(2.0);
+        double g1, g2, g3, unused;
+        gal_box_muller(rng, &g1, &g2);
+        gal_box_muller(rng, &g3, &unused);
+        gal_push(o, px, py, pz, sigma * g1, sigma * g2, sigma * g3, mpp, 3);
+    }
+}
+
+/* galaxy_collision.rs:75-157 generate; n = 2 * n_per_galaxy. Every array
+ * must hold n entries. com_shift = 0 skips the final shift_to_com_frame
+ * (a test hook: the raw samples are checked against the profiles). */
+void orc_galaxy_collision(const orc_galaxy_params* p, int com_shift, double* x, double* y,
+                          double* z, double* vx, double* vy, double* vz, double* m,
+                          uint8_t* type) {
+    orc_rng rng;
+    orc_rng_seed_u64(&rng, p->seed);
+    double m_disk = p->baryon_mass * p->disk_fraction;
+    double m_bulge = p->baryon_mass * (1.0 - p->disk_fraction);
+    double m_halo = p->baryon_mass * p->halo_mass_ratio;
+    double m_total = m_disk + m_bulge + m_halo;
+    size_t n_disk = (size_t)round((double)p->n_per_galaxy * m_disk / m_total);
+    size_t n_bulge = (size_t)round((double)p->n_per_galaxy * m_bulge / m_total);
+    size_t n_halo = p->n_per_galaxy - n_disk - n_bulge;
+    gal_out o = {x, y, z, vx, vy, vz, m, type, 0};
+    size_t bounds[3] = {0, 0, 0};
+    for (int g = 0; g < 2; ++g) {
+        gal_disk(p, &rng, n_disk, m_disk, m_bulge, m_halo, &o);
+        gal_bulge(p, &rng, n_bulge, m_bulge, &o);
+        gal_halo(p, &rng, n_halo, m_disk, m_bulge, m_halo, &o);
+        bounds[g + 1] = o.count;
+    }
+    double half_sep = p->separation / 2.0;
+    double half_imp = p->impact_parameter / 2.0;
+    double half_vel = p->approach_velocity / 2.0;
+    for (size_t i = bounds[0]; i < bounds[1]; ++i) {
+        x[i] -= half_sep; y[i] -= half_imp; vx[i] += half_vel;
+    }
+    for (size_t i = bounds[1]; i < bounds[2]; ++i) {
+        x[i] += half_sep; y[i] += half_imp; vx[i] -= half_vel;
+    }
+    if (com_shift) orc_shift_to_com(o.count, x, y, z, vx, vy, vz, m);
+}
+
+/* galaxy_collision.rs:159-185 */
+double orc_galaxy_suggested_dt(const orc_galaxy_params* p) {
+    double m_enc_disk = 0.26 * p->baryon_mass * p->disk_fraction;
+    double r = p->disk_scale_radius;
+    double a_b = p->bulge_scale_radius;
+    double m_bulge = p->baryon_mass * (1.0 - p->disk_fraction);
+    double m_enc_bulge = m_bulge * (r * r * r) / pow(r * r + a_b * a_b, 1.5);
+    double a_h = p->halo_scale_radius;
+    double m_halo = p->baryon_mass * p->halo_mass_ratio;
+    double x = r / a_h;
+    double m_enc_halo = m_halo * x * x / ((1.0 + x) * (1.0 + x));
+    double m_enc = m_enc_disk + m_enc_bulge + m_enc_halo;
+    double v_c = sqrt(1.0 * m_enc / r);
+    double t_orb = 2.0 * ORC_PI * r / v_c;
+    return t_orb / 500.0;
+}
 
 /* ------------------------------------------------------------------------ */
 /* Brute force (gravity.rs:47-80): i ascending, j>i ascending, a_i then a_j. */

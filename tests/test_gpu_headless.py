@@ -107,3 +107,16 @@ def test_evrard_collapse_through_the_cli(tmp_path):
     assert float(rs[0]["thermal_energy"]) == pytest.approx(0.05, rel=1e-12)
     assert float(rs[-1]["kinetic_energy"]) > 0.0
     assert "Scenario:   Evrard Collapse" in r.stderr
+
+
+def test_galaxy_collision_through_the_cli(tmp_path):
+    out = str(tmp_path / "g.csv")
+    r = run("--scenario", "galaxy-collision", "-n", "4001", "--steps", "50", "--diag-interval",
+            "25", "--algorithm", "barnes-hut", "--diag-csv", out)
+    err = r.stderr.splitlines()
+    assert err[0] == "Scenario:   Galaxy Collision"
+    assert err[1] == "Particles:  4000"
+    rs = rows(out)
+    assert [x["step"] for x in rs] == ["0", "25", "50"]
+    e0, e1 = float(rs[0]["total_energy"]), float(rs[-1]["total_energy"])
+    assert abs(e1 - e0) / abs(e0) < 1e-3
