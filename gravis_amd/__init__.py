@@ -66,6 +66,23 @@ class Context:
     def set_tuning(self, key: str, value: int):
         self.check(self._L.gravis_set_tuning(self.handle, key.encode(), int(value)))
 
+    def tuning(self) -> dict:
+        """Current launch shapes (gravis_get_tuning)."""
+        t = _lib.Tuning()
+        self.check(self._L.gravis_get_tuning(self.handle, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in _lib.Tuning._fields_ if k != "best_ms"}
+
+    def autotune(self, gravity: "GravitySolver", p: "Particles") -> dict:
+        """Time the candidate launch shapes of `gravity` on p's positions and
+        masses and keep the fastest in this context (gravis_autotune)."""
+        t = _lib.Tuning()
+        d = gravity.desc()
+        self.check(self._L.gravis_autotune(self.handle, C.byref(d), p.count,
+                                           *[_ptr(getattr(p, f)) for f in ("x", "y", "z",
+                                                                           "mass")],
+                                           C.byref(t)))
+        return {k: getattr(t, k) for k, _ in _lib.Tuning._fields_}
+
     def bh_stats(self) -> dict:
         """Work counters of the last Barnes-Hut call."""
         out = (C.c_uint64 * 5)()

@@ -35,7 +35,7 @@ EXPORTS = (
     "gravis_sph_accel_device", "gravis_snapshot_write", "gravis_snapshot_info",
     "gravis_snapshot_read", "gravis_scenario_cold_collapse", "gravis_galaxy_default_params",
     "gravis_scenario_galaxy_collision", "gravis_galaxy_suggested_dt",
-    "gravis_galaxy_suggested_softening",
+    "gravis_galaxy_suggested_softening", "gravis_get_tuning", "gravis_autotune",
 )
 
 
@@ -50,6 +50,11 @@ class Diag(C.Structure):
                 ("momentum", C.c_double * 3), ("angular_momentum", C.c_double * 3),
                 ("center_of_mass", C.c_double * 3), ("center_of_mass_velocity", C.c_double * 3),
                 ("virial_ratio", C.c_double), ("thermal_energy", C.c_double)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("bf_ipt", C.c_int32), ("bf_tile", C.c_int32), ("bf_waves_per_cu", C.c_int32),
+                ("bh_group_size", C.c_uint32), ("best_ms", C.c_double)]
 
 
 class SphParams(C.Structure):
@@ -168,6 +173,8 @@ def load() -> C.CDLL:
                                       dp, C.POINTER(C.c_uint64)]),
         "gravis_snapshot_read": (st, [C.c_char_p, C.POINTER(SphState)]),
         "gravis_galaxy_default_params": (None, [C.POINTER(GalaxyParams)]),
+        "gravis_get_tuning": (st, [vp, C.POINTER(Tuning)]),
+        "gravis_autotune": (st, [vp, C.POINTER(Solver), sz] + [vp] * 4 + [C.POINTER(Tuning)]),
         "gravis_scenario_galaxy_collision": (st, [C.POINTER(GalaxyParams),
                                                   C.POINTER(SphState)]),
         "gravis_galaxy_suggested_dt": (d, [C.POINTER(GalaxyParams)]),

@@ -587,7 +587,127 @@ gravis_status gravis_set_tuning(gravis_ctx* g, const char* key, int64_t value) {
         c->sph_range_cap = (uint32_t)value;
         return GRAVIS_OK;
     }
+    if (std::strcmp(key, "bf_ipt") == 0) {
+        if (value != 1 && value != 2)
+            return fail(c, GRAVIS_INVALID_ARG, "bf_ipt must be 1 or 2");
+        c->bf_ipt = (int)value;
+        return GRAVIS_OK;
+    }
+    if (std::strcmp(key, "bf_near_64ths") == 0) {
+        if (value < 0 || value > 64)
+            return fail(c, GRAVIS_INVALID_ARG, "bf_near_64ths must be in [0, 64]");
+        c->bf_near_fraction = (double)value / 64.0;
+        return GRAVIS_OK;
+    }
+    if (std::strcmp(key, "bf_tile") == 0) {
+        if (value != 128 && value != 256)
+            return fail(c, GRAVIS_INVALID_ARG, "bf_tile must be 128 or 256");
+        c->bf_tile = (int)value;
+        return GRAVIS_OK;
+    }
+    if (std::strcmp(key, "bh_group_size") == 0) {
+        if (value < 1 || value > 64)
+            return fail(c, GRAVIS_INVALID_ARG, "bh_group_size must be in [1, 64]");
+        c->bh_group = (uint32_t)value;
+        return GRAVIS_OK;
+    }
+    if (std::strcmp(key, "bf_waves_per_cu") == 0) {
+        if (value < 1 || value > 1024)
+            return fail(c, GRAVIS_INVALID_ARG, "bf_waves_per_cu must be in [1, 1024]");
+        c->bf_waves_per_cu = (int)value;
+        return GRAVIS_OK;
+    }
     return fail(c, GRAVIS_INVALID_ARG, "unknown tuning key '%s'", key);
+}
+
+gravis_status gravis_get_tuning(gravis_ctx* g, gravis_tuning* out) {
+    if (!g || !out) return GRAVIS_INVALID_ARG;
+    const Ctx* c = &g->c;
+    out->bf_ipt = c->bf_ipt;
+    out->bf_tile = c->bf_tile;
+    out->bf_waves_per_cu = c->bf_waves_per_cu;
+    out->bh_group_size = c->bh_group;
+    out->best_ms = 0.0;
+    return GRAVIS_OK;
+}
+
+gravis_status gravis_autotune(gravis_ctx* g, const gravis_solver* s, size_t n, const double* x,
+                              const double* y, const double* z, const double* m,
+                              gravis_tuning* out) {
+    if (!g) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    GRAVIS_TRY(check_solver(c, s));
+    if (n == 0) return fail(c, GRAVIS_INVALID_ARG, "autotune needs particles");
+    if (any_null({x, y, z, m})) return fail(c, GRAVIS_INVALID_ARG, "null array");
+    GRAVIS_HIP(c, hipSetDevice(c->device));
+    double* base;
+    GRAVIS_TRY(gravis::buf(c, "tune_state", 7 * n, &base));
+    double *dx = base, *dy = base + n, *dz = base + 2 * n, *dm = base + 3 * n;
+    GRAVIS_TRY(h2d(c, dx, x, n)); GRAVIS_TRY(h2d(c, dy, y, n)); GRAVIS_TRY(h2d(c, dz, z, n));
+    GRAVIS_TRY(h2d(c, dm, m, n));
+    gravis::FinishArgs fa;
+    fa.mode = gravis::kAccumulate;
+    fa.ax = base + 4 * n; fa.ay = base + 5 * n; fa.az = base + 6 * n;
+    GRAVIS_HIP(c, hipMemsetAsync(fa.ax, 0, 3 * n * sizeof(double), c->stream));
+
+    // one candidate: warm-up call, then the mean of two timed calls
+    auto time_it = [&](double* ms) -> gravis_status {
+        GRAVIS_TRY(force_device(c, s, n, dx, dy, dz, dm, fa));
+        GRAVIS_HIP(c, hipEventRecord(c->ev0, c->stream));
+        for (int r = 0; r < 2; ++r) GRAVIS_TRY(force_device(c, s, n, dx, dy, dz, dm, fa));
+        GRAVIS_HIP(c, hipEventRecord(c->ev1, c->stream));
+        GRAVIS_HIP(c, hipEventSynchronize(c->ev1));
+        float t = 0.f;
+        GRAVIS_HIP(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
+        *ms = 0.5 * (double)t;
+        return GRAVIS_OK;
+    };
+    const bool profiling = c->profiling;
+    c->profiling = false;  // the timed calls must not touch the caller's kernel timers
+    double best = -1.0;
+    gravis_status st = GRAVIS_OK;
+    if (s->algorithm == GRAVIS_BRUTE_FORCE && s->precision == GRAVIS_MIXED) {
+        int bi = c->bf_ipt, bt = c->bf_tile, bw = c->bf_waves_per_cu;
+        for (int ipt : {1, 2})
+            for (int tile : {128, 256})
+                for (int wpc : {64, 128, 256, 512}) {
+                    c->bf_ipt = ipt; c->bf_tile = tile; c->bf_waves_per_cu = wpc;
+                    double ms;
+                    st = time_it(&ms);
+                    if (st != GRAVIS_OK) break;
+                    if (best < 0.0 || ms < best) { best = ms; bi = ipt; bt = tile; bw = wpc; }
+                }
+        c->bf_ipt = bi; c->bf_tile = bt; c->bf_waves_per_cu = bw;
+    } else if (s->algorithm == GRAVIS_BRUTE_FORCE) {
+        int bw = c->bf_waves_per_cu;
+        for (int wpc : {32, 64, 128, 256, 512}) {
+            c->bf_waves_per_cu = wpc;
+            double ms;
+            st = time_it(&ms);
+            if (st != GRAVIS_OK) break;
+            if (best < 0.0 || ms < best) { best = ms; bw = wpc; }
+        }
+        c->bf_waves_per_cu = bw;
+    } else if (s->algorithm == GRAVIS_BARNES_HUT && s->group_size == 0) {
+        uint32_t bg = c->bh_group;
+        for (uint32_t gs : {16u, 32u, 64u}) {
+            c->bh_group = gs;
+            double ms;
+            st = time_it(&ms);
+            if (st != GRAVIS_OK) break;
+            if (best < 0.0 || ms < best) { best = ms; bg = gs; }
+        }
+        c->bh_group = bg;
+    } else {
+        st = time_it(&best);
+    }
+    c->profiling = profiling;
+    GRAVIS_TRY(st);
+    if (out) {
+        GRAVIS_TRY(gravis_get_tuning(g, out));
+        out->best_ms = best;
+    }
+    return GRAVIS_OK;
 }
 
 gravis_status gravis_bh_stats(gravis_ctx* g, uint64_t out[5]) {

@@ -6,18 +6,21 @@
 //
 // The reference visits pairs i<j once (Newton's third law, sequential). Here
 // every target i gathers over all j (no write conflicts, no atomics): a
-// workgroup owns BLOCK*IPT targets of the Morton-sorted particle list and a
-// contiguous j range; j records are staged through LDS in TJ-wide tiles and
-// read back as wavefront-uniform broadcasts. Pairwise gravity is not a dense
-// contraction, so this is VALU work, not MFMA work. The j range is split in
-// S chunks so the grid holds many workgroups per CU; the chunk partials are
-// summed in a fixed order by the finishing kernel (deterministic).
+// workgroup owns a run of the Morton-sorted particle list as targets and a
+// contiguous j range; j records are staged through LDS in tiles and read
+// back as wavefront-uniform broadcasts. Pairwise gravity is not a dense
+// contraction, so this is VALU work (the matrix cores were measured: with
+// four useful output columns v_mfma_f32_4x4x1 issues at a quarter of the
+// FP32 matrix rate and saves about 5 %). The j range is split in S chunks so
+// the grid holds many wavefronts per CU; the chunk partials are summed in a
+// fixed order by the finishing kernel (deterministic).
 //
 // Mixed precision: separations are formed in FP64 relative to the centre of
-// the workgroup's target box, rounded to FP32, and the pair force is FP32
-// (v_rsq_f32). Each thread sums a tile of TJ pair forces in FP32 and adds the
-// tile sum into a Kahan-compensated FP32 accumulator; the result leaves as
-// FP64 sum - compensation.
+// the wavefront's target box, rounded to FP32, and the pair force is FP32
+// (v_rsq_f32); sources close to the group get FP64 separations per pair.
+// Each lane sums a tile of pair forces in FP32 and adds the tile sum into a
+// Kahan-compensated FP32 accumulator; the result leaves as FP64
+// sum - compensation.
 #include "internal.h"
 
 namespace gravis {
@@ -50,45 +53,6 @@ __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
     const float t = s + y;
     c = (t - s) - y;
     s = t;
-}
-
-// Block-wide bounding-box centre of the targets (FP64).
-template <int BLOCK, int IPT>
-__device__ __forceinline__ void block_box(const double (&x)[IPT], const double (&y)[IPT],
-                                          const double (&z)[IPT], double (&lo)[3],
-                                          double (&hi)[3]) {
-    lo[0] = hi[0] = x[0];
-    lo[1] = hi[1] = y[0];
-    lo[2] = hi[2] = z[0];
-#pragma unroll
-    for (int k = 1; k < IPT; ++k) {
-        lo[0] = fmin(lo[0], x[k]); hi[0] = fmax(hi[0], x[k]);
-        lo[1] = fmin(lo[1], y[k]); hi[1] = fmax(hi[1], y[k]);
-        lo[2] = fmin(lo[2], z[k]); hi[2] = fmax(hi[2], z[k]);
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            lo[d] = fmin(lo[d], __shfl_xor(lo[d], off));
-            hi[d] = fmax(hi[d], __shfl_xor(hi[d], off));
-        }
-    }
-    __shared__ double s[BLOCK / kWave][6];
-    const int w = threadIdx.x / kWave;
-    if ((threadIdx.x & (kWave - 1)) == 0)
-        for (int d = 0; d < 3; ++d) { s[w][d] = lo[d]; s[w][3 + d] = hi[d]; }
-    __syncthreads();
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        lo[d] = s[0][d];
-        hi[d] = s[0][3 + d];
-    }
-    for (int k = 1; k < BLOCK / kWave; ++k)
-        for (int d = 0; d < 3; ++d) {
-            lo[d] = fmin(lo[d], s[k][d]);
-            hi[d] = fmax(hi[d], s[k][3 + d]);
-        }
 }
 
 // Bounding box of each TJ-wide tile of the sorted records: tbox[6 t ..] =
@@ -130,55 +94,105 @@ __global__ __launch_bounds__(256) void tile_box_kernel(const double4* __restrict
     }
 }
 
-// A j tile is "near" the target box when the gap between the two boxes is
-// smaller than the larger box diagonal. Near tiles get separations formed
-// in FP64 per pair (and a Newton step on rsqrt): there the FP32 rounding of
-// box-relative coordinates would be comparable to the pair separation. Far
-// tiles keep box-relative FP32 coordinates, whose rounding error is a few
-// ulp of the separation.
+// A wave-uniform double moved to scalar registers (frees two VGPRs).
+__device__ __forceinline__ double uniform(double v) {
+    const uint64_t b = __double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Bounding box of the wavefront's IPT*64 targets (FP64); every lane ends
+// with the same box, held in scalar registers.
+template <int IPT>
+__device__ __forceinline__ void wave_box(const double (&x)[IPT], const double (&y)[IPT],
+                                         const double (&z)[IPT], double (&lo)[3],
+                                         double (&hi)[3]) {
+    lo[0] = hi[0] = x[0];
+    lo[1] = hi[1] = y[0];
+    lo[2] = hi[2] = z[0];
+#pragma unroll
+    for (int k = 1; k < IPT; ++k) {
+        lo[0] = fmin(lo[0], x[k]); hi[0] = fmax(hi[0], x[k]);
+        lo[1] = fmin(lo[1], y[k]); hi[1] = fmax(hi[1], y[k]);
+        lo[2] = fmin(lo[2], z[k]); hi[2] = fmax(hi[2], z[k]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], __shfl_xor(lo[d], off));
+            hi[d] = fmax(hi[d], __shfl_xor(hi[d], off));
+        }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = uniform(lo[d]);
+        hi[d] = uniform(hi[d]);
+    }
+}
+
+// Far/near split of the mixed kernel. Source coordinates are rounded to
+// FP32 relative to the centre c of the target group's box, so a separation
+// d = x_j - x_i carries an absolute error of about u (|x_i - c| + |x_j - c|)
+// <= u (diag + |d|), u = 2^-24. The pair force is m d / (|d|^2 + eps^2)^(3/2),
+// so the error matters only through u diag / max(|d|, eps). A tile is "near"
+// when both its gap to the target box and eps are below q * diag: there the
+// separations are formed in FP64 per pair. Every FP32 pair then has
+// max(|d|, eps) >= q * diag and a relative force error of at most about
+// 3 u (1 + 1 / q). q is the context's bf_near_fraction (default 1/32).
 __device__ __forceinline__ bool tile_is_near(const double* __restrict__ tb, const double (&lo)[3],
-                                             const double (&hi)[3]) {
-    double gap2 = 0.0, di2 = 0.0, dj2 = 0.0;
+                                             const double (&hi)[3], double lim2) {
+    double gap2 = 0.0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const double g = fmax(0.0, fmax(tb[d] - hi[d], lo[d] - tb[3 + d]));
         gap2 = __builtin_fma(g, g, gap2);
-        const double ei = hi[d] - lo[d], ej = tb[3 + d] - tb[d];
-        di2 = __builtin_fma(ei, ei, di2);
-        dj2 = __builtin_fma(ej, ej, dj2);
     }
-    return gap2 < fmax(di2, dj2);
+    return gap2 < lim2;
 }
 
-template <int BLOCK, int IPT, int TJ, bool MASK>
-__global__ __launch_bounds__(BLOCK) void bf_mixed_kernel(const double4* __restrict__ pm, int n,
-                                                         int jchunk, float eps2,
+// Mixed precision: one wavefront per workgroup owns IPT*64 Morton-consecutive
+// targets and one j chunk. Each TJ-record tile of the chunk is staged through
+// LDS by the wave itself: far tiles as FP32 records relative to the wave's
+// own target-box centre (so the rounding error scales with the size of 64 or
+// 128 targets, not of a larger block), near tiles as FP64 records. The far
+// loop is 12 VALU operations and one v_rsq_f32 per pair; the transcendental
+// issues beside the FMAs, so the loop is bound by the 12 FMA-class slots.
+// Tile sums are FP32, added into a Kahan-compensated FP32 accumulator.
+template <int IPT, int TJ, bool MASK>
+__global__ __launch_bounds__(kWave) void bf_mixed_kernel(const double4* __restrict__ pm, int n,
+                                                         int jchunk, float eps2, double near2,
                                                          const double* __restrict__ tbox,
                                                          double* __restrict__ part) {
-    static_assert(TJ % BLOCK == 0, "tile must be a multiple of the block");
-    constexpr int LPT = TJ / BLOCK;  // j records each thread stages per tile
-    // Near tiles are staged as FP64 records, far tiles as FP32 box-relative
-    // records in the first half of the same buffer.
-    __shared__ double4 tile[2][TJ];
+    static_assert(TJ % kWave == 0, "tile must be a multiple of the wavefront");
+    constexpr int LPT = TJ / kWave;  // j records each lane stages per tile
+    __shared__ double4 tile[TJ];     // near: FP64 records; far: float4 [0, TJ)
 
-    const int ibase = blockIdx.x * BLOCK * IPT;
+    const int lane = threadIdx.x;
+    const int ibase = blockIdx.x * kWave * IPT;
     int iidx[IPT];
     double lo[3], hi[3];
     float xi[IPT], yi[IPT], zi[IPT];
-    double cx, cy, cz;
+    double cx, cy, cz, lim2;
     {
         double xd[IPT], yd[IPT], zd[IPT];
 #pragma unroll
         for (int k = 0; k < IPT; ++k) {
-            int i = ibase + k * BLOCK + threadIdx.x;
+            const int i = ibase + k * kWave + lane;
             iidx[k] = i < n ? i : ibase;  // out-of-range targets mirror a valid one
             const double4 p = pm[iidx[k]];
             xd[k] = p.x; yd[k] = p.y; zd[k] = p.z;
         }
-        block_box<BLOCK, IPT>(xd, yd, zd, lo, hi);
+        wave_box<IPT>(xd, yd, zd, lo, hi);
         cx = 0.5 * (lo[0] + hi[0]);
         cy = 0.5 * (lo[1] + hi[1]);
         cz = 0.5 * (lo[2] + hi[2]);
+        double di2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) di2 = __builtin_fma(hi[d] - lo[d], hi[d] - lo[d], di2);
+        lim2 = near2 * di2;
+        // softening above q * diag bounds every pair's error: no near tiles
+        if ((double)eps2 >= lim2) lim2 = 0.0;
 #pragma unroll
         for (int k = 0; k < IPT; ++k) {
             xi[k] = (float)(xd[k] - cx);
@@ -199,20 +213,20 @@ __global__ __launch_bounds__(BLOCK) void bf_mixed_kernel(const double4* __restri
     auto load = [&](int t) {
 #pragma unroll
         for (int e = 0; e < LPT; ++e) {
-            const int j = j0 + t * TJ + e * BLOCK + threadIdx.x;
+            const int j = j0 + t * TJ + e * kWave + lane;
             // padding: zero mass at the box centre (no contribution)
             stage[e] = j < j1 ? pm[j] : make_double4(cx, cy, cz, 0.0);
         }
     };
-    auto store = [&](int b, bool near) {
+    auto store = [&](bool near) {
         if (near) {
 #pragma unroll
-            for (int e = 0; e < LPT; ++e) tile[b][e * BLOCK + threadIdx.x] = stage[e];
+            for (int e = 0; e < LPT; ++e) tile[e * kWave + lane] = stage[e];
         } else {
-            float4* t32 = reinterpret_cast<float4*>(&tile[b][0]);
+            float4* t32 = reinterpret_cast<float4*>(&tile[0]);
 #pragma unroll
             for (int e = 0; e < LPT; ++e)
-                t32[e * BLOCK + threadIdx.x] =
+                t32[e * kWave + lane] =
                     make_float4((float)(stage[e].x - cx), (float)(stage[e].y - cy),
                                 (float)(stage[e].z - cz), (float)stage[e].w);
         }
@@ -220,24 +234,26 @@ __global__ __launch_bounds__(BLOCK) void bf_mixed_kernel(const double4* __restri
 
     bool near_cur = false;
     if (ntiles > 0) {
-        near_cur = tile_is_near(tbox + (size_t)tile0 * 6, lo, hi);
+        near_cur = tile_is_near(tbox + (size_t)tile0 * 6, lo, hi, lim2);
         load(0);
-        store(0, near_cur);
     }
-    __syncthreads();
     for (int t = 0; t < ntiles; ++t) {
-        const int cur = t & 1;
+        store(near_cur);
+        __syncthreads();
         bool near_next = false;
         if (t + 1 < ntiles) {
-            near_next = tile_is_near(tbox + (size_t)(tile0 + t + 1) * 6, lo, hi);
+            near_next = tile_is_near(tbox + (size_t)(tile0 + t + 1) * 6, lo, hi, lim2);
             load(t + 1);
         }
         float ax[IPT], ay[IPT], az[IPT];
 #pragma unroll
         for (int k = 0; k < IPT; ++k) ax[k] = ay[k] = az[k] = 0.f;
         if (!near_cur) {
-            const float4* t32 = reinterpret_cast<const float4*>(&tile[cur][0]);
-#pragma unroll 8
+            const float4* t32 = reinterpret_cast<const float4*>(&tile[0]);
+            // unroll depth chosen for register pressure: 63 VGPRs (8 waves
+            // per SIMD) at IPT = 1, 83 (6 waves) at IPT = 2
+            constexpr int kUnroll = IPT == 1 ? 4 : 8;
+#pragma unroll kUnroll
             for (int j = 0; j < TJ; ++j) {
                 const float4 s = t32[j];
 #pragma unroll
@@ -256,9 +272,9 @@ __global__ __launch_bounds__(BLOCK) void bf_mixed_kernel(const double4* __restri
                 const double4 p = pm[iidx[k]];
                 px[k] = p.x; py[k] = p.y; pz[k] = p.z;
             }
-#pragma unroll 2
+#pragma unroll 1
             for (int j = 0; j < TJ; ++j) {
-                const double4 s = tile[cur][j];
+                const double4 s = tile[j];
                 const float m = (float)s.w;
 #pragma unroll
                 for (int k = 0; k < IPT; ++k) {
@@ -284,13 +300,12 @@ __global__ __launch_bounds__(BLOCK) void bf_mixed_kernel(const double4* __restri
             kahan_add(sy[k], ky[k], ay[k]);
             kahan_add(sz[k], kz[k], az[k]);
         }
-        if (t + 1 < ntiles) store(cur ^ 1, near_next);
         near_cur = near_next;
         __syncthreads();
     }
 #pragma unroll
     for (int k = 0; k < IPT; ++k) {
-        const int i = ibase + k * BLOCK + threadIdx.x;
+        const int i = ibase + k * kWave + lane;
         if (i < n) {
             const size_t base = (size_t)blockIdx.y * 3 * n;
             part[base + i] = (double)sx[k] - (double)kx[k];
@@ -378,14 +393,15 @@ __global__ __launch_bounds__(BLOCK) void bf_fp64_kernel(const double4* __restric
 
 constexpr size_t kSortMinN = 2048;
 
-template <int BLOCK, int IPT, int TJ>
+template <int IPT, int TJ>
 void launch_mixed(dim3 g, hipStream_t st, bool mask, const double4* pm, int n, int jchunk,
-                  float eps2, const double* tbox, double* part) {
+                  float eps2, double near2, const double* tbox, double* part) {
     if (mask)
-        bf_mixed_kernel<BLOCK, IPT, TJ, true><<<g, BLOCK, 0, st>>>(pm, n, jchunk, eps2, tbox, part);
+        bf_mixed_kernel<IPT, TJ, true><<<g, kWave, 0, st>>>(pm, n, jchunk, eps2, near2, tbox,
+                                                            part);
     else
-        bf_mixed_kernel<BLOCK, IPT, TJ, false><<<g, BLOCK, 0, st>>>(pm, n, jchunk, eps2, tbox,
-                                                                     part);
+        bf_mixed_kernel<IPT, TJ, false><<<g, kWave, 0, st>>>(pm, n, jchunk, eps2, near2, tbox,
+                                                             part);
 }
 
 template <int BLOCK, int IPT, int TJ>
@@ -459,12 +475,16 @@ gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
     GRAVIS_TRY(buf(c, "bf_pm", n, &pm));
     GRAVIS_TRY(gather_sorted(c, n, x, y, z, m, perm, pm));
 
-    const int block = 256;
-    const int ipt = mixed ? 2 : 1;
-    const int tj = 256;
-    const int gx = (ni + block * ipt - 1) / (block * ipt);
-    // Enough workgroups for 8 per CU, but at least one tile per chunk.
-    const int target = 8 * c->num_cus;
+    // FP64: 256-thread workgroups of 256 targets. Mixed: one wavefront per
+    // workgroup with bf_ipt * 64 targets and bf_tile-record tiles.
+    const int ipt = mixed ? c->bf_ipt : 1;
+    const int group = mixed ? kWave * ipt : 256;
+    const int tj = mixed ? c->bf_tile : 256;
+    const int gx = (ni + group - 1) / group;
+    // Split the j range so the grid holds about bf_waves_per_cu wavefronts
+    // per CU, keeping at least one tile per chunk.
+    const int waves_per_group = mixed ? 1 : 4;
+    const int target = (c->bf_waves_per_cu * c->num_cus + waves_per_group - 1) / waves_per_group;
     int S = (target + gx - 1) / gx;
     const int max_s = (ni + tj - 1) / tj;
     if (S > max_s) S = max_s;
@@ -481,14 +501,25 @@ gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
     if (mixed) {
         const int ntile = (ni + tj - 1) / tj;
         GRAVIS_TRY(buf(c, "bf_tbox", (size_t)ntile * 6, &tbox));
-        tile_box_kernel<256><<<ntile, 256, 0, c->stream>>>(pm, ni, tbox);
+        if (tj == 256)
+            tile_box_kernel<256><<<ntile, 256, 0, c->stream>>>(pm, ni, tbox);
+        else
+            tile_box_kernel<128><<<ntile, 256, 0, c->stream>>>(pm, ni, tbox);
         GRAVIS_TRY(check_launch(c, "tile_box"));
     }
     timer_begin(c);
     if (mixed) {
         const float eps2f = (float)eps2;
         const bool mask = !(eps2f >= 1.17549435e-38f);
-        launch_mixed<256, 2, 256>(grid, c->stream, mask, pm, ni, jchunk, eps2f, tbox, part);
+        const double f2 = c->bf_near_fraction * c->bf_near_fraction;
+        if (ipt == 1 && tj == 128)
+            launch_mixed<1, 128>(grid, c->stream, mask, pm, ni, jchunk, eps2f, f2, tbox, part);
+        else if (ipt == 1)
+            launch_mixed<1, 256>(grid, c->stream, mask, pm, ni, jchunk, eps2f, f2, tbox, part);
+        else if (tj == 128)
+            launch_mixed<2, 128>(grid, c->stream, mask, pm, ni, jchunk, eps2f, f2, tbox, part);
+        else
+            launch_mixed<2, 256>(grid, c->stream, mask, pm, ni, jchunk, eps2f, f2, tbox, part);
         timer_end(c, "bf_mixed");
     } else {
         const bool mask = !(eps2 > 0.0);

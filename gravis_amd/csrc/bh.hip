@@ -354,7 +354,7 @@ gravis_status bh_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
                        const double* y, const double* z, const double* m, const FinishArgs& fa) {
     if (n == 0) return GRAVIS_OK;
     const uint32_t leaf = s->leaf_size ? s->leaf_size : 16u;
-    const uint32_t gs = s->group_size ? s->group_size : (uint32_t)kWave;
+    const uint32_t gs = s->group_size ? s->group_size : c->bh_group;
     Octree t;
     GRAVIS_TRY(build_octree(c, n, x, y, z, m, leaf, gs, &t));
     return bh_walk(c, s, t, fa);

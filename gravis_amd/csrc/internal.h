@@ -53,6 +53,19 @@ struct Ctx {
     // SPH neighbour search: overlapping leaves one wave may hold in LDS
     // before its group is searched depth first (at most 512)
     uint32_t sph_range_cap = 512;
+    // Brute-force launch shape (gravis_set_tuning / gravis_autotune):
+    // targets per lane of the mixed kernel (wavefront groups of 64 or 128),
+    // j records per staged tile (128 or 256), and the wavefronts per CU the
+    // j-chunk split aims for.
+    int bf_ipt = 2;
+    int bf_tile = 128;
+    int bf_waves_per_cu = 512;
+    // Barnes-Hut target group when the solver leaves group_size at 0
+    uint32_t bh_group = 64;
+    // Mixed brute force: a source tile closer to the target box than this
+    // fraction q of the box diagonal gets FP64 pair separations, unless the
+    // softening exceeds q times the diagonal.
+    double bf_near_fraction = 1.0 / 32.0;
 };
 
 // Status helpers ----------------------------------------------------------

@@ -47,7 +47,8 @@ typedef enum gravis_algorithm {
 
 typedef enum gravis_precision {
     GRAVIS_FP64 = 0,  /* f64 throughout, as sim-core */
-    GRAVIS_MIXED = 1  /* FP32 pair forces on FP64 tile-relative separations,
+    GRAVIS_MIXED = 1  /* FP32 pair forces on separations formed in FP64
+                         relative to each wavefront's target-box centre,
                          compensated FP32 accumulation, FP64 state */
 } gravis_precision;
 
@@ -59,7 +60,8 @@ typedef struct gravis_solver {
     double softening;    /* Plummer softening eps >= 0 */
     double theta;        /* Barnes-Hut opening angle >= 0 */
     uint32_t leaf_size;  /* Barnes-Hut leaf capacity (0 = 16) */
-    uint32_t group_size; /* Barnes-Hut target group (0 = 64, one wavefront) */
+    uint32_t group_size; /* Barnes-Hut target group, at most 64 (0 = the
+                            context's bh_group_size, default 64) */
 } gravis_solver;
 
 /* Diagnostics (crates/sim-core/src/diagnostics.rs:5-30). */
@@ -313,6 +315,37 @@ gravis_status gravis_kernel_time(gravis_ctx* ctx, const char* name, double* ms_t
  * neighbour search holds (1..512, default 512; a group needing more is
  * searched one target at a time, depth first). */
 gravis_status gravis_set_tuning(gravis_ctx* ctx, const char* key, int64_t value);
+
+/* Launch shapes of the context (no reference counterpart). Brute force:
+ * targets per lane of the mixed kernel (1 or 2: 64 or 128 targets per
+ * wavefront), j records per staged tile (128 or 256), wavefronts per CU the
+ * j-range split aims for; Barnes-Hut: target group size used when the
+ * solver's group_size is 0. Set one at a time with gravis_set_tuning keys
+ * "bf_ipt", "bf_tile", "bf_waves_per_cu", "bh_group_size"; "bf_near_64ths"
+ * (0..64, default 2) is the near-tile fraction q of the mixed kernel: a
+ * source tile whose gap to the target box and the softening are both below
+ * q times the box diagonal gets FP64 pair separations. */
+typedef struct gravis_tuning {
+    int32_t bf_ipt;
+    int32_t bf_tile;
+    int32_t bf_waves_per_cu;
+    uint32_t bh_group_size;
+    double best_ms; /* gravis_autotune: mean time of one force call, chosen shape */
+} gravis_tuning;
+
+gravis_status gravis_get_tuning(gravis_ctx* ctx, gravis_tuning* out);
+
+/* Autotuner (no reference counterpart): times the candidate launch shapes
+ * for solver s on the given particles (host arrays, copied once) and keeps
+ * the fastest in the context. Mixed brute force: bf_ipt x bf_tile x
+ * bf_waves_per_cu (16 shapes); FP64 brute force: bf_waves_per_cu;
+ * Barnes-Hut with group_size 0: group sizes 16, 32, 64. Each shape costs
+ * three force calls. Shapes change the summation grouping, so forces move in
+ * their last bits (mixed) or within the opening-angle error (Barnes-Hut
+ * groups). out may be NULL. */
+gravis_status gravis_autotune(gravis_ctx* ctx, const gravis_solver* s, size_t n,
+                              const double* x, const double* y, const double* z,
+                              const double* m, gravis_tuning* out);
 
 /* Work counters of the last Barnes-Hut force call: out[0] frontier node
  * tests, out[1] accepted (monopole) nodes, out[2] leaf particles used as

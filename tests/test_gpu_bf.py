@@ -121,6 +121,50 @@ def test_bf_mixed_error_budget(g, n):
     assert mx <= 1e-4
 
 
+@pytest.fixture(scope="module")
+def bf_ref():
+    """FP64 GPU forces of the inputs the kernel-variant test uses (cached)."""
+    return {}
+
+
+@pytest.mark.parametrize("n,eps", [(4099, 0.01), (65537, 0.01), (20011, 0.0), (1500, 0.05)])
+@pytest.mark.parametrize("ipt,tile", [(1, 128), (2, 128), (1, 256), (2, 256)])
+def test_bf_mixed_launch_shapes(g, bf_ref, ipt, tile, n, eps):
+    """Every mixed launch shape (64 or 128 targets per wavefront, 128 or 256
+    records per tile, j split for 8 or 32 wavefronts per CU) against the FP64
+    path: RMS |da|/|a| <= 1e-6, max <= 1e-4. eps = 0 takes the masked
+    self-pair path (positions distinct) and the FP64 near tiles; n = 1500
+    runs unsorted (below the Morton-sort threshold)."""
+    say(f"bf mixed ipt={ipt} tile={tile} n={n} eps={eps}")
+    s = O.plummer(n, 7)
+    key = (n, eps)
+    if key not in bf_ref:
+        p64 = to_particles(g, s)
+        g.BruteForce(eps).compute_accelerations(p64)
+        bf_ref[key] = p64.acc()
+    ctx = g.Context(0)
+    ctx.set_tuning("bf_ipt", ipt)
+    ctx.set_tuning("bf_tile", tile)
+    for wpc in (8, 32):
+        ctx.set_tuning("bf_waves_per_cu", wpc)
+        pm = to_particles(g, s)
+        g.BruteForce(eps, "mixed", ctx=ctx).compute_accelerations(pm)
+        rms, mx = rel_errors(bf_ref[key], pm.acc())
+        say(f"  waves/CU {wpc}: rms={rms:.3e} max={mx:.3e}")
+        assert rms <= 1e-6
+        assert mx <= 1e-4
+    ctx.close()
+
+
+def test_bf_tuning_keys_validate(g):
+    ctx = g.Context(0)
+    for key, bad in (("bf_ipt", 3), ("bf_waves_per_cu", 0), ("bf_kernel", 1), ("bf_tile", 64),
+                     ("bh_group_size", 65), ("bf_near_64ths", 65)):
+        with pytest.raises(g.GravisError):
+            ctx.set_tuning(key, bad)
+    ctx.close()
+
+
 def test_bf_invalid_args(g):
     say("bf invalid args")
     p = g.Particles.from_arrays([0.0, 1.0], [0, 0], [0, 0], [0, 0], [0, 0], [0, 0], [1, 1])
@@ -252,3 +296,36 @@ def test_conservation_momentum_com(g):
             assert np.linalg.norm(d.momentum) < 1e-12
         else:
             assert np.linalg.norm(d.center_of_mass) < 1e-12
+
+
+def test_autotune_brute_force_and_barnes_hut(g):
+    """The autotuner picks shapes from its candidate sets, keeps them in the
+    context, and the tuned forces still meet the mixed error bars (brute
+    force) and the reference's theta = 0.5 error (Barnes-Hut, m3_accuracy:
+    RMS 4.942351e-3)."""
+    say("autotune bf mixed / bf fp64 / bh, n=20000")
+    s = O.plummer(20000, 5)
+    p64 = to_particles(g, s)
+    g.BruteForce(0.01).compute_accelerations(p64)
+    ref = p64.acc()
+    ctx = g.Context(0)
+    t = ctx.autotune(g.BruteForce(0.01, "mixed", ctx=ctx), to_particles(g, s))
+    say(f"  mixed: {t}")
+    assert t["bf_ipt"] in (1, 2) and t["bf_tile"] in (128, 256)
+    assert t["bf_waves_per_cu"] in (64, 128, 256, 512) and t["best_ms"] > 0
+    assert ctx.tuning() == {k: v for k, v in t.items() if k != "best_ms"}
+    pm = to_particles(g, s)
+    g.BruteForce(0.01, "mixed", ctx=ctx).compute_accelerations(pm)
+    rms, mx = rel_errors(ref, pm.acc())
+    assert rms <= 1e-6 and mx <= 1e-4
+    t = ctx.autotune(g.BruteForce(0.01, ctx=ctx), to_particles(g, s))
+    say(f"  fp64: {t}")
+    assert t["bf_waves_per_cu"] in (32, 64, 128, 256, 512)
+    t = ctx.autotune(g.BarnesHut(0.01, 0.5, ctx=ctx), to_particles(g, s))
+    say(f"  bh: {t}")
+    assert t["bh_group_size"] in (16, 32, 64)
+    pb = to_particles(g, s)
+    g.BarnesHut(0.01, 0.5, ctx=ctx).compute_accelerations(pb)
+    rms, _ = rel_errors(ref, pb.acc())
+    assert rms <= 4.942351e-3
+    ctx.close()
