@@ -62,6 +62,9 @@ def parse():
     ap.add_argument("--c4-n", type=int, default=1048576)
     ap.add_argument("--c4-steps", type=int, default=200)
     ap.add_argument("--no-c4", action="store_true")
+    ap.add_argument("--bf-shape", default=None,
+                    help="ipt,tile,waves_per_cu: pin the mixed brute-force launch shape "
+                         "instead of autotuning (profiling runs)")
     ap.add_argument("--probe-copy", action="store_true",
                     help="only run a 1 GiB device copy (PMC byte-counter calibration)")
     return ap.parse_args()
@@ -113,7 +116,8 @@ def probe_copy():
 def run_bh(args, g, torch, ctx, precision="mixed", steps=None):
     """Barnes-Hut KDK steps on two Plummer spheres of n_each particles (seeds
     42/43, a = 1, M = 1 each) 10 a apart on a parabolic approach, eps 0.01,
-    dt 5e-4, theta 0.5, leaf 16, groups of at most 64. precision 'mixed':
+    dt 5e-4, theta 0.5, leaf 16, target groups of the autotuned size (at most
+    64). precision 'mixed':
     FP32 pair forces on FP64 group-relative separations (FP64 tree, state and
     opening test); 'fp64': the reference arithmetic."""
     steps = steps or args.bh_steps
@@ -128,7 +132,11 @@ def run_bh(args, g, torch, ctx, precision="mixed", steps=None):
     for f in ("ax", "ay", "az"):
         T[f] = torch.zeros(n, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
-    bh = g.BarnesHut(0.01, 0.5, 16, 64, precision=precision, ctx=ctx)
+    # group size 0: the context's, chosen by the autotuner on the mixed run
+    bh = g.BarnesHut(0.01, 0.5, 16, 0, precision=precision, ctx=ctx)
+    if precision == "mixed":
+        log(f"bench: autotuned BH: {ctx.autotune(bh, p)}")
+    group = ctx.tuning()["bh_group_size"]
     L = ctx._L
     keys = ("x", "y", "z", "vx", "vy", "vz", "mass", "ax", "ay", "az")
     d = bh.desc()
@@ -160,7 +168,7 @@ def run_bh(args, g, torch, ctx, precision="mixed", steps=None):
     walk_bytes = st["node_tests"] * 48 + (st["accepted_nodes"] + st["leaf_particles"]) * 32 \
         + n * 56
     return {"n": n, "precision": precision, "steps_per_s": round(1.0 / t_step, 3),
-            "ms_per_step": round(t_step * 1e3, 3), "steps": steps, "theta": 0.5, "leaf_size": 16, "group_size": 64,
+            "ms_per_step": round(t_step * 1e3, 3), "steps": steps, "theta": 0.5, "leaf_size": 16, "group_size": group,
             "softening": 0.01, "dt": 5e-4,
             "walk_kernel_ms": round(walk_s * 1e3, 3),
             "walk_stats": st,
@@ -449,6 +457,17 @@ def main():
                                           ("x", "y", "z", "mass")],
                                         *[C.c_void_p(o.data_ptr()) for o in out]))
 
+    # Launch shapes of the mixed kernel picked by the library's autotuner on
+    # these particles (outside the timed region).
+    if args.bf_shape:
+        ipt, tile, wpc = (int(v) for v in args.bf_shape.split(","))
+        ctx.set_tuning("bf_ipt", ipt)
+        ctx.set_tuning("bf_tile", tile)
+        ctx.set_tuning("bf_waves_per_cu", wpc)
+        tuning = dict(ctx.tuning(), pinned=True)
+    else:
+        tuning = ctx.autotune(mixed, p)
+    log(f"bench: mixed brute-force launch shape: {tuning}")
     # a(t0), as headless does before the loop (main.rs:151-152)
     accel(mixed, (T["ax"], T["ay"], T["az"]))
     log(f"bench: warmup {args.warmup} mixed KDK steps")
@@ -542,8 +561,9 @@ def main():
         "dtype": "f32 pair forces / f64 state",
         "data": "synthetic: Plummer sphere, sim-core ChaCha20 ICs, seed 42",
         "config": {"workload": f"Plummer N={n}, eps={eps}, dt={dt}, leapfrog KDK, mixed-precision "
-                               "brute-force gravity (FP32 pair forces on FP64 tile-relative "
-                               "separations, compensated accumulation)",
+                               "brute-force gravity (FP32 pair forces on separations formed "
+                               "in FP64 relative to each wavefront's target box, compensated "
+                               "accumulation)",
                    "n": n, "softening": eps, "dt": dt, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
@@ -551,6 +571,7 @@ def main():
                      "kernel_ms": round(k_mixed_s * 1e3, 4),
                      "flop_per_interaction": FLOP_PER_INTERACTION},
         "cpu_baseline": cpu,
+        "tuning": tuning,
         "fp64": {"value": round(inter / t_fp64 / 1e9, 3), "unit": "G-interactions/s",
                  "ms_per_step": round(t_fp64 * 1e3, 4), "kernel_ms": round(k_fp64_s * 1e3, 4),
                  "achieved_tflops": round(FLOP_PER_INTERACTION * inter / k_fp64_s / 1e12, 3),

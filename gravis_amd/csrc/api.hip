@@ -4,6 +4,7 @@
 // into a gravis_status plus a message, and never throws across the ABI.
 #include <cmath>
 #include <cstdarg>
+#include <algorithm>
 #include <cstring>
 #include <new>
 
@@ -666,28 +667,39 @@ gravis_status gravis_autotune(gravis_ctx* g, const gravis_solver* s, size_t n, c
     c->profiling = false;  // the timed calls must not touch the caller's kernel timers
     double best = -1.0;
     gravis_status st = GRAVIS_OK;
-    if (s->algorithm == GRAVIS_BRUTE_FORCE && s->precision == GRAVIS_MIXED) {
-        int bi = c->bf_ipt, bt = c->bf_tile, bw = c->bf_waves_per_cu;
-        for (int ipt : {1, 2})
-            for (int tile : {128, 256})
-                for (int wpc : {64, 128, 256, 512}) {
-                    c->bf_ipt = ipt; c->bf_tile = tile; c->bf_waves_per_cu = wpc;
+    // Brute force: a finer j split balances better but writes more chunk
+    // partials (3 doubles per particle per chunk), so among the shapes within
+    // 2 % of the fastest the one with the fewest chunks wins; candidates are
+    // listed in that order of preference.
+    struct Shape { int ipt, tile, wpc; double ms; };
+    auto pick = [&](std::vector<Shape>& v) {
+        double fastest = v[0].ms;
+        for (const Shape& x : v) fastest = std::min(fastest, x.ms);
+        for (const Shape& x : v)
+            if (x.ms <= 1.02 * fastest) return x;
+        return v[0];
+    };
+    if (s->algorithm == GRAVIS_BRUTE_FORCE) {
+        const bool mx = s->precision == GRAVIS_MIXED;
+        std::vector<Shape> v;
+        for (int wpc : {32, 64, 128, 256, 512})
+            for (int ipt : {2, 1})
+                for (int tile : {128, 256}) {
+                    if (!mx && (ipt != 1 || tile != 256)) continue;  // FP64: split only
+                    if (mx && wpc == 32) continue;
+                    c->bf_ipt = mx ? ipt : c->bf_ipt;
+                    c->bf_tile = mx ? tile : c->bf_tile;
+                    c->bf_waves_per_cu = wpc;
                     double ms;
                     st = time_it(&ms);
                     if (st != GRAVIS_OK) break;
-                    if (best < 0.0 || ms < best) { best = ms; bi = ipt; bt = tile; bw = wpc; }
+                    v.push_back({c->bf_ipt, c->bf_tile, wpc, ms});
                 }
-        c->bf_ipt = bi; c->bf_tile = bt; c->bf_waves_per_cu = bw;
-    } else if (s->algorithm == GRAVIS_BRUTE_FORCE) {
-        int bw = c->bf_waves_per_cu;
-        for (int wpc : {32, 64, 128, 256, 512}) {
-            c->bf_waves_per_cu = wpc;
-            double ms;
-            st = time_it(&ms);
-            if (st != GRAVIS_OK) break;
-            if (best < 0.0 || ms < best) { best = ms; bw = wpc; }
+        if (st == GRAVIS_OK) {
+            const Shape b = pick(v);
+            c->bf_ipt = b.ipt; c->bf_tile = b.tile; c->bf_waves_per_cu = b.wpc;
+            best = b.ms;
         }
-        c->bf_waves_per_cu = bw;
     } else if (s->algorithm == GRAVIS_BARNES_HUT && s->group_size == 0) {
         uint32_t bg = c->bh_group;
         for (uint32_t gs : {16u, 32u, 64u}) {

@@ -59,7 +59,7 @@ struct Ctx {
     // j-chunk split aims for.
     int bf_ipt = 2;
     int bf_tile = 128;
-    int bf_waves_per_cu = 512;
+    int bf_waves_per_cu = 256;
     // Barnes-Hut target group when the solver leaves group_size at 0
     uint32_t bh_group = 64;
     // Mixed brute force: a source tile closer to the target box than this

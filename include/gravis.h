@@ -337,12 +337,14 @@ gravis_status gravis_get_tuning(gravis_ctx* ctx, gravis_tuning* out);
 
 /* Autotuner (no reference counterpart): times the candidate launch shapes
  * for solver s on the given particles (host arrays, copied once) and keeps
- * the fastest in the context. Mixed brute force: bf_ipt x bf_tile x
- * bf_waves_per_cu (16 shapes); FP64 brute force: bf_waves_per_cu;
- * Barnes-Hut with group_size 0: group sizes 16, 32, 64. Each shape costs
- * three force calls. Shapes change the summation grouping, so forces move in
- * their last bits (mixed) or within the opening-angle error (Barnes-Hut
- * groups). out may be NULL. */
+ * the chosen one in the context. Mixed brute force: bf_waves_per_cu
+ * 64..512 x bf_ipt x bf_tile (16 shapes); FP64 brute force:
+ * bf_waves_per_cu 32..512; among brute-force shapes within 2 % of the
+ * fastest, the one with the fewest j chunks (least partial-sum memory) is
+ * kept. Barnes-Hut with group_size 0: the fastest of group sizes 16, 32, 64.
+ * Each shape costs three force calls. Shapes change the summation grouping,
+ * so forces move in their last bits (brute force) or within the
+ * opening-angle error (Barnes-Hut groups). out may be NULL. */
 gravis_status gravis_autotune(gravis_ctx* ctx, const gravis_solver* s, size_t n,
                               const double* x, const double* y, const double* z,
                               const double* m, gravis_tuning* out);
