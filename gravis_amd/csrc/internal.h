@@ -99,6 +99,34 @@ inline gravis_status buf(Ctx* c, const char* name, size_t count, T** out) {
 // Kernel launch check (launch errors are reported by hipGetLastError).
 gravis_status check_launch(Ctx* c, const char* what);
 
+// Exclusive scan of one u32 per thread over a block of BLOCK threads (a
+// multiple of the wavefront); *total receives the block sum. Every thread of
+// the block must call it (it synchronises twice).
+template <int BLOCK>
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
+    __shared__ uint32_t wsum[BLOCK / kWave];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = threadIdx.x / kWave;
+    uint32_t inc = v;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t t = __shfl_up(inc, off);
+        if (lane >= off) inc += t;
+    }
+    if (lane == kWave - 1) wsum[w] = inc;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int k = 0; k < BLOCK / kWave; ++k) {
+        const uint32_t x = wsum[k];
+        before += k < w ? x : 0u;
+        all += x;
+    }
+    __syncthreads();  // wsum is reused by the next call
+    *total = all;
+    return before + inc - v;
+}
+
 // Optional event bracket around the dominant kernel of a call.
 void timer_begin(Ctx* c);
 void timer_end(Ctx* c, const char* name);

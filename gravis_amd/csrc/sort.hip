@@ -161,28 +161,47 @@ __global__ __launch_bounds__(kRsBlock) void radix_hist_kernel(const uint64_t* __
     counts[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of m counts in one block of 1024 threads.
-__global__ __launch_bounds__(1024) void radix_scan_kernel(uint32_t* __restrict__ v, size_t m) {
-    __shared__ uint32_t s[1024];
-    const size_t per = (m + 1023) / 1024;
-    const size_t b = threadIdx.x * per;
-    const size_t e = b + per < m ? b + per : m;
-    uint32_t sum = 0;
-    for (size_t i = b; i < e; ++i) sum += v[i];
-    s[threadIdx.x] = sum;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        uint32_t t = threadIdx.x >= (unsigned)off ? s[threadIdx.x - off] : 0u;
-        __syncthreads();
-        s[threadIdx.x] += t;
-        __syncthreads();
+// Exclusive scan of the per-tile counts in (digit, tile) order, in two
+// launches: one block per digit scans that digit's row of nblocks counts in
+// place and stores the row total; one block then turns the 256 totals into
+// exclusive digit bases. The scatter adds base[digit] to the row offsets.
+constexpr int kScanBlock = 256;
+constexpr int kScanPer = 4;  // consecutive counts per thread per chunk
+static_assert(kScanBlock == kRsDigits, "one thread per digit in the digit-base scan");
+
+__global__ __launch_bounds__(kScanBlock) void radix_row_scan_kernel(uint32_t* __restrict__ counts,
+                                                                    int nblocks,
+                                                                    uint32_t* __restrict__ totals) {
+    uint32_t* row = counts + (size_t)blockIdx.x * nblocks;
+    uint32_t carry = 0;
+    for (int base = 0; base < nblocks; base += kScanBlock * kScanPer) {
+        const int i0 = base + threadIdx.x * kScanPer;
+        uint32_t v[kScanPer];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int e = 0; e < kScanPer; ++e) {
+            v[e] = i0 + e < nblocks ? row[i0 + e] : 0u;
+            sum += v[e];
+        }
+        uint32_t chunk_total;
+        uint32_t run = carry + block_exclusive_scan<kScanBlock>(sum, &chunk_total);
+#pragma unroll
+        for (int e = 0; e < kScanPer; ++e)
+            if (i0 + e < nblocks) {
+                row[i0 + e] = run;
+                run += v[e];
+            }
+        carry += chunk_total;
     }
-    uint32_t run = s[threadIdx.x] - sum;  // exclusive
-    for (size_t i = b; i < e; ++i) {
-        uint32_t c = v[i];
-        v[i] = run;
-        run += c;
-    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
+__global__ __launch_bounds__(kScanBlock) void radix_digit_base_kernel(
+    uint32_t* __restrict__ totals) {
+    uint32_t all;
+    const uint32_t v = totals[threadIdx.x];
+    const uint32_t ex = block_exclusive_scan<kScanBlock>(v, &all);
+    totals[threadIdx.x] = ex;
 }
 
 __device__ __forceinline__ uint64_t digit_match(uint32_t d, uint64_t active) {
@@ -198,8 +217,8 @@ __device__ __forceinline__ uint64_t digit_match(uint32_t d, uint64_t active) {
 
 __global__ __launch_bounds__(kRsBlock) void radix_scatter_kernel(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, size_t n, int shift,
-    const uint32_t* __restrict__ offsets, int nblocks, uint64_t* __restrict__ kout,
-    uint32_t* __restrict__ vout) {
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ dbase, int nblocks,
+    uint64_t* __restrict__ kout, uint32_t* __restrict__ vout) {
     __shared__ uint32_t cnt[kRsWaves][kRsDigits];
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
@@ -230,7 +249,7 @@ __global__ __launch_bounds__(kRsBlock) void radix_scatter_kernel(
     // Phase B: global base of each (wave, digit): offsets + earlier waves.
     {
         const int d = threadIdx.x;
-        uint32_t run = offsets[(size_t)d * nblocks + blockIdx.x];
+        uint32_t run = dbase[d] + offsets[(size_t)d * nblocks + blockIdx.x];
         for (int k = 0; k < kRsWaves; ++k) {
             const uint32_t c = cnt[k][d];
             cnt[k][d] = run;
@@ -287,7 +306,7 @@ gravis_status morton_sort(Ctx* c, size_t n, const double* x, const double* y, co
                           const double* d_box, uint64_t** d_keys_sorted, uint32_t** d_perm) {
     if (n > 0xFFFFFFFFull) return fail(c, GRAVIS_INVALID_ARG, "n too large for 32-bit indices");
     uint64_t *k0, *k1;
-    uint32_t *v0, *v1, *counts;
+    uint32_t *v0, *v1, *counts, *dbase;
     const int nblocks = (int)((n + kRsTile - 1) / kRsTile);
     const size_t m = (size_t)kRsDigits * nblocks;
     GRAVIS_TRY(buf(c, "sort_k0", n, &k0));
@@ -295,13 +314,15 @@ gravis_status morton_sort(Ctx* c, size_t n, const double* x, const double* y, co
     GRAVIS_TRY(buf(c, "sort_v0", n, &v0));
     GRAVIS_TRY(buf(c, "sort_v1", n, &v1));
     GRAVIS_TRY(buf(c, "sort_counts", m, &counts));
+    GRAVIS_TRY(buf(c, "sort_dbase", kRsDigits, &dbase));
     morton_kernel<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(x, y, z, n, d_box, k0, v0);
     for (int pass = 0; pass < 8; ++pass) {
         const int shift = 8 * pass;
         radix_hist_kernel<<<nblocks, kRsBlock, 0, c->stream>>>(k0, n, shift, counts, nblocks);
-        radix_scan_kernel<<<1, 1024, 0, c->stream>>>(counts, m);
+        radix_row_scan_kernel<<<kRsDigits, kScanBlock, 0, c->stream>>>(counts, nblocks, dbase);
+        radix_digit_base_kernel<<<1, kScanBlock, 0, c->stream>>>(dbase);
         radix_scatter_kernel<<<nblocks, kRsBlock, 0, c->stream>>>(k0, v0, n, shift, counts,
-                                                                  nblocks, k1, v1);
+                                                                  dbase, nblocks, k1, v1);
         std::swap(k0, k1);
         std::swap(v0, v1);
     }

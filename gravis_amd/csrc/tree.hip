@@ -76,37 +76,75 @@ __global__ __launch_bounds__(kTreeBlock) void tree_count_kernel(const uint64_t* 
     }
 }
 
-// Exclusive scan of the level's child counts (one block) and the range of
-// level L+1.
-__global__ __launch_bounds__(1024) void tree_scan_kernel(const uint32_t* __restrict__ nch,
-                                                         uint32_t* __restrict__ off,
-                                                         uint32_t* __restrict__ lvl, int L) {
-    __shared__ uint32_t s[1024];
-    const uint32_t b = lvl[2 * L], e = lvl[2 * L + 1];
-    const uint32_t m = e - b;
-    const uint32_t per = (m + 1023) / 1024;
-    const uint32_t lo = threadIdx.x * per;
-    const uint32_t hi = min(m, lo + per);
-    uint32_t sum = 0;
-    for (uint32_t i = lo; i < hi; ++i) sum += nch[i];
-    s[threadIdx.x] = sum;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        const uint32_t t = threadIdx.x >= (unsigned)o ? s[threadIdx.x - o] : 0u;
-        __syncthreads();
-        s[threadIdx.x] += t;
-        __syncthreads();
-    }
-    uint32_t run = s[threadIdx.x] - sum;
-    for (uint32_t i = lo; i < hi; ++i) {
-        off[i] = run;
-        run += nch[i];
-    }
-    if (threadIdx.x == 1023) {
+// Exclusive scan of the level's child counts and the node range of level
+// L+1, over a fixed grid of kTsBlocks blocks whose ranges split the level
+// (sizes live on the device): per-block sums, one block scanning those, and
+// a down-sweep writing the offsets.
+constexpr int kTsBlocks = 256;
+constexpr int kTsThreads = 256;
+constexpr int kTsPer = 4;
+
+__device__ __forceinline__ void ts_range(uint32_t m, uint32_t* lo, uint32_t* hi) {
+    const uint32_t per = (m + kTsBlocks - 1) / kTsBlocks;
+    *lo = min(m, blockIdx.x * per);
+    *hi = min(m, *lo + per);
+}
+
+__global__ __launch_bounds__(kTsThreads) void tree_scan_reduce_kernel(
+    const uint32_t* __restrict__ nch, const uint32_t* __restrict__ lvl, int L,
+    uint32_t* __restrict__ bsum) {
+    const uint32_t m = lvl[2 * L + 1] - lvl[2 * L];
+    uint32_t lo, hi;
+    ts_range(m, &lo, &hi);
+    uint32_t s = 0;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += kTsThreads) s += nch[i];
+    uint32_t all;
+    block_exclusive_scan<kTsThreads>(s, &all);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = all;
+}
+
+__global__ __launch_bounds__(kTsBlocks) void tree_scan_top_kernel(uint32_t* __restrict__ bsum,
+                                                                  uint32_t* __restrict__ lvl,
+                                                                  int L) {
+    uint32_t all;
+    const uint32_t ex = block_exclusive_scan<kTsBlocks>(bsum[threadIdx.x], &all);
+    bsum[threadIdx.x] = ex;
+    if (threadIdx.x == 0) {
+        const uint32_t e = lvl[2 * L + 1];
         lvl[2 * (L + 1)] = e;
-        lvl[2 * (L + 1) + 1] = e + s[1023];
+        lvl[2 * (L + 1) + 1] = e + all;
     }
 }
+
+__global__ __launch_bounds__(kTsThreads) void tree_scan_down_kernel(
+    const uint32_t* __restrict__ nch, uint32_t* __restrict__ off,
+    const uint32_t* __restrict__ lvl, int L, const uint32_t* __restrict__ bsum) {
+    const uint32_t m = lvl[2 * L + 1] - lvl[2 * L];
+    uint32_t lo, hi;
+    ts_range(m, &lo, &hi);
+    uint32_t carry = bsum[blockIdx.x];
+    for (uint32_t base = lo; base < hi; base += kTsThreads * kTsPer) {
+        const uint32_t i0 = base + threadIdx.x * kTsPer;
+        uint32_t v[kTsPer];
+        uint32_t sum = 0;
+#pragma unroll
+        for (int e = 0; e < kTsPer; ++e) {
+            v[e] = i0 + e < hi ? nch[i0 + e] : 0u;
+            sum += v[e];
+        }
+        uint32_t chunk;
+        uint32_t run = carry + block_exclusive_scan<kTsThreads>(sum, &chunk);
+#pragma unroll
+        for (int e = 0; e < kTsPer; ++e)
+            if (i0 + e < hi) {
+                off[i0 + e] = run;
+                run += v[e];
+            }
+        carry += chunk;
+    }
+}
+
+static_assert(kTsBlocks % kWave == 0 && kTsThreads % kWave == 0, "whole wavefronts");
 
 __global__ __launch_bounds__(kTreeBlock) void tree_write_kernel(
     const uint64_t* __restrict__ keys, uint4* __restrict__ meta, const uint32_t* __restrict__ lvl,
@@ -309,7 +347,7 @@ gravis_status build_octree(Ctx* c, size_t n, const double* x, const double* y, c
     if (cap > 0x7FFFFFFFull) return fail(c, GRAVIS_INVALID_ARG, "tree too large");
     double4 *src, *sums, *bmin, *bmax;
     uint4* meta;
-    uint32_t *lvl, *nch, *off, *flags, *ngroups;
+    uint32_t *lvl, *nch, *off, *flags, *ngroups, *bsum;
     uint2* groups;
     GRAVIS_TRY(buf(c, "tree_src", n + cap, &src));
     GRAVIS_TRY(buf(c, "tree_sums", cap, &sums));
@@ -319,6 +357,7 @@ gravis_status build_octree(Ctx* c, size_t n, const double* x, const double* y, c
     GRAVIS_TRY(buf(c, "tree_lvl", 2 * (kMaxLevel + 2), &lvl));
     GRAVIS_TRY(buf(c, "tree_nch", cap, &nch));
     GRAVIS_TRY(buf(c, "tree_off", cap, &off));
+    GRAVIS_TRY(buf(c, "tree_bsum", kTsBlocks, &bsum));
     GRAVIS_TRY(buf(c, "tree_flags", 4, &flags));
     GRAVIS_TRY(buf(c, "tree_ngroups", 1, &ngroups));
     GRAVIS_TRY(buf(c, "tree_groups", n, &groups));
@@ -329,13 +368,16 @@ gravis_status build_octree(Ctx* c, size_t n, const double* x, const double* y, c
     const unsigned tg = grid_for(cap, kTreeBlock, kTreeGrid);
     for (int L = 0; L < kMaxLevel; ++L) {
         tree_count_kernel<<<tg, kTreeBlock, 0, st>>>(keys, meta, lvl, L, leaf, nch);
-        tree_scan_kernel<<<1, 1024, 0, st>>>(nch, off, lvl, L);
+        tree_scan_reduce_kernel<<<kTsBlocks, kTsThreads, 0, st>>>(nch, lvl, L, bsum);
+        tree_scan_top_kernel<<<1, kTsBlocks, 0, st>>>(bsum, lvl, L);
+        tree_scan_down_kernel<<<kTsBlocks, kTsThreads, 0, st>>>(nch, off, lvl, L, bsum);
         tree_write_kernel<<<tg, kTreeBlock, 0, st>>>(keys, meta, lvl, L, nch, off, (uint32_t)cap,
                                                     group, flags + 1);
     }
     // level 21 holds leaves only (zero children); its scan closes the node range
     tree_count_kernel<<<tg, kTreeBlock, 0, st>>>(keys, meta, lvl, kMaxLevel, leaf, nch);
-    tree_scan_kernel<<<1, 1024, 0, st>>>(nch, off, lvl, kMaxLevel);
+    tree_scan_reduce_kernel<<<kTsBlocks, kTsThreads, 0, st>>>(nch, lvl, kMaxLevel, bsum);
+    tree_scan_top_kernel<<<1, kTsBlocks, 0, st>>>(bsum, lvl, kMaxLevel);
     GRAVIS_TRY(check_launch(c, "tree build"));
     for (int L = kMaxLevel; L >= 0; --L)
         tree_sums_kernel<<<tg, kTreeBlock, 0, st>>>(src, meta, lvl, L, sums, bmin, bmax);
