@@ -94,14 +94,6 @@ __global__ __launch_bounds__(256) void tile_box_kernel(const double4* __restrict
     }
 }
 
-// A wave-uniform double moved to scalar registers (frees two VGPRs).
-__device__ __forceinline__ double uniform(double v) {
-    const uint64_t b = __double_as_longlong(v);
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
-    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
-}
-
 // Bounding box of the wavefront's IPT*64 targets (FP64); every lane ends
 // with the same box, held in scalar registers.
 template <int IPT>

@@ -92,10 +92,17 @@ __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
         const uint2 gr = groups[g];
         const int i0 = (int)gr.x;
         const int cnt = (int)gr.y;
-        const bool active = lane < cnt;
-        const double4 me = src[i0 + (active ? lane : 0)];
+        // MIXED: a group of at most 32 (16) targets is held twice (four
+        // times) across the wave; replica r sums every R-th source of each
+        // flush round and the replicas are added in a fixed order at the end.
+        const int R = MIXED ? (cnt <= 16 ? 4 : (cnt <= 32 ? 2 : 1)) : 1;
+        const int tpr = kWave / R;          // lanes per replica
+        const int tl = lane % tpr;          // this lane's target
+        const int rep = lane / tpr;         // this lane's replica
+        const bool active = tl < cnt;
+        const double4 me = src[i0 + (active ? tl : 0)];
         // softening of this lane's target: one value, or one per target
-        const double eps2 = eps2_t ? eps2_t[i0 + (active ? lane : 0)] : eps2_all;
+        const double eps2 = eps2_t ? eps2_t[i0 + (active ? tl : 0)] : eps2_all;
         const float eps2f = (float)eps2;
         double glo[3] = {me.x, me.y, me.z}, ghi[3] = {me.x, me.y, me.z};
 #pragma unroll
@@ -105,6 +112,11 @@ __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
                 glo[d] = fmin(glo[d], __shfl_xor(glo[d], o));
                 ghi[d] = fmax(ghi[d], __shfl_xor(ghi[d], o));
             }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            glo[d] = uniform(glo[d]);
+            ghi[d] = uniform(ghi[d]);
+        }
         // MIXED: records and target are rounded to FP32 relative to the
         // group's box centre
         const double gcx = 0.5 * (glo[0] + ghi[0]), gcy = 0.5 * (glo[1] + ghi[1]),
@@ -133,8 +145,11 @@ __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
                 const uint32_t m = min(nl - q0, (uint32_t)kWave);
                 if (MIXED) {
                     float tx = 0.f, ty = 0.f, tz = 0.f;
-                    for (uint32_t k = 0; k < m; ++k) {
-                        const float4 r = stage32[k];
+                    for (uint32_t k0 = 0; k0 < m; k0 += (uint32_t)R) {
+                        const uint32_t k = k0 + (uint32_t)rep;
+                        // past the round's end: a zero-mass copy of record 0
+                        float4 r = stage32[k < m ? k : 0];
+                        r.w = k < m ? r.w : 0.f;
                         const float dx = r.x - fx, dy = r.y - fy, dz = r.z - fz;
                         const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2f)));
                         float ri = __builtin_amdgcn_rsqf(r2);
@@ -276,7 +291,7 @@ __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
             for (int l = 0; l < cnt; ++l) {
                 const double px = __shfl(me.x, l), py = __shfl(me.y, l), pz = __shfl(me.z, l);
                 const double p[3] = {px, py, pz};
-                keep = lane == l;
+                keep = tl == l;
                 if (walk(p, p, 1) && lane == 0) atomicOr(flag, 1u);
             }
             keep = true;
@@ -285,9 +300,16 @@ __global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
             ax = (double)sx - (double)kx;
             ay = (double)sy - (double)ky;
             az = (double)sz - (double)kz;
+            // replicas in a fixed pairing: (r0 + r1) [+ (r2 + r3)]; each sum
+            // is formed identically in both lanes (addition commutes)
+            for (int o = tpr; o < kWave; o <<= 1) {
+                ax += __shfl_xor(ax, o);
+                ay += __shfl_xor(ay, o);
+                az += __shfl_xor(az, o);
+            }
         }
-        if (active) {
-            const size_t i = (size_t)i0 + lane;
+        if (active && rep == 0) {
+            const size_t i = (size_t)i0 + tl;
             part[i] = ax;
             part[(size_t)n + i] = ay;
             part[2 * (size_t)n + i] = az;

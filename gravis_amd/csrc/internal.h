@@ -99,6 +99,14 @@ inline gravis_status buf(Ctx* c, const char* name, size_t count, T** out) {
 // Kernel launch check (launch errors are reported by hipGetLastError).
 gravis_status check_launch(Ctx* c, const char* what);
 
+// A wave-uniform double moved to scalar registers (frees two VGPRs).
+__device__ __forceinline__ double uniform(double v) {
+    const uint64_t b = __double_as_longlong(v);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
 // Exclusive scan of one u32 per thread over a block of BLOCK threads (a
 // multiple of the wavefront); *total receives the block sum. Every thread of
 // the block must call it (it synchronises twice).

@@ -163,20 +163,20 @@ __global__ __launch_bounds__(256) void sph_prep_kernel(
 // on to kQDensity (kQDone if their iteration already ended); a group that
 // exceeds the LDS capacities is appended to dfs_groups instead. All control
 // flow is wave uniform: g and every count are scalar values.
-__device__ __forceinline__ void query_group(
-    uint32_t g, int lane, const double4* __restrict__ pos, const uint4* __restrict__ meta,
+// Returns false (and writes nothing) when the search exceeds the LDS
+// capacities; the caller then splits the range or searches depth first.
+__device__ __forceinline__ bool query_range(
+    uint2 gr, int lane, const double4* __restrict__ pos, const uint4* __restrict__ meta,
     const double4* __restrict__ bmin, const double4* __restrict__ bmax,
-    const uint2* __restrict__ groups, uint8_t* __restrict__ qstate,
-    const uint8_t* __restrict__ iter, const double* __restrict__ radius,
-    uint2* __restrict__ nbr, uint32_t* __restrict__ pool, uint32_t pool_cap,
-    uint32_t* __restrict__ ctl, uint32_t* __restrict__ dfs_groups, uint32_t rcap, int maxit,
-    uint32_t* fa, uint32_t* fb, uint2* rg, double4* stage) {
+    uint8_t* __restrict__ qstate, const uint8_t* __restrict__ iter,
+    const double* __restrict__ radius, uint2* __restrict__ nbr, uint32_t* __restrict__ pool,
+    uint32_t pool_cap, uint32_t* __restrict__ ctl, uint32_t rcap, int maxit, uint32_t* fa,
+    uint32_t* fb, uint2* rg, double4* stage) {
     const uint64_t below = (1ull << lane) - 1ull;
-    const uint2 gr = groups[g];
     const uint32_t s = gr.x + (uint32_t)lane;
     const bool inside = (uint32_t)lane < gr.y;
     const bool act = inside && qstate[s] == kQQuery;
-    if (__ballot(act) == 0ull) return;
+    if (__ballot(act) == 0ull) return true;
     const double4 me = pos[inside ? s : gr.x];
     const double R = act ? radius[s] : 0.0;
     const double R2 = R * R;
@@ -239,10 +239,7 @@ __device__ __forceinline__ void query_group(
         fb = t;
         nf = nn;
     }
-    if (ovf) {
-        if (lane == 0) dfs_groups[atomicAdd(&ctl[2], 1u)] = g;
-        return;
-    }
+    if (ovf) return false;
 
     // sort the leaves by first particle (bitonic, one wave)
     uint32_t np = 1;
@@ -327,6 +324,7 @@ __device__ __forceinline__ void query_group(
         }
     }
     if (act && !failed) qstate[s] = iter[s] >= maxit ? kQDone : kQDensity;
+    return true;
 }
 
 __global__ __launch_bounds__(kQWaves* kWave) void sph_query_kernel(
@@ -349,28 +347,44 @@ __global__ __launch_bounds__(kQWaves* kWave) void sph_query_kernel(
         const uint32_t g =
             __builtin_amdgcn_readfirstlane(atomicAdd(counter, lane == 0 ? 1u : 0u));
         if (g >= ng) break;
-        query_group(g, lane, pos, meta, bmin, bmax, groups, qstate, iter, radius, nbr, pool,
-                    pool_cap, ctl, dfs_groups, rcap, maxit, fronts[w][0], fronts[w][1], leaves[w],
-                    stages[w]);
+        const bool ok = query_range(groups[g], lane, pos, meta, bmin, bmax, qstate, iter,
+                                    radius, nbr, pool, pool_cap, ctl, rcap, maxit, fronts[w][0],
+                                    fronts[w][1], leaves[w], stages[w]);
+        if (!ok && lane == 0) dfs_groups[atomicAdd(&ctl[2], 1u)] = g;
     }
 }
 
-// Depth-first search, one target at a time, for the groups the
-// breadth-first kernel could not hold in LDS. Children are pushed in reverse
-// digit order, so leaves are reached in ascending Morton order.
+// Groups the breadth-first kernel could not hold in LDS are split into
+// ranges of kQSub targets, each searched breadth first again with its own
+// (smaller) box; a range that still does not fit is searched depth first,
+// one target at a time. Children are pushed in reverse digit order, so
+// leaves are reached in ascending Morton order.
+constexpr uint32_t kQSub = 8;
+
 __global__ __launch_bounds__(kWave) void sph_query_dfs_kernel(
     const double4* __restrict__ pos, const uint4* __restrict__ meta,
     const double4* __restrict__ bmin, const double4* __restrict__ bmax,
     const uint2* __restrict__ groups, const uint32_t* __restrict__ dfs_groups,
     uint8_t* __restrict__ qstate, const uint8_t* __restrict__ iter,
     const double* __restrict__ radius, uint2* __restrict__ nbr, uint32_t* __restrict__ pool,
-    uint32_t pool_cap, uint32_t* __restrict__ ctl, int maxit) {
+    uint32_t pool_cap, uint32_t* __restrict__ ctl, uint32_t rcap, int maxit) {
     __shared__ uint32_t stack[kQStack];
+    __shared__ uint32_t fronts[2][kQFront];
+    __shared__ uint2 leaves[kQRanges];
+    __shared__ double4 stage[kWave];
     const int lane = threadIdx.x;
     const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t ndfs = ctl[2];
-    for (uint32_t gi = blockIdx.x; gi < ndfs; gi += gridDim.x) {
-        const uint2 gr = groups[dfs_groups[gi]];
+    constexpr uint32_t kSubs = kWave / kQSub;
+    for (uint32_t item = blockIdx.x; item < ndfs * kSubs; item += gridDim.x) {
+        const uint2 grp = groups[dfs_groups[item / kSubs]];
+        const uint32_t off = (item % kSubs) * kQSub;
+        if (off >= grp.y) continue;
+        const uint2 gr = make_uint2(grp.x + off, min(kQSub, grp.y - off));
+        if (query_range(gr, lane, pos, meta, bmin, bmax, qstate, iter, radius, nbr, pool,
+                        pool_cap, ctl, rcap, maxit, fronts[0], fronts[1], leaves, stage))
+            continue;
+        if (lane == 0) atomicAdd(&ctl[4], 1u);
         for (uint32_t l = 0; l < gr.y; ++l) {
             const uint32_t s = gr.x + l;
             if (qstate[s] != kQQuery) continue;
@@ -927,6 +941,7 @@ __global__ void sph_query_reset_kernel(uint32_t* __restrict__ counter, uint32_t*
     ctl[1] = 0u;
     ctl[2] = 0u;
     ctl[3] = 0u;
+    ctl[4] = 0u;
 }
 
 __global__ void sph_pool_rewind_kernel(uint32_t* __restrict__ ctl, uint32_t used) {
@@ -1045,10 +1060,10 @@ gravis_status sph_compute(Ctx* c, const gravis_sph_params* P, const gravis_sph_s
             t.src, t.meta, t.bmin, t.bmax, t.groups, t.ngroups, counter, qstate, iter, radius,
             nbr, pool, (uint32_t)pool_cap, ctl, dfs_groups,
             std::min<uint32_t>(c->sph_range_cap, kQRanges), P->max_h_iter);
-        sph_query_dfs_kernel<<<dblocks, kWave, 0, st>>>(t.src, t.meta, t.bmin, t.bmax, t.groups,
-                                                        dfs_groups, qstate, iter, radius, nbr,
-                                                        pool, (uint32_t)pool_cap, ctl,
-                                                        P->max_h_iter);
+        sph_query_dfs_kernel<<<dblocks, kWave, 0, st>>>(
+            t.src, t.meta, t.bmin, t.bmax, t.groups, dfs_groups, qstate, iter, radius, nbr, pool,
+            (uint32_t)pool_cap, ctl, std::min<uint32_t>(c->sph_range_cap, kQRanges),
+            P->max_h_iter);
         timer_end(c, "sph_query");
         GRAVIS_TRY(check_launch(c, "sph_query"));
         timer_begin(c);

@@ -218,6 +218,27 @@ def test_bh_mixed_matches_fp64_walk(g, n, theta):
     assert rms <= 1e-6 and mx <= 1e-4
 
 
+@pytest.mark.parametrize("group", [16, 32, 64])
+def test_bh_mixed_replicas_and_fallback(g, group):
+    """The mixed walk holds groups of <= 32 (<= 16) targets two (four) times
+    across the wavefront and splits each source round between the copies;
+    with groups capped at 16/32/64 targets and a frontier capacity that
+    forces per-target fallback walks, it still matches the FP64 walk of the
+    same context (RMS |da|/|a| <= 1e-6, max <= 1e-4)."""
+    say(f"bh mixed replicas group={group} + fallback (cap 350)")
+    s = O.plummer(5000, 12)
+    ctx = g.Context(0)
+    ctx.set_tuning("bh_frontier_cap", 350)
+    p64 = to_particles(g, s)
+    g.BarnesHut(0.05, 0.5, 16, group, ctx=ctx).compute_accelerations(p64)
+    pm = to_particles(g, s)
+    g.BarnesHut(0.05, 0.5, 16, group, precision="mixed", ctx=ctx).compute_accelerations(pm)
+    rms, mx = rel_errors(p64.acc(), pm.acc())
+    say(f"  rms={rms:.3e} max={mx:.3e}")
+    assert rms <= 1e-6 and mx <= 1e-4
+    ctx.close()
+
+
 def test_bh_frontier_fallback_bit_exact(g):
     """With a small frontier capacity (forced from the host: 350 nodes, above
     the largest single-target frontier of this input, 327, below the largest
