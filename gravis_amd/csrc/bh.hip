@@ -61,8 +61,11 @@ __device__ __forceinline__ void pair_ref(const double4 s, double px, double py, 
 // centre (separations formed in FP64 first), pair forces are FP32 (v_rsq_f32)
 // and every round's FP32 sum is Kahan-added; the opening test, tree sums and
 // output stay FP64.
+// MIXED fits 96 VGPRs (5 wavefronts per SIMD; the FP64 walk needs more and
+// runs 4)
 template <bool MIXED>
-__global__ __launch_bounds__(kWalkWaves* kWave) void bh_walk_kernel(
+__global__ __launch_bounds__(kWalkWaves* kWave)
+__attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
     const double4* __restrict__ src, const uint4* __restrict__ meta, int n,
     const uint2* __restrict__ groups, const uint32_t* __restrict__ ngroups,
     const double* __restrict__ box, double theta2, double eps2_all,
@@ -330,8 +333,9 @@ gravis_status bh_walk(Ctx* c, const gravis_solver* s, const Octree& t, const Fin
     const size_t n = t.n;
     const int ni = (int)n;
     hipStream_t st = c->stream;
+    const int per_cu = s->precision == GRAVIS_MIXED ? 5 : 4;  // resident blocks per CU
     const int blocks = (int)std::min<long>((long)(n / 64 / kWalkWaves) + 1,
-                                           (long)c->num_cus * 4);
+                                           (long)c->num_cus * per_cu);
     const uint32_t fcap = (uint32_t)std::min<size_t>(t.cap, c->bh_frontier_cap);
     uint32_t *front, *counter;
     GRAVIS_TRY(buf(c, "bh_front", (size_t)blocks * kWalkWaves * 2 * fcap, &front));

@@ -464,6 +464,42 @@ __global__ __launch_bounds__(kWave) void sph_query_dfs_kernel(
     }
 }
 
+// Gathers of kGather list members: the indices first, then the records, so
+// the dependent index -> record loads of a batch overlap. Members past the
+// list end read record 0 and are skipped by the caller.
+constexpr uint32_t kGather = 4;
+
+__device__ __forceinline__ void gather_batch(const double4* __restrict__ pos,
+                                             const uint32_t* __restrict__ pool, uint2 nb,
+                                             uint32_t k0, double4 (&qb)[kGather]) {
+    uint32_t jb[kGather];
+#pragma unroll
+    for (uint32_t e = 0; e < kGather; ++e) jb[e] = k0 + e < nb.y ? pool[nb.x + k0 + e] : 0u;
+#pragma unroll
+    for (uint32_t e = 0; e < kGather; ++e) qb[e] = pos[jb[e]];
+}
+
+// As gather_batch for the force pass: kFGather members' index, position,
+// velocity/h, (rho, P, c, alpha) and (Omega, u) records.
+constexpr uint32_t kFGather = 2;
+
+__device__ __forceinline__ void gather_force_batch(
+    const double4* __restrict__ pos, const double4* __restrict__ vh,
+    const double4* __restrict__ aux, const double2* __restrict__ aux2,
+    const uint32_t* __restrict__ pool, uint2 nb, uint32_t k0, uint32_t (&jb)[kFGather],
+    double4 (&pb)[kFGather], double4 (&vb)[kFGather], double4 (&ab)[kFGather],
+    double2 (&bb)[kFGather]) {
+#pragma unroll
+    for (uint32_t e = 0; e < kFGather; ++e) jb[e] = k0 + e < nb.y ? pool[nb.x + k0 + e] : 0u;
+#pragma unroll
+    for (uint32_t e = 0; e < kFGather; ++e) {
+        pb[e] = pos[jb[e]];
+        vb[e] = vh[jb[e]];
+        ab[e] = aux[jb[e]];
+        bb[e] = aux2[jb[e]];
+    }
+}
+
 // Density sums over a list at smoothing length h: rho = sum m_j W(r, h) and
 // S = sum m_j dW/dh(r, h) over list members with r < 2h (density.rs:130-157).
 // K = 0: Wendland C2 in sim-core's operation order (only sigma / h^3 and
@@ -481,8 +517,15 @@ __device__ __forceinline__ void density_sums(const double4 me, double h,
         const double nh3 = (K == 0 ? kNorm3d : kCsNorm) / (h * h * h);
         const double nh4 = kCsNorm / (h * h * h * h);
         const double m3h = -3.0 / h;
-        for (uint32_t k = 0; k < nb.y; ++k) {
-            const double4 q4 = pos[pool[nb.x + k]];
+        // loads in batches of kGather (indices, then records) so that the
+        // gathers of a batch are in flight together; pairs in list order
+        for (uint32_t k0 = 0; k0 < nb.y; k0 += kGather) {
+            double4 qb[kGather];
+            gather_batch(pos, pool, nb, k0, qb);
+#pragma unroll
+            for (uint32_t e = 0; e < kGather; ++e) {
+            if (k0 + e >= nb.y) break;
+            const double4 q4 = qb[e];
             const double dx = me.x - q4.x, dy = me.y - q4.y, dz = me.z - q4.z;
             const double r = sqrt(dx * dx + dy * dy + dz * dz);
             if (r < h2) {
@@ -506,6 +549,7 @@ __device__ __forceinline__ void density_sums(const double4 me, double h,
                 rs += q4.w * wv;
                 ds += q4.w * dwh;
             }
+            }
         }
     } else {
         const float hf = (float)h;
@@ -513,8 +557,13 @@ __device__ __forceinline__ void density_sums(const double4 me, double h,
         const float h2sq = 4.0f * hf * hf;
         const float nh3 = (float)(K == 0 ? kNorm3d : kCsNorm) * ih * ih * ih;
         const float nh4 = nh3 * ih;
-        for (uint32_t k = 0; k < nb.y; ++k) {
-            const double4 q4 = pos[pool[nb.x + k]];
+        for (uint32_t k0 = 0; k0 < nb.y; k0 += kGather) {
+            double4 qb[kGather];
+            gather_batch(pos, pool, nb, k0, qb);
+#pragma unroll
+            for (uint32_t e = 0; e < kGather; ++e) {
+            if (k0 + e >= nb.y) break;
+            const double4 q4 = qb[e];
             const float dx = (float)(me.x - q4.x), dy = (float)(me.y - q4.y),
                         dz = (float)(me.z - q4.z);
             const float r2 = dx * dx + dy * dy + dz * dz;
@@ -536,6 +585,7 @@ __device__ __forceinline__ void density_sums(const double4 me, double h,
                 const float mj = (float)q4.w;
                 rs += (double)(mj * wv);
                 ds += (double)(mj * dwh);
+            }
             }
         }
     }
@@ -710,19 +760,26 @@ __global__ __launch_bounds__(256) void sph_force_kernel(
     double axs = 0.0, ays = 0.0, azs = 0.0, du = 0.0, dv = 0.0, mmu = 0.0;
     const uint2 nb = nbr[s];
     if (!F32) {
-        for (uint32_t k = 0; k < nb.y; ++k) {
-            const uint32_t j = pool[nb.x + k];
+        for (uint32_t k0 = 0; k0 < nb.y; k0 += kFGather) {
+            uint32_t jb[kFGather];
+            double4 pb[kFGather], vb[kFGather], ab[kFGather];
+            double2 bb[kFGather];
+            gather_force_batch(pos, vh, aux, aux2, pool, nb, k0, jb, pb, vb, ab, bb);
+#pragma unroll
+            for (uint32_t e = 0; e < kFGather; ++e) {
+            if (k0 + e >= nb.y) break;
+            const uint32_t j = jb[e];
             if (j == (uint32_t)s) continue;
-            const double4 pj = pos[j];
+            const double4 pj = pb[e];
             const double dx = me.x - pj.x, dy = me.y - pj.y, dz = me.z - pj.z;
             const double r = sqrt(dx * dx + dy * dy + dz * dz);
-            const double4 vj = vh[j];
+            const double4 vj = vb[e];
             const double hj = vj.w;
             const double havg = 0.5 * (hi + hj);
             if (r >= 2.0 * hi && r >= 2.0 * hj) continue;
             const double mj = pj.w;
-            const double4 aj = aux[j];
-            const double2 bj = aux2[j];
+            const double4 aj = ab[e];
+            const double2 bj = bb[e];
             const double rho_j = fmax(aj.x, kDensityFloor);
             const double p_j = aj.y, c_j = aj.z, alpha_j = aj.w, omega_j = bj.x;
             const double dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
@@ -761,6 +818,7 @@ __global__ __launch_bounds__(256) void sph_force_kernel(
                     du += mj * P.alpha_cond * vsig * (u_i - u_j) / rsum * agw;
                 }
             }
+            }
         }
     } else {
         const float hif = (float)hi, ihi = 1.0f / hif;
@@ -769,21 +827,28 @@ __global__ __launch_bounds__(256) void sph_force_kernel(
         const float beta = (float)P.beta_visc, acond = (float)P.alpha_cond;
         const float vxi = (float)vi.x, vyi = (float)vi.y, vzi = (float)vi.z;
         float mmuf = 0.0f;
-        for (uint32_t k = 0; k < nb.y; ++k) {
-            const uint32_t j = pool[nb.x + k];
+        for (uint32_t k0 = 0; k0 < nb.y; k0 += kFGather) {
+            uint32_t jb[kFGather];
+            double4 pb[kFGather], vb[kFGather], ab[kFGather];
+            double2 bb[kFGather];
+            gather_force_batch(pos, vh, aux, aux2, pool, nb, k0, jb, pb, vb, ab, bb);
+#pragma unroll
+            for (uint32_t e = 0; e < kFGather; ++e) {
+            if (k0 + e >= nb.y) break;
+            const uint32_t j = jb[e];
             if (j == (uint32_t)s) continue;
-            const double4 pj = pos[j];
+            const double4 pj = pb[e];
             const float dx = (float)(me.x - pj.x), dy = (float)(me.y - pj.y),
                         dz = (float)(me.z - pj.z);
             const float r2 = dx * dx + dy * dy + dz * dz;
             const float r = __builtin_sqrtf(r2);
-            const double4 vj = vh[j];
+            const double4 vj = vb[e];
             const float hj = (float)vj.w;
             if (r >= 2.0f * hif && r >= 2.0f * hj) continue;
             const float havg = 0.5f * (hif + hj);
             const float mj = (float)pj.w;
-            const double4 aj = aux[j];
-            const double2 bj = aux2[j];
+            const double4 aj = ab[e];
+            const double2 bj = bb[e];
             const float rhoj = fmaxf((float)aj.x, (float)kDensityFloor);
             const float pj_ = (float)aj.y, cj = (float)aj.z, alj = (float)aj.w;
             const float omj = (float)bj.x;
@@ -814,6 +879,7 @@ __global__ __launch_bounds__(256) void sph_force_kernel(
                 const float vsig = __builtin_sqrtf(fabsf(pif - pj_) / (0.5f * rsum));
                 const float agw = -fa * r;  // |grad W(h_avg)|
                 du += (double)(mj * acond * vsig * (ui - (float)bj.y) / rsum * agw);
+            }
             }
         }
         mmu = (double)mmuf;
