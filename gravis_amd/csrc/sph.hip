@@ -467,7 +467,7 @@ __global__ __launch_bounds__(kWave) void sph_query_dfs_kernel(
 // Gathers of kGather list members: the indices first, then the records, so
 // the dependent index -> record loads of a batch overlap. Members past the
 // list end read record 0 and are skipped by the caller.
-constexpr uint32_t kGather = 4;
+constexpr uint32_t kGather = 8;
 
 __device__ __forceinline__ void gather_batch(const double4* __restrict__ pos,
                                              const uint32_t* __restrict__ pool, uint2 nb,
