@@ -26,17 +26,6 @@
 namespace gravis {
 namespace {
 
-// Pair force factor in FP32: m * r^-3 with r^2 = d.d + eps^2.
-template <bool MASK>
-__device__ __forceinline__ float pair_f32(float dx, float dy, float dz, float m, float eps2) {
-    const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
-    float ri = __builtin_amdgcn_rsqf(r2);
-    if (MASK) ri = r2 > 0.0f ? ri : 0.0f;
-    const float mr = m * ri;
-    const float ri2 = ri * ri;
-    return mr * ri2;
-}
-
 template <bool MASK>
 __device__ __forceinline__ double pair_f64(double dx, double dy, double dz, double m,
                                            double eps2) {
@@ -46,13 +35,6 @@ __device__ __forceinline__ double pair_f64(double dx, double dy, double dz, doub
     const double mr = m * ri;
     const double ri2 = ri * ri;
     return mr * ri2;
-}
-
-__device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
-    const float y = v - c;
-    const float t = s + y;
-    c = (t - s) - y;
-    s = t;
 }
 
 // Bounding box of each TJ-wide tile of the sorted records: tbox[6 t ..] =
@@ -94,35 +76,6 @@ __global__ __launch_bounds__(256) void tile_box_kernel(const double4* __restrict
     }
 }
 
-// Bounding box of the wavefront's IPT*64 targets (FP64); every lane ends
-// with the same box, held in scalar registers.
-template <int IPT>
-__device__ __forceinline__ void wave_box(const double (&x)[IPT], const double (&y)[IPT],
-                                         const double (&z)[IPT], double (&lo)[3],
-                                         double (&hi)[3]) {
-    lo[0] = hi[0] = x[0];
-    lo[1] = hi[1] = y[0];
-    lo[2] = hi[2] = z[0];
-#pragma unroll
-    for (int k = 1; k < IPT; ++k) {
-        lo[0] = fmin(lo[0], x[k]); hi[0] = fmax(hi[0], x[k]);
-        lo[1] = fmin(lo[1], y[k]); hi[1] = fmax(hi[1], y[k]);
-        lo[2] = fmin(lo[2], z[k]); hi[2] = fmax(hi[2], z[k]);
-    }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            lo[d] = fmin(lo[d], __shfl_xor(lo[d], off));
-            hi[d] = fmax(hi[d], __shfl_xor(hi[d], off));
-        }
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        lo[d] = uniform(lo[d]);
-        hi[d] = uniform(hi[d]);
-    }
-}
-
 // Far/near split of the mixed kernel. Source coordinates are rounded to
 // FP32 relative to the centre c of the target group's box, so a separation
 // d = x_j - x_i carries an absolute error of about u (|x_i - c| + |x_j - c|)
@@ -131,18 +84,7 @@ __device__ __forceinline__ void wave_box(const double (&x)[IPT], const double (&
 // when both its gap to the target box and eps are below q * diag: there the
 // separations are formed in FP64 per pair. Every FP32 pair then has
 // max(|d|, eps) >= q * diag and a relative force error of at most about
-// 3 u (1 + 1 / q). q is the context's bf_near_fraction (default 1/32).
-__device__ __forceinline__ bool tile_is_near(const double* __restrict__ tb, const double (&lo)[3],
-                                             const double (&hi)[3], double lim2) {
-    double gap2 = 0.0;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const double g = fmax(0.0, fmax(tb[d] - hi[d], lo[d] - tb[3 + d]));
-        gap2 = __builtin_fma(g, g, gap2);
-    }
-    return gap2 < lim2;
-}
-
+// 3 u (1 + 1 / q). q is the context's bf_near_fraction (default 1/64).
 // Mixed precision: one wavefront per workgroup owns IPT*64 Morton-consecutive
 // targets and one j chunk. Each TJ-record tile of the chunk is staged through
 // LDS by the wave itself: far tiles as FP32 records relative to the wave's

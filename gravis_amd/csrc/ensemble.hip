@@ -118,6 +118,232 @@ __global__ __launch_bounds__(kEnsBlock) void ens_force_kernel(
     }
 }
 
+// ---- mixed forward in per-system Morton order --------------------------------
+// gravis_ensemble_kdk with mixed precision sorts every system's particles
+// by Morton key inside the system's own cube (the system index is the key's
+// high part, so systems keep their ranges), runs the steps in that order and
+// restores the caller's order at the end. In sorted order a wavefront's 64
+// targets are spatially compact, so ens_wave_kernel rounds sources to FP32
+// relative to the wavefront's own box centre (as bf_mixed_kernel does): 12
+// VALU operations and one v_rsq_f32 per pair instead of forming every
+// separation in FP64. Near tiles (gap to the target box and eps both below
+// q times its diagonal, q = the context's bf_near_fraction) keep FP64
+// separations.
+constexpr int kEnsTile = 128;       // sources per staged tile
+constexpr int kEnsMortonBits = 14;  // per axis inside a system's cube
+
+// Cube of each system: centre and half width of its bounding box.
+__global__ __launch_bounds__(256) void ens_cube_kernel(const EnsBlock* __restrict__ sys,
+                                                       const double* __restrict__ x,
+                                                       const double* __restrict__ y,
+                                                       const double* __restrict__ z,
+                                                       double4* __restrict__ cube) {
+    const EnsBlock b = sys[blockIdx.x];
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = b.j0 + threadIdx.x; i < b.j1; i += blockDim.x) {
+        lo[0] = fmin(lo[0], x[i]); hi[0] = fmax(hi[0], x[i]);
+        lo[1] = fmin(lo[1], y[i]); hi[1] = fmax(hi[1], y[i]);
+        lo[2] = fmin(lo[2], z[i]); hi[2] = fmax(hi[2], z[i]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], __shfl_xor(lo[d], off));
+            hi[d] = fmax(hi[d], __shfl_xor(hi[d], off));
+        }
+    __shared__ double sh[4][6];
+    const int w = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0)
+        for (int d = 0; d < 3; ++d) { sh[w][d] = lo[d]; sh[w][3 + d] = hi[d]; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)(blockDim.x / kWave); ++k)
+            for (int d = 0; d < 3; ++d) {
+                sh[0][d] = fmin(sh[0][d], sh[k][d]);
+                sh[0][3 + d] = fmax(sh[0][3 + d], sh[k][3 + d]);
+            }
+        double ext = 0.0;
+        for (int d = 0; d < 3; ++d) ext = fmax(ext, sh[0][3 + d] - sh[0][d]);
+        const double hw = b.j1 > b.j0 ? 0.5 * ext + 1e-300 : 1.0;
+        cube[blockIdx.x] = make_double4(0.5 * (sh[0][0] + sh[0][3]), 0.5 * (sh[0][1] + sh[0][4]),
+                                        0.5 * (sh[0][2] + sh[0][5]), hw);
+    }
+}
+
+// key = system index << 42 | 42-bit Morton key inside the system's cube.
+__global__ __launch_bounds__(256) void ens_key_kernel(const EnsBlock* __restrict__ sys,
+                                                      const double* __restrict__ x,
+                                                      const double* __restrict__ y,
+                                                      const double* __restrict__ z,
+                                                      const double4* __restrict__ cube,
+                                                      uint64_t* __restrict__ keys,
+                                                      uint32_t* __restrict__ vals) {
+    const EnsBlock b = sys[blockIdx.x];
+    const double4 cb = cube[blockIdx.x];
+    const double cells = (double)(1u << kEnsMortonBits);
+    const double inv = cells / (2.0 * cb.w);
+    auto q = [&](double v, double c) -> uint32_t {
+        const double t = (v - (c - cb.w)) * inv;
+        if (!(t >= 0.0)) return 0u;
+        if (t >= cells - 1.0) return (1u << kEnsMortonBits) - 1u;
+        return (uint32_t)t;
+    };
+    for (int i = b.j0 + threadIdx.x; i < b.j1; i += blockDim.x) {
+        const uint64_t mk = morton_spread3(q(x[i], cb.x)) | (morton_spread3(q(y[i], cb.y)) << 1) |
+                            (morton_spread3(q(z[i], cb.z)) << 2);
+        keys[i] = ((uint64_t)blockIdx.x << (3 * kEnsMortonBits)) | mk;
+        vals[i] = (uint32_t)i;
+    }
+}
+
+// dst[k][i] = src[k][perm[i]] for the state arrays (gather into sorted
+// order), or dst[k][perm[i]] = src[k][i] (back to the caller's order).
+struct EnsArrays {
+    const double* src[10];
+    double* dst[10];
+    int count;
+};
+
+__global__ __launch_bounds__(256) void ens_permute_kernel(EnsArrays a, const uint32_t* __restrict__ perm,
+                                                          size_t n, int scatter) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t p = perm[i];
+        for (int k = 0; k < a.count; ++k) {
+            if (scatter) a.dst[k][p] = a.src[k][i];
+            else a.dst[k][i] = a.src[k][p];
+        }
+    }
+}
+
+// One wavefront per item: targets [i0, i1) (at most 64) of one system,
+// sources [j0, j1) = that system; assigns a, optionally kicks v by h a.
+template <bool MASK>
+__global__ __launch_bounds__(kWave) void ens_wave_kernel(
+    const EnsBlock* __restrict__ items, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ m,
+    float eps2, double near2, double h, int kick, double* __restrict__ ax,
+    double* __restrict__ ay, double* __restrict__ az, double* __restrict__ vx,
+    double* __restrict__ vy, double* __restrict__ vz) {
+    constexpr int LPT = kEnsTile / kWave;
+    __shared__ double4 tile[kEnsTile];  // near: FP64 records; far: float4 [0, kEnsTile)
+    const EnsBlock b = items[blockIdx.x];
+    const int lane = threadIdx.x;
+    const int i = b.i0 + lane;
+    const bool active = i < b.i1;
+    const int ii = active ? i : b.i0;
+    double lo[3], hi[3];
+    {
+        const double xd[1] = {x[ii]}, yd[1] = {y[ii]}, zd[1] = {z[ii]};
+        wave_box<1>(xd, yd, zd, lo, hi);
+    }
+    const double px = x[ii], py = y[ii], pz = z[ii];
+    const double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]), cz = 0.5 * (lo[2] + hi[2]);
+    double di2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) di2 = __builtin_fma(hi[d] - lo[d], hi[d] - lo[d], di2);
+    double lim2 = near2 * di2;
+    if ((double)eps2 >= lim2) lim2 = 0.0;  // softening bounds every pair's error
+    const float xi = (float)(px - cx), yi = (float)(py - cy), zi = (float)(pz - cz);
+    float sx = 0.f, sy = 0.f, sz = 0.f, kx = 0.f, ky = 0.f, kz = 0.f;
+    // records of the next tile are loaded while the current one is computed
+    double4 nxt[LPT];
+    auto load = [&](int jb) {
+#pragma unroll
+        for (int e = 0; e < LPT; ++e) {
+            const int j = jb + e * kWave + lane;
+            // padding: zero mass at the box centre (no contribution)
+            nxt[e] = j < b.j1 ? make_double4(x[j], y[j], z[j], m[j])
+                              : make_double4(cx, cy, cz, 0.0);
+        }
+    };
+    if (b.j0 < b.j1) load(b.j0);
+    for (int jb = b.j0; jb < b.j1; jb += kEnsTile) {
+        double4 rec[LPT];
+        double tb[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+#pragma unroll
+        for (int e = 0; e < LPT; ++e) {
+            const int j = jb + e * kWave + lane;
+            const bool ok = j < b.j1;
+            rec[e] = nxt[e];
+            if (ok) {
+                tb[0] = fmin(tb[0], rec[e].x); tb[3] = fmax(tb[3], rec[e].x);
+                tb[1] = fmin(tb[1], rec[e].y); tb[4] = fmax(tb[4], rec[e].y);
+                tb[2] = fmin(tb[2], rec[e].z); tb[5] = fmax(tb[5], rec[e].z);
+            }
+        }
+        bool near = false;
+        if (lim2 > 0.0) {
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    tb[d] = fmin(tb[d], __shfl_xor(tb[d], off));
+                    tb[3 + d] = fmax(tb[3 + d], __shfl_xor(tb[3 + d], off));
+                }
+            near = tile_is_near(tb, lo, hi, lim2);
+        }
+        near = __builtin_amdgcn_readfirstlane((int)near) != 0;
+        if (near) {
+#pragma unroll
+            for (int e = 0; e < LPT; ++e) tile[e * kWave + lane] = rec[e];
+        } else {
+            float4* t32 = reinterpret_cast<float4*>(&tile[0]);
+#pragma unroll
+            for (int e = 0; e < LPT; ++e)
+                t32[e * kWave + lane] = make_float4((float)(rec[e].x - cx), (float)(rec[e].y - cy),
+                                                    (float)(rec[e].z - cz), (float)rec[e].w);
+        }
+        __syncthreads();
+        if (jb + kEnsTile < b.j1) load(jb + kEnsTile);
+        const int cnt = min(kEnsTile, b.j1 - jb);
+        float tx = 0.f, ty = 0.f, tz = 0.f;
+        if (!near) {
+            const float4* t32 = reinterpret_cast<const float4*>(&tile[0]);
+#pragma unroll 4
+            for (int k = 0; k < cnt; ++k) {
+                const float4 s4 = t32[k];
+                const float dx = s4.x - xi, dy = s4.y - yi, dz = s4.z - zi;
+                const float f = pair_f32<MASK>(dx, dy, dz, s4.w, eps2);
+                tx = __builtin_fmaf(f, dx, tx);
+                ty = __builtin_fmaf(f, dy, ty);
+                tz = __builtin_fmaf(f, dz, tz);
+            }
+        } else {
+            for (int k = 0; k < cnt; ++k) {
+                const double4 s4 = tile[k];
+                const float dx = (float)(s4.x - px), dy = (float)(s4.y - py),
+                            dz = (float)(s4.z - pz);
+                const float r2 =
+                    __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+                float ri = __builtin_amdgcn_rsqf(r2);
+                const float hh = (r2 * ri) * ri;
+                ri = ri * __builtin_fmaf(-0.5f, hh, 1.5f);
+                if (MASK) ri = r2 > 0.0f ? ri : 0.0f;
+                const float f = ((float)s4.w * ri) * (ri * ri);
+                tx = __builtin_fmaf(f, dx, tx);
+                ty = __builtin_fmaf(f, dy, ty);
+                tz = __builtin_fmaf(f, dz, tz);
+            }
+        }
+        kahan_add(sx, kx, tx);
+        kahan_add(sy, ky, ty);
+        kahan_add(sz, kz, tz);
+        __syncthreads();
+    }
+    if (active) {
+        const double a0 = (double)sx - (double)kx, a1 = (double)sy - (double)ky,
+                     a2 = (double)sz - (double)kz;
+        ax[i] = a0; ay[i] = a1; az[i] = a2;
+        if (kick) {
+            vx[i] = vx[i] + a0 * h;
+            vy[i] = vy[i] + a1 * h;
+            vz[i] = vz[i] + a2 * h;
+        }
+    }
+}
+
 // lx += J_x^T u, lm += J_m^T u (FP64).
 __global__ __launch_bounds__(kEnsBlock) void ens_vjp_kernel(
     const EnsBlock* __restrict__ blocks, const double* __restrict__ x,
@@ -229,6 +455,7 @@ __global__ __launch_bounds__(256) void adj_loss_kernel(const EnsBlock* __restric
 
 struct EnsPlan {
     std::vector<EnsBlock> blocks;  // force/vjp work items, largest systems first
+    std::vector<EnsBlock> waves;   // 64-target items of ens_wave_kernel, same order
     std::vector<EnsBlock> systems; // one per system (j0, j1)
     size_t n = 0;
 };
@@ -252,6 +479,8 @@ gravis_status make_plan(Ctx* c, int nsys, const uint64_t* offsets, EnsPlan* p) {
         const int j0 = (int)offsets[s], j1 = (int)offsets[s + 1];
         for (int i0 = j0; i0 < j1; i0 += kEnsBlock)
             p->blocks.push_back({i0, std::min(i0 + kEnsBlock, j1), j0, j1});
+        for (int i0 = j0; i0 < j1; i0 += kWave)
+            p->waves.push_back({i0, std::min(i0 + kWave, j1), j0, j1});
     }
     p->n = offsets[nsys];
     return GRAVIS_OK;
@@ -289,6 +518,47 @@ gravis_status ens_step(Ctx* c, bool mixed, const EnsBlock* d_blocks, int nblocks
     return ens_force(c, mixed, d_blocks, nblocks, a, eps2, h, 1);
 }
 
+// Mixed force on state in per-system Morton order (one wavefront per item).
+gravis_status ens_wave_force(Ctx* c, const EnsBlock* d_waves, int nwaves, const EnsArgs& a,
+                             double eps2, double h, int kick) {
+    if (nwaves == 0) return GRAVIS_OK;
+    const float e2 = (float)eps2;
+    const double q2 = c->bf_near_fraction * c->bf_near_fraction;
+    if (!(e2 >= 1.17549435e-38f))
+        ens_wave_kernel<true><<<nwaves, kWave, 0, c->stream>>>(
+            d_waves, a.x, a.y, a.z, a.m, e2, q2, h, kick, a.ax, a.ay, a.az, a.vx, a.vy, a.vz);
+    else
+        ens_wave_kernel<false><<<nwaves, kWave, 0, c->stream>>>(
+            d_waves, a.x, a.y, a.z, a.m, e2, q2, h, kick, a.ax, a.ay, a.az, a.vx, a.vy, a.vz);
+    return check_launch(c, "ens_wave_force");
+}
+
+gravis_status ens_wave_step(Ctx* c, const EnsBlock* d_waves, int nwaves, size_t n,
+                            const EnsArgs& a, double eps2, double dt) {
+    const double h = 0.5 * dt;
+    GRAVIS_TRY(kick_drift(c, n, a.x, a.y, a.z, a.vx, a.vy, a.vz, a.ax, a.ay, a.az, h, dt));
+    return ens_wave_force(c, d_waves, nwaves, a, eps2, h, 1);
+}
+
+// Permutation that sorts every system's particles by Morton key within the
+// system (systems keep their ranges): perm[sorted position] = caller index.
+gravis_status ens_sort(Ctx* c, const EnsPlan& plan, const EnsBlock* d_sys, const double* x,
+                       const double* y, const double* z, uint32_t** perm) {
+    const int nsys = (int)plan.systems.size();
+    double4* cube;
+    GRAVIS_TRY(buf(c, "ens_cube", (size_t)nsys, &cube));
+    ens_cube_kernel<<<nsys, 256, 0, c->stream>>>(d_sys, x, y, z, cube);
+    uint64_t* keys;
+    uint32_t* vals;
+    GRAVIS_TRY(sort_buffers(c, plan.n, &keys, &vals));
+    ens_key_kernel<<<nsys, 256, 0, c->stream>>>(d_sys, x, y, z, cube, keys, vals);
+    GRAVIS_TRY(check_launch(c, "ens_sort keys"));
+    int bits = 3 * kEnsMortonBits;
+    while ((1ll << (bits - 3 * kEnsMortonBits)) < nsys) ++bits;
+    uint64_t* sk;
+    return radix_sort(c, plan.n, (bits + 7) / 8, &sk, perm);
+}
+
 gravis_status check_solver_ens(Ctx* c, const gravis_solver* s) {
     if (!s) return fail(c, GRAVIS_INVALID_ARG, "solver is NULL");
     if (s->algorithm != GRAVIS_BRUTE_FORCE)
@@ -312,20 +582,63 @@ gravis_status ensemble_kdk(Ctx* c, const gravis_solver* s, int nsys, const uint6
     if (plan.n == 0 || nsteps == 0) return GRAVIS_OK;
     EnsBlock* d_blocks;
     GRAVIS_TRY(upload(c, "ens_blocks", plan.blocks, &d_blocks));
-    const EnsArgs a{x, y, z, vx, vy, vz, ax, ay, az, m};
     const bool mixed = s->precision == GRAVIS_MIXED;
     const double eps2 = s->softening * s->softening;
     const int nb = (int)plan.blocks.size();
+    const size_t n = plan.n;
+    EnsArgs a{x, y, z, vx, vy, vz, ax, ay, az, m};
+    // Mixed: run in per-system Morton order on a sorted copy of the state.
+    EnsBlock* d_waves = nullptr;
+    uint32_t* perm = nullptr;
+    double* sorted = nullptr;
+    // (the sort key holds the system index in 21 bits above the Morton key)
+    const bool sorted_run = mixed && plan.systems.size() <= (1u << 21);
+    if (sorted_run) {
+        EnsBlock* d_sys;
+        GRAVIS_TRY(upload(c, "ens_sys", plan.systems, &d_sys));
+        GRAVIS_TRY(upload(c, "ens_waves", plan.waves, &d_waves));
+        GRAVIS_TRY(ens_sort(c, plan, d_sys, x, y, z, &perm));
+        GRAVIS_TRY(buf(c, "ens_sorted", 10 * n, &sorted));
+        EnsArrays pa;
+        const double* src[10] = {x, y, z, vx, vy, vz, ax, ay, az, m};
+        for (int k = 0; k < 10; ++k) {
+            pa.src[k] = src[k];
+            pa.dst[k] = sorted + (size_t)k * n;
+        }
+        pa.count = 10;
+        ens_permute_kernel<<<grid_for(n, 256, 4096), 256, 0, c->stream>>>(pa, perm, n, 0);
+        GRAVIS_TRY(check_launch(c, "ens_permute"));
+        a = EnsArgs{sorted, sorted + n, sorted + 2 * n, sorted + 3 * n, sorted + 4 * n,
+                    sorted + 5 * n, sorted + 6 * n, sorted + 7 * n, sorted + 8 * n,
+                    sorted + 9 * n};
+    }
+    const int nw = (int)plan.waves.size();
+    auto step = [&]() -> gravis_status {
+        return sorted_run ? ens_wave_step(c, d_waves, nw, n, a, eps2, dt)
+                          : ens_step(c, mixed, d_blocks, nb, n, a, eps2, dt);
+    };
+    // back to the caller's order (positions, velocities, accelerations)
+    auto restore = [&]() -> gravis_status {
+        if (!sorted_run) return GRAVIS_OK;
+        EnsArrays pa;
+        double* dst[9] = {x, y, z, vx, vy, vz, ax, ay, az};
+        for (int k = 0; k < 9; ++k) {
+            pa.src[k] = sorted + (size_t)k * n;
+            pa.dst[k] = dst[k];
+        }
+        pa.count = 9;
+        ens_permute_kernel<<<grid_for(n, 256, 4096), 256, 0, c->stream>>>(pa, perm, n, 1);
+        return check_launch(c, "ens_permute");
+    };
     if (!use_graph) {
-        for (uint64_t t = 0; t < nsteps; ++t)
-            GRAVIS_TRY(ens_step(c, mixed, d_blocks, nb, plan.n, a, eps2, dt));
-        return GRAVIS_OK;
+        for (uint64_t t = 0; t < nsteps; ++t) GRAVIS_TRY(step());
+        return restore();
     }
     // Capture one step (kick-drift + force/kick) and replay it.
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     GRAVIS_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    gravis_status st = ens_step(c, mixed, d_blocks, nb, plan.n, a, eps2, dt);
+    gravis_status st = step();
     hipError_t e = hipStreamEndCapture(c->stream, &graph);
     if (st != GRAVIS_OK) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -341,7 +654,7 @@ gravis_status ensemble_kdk(Ctx* c, const gravis_solver* s, int nsys, const uint6
     (void)hipGraphExecDestroy(exec);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return fail(c, GRAVIS_HIP_ERROR, "graph launch: %s", hipGetErrorString(e));
-    return GRAVIS_OK;
+    return restore();
 }
 
 gravis_status ensemble_accel(Ctx* c, const gravis_solver* s, int nsys, const uint64_t* offsets,

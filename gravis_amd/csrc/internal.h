@@ -65,7 +65,7 @@ struct Ctx {
     // Mixed brute force: a source tile closer to the target box than this
     // fraction q of the box diagonal gets FP64 pair separations, unless the
     // softening exceeds q times the diagonal.
-    double bf_near_fraction = 1.0 / 32.0;
+    double bf_near_fraction = 1.0 / 64.0;
 };
 
 // Status helpers ----------------------------------------------------------
@@ -105,6 +105,67 @@ __device__ __forceinline__ double uniform(double v) {
     const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
     const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// ---- pair-force helpers shared by the brute-force kernels ----------------
+// Pair force factor in FP32: m * r^-3 with r^2 = d.d + eps^2.
+template <bool MASK>
+__device__ __forceinline__ float pair_f32(float dx, float dy, float dz, float m, float eps2) {
+    const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+    float ri = __builtin_amdgcn_rsqf(r2);
+    if (MASK) ri = r2 > 0.0f ? ri : 0.0f;
+    const float mr = m * ri;
+    const float ri2 = ri * ri;
+    return mr * ri2;
+}
+
+__device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
+    const float y = v - c;
+    const float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// Bounding box of the wavefront's IPT*64 targets (FP64); every lane ends
+// with the same box, held in scalar registers.
+template <int IPT>
+__device__ __forceinline__ void wave_box(const double (&x)[IPT], const double (&y)[IPT],
+                                         const double (&z)[IPT], double (&lo)[3],
+                                         double (&hi)[3]) {
+    lo[0] = hi[0] = x[0];
+    lo[1] = hi[1] = y[0];
+    lo[2] = hi[2] = z[0];
+#pragma unroll
+    for (int k = 1; k < IPT; ++k) {
+        lo[0] = fmin(lo[0], x[k]); hi[0] = fmax(hi[0], x[k]);
+        lo[1] = fmin(lo[1], y[k]); hi[1] = fmax(hi[1], y[k]);
+        lo[2] = fmin(lo[2], z[k]); hi[2] = fmax(hi[2], z[k]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], __shfl_xor(lo[d], off));
+            hi[d] = fmax(hi[d], __shfl_xor(hi[d], off));
+        }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = uniform(lo[d]);
+        hi[d] = uniform(hi[d]);
+    }
+}
+
+// True when the gap between a source box tb = (lo xyz, hi xyz) and the
+// target box (lo, hi) is below sqrt(lim2).
+__device__ __forceinline__ bool tile_is_near(const double* __restrict__ tb, const double (&lo)[3],
+                                             const double (&hi)[3], double lim2) {
+    double gap2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double g = fmax(0.0, fmax(tb[d] - hi[d], lo[d] - tb[3 + d]));
+        gap2 = __builtin_fma(g, g, gap2);
+    }
+    return gap2 < lim2;
 }
 
 // Exclusive scan of one u32 per thread over a block of BLOCK threads (a
@@ -148,6 +209,22 @@ gravis_status root_box(Ctx* c, size_t n, const double* x, const double* y, const
 // Outputs: d_keys_sorted[n], d_perm[n] (sorted position -> original index).
 gravis_status morton_sort(Ctx* c, size_t n, const double* x, const double* y, const double* z,
                           const double* d_box, uint64_t** d_keys_sorted, uint32_t** d_perm);
+// Stable LSD radix sort of n 64-bit keys with u32 values over the low
+// 8 * passes bits. The caller fills the buffers of sort_buffers first; the
+// sorted keys and values are returned (either set of ping-pong buffers).
+gravis_status sort_buffers(Ctx* c, size_t n, uint64_t** keys, uint32_t** vals);
+gravis_status radix_sort(Ctx* c, size_t n, int passes, uint64_t** d_keys_sorted,
+                         uint32_t** d_vals_sorted);
+// Spreads the low 21 bits of v to every third bit (Morton interleave).
+__device__ __forceinline__ uint64_t morton_spread3(uint32_t v) {
+    uint64_t x = v & 0x1FFFFFu;
+    x = (x | (x << 32)) & 0x1F00000000FFFFull;
+    x = (x | (x << 16)) & 0x1F0000FF0000FFull;
+    x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+    x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
 // Gather x,y,z,m into double4 records in d_perm order (identity if null).
 gravis_status gather_sorted(Ctx* c, size_t n, const double* x, const double* y,
                             const double* z, const double* m, const uint32_t* d_perm,

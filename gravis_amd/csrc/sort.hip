@@ -102,16 +102,6 @@ __global__ __launch_bounds__(kBoxBlock) void box_final_kernel(const double* __re
     box[3] = 0.5 * ext * 1.001 + 1e-10;
 }
 
-__device__ __forceinline__ uint64_t spread3(uint32_t v) {
-    uint64_t x = v & 0x1FFFFFu;
-    x = (x | (x << 32)) & 0x1F00000000FFFFull;
-    x = (x | (x << 16)) & 0x1F0000FF0000FFull;
-    x = (x | (x << 8)) & 0x100F00F00F00F00Full;
-    x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
-    x = (x | (x << 2)) & 0x1249249249249249ull;
-    return x;
-}
-
 __device__ __forceinline__ uint32_t quantise(double v, double lo, double inv) {
     double q = (v - lo) * inv;
     if (!(q >= 0.0)) return 0u;
@@ -132,7 +122,7 @@ __global__ __launch_bounds__(256) void morton_kernel(const double* __restrict__ 
          i += (size_t)gridDim.x * blockDim.x) {
         uint32_t qx = quantise(x[i], lx, inv), qy = quantise(y[i], ly, inv),
                  qz = quantise(z[i], lz, inv);
-        keys[i] = spread3(qx) | (spread3(qy) << 1) | (spread3(qz) << 2);
+        keys[i] = morton_spread3(qx) | (morton_spread3(qy) << 1) | (morton_spread3(qz) << 2);
         vals[i] = static_cast<uint32_t>(i);
     }
 }
@@ -302,9 +292,17 @@ gravis_status root_box(Ctx* c, size_t n, const double* x, const double* y, const
     return check_launch(c, "root_box");
 }
 
-gravis_status morton_sort(Ctx* c, size_t n, const double* x, const double* y, const double* z,
-                          const double* d_box, uint64_t** d_keys_sorted, uint32_t** d_perm) {
+gravis_status sort_buffers(Ctx* c, size_t n, uint64_t** keys, uint32_t** vals) {
     if (n > 0xFFFFFFFFull) return fail(c, GRAVIS_INVALID_ARG, "n too large for 32-bit indices");
+    GRAVIS_TRY(buf(c, "sort_k0", n, keys));
+    GRAVIS_TRY(buf(c, "sort_v0", n, vals));
+    return GRAVIS_OK;
+}
+
+gravis_status radix_sort(Ctx* c, size_t n, int passes, uint64_t** d_keys_sorted,
+                         uint32_t** d_vals_sorted) {
+    if (n > 0xFFFFFFFFull) return fail(c, GRAVIS_INVALID_ARG, "n too large for 32-bit indices");
+    if (passes < 1 || passes > 8) return fail(c, GRAVIS_INTERNAL, "radix passes must be 1..8");
     uint64_t *k0, *k1;
     uint32_t *v0, *v1, *counts, *dbase;
     const int nblocks = (int)((n + kRsTile - 1) / kRsTile);
@@ -315,8 +313,7 @@ gravis_status morton_sort(Ctx* c, size_t n, const double* x, const double* y, co
     GRAVIS_TRY(buf(c, "sort_v1", n, &v1));
     GRAVIS_TRY(buf(c, "sort_counts", m, &counts));
     GRAVIS_TRY(buf(c, "sort_dbase", kRsDigits, &dbase));
-    morton_kernel<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(x, y, z, n, d_box, k0, v0);
-    for (int pass = 0; pass < 8; ++pass) {
+    for (int pass = 0; pass < passes; ++pass) {
         const int shift = 8 * pass;
         radix_hist_kernel<<<nblocks, kRsBlock, 0, c->stream>>>(k0, n, shift, counts, nblocks);
         radix_row_scan_kernel<<<kRsDigits, kScanBlock, 0, c->stream>>>(counts, nblocks, dbase);
@@ -326,10 +323,19 @@ gravis_status morton_sort(Ctx* c, size_t n, const double* x, const double* y, co
         std::swap(k0, k1);
         std::swap(v0, v1);
     }
-    // 8 passes: the result is back in sort_k0/sort_v0.
     *d_keys_sorted = k0;
-    *d_perm = v0;
-    return check_launch(c, "morton_sort");
+    *d_vals_sorted = v0;
+    return check_launch(c, "radix_sort");
+}
+
+gravis_status morton_sort(Ctx* c, size_t n, const double* x, const double* y, const double* z,
+                          const double* d_box, uint64_t** d_keys_sorted, uint32_t** d_perm) {
+    uint64_t* k0;
+    uint32_t* v0;
+    GRAVIS_TRY(sort_buffers(c, n, &k0, &v0));
+    morton_kernel<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(x, y, z, n, d_box, k0, v0);
+    // 8 passes: the result is back in the first buffers
+    return radix_sort(c, n, 8, d_keys_sorted, d_perm);
 }
 
 gravis_status gather_sorted(Ctx* c, size_t n, const double* x, const double* y,

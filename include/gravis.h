@@ -322,9 +322,10 @@ gravis_status gravis_set_tuning(gravis_ctx* ctx, const char* key, int64_t value)
  * j-range split aims for; Barnes-Hut: target group size used when the
  * solver's group_size is 0. Set one at a time with gravis_set_tuning keys
  * "bf_ipt", "bf_tile", "bf_waves_per_cu", "bh_group_size"; "bf_near_64ths"
- * (0..64, default 2) is the near-tile fraction q of the mixed kernel: a
- * source tile whose gap to the target box and the softening are both below
- * q times the box diagonal gets FP64 pair separations. */
+ * (0..64, default 1) is the near-tile fraction q, in 64ths, of the mixed
+ * brute-force and ensemble kernels: a source tile whose gap to the
+ * wavefront's target box and the softening are both below q times the box
+ * diagonal gets FP64 pair separations. */
 typedef struct gravis_tuning {
     int32_t bf_ipt;
     int32_t bf_tile;
