@@ -116,8 +116,8 @@ def probe_copy():
 def run_bh(args, g, torch, ctx, precision="mixed", steps=None):
     """Barnes-Hut KDK steps on two Plummer spheres of n_each particles (seeds
     42/43, a = 1, M = 1 each) 10 a apart on a parabolic approach, eps 0.01,
-    dt 5e-4, theta 0.5, leaf 16, target groups of the autotuned size (at most
-    64). precision 'mixed':
+    dt 5e-4, theta 0.5, leaf 16, target groups of the size the autotuner
+    picks for this precision (at most 64). precision 'mixed':
     FP32 pair forces on FP64 group-relative separations (FP64 tree, state and
     opening test); 'fp64': the reference arithmetic."""
     steps = steps or args.bh_steps
@@ -134,8 +134,7 @@ def run_bh(args, g, torch, ctx, precision="mixed", steps=None):
     torch.cuda.synchronize()
     # group size 0: the context's, chosen by the autotuner on the mixed run
     bh = g.BarnesHut(0.01, 0.5, 16, 0, precision=precision, ctx=ctx)
-    if precision == "mixed":
-        log(f"bench: autotuned BH: {ctx.autotune(bh, p)}")
+    log(f"bench: autotuned BH ({precision}): {ctx.autotune(bh, p)}")
     group = ctx.tuning()["bh_group_size"]
     L = ctx._L
     keys = ("x", "y", "z", "vx", "vy", "vz", "mass", "ax", "ay", "az")
