@@ -701,15 +701,21 @@ gravis_status gravis_autotune(gravis_ctx* g, const gravis_solver* s, size_t n, c
             best = b.ms;
         }
     } else if (s->algorithm == GRAVIS_BARNES_HUT && s->group_size == 0) {
-        uint32_t bg = c->bh_group;
-        for (uint32_t gs : {16u, 32u, 64u}) {
+        // full wavefronts first: a smaller group is kept only when it is
+        // more than 2 % faster
+        std::vector<Shape> v;
+        for (uint32_t gs : {64u, 32u, 16u}) {
             c->bh_group = gs;
             double ms;
             st = time_it(&ms);
             if (st != GRAVIS_OK) break;
-            if (best < 0.0 || ms < best) { best = ms; bg = gs; }
+            v.push_back({(int)gs, 0, 0, ms});
         }
-        c->bh_group = bg;
+        if (st == GRAVIS_OK) {
+            const Shape b = pick(v);
+            c->bh_group = (uint32_t)b.ipt;
+            best = b.ms;
+        }
     } else {
         st = time_it(&best);
     }

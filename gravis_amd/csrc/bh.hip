@@ -19,10 +19,13 @@
 // boxes). Accepted nodes and the particles of opened leaves are appended, in
 // frontier order, as source indices to a per-wave LDS list; each flush round
 // stages 64 source records in LDS (one load per lane) that all 64 lanes then
-// read back as broadcasts. Pair arithmetic is barnes_hut.rs:114-135 in FP64
-// (sqrt and divide correctly rounded, no contraction); a source at r^2 = 0 is
-// skipped. The accumulation order is the list order, so results are
-// bit-for-bit the CPU oracle's group walk with the same groups.
+// read back as broadcasts. A group of at most 32 (16) targets is held by 2
+// (4) lanes per target: copy r sums the list positions p with p % R == r,
+// in order, and the copies are added as (r0 + r1) [+ (r2 + r3)]. Pair
+// arithmetic is barnes_hut.rs:114-135 in FP64 (sqrt and divide correctly
+// rounded, no contraction); a source at r^2 = 0 is skipped. The oracle's
+// group walk restates the list positions (64-node chunks, 1024-entry list)
+// and the copies, so results are bit-for-bit the same with the same groups.
 #include <algorithm>
 
 #include "internal.h"
@@ -95,10 +98,10 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
         const uint2 gr = groups[g];
         const int i0 = (int)gr.x;
         const int cnt = (int)gr.y;
-        // MIXED: a group of at most 32 (16) targets is held twice (four
-        // times) across the wave; replica r sums every R-th source of each
-        // flush round and the replicas are added in a fixed order at the end.
-        const int R = MIXED ? (cnt <= 16 ? 4 : (cnt <= 32 ? 2 : 1)) : 1;
+        // A group of at most 32 (16) targets is held twice (four times)
+        // across the wave; replica r sums the sources at list positions
+        // p % R == r, and the replicas are added in a fixed order at the end.
+        const int R = cnt <= 16 ? 4 : (cnt <= 32 ? 2 : 1);
         const int tpr = kWave / R;          // lanes per replica
         const int tl = lane % tpr;          // this lane's target
         const int rep = lane / tpr;         // this lane's replica
@@ -169,8 +172,14 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
                     }
                 } else {
                     double tx = ax, ty = ay, tz = az;
-                    for (uint32_t k = 0; k < m; ++k)
-                        pair_ref(stage[k], me.x, me.y, me.z, eps2, tx, ty, tz);
+                    for (uint32_t k0 = 0; k0 < m; k0 += (uint32_t)R) {
+                        const uint32_t k = k0 + (uint32_t)rep;
+                        // past the round's end: a zero-mass copy of record 0
+                        // (adds exactly zero)
+                        double4 r = stage[k < m ? k : 0];
+                        r.w = k < m ? r.w : 0.0;
+                        pair_ref(r, me.x, me.y, me.z, eps2, tx, ty, tz);
+                    }
                     if (keep) {
                         ax = tx;
                         ay = ty;
@@ -303,13 +312,13 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
             ax = (double)sx - (double)kx;
             ay = (double)sy - (double)ky;
             az = (double)sz - (double)kz;
-            // replicas in a fixed pairing: (r0 + r1) [+ (r2 + r3)]; each sum
-            // is formed identically in both lanes (addition commutes)
-            for (int o = tpr; o < kWave; o <<= 1) {
-                ax += __shfl_xor(ax, o);
-                ay += __shfl_xor(ay, o);
-                az += __shfl_xor(az, o);
-            }
+        }
+        // replicas in a fixed pairing: (r0 + r1) [+ (r2 + r3)]; each sum is
+        // formed identically in both lanes (addition commutes)
+        for (int o = tpr; o < kWave; o <<= 1) {
+            ax += __shfl_xor(ax, o);
+            ay += __shfl_xor(ay, o);
+            az += __shfl_xor(az, o);
         }
         if (active && rep == 0) {
             const size_t i = (size_t)i0 + tl;

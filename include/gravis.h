@@ -342,7 +342,8 @@ gravis_status gravis_get_tuning(gravis_ctx* ctx, gravis_tuning* out);
  * 64..512 x bf_ipt x bf_tile (16 shapes); FP64 brute force:
  * bf_waves_per_cu 32..512; among brute-force shapes within 2 % of the
  * fastest, the one with the fewest j chunks (least partial-sum memory) is
- * kept. Barnes-Hut with group_size 0: the fastest of group sizes 16, 32, 64.
+ * kept. Barnes-Hut with group_size 0: group sizes 64, 32, 16, keeping a
+ * smaller one only when it is more than 2 % faster.
  * Each shape costs three force calls. Shapes change the summation grouping,
  * so forces move in their last bits (brute force) or within the
  * opening-angle error (Barnes-Hut groups). out may be NULL. */

@@ -904,47 +904,93 @@ size_t orc_linear_tree(size_t n, const double* x, const double* y, const double*
  * arithmetic (barnes_hut.rs:114-135); a source at r2 == 0 is skipped. */
 static size_t g_max_frontier = 0;
 
+/* The device walk's interaction-list bookkeeping, restated so the sums are
+ * formed in the same order: frontier nodes are taken in chunks of 64; a
+ * chunk's sources (1 per accepted node, the particles of an opened leaf)
+ * get consecutive positions in a list of LIST_CAP entries, which restarts
+ * at 0 (a flush) when the chunk would not fit, or, for a chunk larger than
+ * the list, whenever it is full. A group of at most 32 (16) targets is held
+ * by 2 (4) lanes per target; lane copy r sums the sources at positions
+ * p with p % R == r, and the copies are added as (r0 + r1) [+ (r2 + r3)]. */
+#define LIST_CAP 1024u
+#define WAVE 64u
+
+static int group_replicas(size_t cnt) { return cnt <= 16 ? 4 : (cnt <= 32 ? 2 : 1); }
+
+typedef struct {
+    double* p;   /* (g1 - g0) * R * 3 partial sums: target, copy, component */
+    int R;
+    uint32_t nl; /* list position */
+} rsum;
+
+static void apply_source(const double* sx, const double* sy, const double* sz, double eps2,
+                         const double* e2s, size_t g0, size_t g1, double srcx, double srcy,
+                         double srcz, double srcm, rsum* acc, int r) {
+    for (size_t i = g0; i < g1; ++i) {
+        double dx = srcx - sx[i], dy = srcy - sy[i], dz = srcz - sz[i];
+        double r2 = dx * dx + dy * dy + dz * dz + (e2s ? e2s[i] : eps2);
+        if (r2 == 0.0) continue;
+        double inv_r3 = 1.0 / (r2 * sqrt(r2));
+        double fm = 1.0 * srcm * inv_r3;
+        double* q = acc->p + ((i - g0) * acc->R + r) * 3;
+        q[0] = q[0] + fm * dx; q[1] = q[1] + fm * dy; q[2] = q[2] + fm * dz;
+    }
+}
+
 /* One breadth-first walk for targets [g0, g1) with opening box (lo, hi).
- * Returns 1 (and leaves partial sums) if a next frontier exceeds fcap. */
+ * Returns 1 if a next frontier exceeds fcap (the sums are then discarded). */
 static int walk_box(const ltree* t, double side0, double theta2, double eps2, const double* e2s,
-                    const double* sx,
-                    const double* sy, const double* sz, const double* sm, size_t g0, size_t g1,
-                    const double lo[3], const double hi[3], double* ax, double* ay, double* az,
+                    const double* sx, const double* sy, const double* sz, const double* sm,
+                    size_t g0, size_t g1, const double lo[3], const double hi[3], rsum* acc,
                     uint32_t* fa, uint32_t* fb, uint32_t fcap, uint64_t* ninter) {
     size_t nf = 1;
     fa[0] = 0;
+    acc->nl = 0;
     uint64_t cnt_inter = 0;
     while (nf > 0) {
         size_t nn = 0;
-        for (size_t f = 0; f < nf; ++f) {
-            uint32_t v = fa[f];
-            double cx = t->comx[v], cy = t->comy[v], cz = t->comz[v];
-            double ex = fmax(fmax(lo[0] - cx, cx - hi[0]), 0.0);
-            double ey = fmax(fmax(lo[1] - cy, cy - hi[1]), 0.0);
-            double ez = fmax(fmax(lo[2] - cz, cz - hi[2]), 0.0);
-            double d2 = ex * ex + ey * ey + ez * ez;
-            double s = ldexp(side0, -(int)t->level[v]);
-            int accept = (s * s < theta2 * d2);
-            if (!accept && t->nchild[v] != 0) {
-                for (uint32_t c = 0; c < t->nchild[v]; ++c) fb[nn++] = t->child[v] + c;
-                continue;
+        for (size_t base = 0; base < nf; base += WAVE) {
+            size_t end = base + WAVE < nf ? base + WAVE : nf;
+            int acc_flag[WAVE];
+            uint32_t total = 0;
+            for (size_t f = base; f < end; ++f) {
+                uint32_t v = fa[f];
+                double cx = t->comx[v], cy = t->comy[v], cz = t->comz[v];
+                double ex = fmax(fmax(lo[0] - cx, cx - hi[0]), 0.0);
+                double ey = fmax(fmax(lo[1] - cy, cy - hi[1]), 0.0);
+                double ez = fmax(fmax(lo[2] - cz, cz - hi[2]), 0.0);
+                double d2 = ex * ex + ey * ey + ez * ez;
+                double s = ldexp(side0, -(int)t->level[v]);
+                int accept = (s * s < theta2 * d2);
+                acc_flag[f - base] = accept;
+                if (accept) total += 1;
+                else if (t->nchild[v] == 0) total += t->cnt[v];
             }
-            uint32_t kb, ke;
-            double srcx, srcy, srcz, srcm;
-            int single = accept;
-            if (single) { kb = 0; ke = 1; } else { kb = t->first[v]; ke = kb + t->cnt[v]; }
-            for (uint32_t k = kb; k < ke; ++k) {
-                if (single) { srcx = cx; srcy = cy; srcz = cz; srcm = t->mass[v]; }
-                else { srcx = sx[k]; srcy = sy[k]; srcz = sz[k]; srcm = sm[k]; }
-                for (size_t i = g0; i < g1; ++i) {
-                    double dx = srcx - sx[i], dy = srcy - sy[i], dz = srcz - sz[i];
-                    double r2 = dx * dx + dy * dy + dz * dz + (e2s ? e2s[i] : eps2);
-                    if (r2 == 0.0) continue;
-                    double inv_r3 = 1.0 / (r2 * sqrt(r2));
-                    double fm = 1.0 * srcm * inv_r3;
-                    ax[i] = ax[i] + fm * dx; ay[i] = ay[i] + fm * dy; az[i] = az[i] + fm * dz;
+            if (total > 0 && acc->nl + total > LIST_CAP) acc->nl = 0;
+            for (size_t f = base; f < end; ++f) {
+                uint32_t v = fa[f];
+                int accept = acc_flag[f - base];
+                if (!accept && t->nchild[v] != 0) continue;
+                uint32_t kb, ke;
+                int single = accept;
+                if (single) { kb = 0; ke = 1; } else { kb = t->first[v]; ke = kb + t->cnt[v]; }
+                for (uint32_t k = kb; k < ke; ++k) {
+                    if (total > LIST_CAP && acc->nl == LIST_CAP) acc->nl = 0;
+                    int r = (int)(acc->nl % (uint32_t)acc->R);
+                    acc->nl++;
+                    if (single)
+                        apply_source(sx, sy, sz, eps2, e2s, g0, g1, t->comx[v], t->comy[v],
+                                     t->comz[v], t->mass[v], acc, r);
+                    else
+                        apply_source(sx, sy, sz, eps2, e2s, g0, g1, sx[k], sy[k], sz[k], sm[k],
+                                     acc, r);
+                    cnt_inter += (uint64_t)(g1 - g0);
                 }
-                cnt_inter += (uint64_t)(g1 - g0);
+            }
+            for (size_t f = base; f < end; ++f) {
+                uint32_t v = fa[f];
+                if (!acc_flag[f - base] && t->nchild[v] != 0)
+                    for (uint32_t c = 0; c < t->nchild[v]; ++c) fb[nn++] = t->child[v] + c;
             }
         }
         uint32_t* tmp = fa; fa = fb; fb = tmp;
@@ -964,30 +1010,43 @@ static int walk_box(const ltree* t, double side0, double theta2, double eps2, co
     return 0;
 }
 
-/* Group walk for targets [g0, g1): the group's bounding box opens the tree;
- * if a breadth-first frontier would exceed fcap nodes, the group's sums are
- * discarded and every target is walked alone with its own position as the
- * box (whose frontier the opening geometry bounds). */
+/* copies combined as (r0 + r1) [+ (r2 + r3)] */
+static double combine(const double* q, int R, int c) {
+    if (R == 1) return q[c];
+    double a = q[c] + q[3 + c];
+    if (R == 2) return a;
+    return a + (q[6 + c] + q[9 + c]);
+}
+
 static void gwalk(const ltree* t, double side0, double theta2, double eps2, const double* e2s,
                   const double* sx,
                   const double* sy, const double* sz, const double* sm, size_t g0, size_t g1,
                   double* ax, double* ay, double* az, uint32_t* fa, uint32_t* fb, uint32_t fcap,
-                  uint64_t* ninter) {
+                  uint64_t* ninter, double* scratch) {
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (size_t i = g0; i < g1; ++i) {
         lo[0] = fmin(lo[0], sx[i]); hi[0] = fmax(hi[0], sx[i]);
         lo[1] = fmin(lo[1], sy[i]); hi[1] = fmax(hi[1], sy[i]);
         lo[2] = fmin(lo[2], sz[i]); hi[2] = fmax(hi[2], sz[i]);
     }
-    for (size_t i = g0; i < g1; ++i) { ax[i] = 0; ay[i] = 0; az[i] = 0; }
-    if (!walk_box(t, side0, theta2, eps2, e2s, sx, sy, sz, sm, g0, g1, lo, hi, ax, ay, az, fa, fb,
-                  fcap, ninter))
-        return;
+    rsum acc = {scratch, group_replicas(g1 - g0), 0};
+    const size_t np = (g1 - g0) * (size_t)acc.R * 3;
+    for (size_t k = 0; k < np; ++k) scratch[k] = 0.0;
+    if (walk_box(t, side0, theta2, eps2, e2s, sx, sy, sz, sm, g0, g1, lo, hi, &acc, fa, fb,
+                 fcap, ninter)) {
+        for (size_t k = 0; k < np; ++k) scratch[k] = 0.0;
+        for (size_t i = g0; i < g1; ++i) {
+            double p[3] = {sx[i], sy[i], sz[i]};
+            rsum one = {scratch + (i - g0) * (size_t)acc.R * 3, acc.R, 0};
+            walk_box(t, side0, theta2, eps2, e2s, sx, sy, sz, sm, i, i + 1, p, p, &one, fa, fb,
+                     0xFFFFFFFFu, ninter);
+        }
+    }
     for (size_t i = g0; i < g1; ++i) {
-        ax[i] = 0; ay[i] = 0; az[i] = 0;
-        double p[3] = {sx[i], sy[i], sz[i]};
-        walk_box(t, side0, theta2, eps2, e2s, sx, sy, sz, sm, i, i + 1, p, p, ax, ay, az, fa, fb,
-                 0xFFFFFFFFu, ninter);
+        const double* q = scratch + (i - g0) * (size_t)acc.R * 3;
+        ax[i] = combine(q, acc.R, 0);
+        ay[i] = combine(q, acc.R, 1);
+        az[i] = combine(q, acc.R, 2);
     }
 }
 
@@ -1111,15 +1170,16 @@ static uint64_t bh_group(size_t n, double eps, const double* eps_i, double theta
     {
         uint32_t* fa = (uint32_t*)malloc(t.len * 4 + 4);
         uint32_t* fb = (uint32_t*)malloc(t.len * 4 + 4);
+        double* scratch = (double*)malloc(64 * 4 * 3 * sizeof(double));
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 1)
 #endif
         for (long long g = 0; g < (long long)ng; ++g) {
             size_t g0 = gfirst[g], g1 = g0 + gcnt[g];
             gwalk(&t, side0, theta2, eps2, eps_i ? e2 : NULL, sx, sy, sz, sm, g0, g1, bx, by, bz,
-                  fa, fb, fcap, &total);
+                  fa, fb, fcap, &total, scratch);
         }
-        free(fa); free(fb);
+        free(fa); free(fb); free(scratch);
     }
     for (size_t k = 0; k < n; ++k) {
         ax[perm[k]] += bx[k]; ay[perm[k]] += by[k]; az[perm[k]] += bz[k];
