@@ -67,6 +67,11 @@ constexpr int kQWaves = 4;
 constexpr uint32_t kQFront = 512;   // breadth-first frontier (nodes) per wave
 constexpr uint32_t kQRanges = 512;  // overlapping leaves per wave
 constexpr uint32_t kQStack = 256;   // depth-first stack (>= 1 + 7 * 22)
+// candidate rounds (of 64) whose hit masks stay in registers; their
+// candidate table (4 B per entry) is written over the leaf ranges, which
+// are not needed once the candidates are counted
+constexpr uint32_t kQMaskRounds = 16;
+static_assert(kQMaskRounds * 64 * 4 <= kQRanges * 8, "candidate table must fit the leaf buffer");
 
 __device__ __forceinline__ void lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -274,9 +279,85 @@ __device__ __forceinline__ bool query_range(
     }
     lds_sync();
 
-    // two passes over the candidates: count, then write
     uint32_t mine = 0, wpos = 0;
     bool failed = false;
+    const uint32_t nrounds = (total + kWave - 1) / kWave;
+    // reserves this lane's list in the pool (wave-wide prefix of the counts)
+    auto allocate = [&]() -> bool {
+        uint32_t gtot;
+        const uint32_t ex = wave_excl_scan(act ? mine : 0u, lane, &gtot);
+        uint32_t base = 0;
+        if (lane == 0) {
+            base = atomicAdd(&ctl[0], gtot);
+            if ((uint64_t)base + gtot > (uint64_t)pool_cap) atomicOr(&ctl[1], 1u);
+        }
+        base = __builtin_amdgcn_readfirstlane(base);
+        if ((uint64_t)base + gtot > (uint64_t)pool_cap) return false;
+        wpos = base + ex;
+        if (act) nbr[s] = make_uint2(wpos, mine);
+        return true;
+    };
+    if (nrounds <= kQMaskRounds) {
+        // one pass over the candidates: per round of 64, each lane keeps the
+        // mask of its hits and the candidate it staged; the lists are then
+        // written from the masks through a table of the candidates
+        uint64_t hit[kQMaskRounds];
+        uint32_t cj[kQMaskRounds];
+#pragma unroll
+        for (uint32_t q = 0; q < kQMaskRounds; ++q) {
+            hit[q] = 0ull;
+            cj[q] = 0u;
+            if (q < nrounds) {
+                const uint32_t c = q * kWave + (uint32_t)lane;
+                if (c < total) {
+                    uint32_t a = 0, b = nr;  // last leaf whose offset <= c
+                    while (b - a > 1) {
+                        const uint32_t mid = (a + b) >> 1;
+                        if (rg[mid].y <= c) a = mid;
+                        else b = mid;
+                    }
+                    const uint32_t j = rg[a].x + (c - rg[a].y);
+                    const double4 pj = pos[j];
+                    stage[lane] = make_double4(pj.x, pj.y, pj.z, 0.0);
+                    cj[q] = j;
+                }
+                lds_sync();
+                const uint32_t m = min(total - q * kWave, (uint32_t)kWave);
+                if (act) {
+                    for (uint32_t k = 0; k < m; ++k) {
+                        const double4 c4 = stage[k];
+                        const double dx = c4.x - me.x, dy = c4.y - me.y, dz = c4.z - me.z;
+                        if (dx * dx + dy * dy + dz * dz <= R2) hit[q] |= 1ull << k;
+                    }
+                }
+                mine += (uint32_t)__popcll(hit[q]);
+                lds_sync();
+            }
+        }
+        failed = !allocate();
+        if (!failed) {
+            uint32_t* cand = reinterpret_cast<uint32_t*>(rg);  // kQMaskRounds * 64 entries
+#pragma unroll
+            for (uint32_t q = 0; q < kQMaskRounds; ++q)
+                if (q < nrounds) cand[q * kWave + (uint32_t)lane] = cj[q];
+            lds_sync();
+            if (act) {
+#pragma unroll
+                for (uint32_t q = 0; q < kQMaskRounds; ++q) {
+                    uint64_t mk = hit[q];
+                    while (mk) {
+                        const uint32_t b = (uint32_t)__ffsll((long long)mk) - 1u;
+                        mk &= mk - 1ull;
+                        pool[wpos++] = cand[q * kWave + b];
+                    }
+                }
+            }
+            lds_sync();
+        }
+        if (act && !failed) qstate[s] = iter[s] >= maxit ? kQDone : kQDensity;
+        return true;
+    }
+    // two passes over the candidates: count, then write
     for (int pass = 0; pass < 2; ++pass) {
         if (pass == 1) {
             uint32_t gtot;
