@@ -258,3 +258,27 @@ def test_bh_frontier_fallback_bit_exact(g):
     O.bh_group_accel(normal, 0.05, 0.5)
     assert not np.array_equal(normal.acc(), ref.acc())  # the fallback did run
     ctx.close()
+
+
+def test_repeated_builds_in_one_context(g):
+    """One context rebuilding the octree for alternating inputs (reusing its
+    cached buffers, with other work between calls) matches a fresh
+    context's result bit for bit every time."""
+    say("repeated tree builds in one context")
+    a = O.plummer(20000, 21)
+    b = O.plummer(20000, 22)
+    ctx = g.Context(0)
+
+    def forces(s, c):
+        p = to_particles(g, s)
+        g.BarnesHut(0.01, 0.5, ctx=c).compute_accelerations(p)
+        return p.acc()
+
+    ref_a = forces(a, g.Context(0))
+    ref_b = forces(b, g.Context(0))
+    runs = [(a, ref_a), (a, ref_a), (b, ref_b), (a, ref_a), (b, ref_b), (b, ref_b)]
+    for k, (s, ref) in enumerate(runs):
+        got = forces(s, ctx)
+        assert np.array_equal(got, ref), k
+        g.BruteForce(0.01, ctx=ctx).compute_accelerations(to_particles(g, O.plummer(1000, k)))
+    ctx.close()
