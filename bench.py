@@ -62,6 +62,8 @@ def parse():
     ap.add_argument("--c4-n", type=int, default=1048576)
     ap.add_argument("--c4-steps", type=int, default=200)
     ap.add_argument("--no-c4", action="store_true")
+    ap.add_argument("--c1-steps", type=int, default=10000)
+    ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--bf-shape", default=None,
                     help="ipt,tile,waves_per_cu: pin the mixed brute-force launch shape "
                          "instead of autotuning (profiling runs)")
@@ -111,6 +113,39 @@ def probe_copy():
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"probe": "copy", "bytes_read": a.numel() * 8, "bytes_written": b.numel() * 8}))
+
+
+def run_c1(args, g, ctx):
+    """Plummer N=1000 (seed 42), eps 0.05, dt 1e-3, FP64 brute force, 10,000
+    fused KDK steps (host arrays in and out); the oracle runs the same loop
+    on one core for a bounded sample of steps."""
+    import oracle as O
+    n, eps, dt, T = 1000, 0.05, 1e-3, args.c1_steps
+    log(f"bench: C1 Plummer N={n}, {T} FP64 KDK steps")
+    p = g.scenarios.plummer(n, 42)
+    gr = g.BruteForce(eps, ctx=ctx)
+    gr.compute_accelerations(p)
+    e0 = g.diagnostics.compute(p, eps, ctx=ctx).total_energy
+    g.LeapfrogKDK().run(p, gr, dt, 10)  # warm-up
+    t0 = time.perf_counter()
+    g.LeapfrogKDK().run(p, gr, dt, T)
+    t_gpu = time.perf_counter() - t0
+    d = g.diagnostics.compute(p, eps, ctx=ctx)
+    s = O.plummer(n, 42)
+    O.bf_accel(s, eps)
+    cpu_steps = min(T, 2000)
+    t0 = time.perf_counter()
+    O.kdk_steps(s, dt, cpu_steps, eps, nthreads=1)
+    t_cpu = (time.perf_counter() - t0) / cpu_steps
+    return {"n": n, "steps": T, "steps_per_s": round(T / t_gpu, 1),
+            "ms_per_step": round(t_gpu / T * 1e3, 4),
+            "energy_drift": (d.total_energy - e0) / abs(e0), "virial_ratio": d.virial_ratio,
+            "cpu_baseline": {"value": round(1.0 / t_cpu, 1), "unit": "steps/s", "cores": 1,
+                             "kind": "port",
+                             "sample": f"{cpu_steps} steps of the oracle's sim-core loop "
+                                       f"(i<j brute force, f64, 1 thread)"},
+            "workload": "Plummer N=1000, seed 42, eps 0.05, dt 1e-3, FP64 brute force, "
+                        "leapfrog KDK fused on the device"}
 
 
 def run_bh(args, g, torch, ctx, precision="mixed", steps=None):
@@ -529,6 +564,12 @@ def main():
         log(f"bench: CPU baseline on {args.cpu_sample} particles")
         cpu = cpu_baseline(args.cpu_sample, eps)
 
+    # C1: the reference's own CPU case (Plummer N=1000, eps 0.05, dt 1e-3,
+    # 10,000 FP64 KDK steps) on the GPU, the oracle beside it
+    c1 = None
+    if not args.no_c1:
+        c1 = run_c1(args, g, ctx)
+
     bh = bh64 = None
     if not args.no_bh:
         bh = run_bh(args, g, torch, ctx, "mixed")
@@ -578,6 +619,7 @@ def main():
                                          / PEAK_FP64_TFLOPS, 4)},
         "mixed_over_fp64": round(t_fp64 / t_mixed, 3),
         "force_error_vs_fp64": {"rms": err_rms, "max": err_max},
+        "c1": c1,
         "barnes_hut": bh,
         "barnes_hut_fp64": bh64,
         "ensemble": ens,

@@ -329,3 +329,31 @@ def test_autotune_brute_force_and_barnes_hut(g):
     rms, _ = rel_errors(ref, pb.acc())
     assert rms <= 4.942351e-3
     ctx.close()
+
+
+def test_c1_plummer_1k_10k_steps(g):
+    """Plummer N=1000 (seed 42), eps 0.05, dt 1e-3, 10,000 KDK steps with
+    FP64 brute force, against the oracle's sim-core loop (gravity.rs i<j,
+    integrator.rs): forces at t0 within 1e-12 (condition-normalised), and
+    after 10,000 steps the total energy within 1e-7 relative of the
+    oracle's and the virial ratio within 0.01 (the trajectories separate
+    chaotically; the energy does not), both drifting less than 1e-4."""
+    say("C1: plummer 1000, 10000 steps")
+    eps, dt, T = 0.05, 1e-3, 10000
+    s = O.plummer(1000, 42)
+    ref = s.copy()
+    O.bf_accel(ref, eps)
+    p = to_particles(g, s)
+    gr = g.BruteForce(eps)
+    gr.compute_accelerations(p)
+    worst, _ = cond_errors(s, eps, p.acc())
+    assert worst <= 1e-12
+    e0 = O.diagnostics(ref, eps)["E"]
+    O.kdk_steps(ref, dt, T, eps)
+    g.LeapfrogKDK().run(p, gr, dt, T)
+    dr = O.diagnostics(ref, eps)
+    dg = g.diagnostics.compute(p, eps)
+    say(f"  dE/E gpu {(dg.total_energy - e0) / abs(e0):.3e} oracle {(dr['E'] - e0) / abs(e0):.3e}")
+    assert abs(dg.total_energy - dr["E"]) <= 1e-7 * abs(dr["E"])
+    assert abs(dg.virial_ratio - dr["virial"]) <= 0.01
+    assert abs((dg.total_energy - e0) / e0) < 1e-4
