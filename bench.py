@@ -65,21 +65,27 @@ def parse():
     ap.add_argument("--c1-steps", type=int, default=10000)
     ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--bf-shape", default=None,
-                    help="ipt,tile,waves_per_cu: pin the mixed brute-force launch shape "
-                         "instead of autotuning (profiling runs)")
+                    help="ipt,tile,waves_per_cu (ipt 1 or 2, or m for the matrix-core "
+                         "kernel): pin the mixed brute-force launch shape instead of "
+                         "autotuning (profiling runs)")
+    ap.add_argument("--probe-shapes", default=None,
+                    help="only run three mixed brute-force calls per shape, shapes as "
+                         "'ipt,tile,waves_per_cu' joined by ';' (per-shape PMC traffic runs)")
     ap.add_argument("--probe-copy", action="store_true",
                     help="only run a 1 GiB device copy (PMC byte-counter calibration)")
     return ap.parse_args()
 
 
-def traffic_from_profiles(name):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+def traffic_from_profiles(name, key):
+    """HBM bytes per launch of launch shape `key` from the committed rocprofv3
+    --pmc summary; None when that shape was not profiled (the partial-sum
+    writes depend on the shape's j split)."""
     path = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
     if not os.path.exists(path):
         return None
     try:
         d = json.load(open(path))
-        return float(d["hbm_bytes_per_launch"])
+        return float(d["shapes"][key]["hbm_bytes_per_launch"])
     except Exception:
         return None
 
@@ -113,6 +119,38 @@ def probe_copy():
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"probe": "copy", "bytes_read": a.numel() * 8, "bytes_written": b.numel() * 8}))
+
+
+def shape_key(t):
+    """Mixed brute-force launch shape as 'ipt,tile,waves_per_cu' ('m' for the
+    matrix-core kernel), the key of the traffic profile."""
+    return f"{'m' if t['bf_matrix'] else t['bf_ipt']},{t['bf_tile']},{t['bf_waves_per_cu']}"
+
+
+def set_shape(ctx, key):
+    ipt, tile, wpc = key.split(",")
+    ctx.set_tuning("bf_matrix", 1 if ipt == "m" else 0)
+    if ipt != "m":
+        ctx.set_tuning("bf_ipt", int(ipt))
+    ctx.set_tuning("bf_tile", int(tile))
+    ctx.set_tuning("bf_waves_per_cu", int(wpc))
+
+
+def probe_shapes(args):
+    """Three mixed brute-force force calls per listed launch shape on the
+    bench's Plummer ICs (rocprofv3 --pmc runs of the per-shape HBM traffic)."""
+    import gravis_amd as g
+    p = g.scenarios.plummer(args.n, 42)
+    ctx = g.Context(0)
+    for key in args.probe_shapes.split(";"):
+        set_shape(ctx, key)
+        bf = g.BruteForce(args.eps, "mixed", ctx=ctx)
+        for _ in range(3):
+            p.clear_accelerations()
+            bf.compute_accelerations(p)
+        ctx.synchronize()
+        print(json.dumps({"shape": key, "tuning": ctx.tuning()}), flush=True)
+    ctx.close()
 
 
 def run_c1(args, g, ctx):
@@ -448,9 +486,12 @@ def main():
     if args.probe_copy:
         probe_copy()
         return
+    if args.probe_shapes:
+        probe_shapes(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the multi-GPU layout is a design in DESIGN.md, not built)", file=sys.stderr)
+              "(one process drives one GPU; sharding across GPUs is not built)", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -494,14 +535,13 @@ def main():
     # Launch shapes of the mixed kernel picked by the library's autotuner on
     # these particles (outside the timed region).
     if args.bf_shape:
-        ipt, tile, wpc = (int(v) for v in args.bf_shape.split(","))
-        ctx.set_tuning("bf_ipt", ipt)
-        ctx.set_tuning("bf_tile", tile)
-        ctx.set_tuning("bf_waves_per_cu", wpc)
+        set_shape(ctx, args.bf_shape)
         tuning = dict(ctx.tuning(), pinned=True)
     else:
         tuning = ctx.autotune(mixed, p)
     log(f"bench: mixed brute-force launch shape: {tuning}")
+    # the timer and kernel of the chosen shape
+    kname = "bf_matrix" if tuning["bf_matrix"] else "bf_mixed"
     # a(t0), as headless does before the loop (main.rs:151-152)
     accel(mixed, (T["ax"], T["ay"], T["az"]))
     log(f"bench: warmup {args.warmup} mixed KDK steps")
@@ -525,7 +565,7 @@ def main():
         kdk(mixed, 1)
     ctx.set_profiling(False)
     torch.cuda.synchronize()
-    ms_total, launches = ctx.kernel_time("bf_mixed")
+    ms_total, launches = ctx.kernel_time(kname)
     k_mixed_s = ms_total / 1e3 / max(1, launches)
     achieved = FLOP_PER_INTERACTION * inter / k_mixed_s / 1e12
 
@@ -586,7 +626,7 @@ def main():
     if not args.no_c4:
         c4 = run_sph_c4(args, g, torch, ctx)
 
-    traffic = traffic_from_profiles(f"bf_mixed_n{n}")
+    traffic = traffic_from_profiles(f"bf_mixed_n{n}", shape_key(tuning))
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -607,7 +647,7 @@ def main():
                    "n": n, "softening": eps, "dt": dt, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "traffic": traffic, "kernel": "bf_mixed_kernel",
+                     "traffic": traffic, "kernel": f"{kname}_kernel",
                      "kernel_ms": round(k_mixed_s * 1e3, 4),
                      "flop_per_interaction": FLOP_PER_INTERACTION},
         "cpu_baseline": cpu,

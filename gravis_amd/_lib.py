@@ -54,7 +54,7 @@ class Diag(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("bf_ipt", C.c_int32), ("bf_tile", C.c_int32), ("bf_waves_per_cu", C.c_int32),
-                ("bh_group_size", C.c_uint32), ("best_ms", C.c_double)]
+                ("bh_group_size", C.c_uint32), ("bf_matrix", C.c_int32), ("best_ms", C.c_double)]
 
 
 class SphParams(C.Structure):

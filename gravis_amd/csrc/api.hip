@@ -594,6 +594,12 @@ gravis_status gravis_set_tuning(gravis_ctx* g, const char* key, int64_t value) {
         c->bf_ipt = (int)value;
         return GRAVIS_OK;
     }
+    if (std::strcmp(key, "bf_matrix") == 0) {
+        if (value != 0 && value != 1)
+            return fail(c, GRAVIS_INVALID_ARG, "bf_matrix must be 0 or 1");
+        c->bf_matrix = (int)value;
+        return GRAVIS_OK;
+    }
     if (std::strcmp(key, "bf_near_64ths") == 0) {
         if (value < 0 || value > 64)
             return fail(c, GRAVIS_INVALID_ARG, "bf_near_64ths must be in [0, 64]");
@@ -628,6 +634,7 @@ gravis_status gravis_get_tuning(gravis_ctx* g, gravis_tuning* out) {
     out->bf_tile = c->bf_tile;
     out->bf_waves_per_cu = c->bf_waves_per_cu;
     out->bh_group_size = c->bh_group;
+    out->bf_matrix = c->bf_matrix;
     out->best_ms = 0.0;
     return GRAVIS_OK;
 }
@@ -671,7 +678,7 @@ gravis_status gravis_autotune(gravis_ctx* g, const gravis_solver* s, size_t n, c
     // partials (3 doubles per particle per chunk), so among the shapes within
     // 2 % of the fastest the one with the fewest chunks wins; candidates are
     // listed in that order of preference.
-    struct Shape { int ipt, tile, wpc; double ms; };
+    struct Shape { int ipt, tile, wpc, matrix; double ms; };
     auto pick = [&](std::vector<Shape>& v) {
         double fastest = v[0].ms;
         for (const Shape& x : v) fastest = std::min(fastest, x.ms);
@@ -682,22 +689,27 @@ gravis_status gravis_autotune(gravis_ctx* g, const gravis_solver* s, size_t n, c
     if (s->algorithm == GRAVIS_BRUTE_FORCE) {
         const bool mx = s->precision == GRAVIS_MIXED;
         std::vector<Shape> v;
+        // kernel 0, 1: VALU kernel with bf_ipt 2, 1; kernel 2: matrix core
         for (int wpc : {32, 64, 128, 256, 512})
-            for (int ipt : {2, 1})
+            for (int kern : {0, 1, 2})
                 for (int tile : {128, 256}) {
-                    if (!mx && (ipt != 1 || tile != 256)) continue;  // FP64: split only
+                    if (!mx && (kern != 1 || tile != 256)) continue;  // FP64: split only
                     if (mx && wpc == 32) continue;
-                    c->bf_ipt = mx ? ipt : c->bf_ipt;
-                    c->bf_tile = mx ? tile : c->bf_tile;
+                    if (mx) {
+                        c->bf_matrix = kern == 2 ? 1 : 0;
+                        if (kern < 2) c->bf_ipt = 2 - kern;
+                        c->bf_tile = tile;
+                    }
                     c->bf_waves_per_cu = wpc;
                     double ms;
                     st = time_it(&ms);
                     if (st != GRAVIS_OK) break;
-                    v.push_back({c->bf_ipt, c->bf_tile, wpc, ms});
+                    v.push_back({c->bf_ipt, c->bf_tile, wpc, c->bf_matrix, ms});
                 }
         if (st == GRAVIS_OK) {
             const Shape b = pick(v);
             c->bf_ipt = b.ipt; c->bf_tile = b.tile; c->bf_waves_per_cu = b.wpc;
+            c->bf_matrix = b.matrix;
             best = b.ms;
         }
     } else if (s->algorithm == GRAVIS_BARNES_HUT && s->group_size == 0) {
@@ -709,7 +721,7 @@ gravis_status gravis_autotune(gravis_ctx* g, const gravis_solver* s, size_t n, c
             double ms;
             st = time_it(&ms);
             if (st != GRAVIS_OK) break;
-            v.push_back({(int)gs, 0, 0, ms});
+            v.push_back({(int)gs, 0, 0, 0, ms});
         }
         if (st == GRAVIS_OK) {
             const Shape b = pick(v);

@@ -8,12 +8,15 @@
 // every target i gathers over all j (no write conflicts, no atomics): a
 // workgroup owns a run of the Morton-sorted particle list as targets and a
 // contiguous j range; j records are staged through LDS in tiles and read
-// back as wavefront-uniform broadcasts. Pairwise gravity is not a dense
-// contraction, so this is VALU work (the matrix cores were measured: with
-// four useful output columns v_mfma_f32_4x4x1 issues at a quarter of the
-// FP32 matrix rate and saves about 5 %). The j range is split in S chunks so
-// the grid holds many wavefronts per CU; the chunk partials are summed in a
-// fixed order by the finishing kernel (deterministic).
+// back as wavefront-uniform broadcasts. Pairwise gravity is mostly VALU
+// work: only |x_j - x_i|^2 is a contraction (of length 4), which the
+// matrix-core variant below computes with v_mfma_f32_16x16x4f32. An FP32
+// MFMA occupies the SIMD for its 32 cycles (no overlap with the VALU was
+// measured), so the variant trades six VALU operations per pair for an
+// eighth of an MFMA and gains a few per cent; the autotuner picks between
+// them. The j range is split in S chunks so the grid holds many wavefronts
+// per CU; the chunk partials are summed in a fixed order by the finishing
+// kernel (deterministic).
 //
 // Mixed precision: separations are formed in FP64 relative to the centre of
 // the wavefront's target box, rounded to FP32, and the pair force is FP32
@@ -37,8 +40,10 @@ __device__ __forceinline__ double pair_f64(double dx, double dy, double dz, doub
     return mr * ri2;
 }
 
-// Bounding box of each TJ-wide tile of the sorted records: tbox[6 t ..] =
-// (lo x, lo y, lo z, hi x, hi y, hi z).
+// Bounding box of each TJ-wide tile of the sorted records: tbox[kTileBox t
+// ..] = (lo x, lo y, lo z, hi x, hi y, hi z).
+constexpr int kTileBox = 6;
+
 template <int TJ>
 __global__ __launch_bounds__(256) void tile_box_kernel(const double4* __restrict__ pm, int n,
                                                        double* __restrict__ tbox) {
@@ -72,7 +77,7 @@ __global__ __launch_bounds__(256) void tile_box_kernel(const double4* __restrict
         double v = sh[0][d];
         for (int k = 1; k < (int)(blockDim.x / kWave); ++k)
             v = d < 3 ? fmin(v, sh[k][d]) : fmax(v, sh[k][d]);
-        tbox[(size_t)blockIdx.x * 6 + d] = v;
+        tbox[(size_t)blockIdx.x * kTileBox + d] = v;
     }
 }
 
@@ -90,8 +95,10 @@ __global__ __launch_bounds__(256) void tile_box_kernel(const double4* __restrict
 // LDS by the wave itself: far tiles as FP32 records relative to the wave's
 // own target-box centre (so the rounding error scales with the size of 64 or
 // 128 targets, not of a larger block), near tiles as FP64 records. The far
-// loop is 12 VALU operations and one v_rsq_f32 per pair; the transcendental
-// issues beside the FMAs, so the loop is bound by the 12 FMA-class slots.
+// loop is 12 VALU operations and one v_rsq_f32 per pair. On gfx950 FP32
+// VALU operations with VGPR or literal operands issue at about 103 lane
+// operations per cycle per CU, v_rsq_f32 at about 30, and the two do not
+// overlap (measured), so a pair costs about 12 + 3.4 FMA slots.
 // Tile sums are FP32, added into a Kahan-compensated FP32 accumulator.
 template <int IPT, int TJ, bool MASK>
 __global__ __launch_bounds__(kWave) void bf_mixed_kernel(const double4* __restrict__ pm, int n,
@@ -137,6 +144,7 @@ __global__ __launch_bounds__(kWave) void bf_mixed_kernel(const double4* __restri
     float sx[IPT], sy[IPT], sz[IPT], kx[IPT], ky[IPT], kz[IPT];
 #pragma unroll
     for (int k = 0; k < IPT; ++k) { sx[k] = sy[k] = sz[k] = 0.f; kx[k] = ky[k] = kz[k] = 0.f; }
+    const float e2 = in_vgpr(eps2);  // full-rate operand of the r^2 FMA
 
     const int j0 = blockIdx.y * jchunk;
     const int j1 = min(n, j0 + jchunk);
@@ -168,7 +176,7 @@ __global__ __launch_bounds__(kWave) void bf_mixed_kernel(const double4* __restri
 
     bool near_cur = false;
     if (ntiles > 0) {
-        near_cur = tile_is_near(tbox + (size_t)tile0 * 6, lo, hi, lim2);
+        near_cur = tile_is_near(tbox + (size_t)tile0 * kTileBox, lo, hi, lim2);
         load(0);
     }
     for (int t = 0; t < ntiles; ++t) {
@@ -176,7 +184,7 @@ __global__ __launch_bounds__(kWave) void bf_mixed_kernel(const double4* __restri
         __syncthreads();
         bool near_next = false;
         if (t + 1 < ntiles) {
-            near_next = tile_is_near(tbox + (size_t)(tile0 + t + 1) * 6, lo, hi, lim2);
+            near_next = tile_is_near(tbox + (size_t)(tile0 + t + 1) * kTileBox, lo, hi, lim2);
             load(t + 1);
         }
         float ax[IPT], ay[IPT], az[IPT];
@@ -185,7 +193,7 @@ __global__ __launch_bounds__(kWave) void bf_mixed_kernel(const double4* __restri
         if (!near_cur) {
             const float4* t32 = reinterpret_cast<const float4*>(&tile[0]);
             // unroll depth chosen for register pressure: 63 VGPRs (8 waves
-            // per SIMD) at IPT = 1, 83 (6 waves) at IPT = 2
+            // per SIMD) at IPT = 1, 84 (6 waves) at IPT = 2
             constexpr int kUnroll = IPT == 1 ? 4 : 8;
 #pragma unroll kUnroll
             for (int j = 0; j < TJ; ++j) {
@@ -193,7 +201,7 @@ __global__ __launch_bounds__(kWave) void bf_mixed_kernel(const double4* __restri
 #pragma unroll
                 for (int k = 0; k < IPT; ++k) {
                     const float dx = s.x - xi[k], dy = s.y - yi[k], dz = s.z - zi[k];
-                    const float f = pair_f32<MASK>(dx, dy, dz, s.w, eps2);
+                    const float f = pair_f32<MASK>(dx, dy, dz, s.w, e2);
                     ax[k] = __builtin_fmaf(f, dx, ax[k]);
                     ay[k] = __builtin_fmaf(f, dy, ay[k]);
                     az[k] = __builtin_fmaf(f, dz, az[k]);
@@ -216,7 +224,7 @@ __global__ __launch_bounds__(kWave) void bf_mixed_kernel(const double4* __restri
                     const float dy = (float)(s.y - py[k]);
                     const float dz = (float)(s.z - pz[k]);
                     const float r2 =
-                        __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+                        __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, e2)));
                     float ri = __builtin_amdgcn_rsqf(r2);
                     const float h = (r2 * ri) * ri;
                     ri = ri * __builtin_fmaf(-0.5f, h, 1.5f);
@@ -249,6 +257,254 @@ __global__ __launch_bounds__(kWave) void bf_mixed_kernel(const double4* __restri
     }
 }
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// One source block of the matrix-core far loop: the pair forces of the
+// current block (products dc, source terms qc) are summed into acc while the
+// products and source terms of block sbn are issued (MORE), one product per
+// 16-target block between the VALU work of the previous one.
+template <bool MORE, bool MASK>
+__device__ __forceinline__ void mx_block(const float* __restrict__ aop,
+                                         const float4* __restrict__ qop,
+                                         const float* __restrict__ bbq, int sbn, int lane, int grp,
+                                         const float (&bop)[4], const f32x4 (&dc)[4],
+                                         const float4 (&qc)[4], f32x4 (&dn)[4], float4 (&qn)[4],
+                                         float (&acc)[4][4]) {
+    float avn = 0.f;
+    f32x4 cbn = {0.f, 0.f, 0.f, 0.f};
+    if (MORE) {
+        avn = aop[sbn * 64 + lane];
+        const float4 c4 = *reinterpret_cast<const float4*>(bbq + sbn * 16 + 4 * grp);
+        cbn = f32x4{c4.x, c4.y, c4.z, c4.w};
+#pragma unroll
+        for (int v = 0; v < 4; ++v) qn[v] = qop[sbn * 16 + 4 * grp + v];
+    }
+#pragma unroll
+    for (int tb = 0; tb < 4; ++tb) {
+        if (MORE) dn[tb] = __builtin_amdgcn_mfma_f32_16x16x4f32(avn, bop[tb], cbn, 0, 0, 0);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const float r2 = dc[tb][v];
+            float ri = __builtin_amdgcn_rsqf(r2);
+            if (MASK) ri = r2 > 0.0f ? ri : 0.0f;
+            const float f = (ri * ri) * ri;
+            acc[tb][0] = __builtin_fmaf(f, qc[v].x, acc[tb][0]);
+            acc[tb][1] = __builtin_fmaf(f, qc[v].y, acc[tb][1]);
+            acc[tb][2] = __builtin_fmaf(f, qc[v].z, acc[tb][2]);
+            acc[tb][3] = __builtin_fmaf(f, qc[v].w, acc[tb][3]);
+        }
+    }
+    if (MORE) {
+#pragma unroll
+        for (int tb = 0; tb < 4; ++tb) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, MASK ? 32 : 28, 0);
+        }
+    }
+}
+
+// Matrix-core variant of the mixed kernel. With a = x_i - c and b = x_j - c
+// (FP32, c the wavefront's target-box centre),
+//   |x_j - x_i|^2 + eps^2 = (b, 1) . (-2 a, |a|^2 + eps^2) + |b|^2,
+// one v_mfma_f32_16x16x4f32 per 16 sources x 16 targets: A rows are sources
+// (b_x, b_y, b_z, 1), B columns are targets (-2a_x, -2a_y, -2a_z, |a|^2 +
+// eps^2), C is |b|^2 of the row's source. The acceleration is then
+//   sum_j f_ij m_j (b_j - a_i) = sum_j f_ij (m_j b_j) - a_i sum_j f_ij m_j,
+// f = r^-3, so each pair costs one v_rsq_f32, two multiplies and four FMAs
+// on the VALU (six operations instead of twelve). Lane l of the product holds
+// rows 4 (l / 16) + v, v = 0..3, of target column l % 16: each lane sums for
+// one target per 16-target block (64 targets, four blocks) over four sources
+// per block; the four lane groups are added at the end of every tile and the
+// lane of target l keeps its Kahan-compensated sums. The products of the next
+// source block are issued between the VALU work of the current one, so the
+// matrix core runs beside the VALU.
+// Error: the expansion rounds r^2 + eps^2 with an absolute error of about
+// u (|a| + |b|)^2 <= u (diag + |d|)^2, i.e. a relative error of
+// u (1 + 1 / q')^2 when max(|d|, eps) >= q' diag. The caller passes q' with
+// (1 + 1 / q')^2 = 2 (1 + 1 / q), which keeps the per-pair bound of the VALU
+// kernel for the context's q; nearer tiles take the FP64-separation path.
+template <int TJ, bool MASK, int WPE = (TJ == 128 ? 4 : 3)>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void bf_matrix_kernel(const double4* __restrict__ pm, int n,
+                                                          int jchunk, float eps2, double near2,
+                                                          const double* __restrict__ tbox,
+                                                          double* __restrict__ part) {
+    static_assert(TJ % kWave == 0, "tile must be a multiple of the wavefront");
+    constexpr int LPT = TJ / kWave;  // records each lane stages per tile
+    constexpr int SB = TJ / 16;      // 16-source blocks per tile (even)
+    // near: TJ FP64 records; far: A operands [SB][4][16], (m b, m) and |b|^2
+    // per source (36 TJ bytes)
+    __shared__ float4 lds[TJ * 9 / 4];
+    double4* tile = reinterpret_cast<double4*>(&lds[0]);
+    float* aop = reinterpret_cast<float*>(&lds[0]);
+    float4* qop = &lds[TJ];
+    float* bbq = reinterpret_cast<float*>(&lds[2 * TJ]);
+
+    const int lane = threadIdx.x;
+    const int col = lane & 15, grp = lane >> 4;
+    const int ibase = blockIdx.x * kWave;
+    const int iself = ibase + lane < n ? ibase + lane : ibase;
+    double lo[3], hi[3];
+    double cx, cy, cz, lim2;
+    float ax, ay, az;  // this lane's target relative to c (set-up only)
+    {
+        const double4 p = pm[iself];
+        const double xd[1] = {p.x}, yd[1] = {p.y}, zd[1] = {p.z};
+        wave_box<1>(xd, yd, zd, lo, hi);
+        cx = 0.5 * (lo[0] + hi[0]);
+        cy = 0.5 * (lo[1] + hi[1]);
+        cz = 0.5 * (lo[2] + hi[2]);
+        double di2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) di2 = __builtin_fma(hi[d] - lo[d], hi[d] - lo[d], di2);
+        lim2 = near2 * di2;
+        if ((double)eps2 >= lim2) lim2 = 0.0;
+        ax = (float)(p.x - cx);
+        ay = (float)(p.y - cy);
+        az = (float)(p.z - cz);
+    }
+    // B operand of each 16-target block
+    float bop[4];
+#pragma unroll
+    for (int tb = 0; tb < 4; ++tb) {
+        const float tx = __shfl(ax, 16 * tb + col);
+        const float ty = __shfl(ay, 16 * tb + col);
+        const float tz = __shfl(az, 16 * tb + col);
+        const float aa = __builtin_fmaf(tx, tx, __builtin_fmaf(ty, ty, __builtin_fmaf(tz, tz, eps2)));
+        bop[tb] = grp == 0 ? -2.0f * tx : grp == 1 ? -2.0f * ty : grp == 2 ? -2.0f * tz : aa;
+    }
+    float s[4], k[4];  // target ibase + lane: x, y, z, sum f m
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s[q] = k[q] = 0.f;
+
+    const int j0 = blockIdx.y * jchunk;
+    const int j1 = min(n, j0 + jchunk);
+    const int ntiles = (j1 - j0 + TJ - 1) / TJ;
+    const int tile0 = j0 / TJ;
+
+    double4 stage[LPT];
+    auto load = [&](int t) {
+#pragma unroll
+        for (int e = 0; e < LPT; ++e) {
+            const int j = j0 + t * TJ + e * kWave + lane;
+            stage[e] = j < j1 ? pm[j] : make_double4(cx, cy, cz, 0.0);
+        }
+    };
+    auto store = [&](bool near) {
+        if (near) {
+#pragma unroll
+            for (int e = 0; e < LPT; ++e) tile[e * kWave + lane] = stage[e];
+        } else {
+#pragma unroll
+            for (int e = 0; e < LPT; ++e) {
+                const int sj = e * kWave + lane;
+                const float bx = (float)(stage[e].x - cx);
+                const float by = (float)(stage[e].y - cy);
+                const float bz = (float)(stage[e].z - cz);
+                const float m = (float)stage[e].w;
+                float* a = aop + (sj >> 4) * 64 + (sj & 15);
+                a[0] = bx;
+                a[16] = by;
+                a[32] = bz;
+                a[48] = 1.0f;
+                qop[sj] = make_float4(m * bx, m * by, m * bz, m);
+                bbq[sj] = __builtin_fmaf(bx, bx, __builtin_fmaf(by, by, bz * bz));
+            }
+        }
+    };
+
+    // records are loaded at the top of each tile (no register prefetch: the
+    // registers go to occupancy, and other wavefronts cover the latency)
+    for (int t = 0; t < ntiles; ++t) {
+        const bool near_cur = tile_is_near(tbox + (size_t)(tile0 + t) * kTileBox, lo, hi, lim2);
+        load(t);
+        store(near_cur);
+        __syncthreads();
+        float own[4];
+        if (!near_cur) {
+            float acc[4][4];
+#pragma unroll
+            for (int tb = 0; tb < 4; ++tb)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[tb][q] = 0.f;
+            f32x4 dA[4], dB[4];
+            float4 qA[4], qB[4];
+            {
+                const float av = aop[lane];
+                const float4 c4 = *reinterpret_cast<const float4*>(bbq + 4 * grp);
+                const f32x4 cb = {c4.x, c4.y, c4.z, c4.w};
+#pragma unroll
+                for (int tb = 0; tb < 4; ++tb)
+                    dA[tb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bop[tb], cb, 0, 0, 0);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) qA[v] = qop[4 * grp + v];
+            }
+            // blocks in pairs (A current, B next; then B current, A next)
+            int sb = 0;
+#pragma unroll 1
+            for (; sb + 2 < SB; sb += 2) {
+                mx_block<true, MASK>(aop, qop, bbq, sb + 1, lane, grp, bop, dA, qA, dB, qB, acc);
+                mx_block<true, MASK>(aop, qop, bbq, sb + 2, lane, grp, bop, dB, qB, dA, qA, acc);
+            }
+            mx_block<true, MASK>(aop, qop, bbq, SB - 1, lane, grp, bop, dA, qA, dB, qB, acc);
+            mx_block<false, MASK>(aop, qop, bbq, 0, lane, grp, bop, dB, qB, dA, qA, acc);
+            // add the four lane groups; lane l keeps block grp (target l)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+#pragma unroll
+                for (int tb = 0; tb < 4; ++tb) {
+                    float v = acc[tb][q];
+                    v += __shfl_xor(v, 16);
+                    v += __shfl_xor(v, 32);
+                    acc[tb][q] = v;
+                }
+                own[q] = grp == 0 ? acc[0][q] : grp == 1 ? acc[1][q] : grp == 2 ? acc[2][q] : acc[3][q];
+            }
+        } else {
+            // FP64 separations, one target per lane (target ibase + lane)
+            const double4 p = pm[iself];
+            float nx = 0.f, ny = 0.f, nz = 0.f;
+#pragma unroll 1
+            for (int j = 0; j < TJ; ++j) {
+                const double4 sj = tile[j];
+                const float m = (float)sj.w;
+                const float dx = (float)(sj.x - p.x);
+                const float dy = (float)(sj.y - p.y);
+                const float dz = (float)(sj.z - p.z);
+                const float r2 =
+                    __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+                float ri = __builtin_amdgcn_rsqf(r2);
+                const float h = (r2 * ri) * ri;
+                ri = ri * __builtin_fmaf(-0.5f, h, 1.5f);
+                if (MASK) ri = r2 > 0.0f ? ri : 0.0f;
+                const float f = (m * ri) * (ri * ri);
+                nx = __builtin_fmaf(f, dx, nx);
+                ny = __builtin_fmaf(f, dy, ny);
+                nz = __builtin_fmaf(f, dz, nz);
+            }
+            own[0] = nx;
+            own[1] = ny;
+            own[2] = nz;
+            own[3] = 0.f;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) kahan_add(s[q], k[q], own[q]);
+        __syncthreads();
+    }
+    const int i = ibase + lane;
+    if (i < n) {
+        double o[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) o[q] = (double)s[q] - (double)k[q];
+        // the target offset a (FP32, as in the products), formed again here
+        // so it holds no registers through the loop
+        const double4 p = pm[i];
+        const size_t base = (size_t)blockIdx.y * 3 * n;
+        part[base + i] = o[0] - (double)(float)(p.x - cx) * o[3];
+        part[base + n + i] = o[1] - (double)(float)(p.y - cy) * o[3];
+        part[base + 2 * (size_t)n + i] = o[2] - (double)(float)(p.z - cz) * o[3];
+    }
+}
+
 template <int BLOCK, int IPT, int TJ, bool MASK>
 __global__ __launch_bounds__(BLOCK) void bf_fp64_kernel(const double4* __restrict__ pm, int n,
                                                         int jchunk, double eps2,
@@ -272,6 +528,7 @@ __global__ __launch_bounds__(BLOCK) void bf_fp64_kernel(const double4* __restric
     const int ntiles = (j1 - j0 + TJ - 1) / TJ;
     // Padding records: zero mass placed far away (no contribution, r^2 > 0).
     const double4 pad = make_double4(1e150, 1e150, 1e150, 0.0);
+    const double e2 = in_vgpr(eps2);  // VGPR operand of the r^2 FMA
 
     double4 stage[LPT];
     auto load = [&](int t) {
@@ -302,7 +559,7 @@ __global__ __launch_bounds__(BLOCK) void bf_fp64_kernel(const double4* __restric
 #pragma unroll
             for (int k = 0; k < IPT; ++k) {
                 const double dx = s.x - xi[k], dy = s.y - yi[k], dz = s.z - zi[k];
-                const double f = pair_f64<MASK>(dx, dy, dz, s.w, eps2);
+                const double f = pair_f64<MASK>(dx, dy, dz, s.w, e2);
                 ax[k] = __builtin_fma(f, dx, ax[k]);
                 ay[k] = __builtin_fma(f, dy, ay[k]);
                 az[k] = __builtin_fma(f, dz, az[k]);
@@ -336,6 +593,15 @@ void launch_mixed(dim3 g, hipStream_t st, bool mask, const double4* pm, int n, i
     else
         bf_mixed_kernel<IPT, TJ, false><<<g, kWave, 0, st>>>(pm, n, jchunk, eps2, near2, tbox,
                                                              part);
+}
+
+template <int TJ>
+void launch_matrix(dim3 g, hipStream_t st, bool mask, const double4* pm, int n, int jchunk,
+                   float eps2, double near2, const double* tbox, double* part) {
+    if (mask)
+        bf_matrix_kernel<TJ, true><<<g, kWave, 0, st>>>(pm, n, jchunk, eps2, near2, tbox, part);
+    else
+        bf_matrix_kernel<TJ, false><<<g, kWave, 0, st>>>(pm, n, jchunk, eps2, near2, tbox, part);
 }
 
 template <int BLOCK, int IPT, int TJ>
@@ -410,8 +676,10 @@ gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
     GRAVIS_TRY(gather_sorted(c, n, x, y, z, m, perm, pm));
 
     // FP64: 256-thread workgroups of 256 targets. Mixed: one wavefront per
-    // workgroup with bf_ipt * 64 targets and bf_tile-record tiles.
-    const int ipt = mixed ? c->bf_ipt : 1;
+    // workgroup with bf_ipt * 64 targets (64 for the matrix-core kernel) and
+    // bf_tile-record tiles.
+    const bool matrix = mixed && c->bf_matrix;
+    const int ipt = mixed && !matrix ? c->bf_ipt : 1;
     const int group = mixed ? kWave * ipt : 256;
     const int tj = mixed ? c->bf_tile : 256;
     const int gx = (ni + group - 1) / group;
@@ -434,7 +702,7 @@ gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
     double* tbox = nullptr;
     if (mixed) {
         const int ntile = (ni + tj - 1) / tj;
-        GRAVIS_TRY(buf(c, "bf_tbox", (size_t)ntile * 6, &tbox));
+        GRAVIS_TRY(buf(c, "bf_tbox", (size_t)ntile * kTileBox, &tbox));
         if (tj == 256)
             tile_box_kernel<256><<<ntile, 256, 0, c->stream>>>(pm, ni, tbox);
         else
@@ -445,7 +713,21 @@ gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
     if (mixed) {
         const float eps2f = (float)eps2;
         const bool mask = !(eps2f >= 1.17549435e-38f);
-        const double f2 = c->bf_near_fraction * c->bf_near_fraction;
+        const double q = c->bf_near_fraction;
+        if (matrix) {
+            // q' with (1 + 1/q')^2 = 2 (1 + 1/q): the same per-pair bound
+            const double qm = q > 0.0 ? 1.0 / (std::sqrt(2.0 * (1.0 + 1.0 / q)) - 1.0) : 0.0;
+            if (tj == 128)
+                launch_matrix<128>(grid, c->stream, mask, pm, ni, jchunk, eps2f, qm * qm, tbox,
+                                   part);
+            else
+                launch_matrix<256>(grid, c->stream, mask, pm, ni, jchunk, eps2f, qm * qm, tbox,
+                                   part);
+            timer_end(c, "bf_matrix");
+            GRAVIS_TRY(check_launch(c, "bf kernel"));
+            return finish_forces(c, part, S, ni, perm, fa);
+        }
+        const double f2 = q * q;
         if (ipt == 1 && tj == 128)
             launch_mixed<1, 128>(grid, c->stream, mask, pm, ni, jchunk, eps2f, f2, tbox, part);
         else if (ipt == 1)

@@ -109,7 +109,7 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
         const double4 me = src[i0 + (active ? tl : 0)];
         // softening of this lane's target: one value, or one per target
         const double eps2 = eps2_t ? eps2_t[i0 + (active ? tl : 0)] : eps2_all;
-        const float eps2f = (float)eps2;
+        const float eps2f = in_vgpr((float)eps2);  // full-rate FMA operand
         double glo[3] = {me.x, me.y, me.z}, ghi[3] = {me.x, me.y, me.z};
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1)
@@ -157,10 +157,7 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
                         float4 r = stage32[k < m ? k : 0];
                         r.w = k < m ? r.w : 0.f;
                         const float dx = r.x - fx, dy = r.y - fy, dz = r.z - fz;
-                        const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2f)));
-                        float ri = __builtin_amdgcn_rsqf(r2);
-                        ri = r2 > 0.f ? ri : 0.f;
-                        const float f = (r.w * ri) * (ri * ri);
+                        const float f = pair_f32<true>(dx, dy, dz, r.w, eps2f);
                         tx = __builtin_fmaf(f, dx, tx);
                         ty = __builtin_fmaf(f, dy, ty);
                         tz = __builtin_fmaf(f, dz, tz);

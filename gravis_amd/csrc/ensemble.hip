@@ -301,11 +301,12 @@ __global__ __launch_bounds__(kWave) void ens_wave_kernel(
         float tx = 0.f, ty = 0.f, tz = 0.f;
         if (!near) {
             const float4* t32 = reinterpret_cast<const float4*>(&tile[0]);
+            const float e2 = in_vgpr(eps2);  // full-rate operand of the r^2 FMA
 #pragma unroll 4
             for (int k = 0; k < cnt; ++k) {
                 const float4 s4 = t32[k];
                 const float dx = s4.x - xi, dy = s4.y - yi, dz = s4.z - zi;
-                const float f = pair_f32<MASK>(dx, dy, dz, s4.w, eps2);
+                const float f = pair_f32<MASK>(dx, dy, dz, s4.w, e2);
                 tx = __builtin_fmaf(f, dx, tx);
                 ty = __builtin_fmaf(f, dy, ty);
                 tz = __builtin_fmaf(f, dz, tz);

@@ -59,6 +59,9 @@ struct Ctx {
     // j-chunk split aims for.
     int bf_ipt = 2;
     int bf_tile = 128;
+    // 1: the mixed brute force runs the matrix-core kernel (64 targets per
+    // wavefront, r^2 on v_mfma_f32_16x16x4f32); bf_ipt is then unused
+    int bf_matrix = 0;
     int bf_waves_per_cu = 256;
     // Barnes-Hut target group when the solver leaves group_size at 0
     uint32_t bh_group = 64;
@@ -105,6 +108,21 @@ __device__ __forceinline__ double uniform(double v) {
     const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
     const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// A wavefront-uniform float held in a vector register. On gfx950 a VALU
+// operation with an SGPR operand issues at half the FP32 rate of one whose
+// operands are all VGPRs or literals (measured: 57 against 103 lane
+// operations per cycle per CU), so loop-invariant scalars that feed the pair
+// loops are moved to VGPRs once. The empty asm emits no instruction; it only
+// hides the value's uniformity from the compiler.
+__device__ __forceinline__ float in_vgpr(float v) {
+    asm("" : "+v"(v));
+    return v;
+}
+__device__ __forceinline__ double in_vgpr(double v) {
+    asm("" : "+v"(v));
+    return v;
 }
 
 // ---- pair-force helpers shared by the brute-force kernels ----------------
