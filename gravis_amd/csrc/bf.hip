@@ -33,7 +33,7 @@ template <bool MASK>
 __device__ __forceinline__ double pair_f64(double dx, double dy, double dz, double m,
                                            double eps2) {
     const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, eps2)));
-    double ri = rsqrt(r2);
+    double ri = rsqrt_refined(r2);
     if (MASK) ri = r2 > 0.0 ? ri : 0.0;
     const double mr = m * ri;
     const double ri2 = ri * ri;

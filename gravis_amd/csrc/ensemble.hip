@@ -100,7 +100,7 @@ __global__ __launch_bounds__(kEnsBlock) void ens_force_kernel(
                 const double4 s = tile[k];
                 const double dx = s.x - px, dy = s.y - py, dz = s.z - pz;
                 const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, eps2)));
-                const double ri = (jb + k == i) ? 0.0 : rsqrt(r2);
+                const double ri = (jb + k == i) ? 0.0 : rsqrt_refined(r2);
                 const double f = (s.w * ri) * (ri * ri);
                 sx = __builtin_fma(f, dx, sx);
                 sy = __builtin_fma(f, dy, sy);
@@ -379,7 +379,7 @@ __global__ __launch_bounds__(kEnsBlock) void ens_vjp_kernel(
             const double4 uj = tu[q];
             const double dx = s.x - px, dy = s.y - py, dz = s.z - pz;
             const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, eps2)));
-            const double ri = rsqrt(r2);
+            const double ri = rsqrt_refined(r2);
             const double ri2 = ri * ri;
             const double w = ri2 * ri;
             const double qq = 3.0 * w * ri2;
