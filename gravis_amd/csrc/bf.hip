@@ -31,13 +31,10 @@ namespace {
 
 template <bool MASK>
 __device__ __forceinline__ double pair_f64(double dx, double dy, double dz, double m,
-                                           double eps2) {
+                                           double eps2, const Rinv3Consts& k3) {
     const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, eps2)));
-    double ri = rsqrt_refined(r2);
-    if (MASK) ri = r2 > 0.0 ? ri : 0.0;
-    const double mr = m * ri;
-    const double ri2 = ri * ri;
-    return mr * ri2;
+    const double f = m_rinv3(r2, m, k3);
+    return MASK ? (r2 > 0.0 ? f : 0.0) : f;
 }
 
 // Bounding box of each TJ-wide tile of the sorted records: tbox[kTileBox t
@@ -529,6 +526,7 @@ __global__ __launch_bounds__(BLOCK) void bf_fp64_kernel(const double4* __restric
     // Padding records: zero mass placed far away (no contribution, r^2 > 0).
     const double4 pad = make_double4(1e150, 1e150, 1e150, 0.0);
     const double e2 = in_vgpr(eps2);  // VGPR operand of the r^2 FMA
+    const Rinv3Consts k3 = rinv3_consts();
 
     double4 stage[LPT];
     auto load = [&](int t) {
@@ -559,7 +557,7 @@ __global__ __launch_bounds__(BLOCK) void bf_fp64_kernel(const double4* __restric
 #pragma unroll
             for (int k = 0; k < IPT; ++k) {
                 const double dx = s.x - xi[k], dy = s.y - yi[k], dz = s.z - zi[k];
-                const double f = pair_f64<MASK>(dx, dy, dz, s.w, e2);
+                const double f = pair_f64<MASK>(dx, dy, dz, s.w, e2, k3);
                 ax[k] = __builtin_fma(f, dx, ax[k]);
                 ay[k] = __builtin_fma(f, dy, ay[k]);
                 az[k] = __builtin_fma(f, dz, az[k]);

@@ -89,6 +89,7 @@ __global__ __launch_bounds__(kEnsBlock) void ens_force_kernel(
     } else {
         __shared__ double4 tile[kEnsBlock];
         double sx = 0.0, sy = 0.0, sz = 0.0;
+        const Rinv3Consts k3 = rinv3_consts();
         for (int jb = b.j0; jb < b.j1; jb += kEnsBlock) {
             const int j = jb + threadIdx.x;
             __syncthreads();
@@ -100,8 +101,7 @@ __global__ __launch_bounds__(kEnsBlock) void ens_force_kernel(
                 const double4 s = tile[k];
                 const double dx = s.x - px, dy = s.y - py, dz = s.z - pz;
                 const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, eps2)));
-                const double ri = (jb + k == i) ? 0.0 : rsqrt_refined(r2);
-                const double f = (s.w * ri) * (ri * ri);
+                const double f = (jb + k == i) ? 0.0 : m_rinv3(r2, s.w, k3);
                 sx = __builtin_fma(f, dx, sx);
                 sy = __builtin_fma(f, dy, sy);
                 sz = __builtin_fma(f, dz, sz);
