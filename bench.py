@@ -71,6 +71,10 @@ def parse():
     ap.add_argument("--probe-shapes", default=None,
                     help="only run three mixed brute-force calls per shape, shapes as "
                          "'ipt,tile,waves_per_cu' joined by ';' (per-shape PMC traffic runs)")
+    ap.add_argument("--probe-workload", choices=("bh", "bh_fp64", "sph", "c4"), default=None,
+                    help="only run one secondary workload and print its JSON (per-kernel PMC "
+                         "traffic runs; combine with its --*-steps option and "
+                         "--no-cpu-baseline)")
     ap.add_argument("--probe-copy", action="store_true",
                     help="only run a 1 GiB device copy (PMC byte-counter calibration)")
     return ap.parse_args()
@@ -186,6 +190,31 @@ def run_c1(args, g, ctx):
                         "leapfrog KDK fused on the device"}
 
 
+def cpu_threads():
+    """Host threads for the multi-threaded CPU baselines (the GPU box's CPU
+    share is 16)."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def bh_cpu_baseline(p, eps, theta):
+    """The oracle's restatement of the reference Barnes-Hut
+    (barnes_hut.rs:44-192: serial insertion octree, octree.rs:145-239, and a
+    per-particle stack walk run in parallel over particles, as rayon does)
+    on the same ICs: one force evaluation, the cost of one KDK step."""
+    import oracle as O
+    n = p.count
+    s = O.State(n)
+    s.x, s.y, s.z, s.m = [np.ascontiguousarray(getattr(p, f)) for f in ("x", "y", "z", "mass")]
+    th = cpu_threads()
+    t0 = time.perf_counter()
+    O.bh_accel(s, eps, theta, th)
+    dt = time.perf_counter() - t0
+    return {"value": round(1.0 / dt, 4), "unit": "steps/s (one force evaluation per KDK step)",
+            "cores": th, "kind": "port",
+            "sample": f"one reference Barnes-Hut force evaluation (insertion octree + parallel "
+                      f"walk, f64, theta={theta}) on the same {n} particles: {dt:.2f} s"}
+
+
 def run_bh(args, g, torch, ctx, precision="mixed", steps=None):
     """Barnes-Hut KDK steps on two Plummer spheres of n_each particles (seeds
     42/43, a = 1, M = 1 each) 10 a apart on a parabolic approach, eps 0.01,
@@ -239,7 +268,12 @@ def run_bh(args, g, torch, ctx, precision="mixed", steps=None):
     # source a 32 B record read by the wave; per target 32 B in + 24 B out.
     walk_bytes = st["node_tests"] * 48 + (st["accepted_nodes"] + st["leaf_particles"]) * 32 \
         + n * 56
+    cpu = None
+    if precision == "mixed" and not args.no_cpu_baseline:
+        log(f"bench: CPU baseline, reference Barnes-Hut on {n} particles")
+        cpu = bh_cpu_baseline(p, 0.01, 0.5)
     return {"n": n, "precision": precision, "steps_per_s": round(1.0 / t_step, 3),
+            "cpu_baseline": cpu,
             "ms_per_step": round(t_step * 1e3, 3), "steps": steps, "theta": 0.5, "leaf_size": 16, "group_size": group,
             "softening": 0.01, "dt": 5e-4,
             "walk_kernel_ms": round(walk_s * 1e3, 3),
@@ -250,6 +284,32 @@ def run_bh(args, g, torch, ctx, precision="mixed", steps=None):
             "walk_frac_hbm_peak": round(walk_bytes / walk_s / 8.0e12, 4),
             "workload": "two Plummer spheres (seeds 42/43, a=1, M=1 each) 10a apart, parabolic "
                         "approach, KDK with Barnes-Hut theta=0.5, leaf 16"}
+
+
+def ens_cpu_baseline(cnts, steps=10, systems=4):
+    """The oracle's FP64 ensemble forward (gather-form brute force, KDK) and
+    checkpoint-free adjoint on the first `systems` systems of the same
+    ensemble, `steps` steps each; per-step times and pair rates."""
+    import oracle as O
+    pairs = 0.0
+    tf = ta = 0.0
+    for k in range(systems):
+        ns = int(cnts[k])
+        st = O.plummer(ns, 42 + k)
+        pairs += float(ns) * (ns - 1)
+        t0 = time.perf_counter()
+        O.ens_forward(st.copy(), 0.05, 1e-3, steps)
+        tf += time.perf_counter() - t0
+        gx, gy, gz = (np.ones(ns) for _ in range(3))
+        t0 = time.perf_counter()
+        O.ens_adjoint(st.copy(), 0.05, 1e-3, steps, gx, gy, gz)
+        ta += time.perf_counter() - t0
+    ns_list = [int(c) for c in cnts[:systems]]
+    return {"value": round(pairs * steps / tf / 1e9, 4), "unit": "G-interactions/s", "cores": 1,
+            "kind": "port",
+            "adjoint_G_pairs_per_s": round(pairs * steps / ta / 1e9, 4),
+            "sample": f"oracle FP64 forward and adjoint (f64, 1 thread) of systems N = {ns_list}, "
+                      f"{steps} steps each: forward {tf:.2f} s, adjoint {ta:.2f} s"}
 
 
 def run_ensemble(args, g, torch, ctx):
@@ -302,12 +362,18 @@ def run_ensemble(args, g, torch, ctx):
         *[p(out[r]) for r in range(7)]))
     torch.cuda.synchronize()
     t_adj = time.perf_counter() - t0
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("bench: CPU baseline, oracle ensemble forward and adjoint on 4 systems")
+        cpu = ens_cpu_baseline(cnts)
     return {"systems": len(cnts), "n_total": n, "interactions_per_step": inter,
             "steps": args.ens_steps, "ms_per_step": round(t_step * 1e3, 4),
             "G_interactions_per_s": round(inter / t_step / 1e9, 3),
+            "cpu_baseline": cpu,
             "adjoint": {"steps": args.ens_adj_steps, "checkpoint_every": 32,
                         "seconds": round(t_adj, 4),
                         "ms_per_step": round(t_adj / args.ens_adj_steps * 1e3, 4),
+                        "G_pairs_per_s": round(inter * args.ens_adj_steps / t_adj / 1e9, 3),
                         "note": "forward with checkpoints + segment recompute + FP64 pair-VJP"},
             "workload": "256 Plummer spheres, N_s in [512, 8192] (seeded), eps 0.05, dt 1e-3, "
                         "mixed brute force, hipGraph step"}
@@ -501,6 +567,17 @@ def main():
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream(dev)  # default priority; every op below runs on it
     torch.cuda.set_stream(stream)
+    if args.probe_workload:
+        # one workload alone (rocprofv3 --pmc traffic runs of its kernels)
+        ctx = g.Context(0)
+        ctx.set_stream(stream.cuda_stream)
+        run = {"bh": lambda: run_bh(args, g, torch, ctx, "mixed"),
+               "bh_fp64": lambda: run_bh(args, g, torch, ctx, "fp64"),
+               "sph": lambda: run_sph(args, g, torch, ctx),
+               "c4": lambda: run_sph_c4(args, g, torch, ctx)}[args.probe_workload]
+        print(json.dumps({"probe": args.probe_workload, "result": run()}), flush=True)
+        ctx.close()
+        return
     log(f"bench: Plummer N={n} eps={eps} dt={dt}; generating ICs")
     p = g.scenarios.plummer(n, 42)
     T = {f: torch.from_numpy(getattr(p, f)).to(dev) for f in ("x", "y", "z", "vx", "vy", "vz",

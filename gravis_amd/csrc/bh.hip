@@ -36,7 +36,7 @@ namespace {
 __global__ void walk_reset_kernel(uint32_t* __restrict__ counter,
                                   unsigned long long* __restrict__ stats) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
-        counter[0] = 0u;
+        for (uint32_t k = 0; k < kXcds; ++k) counter[k] = 0u;
         for (int k = 0; k < 4; ++k) stats[k] = 0ull;
     }
 }
@@ -89,11 +89,12 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
     const double side0 = 2.0 * box[3];  // root side, octree.rs:81-85
 
     const uint32_t ng = __builtin_amdgcn_readfirstlane(ngroups[0]);
+    // groups are Morton ordered: each XCD drains its own contiguous range
+    // first (L2 reuse of the nodes and leaves near it), then helps the others
+    const uint32_t home = blockIdx.x % kXcds;
+    uint32_t cx = home;
     for (;;) {
-        // every lane takes part (lane 0 adds 1), so the fetch sits in uniform
-        // control flow and g is a scalar
-        const uint32_t g =
-            __builtin_amdgcn_readfirstlane(atomicAdd(counter, lane == 0 ? 1u : 0u));
+        const uint32_t g = xcd_take(counter, ng, home, cx, lane);
         if (g >= ng) break;
         const uint2 gr = groups[g];
         const int i0 = (int)gr.x;
@@ -345,7 +346,7 @@ gravis_status bh_walk(Ctx* c, const gravis_solver* s, const Octree& t, const Fin
     const uint32_t fcap = (uint32_t)std::min<size_t>(t.cap, c->bh_frontier_cap);
     uint32_t *front, *counter;
     GRAVIS_TRY(buf(c, "bh_front", (size_t)blocks * kWalkWaves * 2 * fcap, &front));
-    GRAVIS_TRY(buf(c, "bh_counter", 1, &counter));
+    GRAVIS_TRY(buf(c, "bh_counter", kXcds, &counter));
     double* part;
     GRAVIS_TRY(buf(c, "bh_part", 3 * n, &part));
     const double theta2 = s->theta * s->theta;

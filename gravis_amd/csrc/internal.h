@@ -64,7 +64,7 @@ struct Ctx {
     int bf_matrix = 0;
     int bf_waves_per_cu = 256;
     // Barnes-Hut target group when the solver leaves group_size at 0
-    uint32_t bh_group = 64;
+    uint32_t bh_group = 32;
     // Mixed brute force: a source tile closer to the target box than this
     // fraction q of the box diagonal gets FP64 pair separations, unless the
     // softening exceeds q times the diagonal.
@@ -108,6 +108,39 @@ __device__ __forceinline__ double uniform(double v) {
     const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
     const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
     return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// Workgroup -> work-range map for gfx950's 8 XCDs. The dispatcher places
+// workgroup b on XCD b % 8, each XCD with its own L2; this bijection gives
+// XCD x the contiguous range of work indices [start_x, start_x + count_x),
+// so kernels whose neighbouring work items read overlapping data (Morton-
+// ordered particles and their neighbour lists) reuse it in one L2 instead
+// of pulling the same lines into all eight.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    const uint32_t per = nb / kXcds, rem = nb % kXcds;
+    const uint32_t x = b % kXcds, k = b / kXcds;
+    return x * per + (x < rem ? x : rem) + k;
+}
+
+// Dynamic work distribution with the same XCD affinity: kXcds counters,
+// counter x handing out the work indices of range x of xcd_block's split
+// (so Morton-consecutive work stays in one L2). A wavefront drains the
+// counter of its workgroup's XCD (home = blockIdx.x % kXcds) first, then
+// moves round the others; each index is handed out exactly once, and the
+// call returns ~0u once every counter is drained. cx carries the counter
+// being drained between calls (start it at home). Every lane of the
+// wavefront calls it; the result is wavefront-uniform.
+__device__ __forceinline__ uint32_t xcd_take(uint32_t* __restrict__ counters, uint32_t n,
+                                             uint32_t home, uint32_t& cx, int lane) {
+    const uint32_t per = n / kXcds, rem = n % kXcds;
+    for (;;) {
+        const uint32_t k = __builtin_amdgcn_readfirstlane(
+            atomicAdd(&counters[cx], lane == 0 ? 1u : 0u));
+        if (k < per + (cx < rem ? 1u : 0u)) return cx * per + (cx < rem ? cx : rem) + k;
+        cx = (cx + 1) % kXcds;
+        if (cx == home) return ~0u;
+    }
 }
 
 // A wavefront-uniform float held in a vector register. On gfx950 a VALU

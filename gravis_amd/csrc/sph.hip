@@ -689,7 +689,7 @@ __global__ __launch_bounds__(256) void sph_density_kernel(
     double* __restrict__ hcur, double* __restrict__ radius, double4* __restrict__ vh,
     double4* __restrict__ aux, double2* __restrict__ aux2, DensityArgs A,
     uint32_t* __restrict__ ctl) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     bool paused = false;
     if (s < n && qstate[s] == kQDensity) {
         const double4 me = pos[s];
@@ -818,7 +818,7 @@ __global__ __launch_bounds__(256) void sph_force_kernel(
     const double2* __restrict__ aux2, const uint8_t* __restrict__ qstate,
     const uint32_t* __restrict__ pool, const uint2* __restrict__ nbr, gravis_sph_params P,
     ForceOut o) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const uint32_t pi = perm[s];
     if (qstate[s] == kQNone) {
