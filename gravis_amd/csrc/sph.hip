@@ -560,24 +560,42 @@ __device__ __forceinline__ void gather_batch(const double4* __restrict__ pos,
     for (uint32_t e = 0; e < kGather; ++e) qb[e] = pos[jb[e]];
 }
 
-// As gather_batch for the force pass: kFGather members' index, position,
-// velocity/h, (rho, P, c, alpha) and (Omega, u) records.
+// Everything the force pass reads of one particle, packed into one 128 B
+// line (sorted order): position and mass, velocity and h, (rho, P, c,
+// alpha), (Omega, u). Written once after the density pass, so a neighbour
+// costs one cache line instead of four.
+struct ForceRec {
+    double4 p, v, a, b;  // b = (Omega, u, 0, 0)
+};
+
+__global__ __launch_bounds__(256) void sph_pack_kernel(int n, const double4* __restrict__ pos,
+                                                       const double4* __restrict__ vh,
+                                                       const double4* __restrict__ aux,
+                                                       const double2* __restrict__ aux2,
+                                                       ForceRec* __restrict__ rec) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const double2 b = aux2[s];
+    rec[s] = ForceRec{pos[s], vh[s], aux[s], make_double4(b.x, b.y, 0.0, 0.0)};
+}
+
+// As gather_batch for the force pass: kFGather members' index and packed
+// record.
 constexpr uint32_t kFGather = 2;
 
 __device__ __forceinline__ void gather_force_batch(
-    const double4* __restrict__ pos, const double4* __restrict__ vh,
-    const double4* __restrict__ aux, const double2* __restrict__ aux2,
-    const uint32_t* __restrict__ pool, uint2 nb, uint32_t k0, uint32_t (&jb)[kFGather],
-    double4 (&pb)[kFGather], double4 (&vb)[kFGather], double4 (&ab)[kFGather],
-    double2 (&bb)[kFGather]) {
+    const ForceRec* __restrict__ rec, const uint32_t* __restrict__ pool, uint2 nb, uint32_t k0,
+    uint32_t (&jb)[kFGather], double4 (&pb)[kFGather], double4 (&vb)[kFGather],
+    double4 (&ab)[kFGather], double2 (&bb)[kFGather]) {
 #pragma unroll
     for (uint32_t e = 0; e < kFGather; ++e) jb[e] = k0 + e < nb.y ? pool[nb.x + k0 + e] : 0u;
 #pragma unroll
     for (uint32_t e = 0; e < kFGather; ++e) {
-        pb[e] = pos[jb[e]];
-        vb[e] = vh[jb[e]];
-        ab[e] = aux[jb[e]];
-        bb[e] = aux2[jb[e]];
+        const ForceRec& r = rec[jb[e]];
+        pb[e] = r.p;
+        vb[e] = r.v;
+        ab[e] = r.a;
+        bb[e] = make_double2(r.b.x, r.b.y);
     }
 }
 
@@ -813,11 +831,9 @@ __device__ __forceinline__ float grad_factor_f(float r, float ir, float ih) {
 // FP64. alpha_switch 0: dalpha/dt = 0.
 template <int K, bool F32>
 __global__ __launch_bounds__(256) void sph_force_kernel(
-    int n, const uint32_t* __restrict__ perm, const double4* __restrict__ pos,
-    const double4* __restrict__ vh, const double4* __restrict__ aux,
-    const double2* __restrict__ aux2, const uint8_t* __restrict__ qstate,
-    const uint32_t* __restrict__ pool, const uint2* __restrict__ nbr, gravis_sph_params P,
-    ForceOut o) {
+    int n, const uint32_t* __restrict__ perm, const ForceRec* __restrict__ rec,
+    const uint8_t* __restrict__ qstate, const uint32_t* __restrict__ pool,
+    const uint2* __restrict__ nbr, gravis_sph_params P, ForceOut o) {
     const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const uint32_t pi = perm[s];
@@ -829,10 +845,10 @@ __global__ __launch_bounds__(256) void sph_force_kernel(
         if (o.nneigh) o.nneigh[pi] = 0u;
         return;
     }
-    const double4 me = pos[s];
-    const double4 vi = vh[s];
-    const double4 ai = aux[s];
-    const double2 bi = aux2[s];
+    const double4 me = rec[s].p;
+    const double4 vi = rec[s].v;
+    const double4 ai = rec[s].a;
+    const double2 bi = make_double2(rec[s].b.x, rec[s].b.y);
     const double hi = vi.w;
     const double rho_i = fmax(ai.x, kDensityFloor);
     const double p_i = ai.y, c_i = ai.z, alpha_i = ai.w;
@@ -845,7 +861,7 @@ __global__ __launch_bounds__(256) void sph_force_kernel(
             uint32_t jb[kFGather];
             double4 pb[kFGather], vb[kFGather], ab[kFGather];
             double2 bb[kFGather];
-            gather_force_batch(pos, vh, aux, aux2, pool, nb, k0, jb, pb, vb, ab, bb);
+            gather_force_batch(rec, pool, nb, k0, jb, pb, vb, ab, bb);
 #pragma unroll
             for (uint32_t e = 0; e < kFGather; ++e) {
             if (k0 + e >= nb.y) break;
@@ -912,7 +928,7 @@ __global__ __launch_bounds__(256) void sph_force_kernel(
             uint32_t jb[kFGather];
             double4 pb[kFGather], vb[kFGather], ab[kFGather];
             double2 bb[kFGather];
-            gather_force_batch(pos, vh, aux, aux2, pool, nb, k0, jb, pb, vb, ab, bb);
+            gather_force_batch(rec, pool, nb, k0, jb, pb, vb, ab, bb);
 #pragma unroll
             for (uint32_t e = 0; e < kFGather; ++e) {
             if (k0 + e >= nb.y) break;
@@ -1252,10 +1268,12 @@ gravis_status sph_compute(Ctx* c, const gravis_sph_params* P, const gravis_sph_s
     }
     ForceOut o{S->ax, S->ay, S->az, S->dudt, S->rho, S->h, S->p, S->c,
                div_v, max_mu, dalpha, omega, nneigh};
+    ForceRec* frec;
+    GRAVIS_TRY(buf(c, "sph_frec", n, &frec));
     timer_begin(c);
-#define GRAVIS_FORCE(K, F)                                                                \
-    sph_force_kernel<K, F><<<g256, 256, 0, st>>>(ni, t.perm, t.src, vh, aux, aux2, qstate, pool, \
-                                                 nbr, *P, o)
+    sph_pack_kernel<<<g256, 256, 0, st>>>(ni, t.src, vh, aux, aux2, frec);
+#define GRAVIS_FORCE(K, F)                                                               \
+    sph_force_kernel<K, F><<<g256, 256, 0, st>>>(ni, t.perm, frec, qstate, pool, nbr, *P, o)
     if (P->kernel == 0 && !P->pair_fp32) GRAVIS_FORCE(0, false);
     else if (P->kernel == 0) GRAVIS_FORCE(0, true);
     else if (!P->pair_fp32) GRAVIS_FORCE(1, false);
