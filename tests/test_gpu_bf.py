@@ -121,6 +121,32 @@ def test_bf_mixed_error_budget(g, n):
     assert mx <= 1e-4
 
 
+@pytest.mark.parametrize("kernel", ["valu", "matrix"])
+def test_c2_mixed_full_size_vs_oracle(g, kernel):
+    """The brute-force benchmark workload at full size (Plummer N=262,144,
+    seed 42, eps 0.01): mixed forces of both kernels on 512 random targets
+    against the oracle's f64 brute force. RMS |da|/|a| <= 1e-6, max <= 1e-4,
+    and the condition-normalised error |da| / sum_j |a_ij| within the
+    per-pair bound 3u(1 + 1/q) = 195 u ~ 1.2e-5 at q = 1/64: the pair errors
+    sum to at most that bound times sum_j |a_ij|. The FP32 tile sums could add
+    up to ~127 u more in the worst case; compensated accumulation keeps the
+    measured errors orders of magnitude below either."""
+    say(f"C2 full size, mixed {kernel} kernel vs oracle, 512 targets")
+    n = 262144
+    s = O.plummer(n, 42)
+    ctx = g.Context(0)
+    ctx.set_tuning("bf_matrix", 1 if kernel == "matrix" else 0)
+    pm = to_particles(g, s)
+    g.BruteForce(0.01, "mixed", ctx=ctx).compute_accelerations(pm)
+    ctx.close()
+    t = np.sort(np.random.default_rng(42).choice(n, 512, replace=False))
+    cmax, ref = cond_errors(s, 0.01, pm.acc(), t)
+    rms, mx = rel_errors(ref, pm.acc()[t])
+    say(f"  rms={rms:.3e} max={mx:.3e} cond_max={cmax:.3e}")
+    assert rms <= 1e-6 and mx <= 1e-4
+    assert cmax <= 195 * 2.0 ** -24
+
+
 @pytest.fixture(scope="module")
 def bf_ref():
     """FP64 GPU forces of the inputs the kernel-variant test uses (cached)."""

@@ -180,8 +180,9 @@ def test_bh_large_n_against_brute_force_sample(g):
 def test_bh_two_sphere_4m_parity(g):
     """The Barnes-Hut benchmark workload at full size: two Plummer spheres of
     2,097,152 particles (seeds 42/43) 10a apart. Morton keys and the sort
-    permutation are bit-exact with the oracle; forces on 256 random targets
-    are within the reference's theta=0.5 error of f64 brute force."""
+    permutation are bit-exact with the oracle; forces of the FP64 and the
+    mixed walk on 256 random targets are within the reference's theta=0.5
+    error of f64 brute force."""
     say("bh two-sphere N=4194304: keys/permutation and force sample")
     ne = 2097152
     p = g.scenarios.plummer_pair(ne, 42, 43, 10.0, 0.0)
@@ -197,8 +198,14 @@ def test_bh_two_sphere_4m_parity(g):
     t = np.sort(rng.choice(p.count, 256, replace=False))
     ref, _ = O.bf_targets(s, 0.01, t, nthreads=16)
     rms, mx = rel_errors(ref, p.acc()[t])
-    say(f"  rms={rms:.3e} max={mx:.3e} nodes={g.default_context().bh_stats()['nodes']}")
+    say(f"  fp64 walk: rms={rms:.3e} max={mx:.3e} nodes={g.default_context().bh_stats()['nodes']}")
     assert rms <= 4.942351e-3 and mx <= 3.588391e-2
+    # the mixed walk (the benchmark's) on the same sample
+    pm = g.scenarios.plummer_pair(ne, 42, 43, 10.0, 0.0)
+    g.BarnesHut(0.01, 0.5, precision="mixed").compute_accelerations(pm)
+    rms_m, mx_m = rel_errors(ref, pm.acc()[t])
+    say(f"  mixed walk: rms={rms_m:.3e} max={mx_m:.3e}")
+    assert rms_m <= 4.942351e-3 and mx_m <= 3.588391e-2
 
 
 @pytest.mark.parametrize("n,theta", [(20000, 0.5), (3000, 0.3)])
