@@ -191,8 +191,8 @@ def run_c1(args, g, ctx):
 
 
 def cpu_threads():
-    """Host threads for the multi-threaded CPU baselines (the GPU box's CPU
-    share is 16)."""
+    """Host threads for the multi-threaded CPU baselines: at most 16, the
+    host share of one GPU's process."""
     return max(1, min(16, os.cpu_count() or 1))
 
 
