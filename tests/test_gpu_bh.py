@@ -6,7 +6,7 @@ Bars:
     orc_bh_group_accel) BIT FOR BIT for the same theta, leaf and group.
   * Accuracy vs brute force at theta 0.3/0.5/0.7/1.0 on the reference's
     N=1000 seed-42 Plummer set is within the reference's own Barnes-Hut
-    error (m3_accuracy.csv) -- the north-star bar for BH forces.
+    error (m3_accuracy.csv), the accuracy bar for BH forces.
   * theta = 0 reduces to brute force (barnes_hut.rs:219-265), |a| = 0.25 for
     two unit masses at distance 2 (barnes_hut.rs:268-283).
   * KDK with BH conserves energy to 1% and keeps the virial ratio within
