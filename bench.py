@@ -65,9 +65,9 @@ def parse():
     ap.add_argument("--c1-steps", type=int, default=10000)
     ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--bf-shape", default=None,
-                    help="ipt,tile,waves_per_cu (ipt 1 or 2, or m for the matrix-core "
-                         "kernel): pin the mixed brute-force launch shape instead of "
-                         "autotuning (profiling runs)")
+                    help="ipt,tile,waves_per_cu (ipt 1 or 2, or m / b for the matrix-core "
+                         "kernel with FP32 / split-bf16 operands): pin the mixed brute-force "
+                         "launch shape instead of autotuning (profiling runs)")
     ap.add_argument("--probe-shapes", default=None,
                     help="only run three mixed brute-force calls per shape, shapes as "
                          "'ipt,tile,waves_per_cu' joined by ';' (per-shape PMC traffic runs)")
@@ -126,15 +126,17 @@ def probe_copy():
 
 
 def shape_key(t):
-    """Mixed brute-force launch shape as 'ipt,tile,waves_per_cu' ('m' for the
-    matrix-core kernel), the key of the traffic profile."""
-    return f"{'m' if t['bf_matrix'] else t['bf_ipt']},{t['bf_tile']},{t['bf_waves_per_cu']}"
+    """Mixed brute-force launch shape as 'ipt,tile,waves_per_cu' ('m' and 'b'
+    for the matrix-core kernel with FP32 and split-bf16 operands), the key of
+    the traffic profile."""
+    kern = {0: str(t["bf_ipt"]), 1: "m", 2: "b"}[t["bf_matrix"]]
+    return f"{kern},{t['bf_tile']},{t['bf_waves_per_cu']}"
 
 
 def set_shape(ctx, key):
     ipt, tile, wpc = key.split(",")
-    ctx.set_tuning("bf_matrix", 1 if ipt == "m" else 0)
-    if ipt != "m":
+    ctx.set_tuning("bf_matrix", {"m": 1, "b": 2}.get(ipt, 0))
+    if ipt not in ("m", "b"):
         ctx.set_tuning("bf_ipt", int(ipt))
     ctx.set_tuning("bf_tile", int(tile))
     ctx.set_tuning("bf_waves_per_cu", int(wpc))

@@ -595,8 +595,8 @@ gravis_status gravis_set_tuning(gravis_ctx* g, const char* key, int64_t value) {
         return GRAVIS_OK;
     }
     if (std::strcmp(key, "bf_matrix") == 0) {
-        if (value != 0 && value != 1)
-            return fail(c, GRAVIS_INVALID_ARG, "bf_matrix must be 0 or 1");
+        if (value < 0 || value > 2)
+            return fail(c, GRAVIS_INVALID_ARG, "bf_matrix must be 0, 1 or 2");
         c->bf_matrix = (int)value;
         return GRAVIS_OK;
     }
@@ -689,14 +689,15 @@ gravis_status gravis_autotune(gravis_ctx* g, const gravis_solver* s, size_t n, c
     if (s->algorithm == GRAVIS_BRUTE_FORCE) {
         const bool mx = s->precision == GRAVIS_MIXED;
         std::vector<Shape> v;
-        // kernel 0, 1: VALU kernel with bf_ipt 2, 1; kernel 2: matrix core
+        // kernel 0, 1: VALU kernel with bf_ipt 2, 1; kernel 2, 3: matrix core
+        // with FP32 and split-bf16 operands
         for (int wpc : {32, 64, 128, 256, 512})
-            for (int kern : {0, 1, 2})
+            for (int kern : {0, 1, 2, 3})
                 for (int tile : {128, 256}) {
                     if (!mx && (kern != 1 || tile != 256)) continue;  // FP64: split only
                     if (mx && wpc == 32) continue;
                     if (mx) {
-                        c->bf_matrix = kern == 2 ? 1 : 0;
+                        c->bf_matrix = kern >= 2 ? kern - 1 : 0;
                         if (kern < 2) c->bf_ipt = 2 - kern;
                         c->bf_tile = tile;
                     }

@@ -24,6 +24,8 @@
 // Each lane sums a tile of pair forces in FP32 and adds the tile sum into a
 // Kahan-compensated FP32 accumulator; the result leaves as FP64
 // sum - compensation.
+#include <type_traits>
+
 #include "internal.h"
 
 namespace gravis {
@@ -256,19 +258,48 @@ __global__ __launch_bounds__(kWave) void bf_mixed_kernel(const double4* __restri
 }
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// The r^2 product of one 16 x 16 block: FP32 operands on
+// v_mfma_f32_16x16x4f32 (K = 4), or bf16 operands on
+// v_mfma_f32_16x16x32_bf16 (K = 32, each FP32 value split in three exact
+// bf16 pieces; half the matrix-core cycles).
+__device__ __forceinline__ f32x4 mx_mfma(float a, float b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x4 mx_mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// x = hi + mid + lo exactly for FP32 x (8 significant bits each; the
+// residuals are exact in FP32).
+__device__ __forceinline__ void split_bf16x3(float x, __bf16& hi, __bf16& mid, __bf16& lo) {
+    hi = (__bf16)x;
+    const float r1 = x - (float)hi;
+    mid = (__bf16)r1;
+    lo = (__bf16)(r1 - (float)mid);
+}
+
+// K slots of the bf16 form: for component c, slots 6c..6c+5 hold the
+// products (b, a') pieces (hi,hi) (hi,mid) (mid,hi) (hi,lo) (mid,mid)
+// (lo,hi) -- every product above 2^-24 of |b||a'| -- slots 18..20 hold
+// 1 x (|a|^2 + eps^2) in three pieces, slots 21..31 are zero.
+__device__ constexpr int kSplitB[6] = {0, 0, 1, 0, 1, 2};
+__device__ constexpr int kSplitA[6] = {0, 1, 0, 2, 1, 0};
 
 // One source block of the matrix-core far loop: the pair forces of the
 // current block (products dc, source terms qc) are summed into acc while the
 // products and source terms of block sbn are issued (MORE), one product per
-// 16-target block between the VALU work of the previous one.
-template <bool MORE, bool MASK>
-__device__ __forceinline__ void mx_block(const float* __restrict__ aop,
+// 16-target block between the VALU work of the previous one. T is the
+// operand type (float or bf16x8).
+template <bool MORE, bool MASK, typename T>
+__device__ __forceinline__ void mx_block(const T* __restrict__ aop,
                                          const float4* __restrict__ qop,
                                          const float* __restrict__ bbq, int sbn, int lane, int grp,
-                                         const float (&bop)[4], const f32x4 (&dc)[4],
+                                         const T (&bop)[4], const f32x4 (&dc)[4],
                                          const float4 (&qc)[4], f32x4 (&dn)[4], float4 (&qn)[4],
                                          float (&acc)[4][4]) {
-    float avn = 0.f;
+    T avn = {};
     f32x4 cbn = {0.f, 0.f, 0.f, 0.f};
     if (MORE) {
         avn = aop[sbn * 64 + lane];
@@ -279,7 +310,7 @@ __device__ __forceinline__ void mx_block(const float* __restrict__ aop,
     }
 #pragma unroll
     for (int tb = 0; tb < 4; ++tb) {
-        if (MORE) dn[tb] = __builtin_amdgcn_mfma_f32_16x16x4f32(avn, bop[tb], cbn, 0, 0, 0);
+        if (MORE) dn[tb] = mx_mfma(avn, bop[tb], cbn);
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const float r2 = dc[tb][v];
@@ -321,7 +352,7 @@ __device__ __forceinline__ void mx_block(const float* __restrict__ aop,
 // u (1 + 1 / q')^2 when max(|d|, eps) >= q' diag. The caller passes q' with
 // (1 + 1 / q')^2 = 2 (1 + 1 / q), which keeps the per-pair bound of the VALU
 // kernel for the context's q; nearer tiles take the FP64-separation path.
-template <int TJ, bool MASK, int WPE = (TJ == 128 ? 4 : 3)>
+template <int TJ, bool MASK, bool BF, int WPE = (!BF && TJ == 128 ? 4 : 3)>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void bf_matrix_kernel(const double4* __restrict__ pm, int n,
                                                           int jchunk, float eps2, double near2,
                                                           const double* __restrict__ tbox,
@@ -329,13 +360,15 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
     static_assert(TJ % kWave == 0, "tile must be a multiple of the wavefront");
     constexpr int LPT = TJ / kWave;  // records each lane stages per tile
     constexpr int SB = TJ / 16;      // 16-source blocks per tile (even)
-    // near: TJ FP64 records; far: A operands [SB][4][16], (m b, m) and |b|^2
-    // per source (36 TJ bytes)
-    __shared__ float4 lds[TJ * 9 / 4];
+    using T = typename std::conditional<BF, bf16x8, float>::type;
+    // near: TJ FP64 records; far: A operands [SB][4][16] of type T, then
+    // (m b, m) and |b|^2 per source (36 or 84 TJ bytes)
+    constexpr int kA = BF ? 4 * TJ : TJ;  // float4 units of the A operands
+    __shared__ float4 lds[kA + TJ + TJ / 4];
     double4* tile = reinterpret_cast<double4*>(&lds[0]);
-    float* aop = reinterpret_cast<float*>(&lds[0]);
-    float4* qop = &lds[TJ];
-    float* bbq = reinterpret_cast<float*>(&lds[2 * TJ]);
+    T* aop = reinterpret_cast<T*>(&lds[0]);
+    float4* qop = &lds[kA];
+    float* bbq = reinterpret_cast<float*>(&lds[kA + TJ]);
 
     const int lane = threadIdx.x;
     const int col = lane & 15, grp = lane >> 4;
@@ -361,14 +394,38 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
         az = (float)(p.z - cz);
     }
     // B operand of each 16-target block
-    float bop[4];
+    T bop[4];
 #pragma unroll
     for (int tb = 0; tb < 4; ++tb) {
         const float tx = __shfl(ax, 16 * tb + col);
         const float ty = __shfl(ay, 16 * tb + col);
         const float tz = __shfl(az, 16 * tb + col);
         const float aa = __builtin_fmaf(tx, tx, __builtin_fmaf(ty, ty, __builtin_fmaf(tz, tz, eps2)));
-        bop[tb] = grp == 0 ? -2.0f * tx : grp == 1 ? -2.0f * ty : grp == 2 ? -2.0f * tz : aa;
+        if constexpr (BF) {
+            // slots 8 grp .. 8 grp + 7 of this lane's target column
+            __bf16 pc[4][3];
+            split_bf16x3(-2.0f * tx, pc[0][0], pc[0][1], pc[0][2]);
+            split_bf16x3(-2.0f * ty, pc[1][0], pc[1][1], pc[1][2]);
+            split_bf16x3(-2.0f * tz, pc[2][0], pc[2][1], pc[2][2]);
+            split_bf16x3(aa, pc[3][0], pc[3][1], pc[3][2]);
+            // every slot index is a compile-time constant (a runtime index
+            // into pc would place it in scratch memory); grp selects
+            bf16x8 v;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                __bf16 c4[4];
+#pragma unroll
+                for (int gs = 0; gs < 4; ++gs) {
+                    const int kk = 8 * gs + j;
+                    c4[gs] = kk < 18 ? pc[kk / 6][kSplitA[kk % 6]]
+                                     : (kk < 21 ? pc[3][kk - 18] : (__bf16)0.0f);
+                }
+                v[j] = grp == 0 ? c4[0] : grp == 1 ? c4[1] : grp == 2 ? c4[2] : c4[3];
+            }
+            bop[tb] = v;
+        } else {
+            bop[tb] = grp == 0 ? -2.0f * tx : grp == 1 ? -2.0f * ty : grp == 2 ? -2.0f * tz : aa;
+        }
     }
     float s[4], k[4];  // target ibase + lane: x, y, z, sum f m
 #pragma unroll
@@ -399,11 +456,31 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
                 const float by = (float)(stage[e].y - cy);
                 const float bz = (float)(stage[e].z - cz);
                 const float m = (float)stage[e].w;
-                float* a = aop + (sj >> 4) * 64 + (sj & 15);
-                a[0] = bx;
-                a[16] = by;
-                a[32] = bz;
-                a[48] = 1.0f;
+                if constexpr (BF) {
+                    // row sj & 15 of block sj >> 4, slot groups 0..3
+                    __bf16 pc[3][3];
+                    split_bf16x3(bx, pc[0][0], pc[0][1], pc[0][2]);
+                    split_bf16x3(by, pc[1][0], pc[1][1], pc[1][2]);
+                    split_bf16x3(bz, pc[2][0], pc[2][1], pc[2][2]);
+                    T* a = aop + (sj >> 4) * 64 + (sj & 15);
+#pragma unroll
+                    for (int kg = 0; kg < 4; ++kg) {
+                        bf16x8 v;
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            const int kk = 8 * kg + j;
+                            v[j] = kk < 18 ? pc[kk / 6][kSplitB[kk % 6]]
+                                           : (kk < 21 ? (__bf16)1.0f : (__bf16)0.0f);
+                        }
+                        a[16 * kg] = v;
+                    }
+                } else {
+                    float* a = aop + (sj >> 4) * 64 + (sj & 15);
+                    a[0] = bx;
+                    a[16] = by;
+                    a[32] = bz;
+                    a[48] = 1.0f;
+                }
                 qop[sj] = make_float4(m * bx, m * by, m * bz, m);
                 bbq[sj] = __builtin_fmaf(bx, bx, __builtin_fmaf(by, by, bz * bz));
             }
@@ -427,12 +504,12 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
             f32x4 dA[4], dB[4];
             float4 qA[4], qB[4];
             {
-                const float av = aop[lane];
+                const T av = aop[lane];
                 const float4 c4 = *reinterpret_cast<const float4*>(bbq + 4 * grp);
                 const f32x4 cb = {c4.x, c4.y, c4.z, c4.w};
 #pragma unroll
                 for (int tb = 0; tb < 4; ++tb)
-                    dA[tb] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bop[tb], cb, 0, 0, 0);
+                    dA[tb] = mx_mfma(av, bop[tb], cb);
 #pragma unroll
                 for (int v = 0; v < 4; ++v) qA[v] = qop[4 * grp + v];
             }
@@ -594,13 +671,15 @@ void launch_mixed(dim3 g, hipStream_t st, bool mask, const double4* pm, int n, i
                                                              part);
 }
 
-template <int TJ>
+template <int TJ, bool BF>
 void launch_matrix(dim3 g, hipStream_t st, bool mask, const double4* pm, int n, int jchunk,
                    float eps2, double near2, const double* tbox, double* part) {
     if (mask)
-        bf_matrix_kernel<TJ, true><<<g, kWave, 0, st>>>(pm, n, jchunk, eps2, near2, tbox, part);
+        bf_matrix_kernel<TJ, true, BF><<<g, kWave, 0, st>>>(pm, n, jchunk, eps2, near2, tbox,
+                                                            part);
     else
-        bf_matrix_kernel<TJ, false><<<g, kWave, 0, st>>>(pm, n, jchunk, eps2, near2, tbox, part);
+        bf_matrix_kernel<TJ, false, BF><<<g, kWave, 0, st>>>(pm, n, jchunk, eps2, near2, tbox,
+                                                             part);
 }
 
 template <int BLOCK, int IPT, int TJ>
@@ -716,12 +795,18 @@ gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
         if (matrix) {
             // q' with (1 + 1/q')^2 = 2 (1 + 1/q): the same per-pair bound
             const double qm = q > 0.0 ? 1.0 / (std::sqrt(2.0 * (1.0 + 1.0 / q)) - 1.0) : 0.0;
-            if (tj == 128)
-                launch_matrix<128>(grid, c->stream, mask, pm, ni, jchunk, eps2f, qm * qm, tbox,
-                                   part);
+            if (tj == 128 && c->bf_matrix == 2)
+                launch_matrix<128, true>(grid, c->stream, mask, pm, ni, jchunk, eps2f, qm * qm,
+                                         tbox, part);
+            else if (tj == 128)
+                launch_matrix<128, false>(grid, c->stream, mask, pm, ni, jchunk, eps2f, qm * qm,
+                                          tbox, part);
+            else if (c->bf_matrix == 2)
+                launch_matrix<256, true>(grid, c->stream, mask, pm, ni, jchunk, eps2f, qm * qm,
+                                         tbox, part);
             else
-                launch_matrix<256>(grid, c->stream, mask, pm, ni, jchunk, eps2f, qm * qm, tbox,
-                                   part);
+                launch_matrix<256, false>(grid, c->stream, mask, pm, ni, jchunk, eps2f, qm * qm,
+                                          tbox, part);
             timer_end(c, "bf_matrix");
             GRAVIS_TRY(check_launch(c, "bf kernel"));
             return finish_forces(c, part, S, ni, perm, fa);

@@ -59,8 +59,10 @@ struct Ctx {
     // j-chunk split aims for.
     int bf_ipt = 2;
     int bf_tile = 128;
-    // 1: the mixed brute force runs the matrix-core kernel (64 targets per
-    // wavefront, r^2 on v_mfma_f32_16x16x4f32); bf_ipt is then unused
+    // 1, 2: the mixed brute force runs the matrix-core kernel (64 targets
+    // per wavefront; r^2 on v_mfma_f32_16x16x4f32, or on
+    // v_mfma_f32_16x16x32_bf16 with FP32 values split in three bf16 pieces);
+    // bf_ipt is then unused
     int bf_matrix = 0;
     int bf_waves_per_cu = 256;
     // Barnes-Hut target group when the solver leaves group_size at 0

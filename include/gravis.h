@@ -322,8 +322,10 @@ gravis_status gravis_set_tuning(gravis_ctx* ctx, const char* key, int64_t value)
  * j-range split aims for; Barnes-Hut: target group size used when the
  * solver's group_size is 0. Set one at a time with gravis_set_tuning keys
  * "bf_ipt", "bf_tile", "bf_waves_per_cu", "bh_group_size", "bf_matrix"
- * (0 or 1: the mixed brute force forms |x_j - x_i|^2 + eps^2 with
- * v_mfma_f32_16x16x4f32, 64 targets per wavefront, bf_ipt unused);
+ * (0: VALU kernel; 1 or 2: the mixed brute force forms |x_j - x_i|^2 +
+ * eps^2 on the matrix cores, 64 targets per wavefront, bf_ipt unused --
+ * 1 with v_mfma_f32_16x16x4f32, 2 with v_mfma_f32_16x16x32_bf16 on FP32
+ * values split exactly in three bf16 pieces);
  * "bf_near_64ths" (0..64, default 1) is the near-tile fraction q, in 64ths,
  * of the mixed brute-force and ensemble kernels: a source tile whose gap to
  * the wavefront's target box and the softening are both below q times the
@@ -335,7 +337,7 @@ typedef struct gravis_tuning {
     int32_t bf_tile;
     int32_t bf_waves_per_cu;
     uint32_t bh_group_size;
-    int32_t bf_matrix; /* 1: mixed brute force on the matrix-core kernel */
+    int32_t bf_matrix; /* 1, 2: mixed brute force on the matrix-core kernel */
     double best_ms;    /* gravis_autotune: mean time of one force call, chosen shape */
 } gravis_tuning;
 
@@ -344,7 +346,7 @@ gravis_status gravis_get_tuning(gravis_ctx* ctx, gravis_tuning* out);
 /* Autotuner (no reference counterpart): times the candidate launch shapes
  * for solver s on the given particles (host arrays, copied once) and keeps
  * the chosen one in the context. Mixed brute force: bf_waves_per_cu
- * 64..512 x (bf_ipt 2, bf_ipt 1, matrix-core kernel) x bf_tile (24
+ * 64..512 x (bf_ipt 2, bf_ipt 1, bf_matrix 1, bf_matrix 2) x bf_tile (32
  * shapes); FP64 brute force: bf_waves_per_cu 32..512; among brute-force
  * shapes within 2 % of the fastest, the first in that order (fewest j
  * chunks, i.e. least partial-sum memory, then the VALU kernels) is kept.

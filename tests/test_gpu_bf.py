@@ -121,7 +121,7 @@ def test_bf_mixed_error_budget(g, n):
     assert mx <= 1e-4
 
 
-@pytest.mark.parametrize("kernel", ["valu", "matrix"])
+@pytest.mark.parametrize("kernel", ["valu", "matrix", "matrix_bf16"])
 def test_c2_mixed_full_size_vs_oracle(g, kernel):
     """The brute-force benchmark workload at full size (Plummer N=262,144,
     seed 42, eps 0.01): mixed forces of both kernels on 512 random targets
@@ -135,7 +135,7 @@ def test_c2_mixed_full_size_vs_oracle(g, kernel):
     n = 262144
     s = O.plummer(n, 42)
     ctx = g.Context(0)
-    ctx.set_tuning("bf_matrix", 1 if kernel == "matrix" else 0)
+    ctx.set_tuning("bf_matrix", {"matrix": 1, "matrix_bf16": 2}.get(kernel, 0))
     pm = to_particles(g, s)
     g.BruteForce(0.01, "mixed", ctx=ctx).compute_accelerations(pm)
     ctx.close()
@@ -156,11 +156,12 @@ def bf_ref():
 @pytest.mark.parametrize("n,eps", [(4099, 0.01), (65537, 0.01), (20011, 0.0), (1500, 0.05),
                                    (64, 0.0), (100, 0.0)])
 @pytest.mark.parametrize("ipt,tile", [(1, 128), (2, 128), (1, 256), (2, 256), ("m", 128),
-                                      ("m", 256)])
+                                      ("m", 256), ("b", 128), ("b", 256)])
 def test_bf_mixed_launch_shapes(g, bf_ref, ipt, tile, n, eps):
     """Every mixed launch shape (64 or 128 targets per wavefront, 128 or 256
-    records per tile, j split for 8 or 32 wavefronts per CU; ipt "m" is the
-    matrix-core kernel) against the FP64 path: RMS |da|/|a| <= 1e-6,
+    records per tile, j split for 8 or 32 wavefronts per CU; ipt "m" / "b"
+    is the matrix-core kernel with FP32 / split-bf16 operands) against the
+    FP64 path: RMS |da|/|a| <= 1e-6,
     max <= 1e-4. eps = 0 takes the masked self-pair path (positions
     distinct) and the FP64 near tiles; n <= 1500 runs unsorted (below the
     Morton-sort threshold); n = 64, 100 are one wavefront and one ragged
@@ -173,8 +174,8 @@ def test_bf_mixed_launch_shapes(g, bf_ref, ipt, tile, n, eps):
         g.BruteForce(eps).compute_accelerations(p64)
         bf_ref[key] = p64.acc()
     ctx = g.Context(0)
-    ctx.set_tuning("bf_matrix", 1 if ipt == "m" else 0)
-    if ipt != "m":
+    ctx.set_tuning("bf_matrix", {"m": 1, "b": 2}.get(ipt, 0))
+    if ipt not in ("m", "b"):
         ctx.set_tuning("bf_ipt", ipt)
     ctx.set_tuning("bf_tile", tile)
     for wpc in (8, 32):
@@ -188,7 +189,7 @@ def test_bf_mixed_launch_shapes(g, bf_ref, ipt, tile, n, eps):
     ctx.close()
 
 
-@pytest.mark.parametrize("kernel", ["valu", "matrix"])
+@pytest.mark.parametrize("kernel", ["valu", "matrix", "matrix_bf16"])
 def test_bf_mixed_unequal_masses(g, kernel):
     """Source masses spread over a factor of three (Plummer positions, masses
     times U(0.5, 1.5)): both mixed kernels against the FP64 path at the
@@ -200,7 +201,7 @@ def test_bf_mixed_unequal_masses(g, kernel):
     p64 = to_particles(g, s)
     g.BruteForce(0.01).compute_accelerations(p64)
     ctx = g.Context(0)
-    ctx.set_tuning("bf_matrix", 1 if kernel == "matrix" else 0)
+    ctx.set_tuning("bf_matrix", {"matrix": 1, "matrix_bf16": 2}.get(kernel, 0))
     pm = to_particles(g, s)
     g.BruteForce(0.01, "mixed", ctx=ctx).compute_accelerations(pm)
     rms, mx = rel_errors(p64.acc(), pm.acc())
@@ -212,7 +213,7 @@ def test_bf_mixed_unequal_masses(g, kernel):
 def test_bf_tuning_keys_validate(g):
     ctx = g.Context(0)
     for key, bad in (("bf_ipt", 3), ("bf_waves_per_cu", 0), ("bf_kernel", 1), ("bf_tile", 64),
-                     ("bh_group_size", 65), ("bf_near_64ths", 65), ("bf_matrix", 2)):
+                     ("bh_group_size", 65), ("bf_near_64ths", 65), ("bf_matrix", 3)):
         with pytest.raises(g.GravisError):
             ctx.set_tuning(key, bad)
     ctx.close()
@@ -364,7 +365,7 @@ def test_autotune_brute_force_and_barnes_hut(g):
     ctx = g.Context(0)
     t = ctx.autotune(g.BruteForce(0.01, "mixed", ctx=ctx), to_particles(g, s))
     say(f"  mixed: {t}")
-    assert t["bf_ipt"] in (1, 2) and t["bf_tile"] in (128, 256) and t["bf_matrix"] in (0, 1)
+    assert t["bf_ipt"] in (1, 2) and t["bf_tile"] in (128, 256) and t["bf_matrix"] in (0, 1, 2)
     assert t["bf_waves_per_cu"] in (64, 128, 256, 512) and t["best_ms"] > 0
     assert ctx.tuning() == {k: v for k, v in t.items() if k != "best_ms"}
     pm = to_particles(g, s)
