@@ -71,7 +71,7 @@ def parse():
     ap.add_argument("--probe-shapes", default=None,
                     help="only run three mixed brute-force calls per shape, shapes as "
                          "'ipt,tile,waves_per_cu' joined by ';' (per-shape PMC traffic runs)")
-    ap.add_argument("--probe-workload", choices=("bh", "bh_fp64", "sph", "c4"), default=None,
+    ap.add_argument("--probe-workload", choices=("bh", "bh_fp64", "sph", "c4", "ens"), default=None,
                     help="only run one secondary workload and print its JSON (per-kernel PMC "
                          "traffic runs; combine with its --*-steps option and "
                          "--no-cpu-baseline)")
@@ -576,7 +576,8 @@ def main():
         run = {"bh": lambda: run_bh(args, g, torch, ctx, "mixed"),
                "bh_fp64": lambda: run_bh(args, g, torch, ctx, "fp64"),
                "sph": lambda: run_sph(args, g, torch, ctx),
-               "c4": lambda: run_sph_c4(args, g, torch, ctx)}[args.probe_workload]
+               "c4": lambda: run_sph_c4(args, g, torch, ctx),
+               "ens": lambda: run_ensemble(args, g, torch, ctx)}[args.probe_workload]
         print(json.dumps({"probe": args.probe_workload, "result": run()}), flush=True)
         ctx.close()
         return
@@ -720,8 +721,9 @@ def main():
         "dtype": "f32 pair forces / f64 state",
         "data": "synthetic: Plummer sphere, sim-core ChaCha20 ICs, seed 42",
         "config": {"workload": f"Plummer N={n}, eps={eps}, dt={dt}, leapfrog KDK, mixed-precision "
-                               "brute-force gravity (FP32 pair forces on separations formed "
-                               "in FP64 relative to each wavefront's target box, compensated "
+                               "brute-force gravity (FP32 pair forces on coordinates formed "
+                               "in FP64 relative to each wavefront's target box, r^2 on the "
+                               "matrix cores or the VALU as autotuned, compensated "
                                "accumulation)",
                    "n": n, "softening": eps, "dt": dt, "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,

@@ -346,6 +346,11 @@ __global__ __launch_bounds__(kWave) void ens_wave_kernel(
 }
 
 // lx += J_x^T u, lm += J_m^T u (FP64).
+// SELF: eps2 = 0, where the self pair must be skipped (r = 0). With eps2 > 0
+// it contributes exactly zero -- v = m_k u_k - m_k u_k is a difference of
+// identical products and every other term carries d = 0 -- so no per-pair
+// branch is taken.
+template <bool SELF>
 __global__ __launch_bounds__(kEnsBlock) void ens_vjp_kernel(
     const EnsBlock* __restrict__ blocks, const double* __restrict__ x,
     const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ m,
@@ -374,7 +379,7 @@ __global__ __launch_bounds__(kEnsBlock) void ens_vjp_kernel(
         __syncthreads();
         const int cnt = min(kEnsBlock, b.j1 - jb);
         for (int q = 0; q < cnt; ++q) {
-            if (jb + q == k) continue;
+            if (SELF && jb + q == k) continue;
             const double4 s = tp[q];
             const double4 uj = tu[q];
             const double dx = s.x - px, dy = s.y - py, dz = s.z - pz;
@@ -388,10 +393,10 @@ __global__ __launch_bounds__(kEnsBlock) void ens_vjp_kernel(
             const double v2 = mk * uj.z - s.w * u2;
             const double dv = __builtin_fma(dx, v0, __builtin_fma(dy, v1, dz * v2));
             const double c = qq * dv;
-            g0 += w * v0 - c * dx;
-            g1 += w * v1 - c * dy;
-            g2 += w * v2 - c * dz;
-            gm -= w * __builtin_fma(uj.x, dx, __builtin_fma(uj.y, dy, uj.z * dz));
+            g0 = __builtin_fma(w, v0, __builtin_fma(-c, dx, g0));
+            g1 = __builtin_fma(w, v1, __builtin_fma(-c, dy, g1));
+            g2 = __builtin_fma(w, v2, __builtin_fma(-c, dz, g2));
+            gm = __builtin_fma(-w, __builtin_fma(uj.x, dx, __builtin_fma(uj.y, dy, uj.z * dz)), gm);
         }
     }
     if (active) {
@@ -746,8 +751,12 @@ gravis_status ensemble_adjoint(Ctx* c, const gravis_solver* s, int nsys, const u
     // top: K2 of the last step at x_T
     if (T > 0) {
         adj_level_kernel<<<eg, 256, 0, S>>>(n, lx, ly, lz, lv, 0.0, h, u);
-        ens_vjp_kernel<<<nb, kEnsBlock, 0, S>>>(d_blocks, a.x, a.y, a.z, m, u, u + n, u + 2 * n,
-                                                eps2, lx, ly, lz, lm);
+        if (eps2 > 0.0)
+            ens_vjp_kernel<false><<<nb, kEnsBlock, 0, S>>>(d_blocks, a.x, a.y, a.z, m, u, u + n,
+                                                           u + 2 * n, eps2, lx, ly, lz, lm);
+        else
+            ens_vjp_kernel<true><<<nb, kEnsBlock, 0, S>>>(d_blocks, a.x, a.y, a.z, m, u, u + n,
+                                                          u + 2 * n, eps2, lx, ly, lz, lm);
         GRAVIS_TRY(check_launch(c, "adj top"));
     }
     for (size_t sg = nseg; sg-- > 0;) {
@@ -763,8 +772,14 @@ gravis_status ensemble_adjoint(Ctx* c, const gravis_solver* s, int nsys, const u
         for (size_t t = t1; t-- > t0;) {
             const double* xt = traj + (t - t0) * 3 * n;
             adj_level_kernel<<<eg, 256, 0, S>>>(n, lx, ly, lz, lv, dt, t > 0 ? 2.0 * h : h, u);
-            ens_vjp_kernel<<<nb, kEnsBlock, 0, S>>>(d_blocks, xt, xt + n, xt + 2 * n, m, u, u + n,
-                                                    u + 2 * n, eps2, lx, ly, lz, lm);
+            if (eps2 > 0.0)
+                ens_vjp_kernel<false><<<nb, kEnsBlock, 0, S>>>(d_blocks, xt, xt + n, xt + 2 * n,
+                                                               m, u, u + n, u + 2 * n, eps2, lx,
+                                                               ly, lz, lm);
+            else
+                ens_vjp_kernel<true><<<nb, kEnsBlock, 0, S>>>(d_blocks, xt, xt + n, xt + 2 * n,
+                                                              m, u, u + n, u + 2 * n, eps2, lx,
+                                                              ly, lz, lm);
         }
         GRAVIS_TRY(check_launch(c, "adj segment"));
     }
