@@ -45,7 +45,36 @@ __device__ __forceinline__ uint32_t digit_of(uint64_t key, int shift) {
     return static_cast<uint32_t>((key >> shift) & 7ull);
 }
 
-// Number of non-empty children of each node of level L.
+// Child runs of one node, 8 lanes per node (lane d = digit d; the node's
+// lanes are 8 consecutive, 8-aligned lanes of one wavefront): lane d finds
+// by binary search the first particle of the node whose next 3-bit digit is
+// >= d, so the 8 searches run side by side instead of one after another.
+// Returns this lane's run [start, stop) (empty when the digit is absent) and
+// the node's bit mask of non-empty digits. split = false: no children.
+struct NodeRun {
+    uint32_t start, stop, bits;
+};
+__device__ __forceinline__ NodeRun node_split(const uint64_t* __restrict__ keys, uint4 md,
+                                              int shift, bool split, uint32_t d, int lane) {
+    const uint32_t end = md.x + md.y;
+    uint32_t lo = md.x, hi = end;
+    if (split && d > 0) {
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (digit_of(keys[mid], shift) >= d) hi = mid;
+            else lo = mid + 1;
+        }
+    }
+    const uint32_t nxt = __shfl_down(lo, 1);
+    const uint32_t stop = d == 7 ? end : nxt;
+    const bool ne = split && stop > lo;
+    const uint64_t bal = __ballot(ne);
+    return {lo, ne ? stop : lo, (uint32_t)(bal >> (lane & ~7)) & 0xFFu};
+}
+
+// Number of non-empty children of each node of level L (8 lanes per node;
+// the grid-stride loop is uniform across each block, so the wavefront-wide
+// shuffles and ballots see every lane).
 __global__ __launch_bounds__(kTreeBlock) void tree_count_kernel(const uint64_t* __restrict__ keys,
                                                                const uint4* __restrict__ meta,
                                                                const uint32_t* __restrict__ lvl,
@@ -53,26 +82,17 @@ __global__ __launch_bounds__(kTreeBlock) void tree_count_kernel(const uint64_t* 
                                                                uint32_t* __restrict__ nch) {
     const uint32_t b = lvl[2 * L], e = lvl[2 * L + 1];
     const int shift = 3 * (kMaxLevel - (L + 1));
-    for (uint32_t v = b + blockIdx.x * kTreeBlock + threadIdx.x; v < e;
-         v += gridDim.x * kTreeBlock) {
-        const uint4 md = meta[v];
-        uint32_t c = 0;
-        if (md.y > leaf && L < kMaxLevel) {
-            uint32_t pos = md.x;
-            const uint32_t end = md.x + md.y;
-            while (pos < end) {
-                const uint32_t d = digit_of(keys[pos], shift);
-                uint32_t lo = pos + 1, hi = end;  // first index with a larger digit
-                while (lo < hi) {
-                    const uint32_t mid = lo + (hi - lo) / 2;
-                    if (digit_of(keys[mid], shift) > d) hi = mid;
-                    else lo = mid + 1;
-                }
-                ++c;
-                pos = lo;
-            }
-        }
-        nch[v - b] = c;
+    const uint32_t total = 8u * (e - b);
+    const int lane = threadIdx.x & (kWave - 1);
+    for (uint32_t t0 = blockIdx.x * kTreeBlock; t0 < total; t0 += gridDim.x * kTreeBlock) {
+        const uint32_t t = t0 + threadIdx.x;
+        const bool valid = t < total;
+        const uint32_t v = b + (valid ? t / 8u : 0u);
+        const uint32_t d = t & 7u;
+        const uint4 md = valid ? meta[v] : make_uint4(0u, 0u, 0u, 0u);
+        const bool split = valid && md.y > leaf && L < kMaxLevel;
+        const NodeRun r = node_split(keys, md, shift, split, d, lane);
+        if (valid && d == 0) nch[v - b] = (uint32_t)__popc(r.bits);
     }
 }
 
@@ -152,32 +172,32 @@ __global__ __launch_bounds__(kTreeBlock) void tree_write_kernel(
     uint32_t gsize, uint32_t* __restrict__ flag) {
     const uint32_t b = lvl[2 * L], e = lvl[2 * L + 1];
     const int shift = 3 * (kMaxLevel - (L + 1));
-    for (uint32_t v = b + blockIdx.x * kTreeBlock + threadIdx.x; v < e;
-         v += gridDim.x * kTreeBlock) {
-        const uint32_t nc = nch[v - b];
+    const uint32_t total = 8u * (e - b);
+    const int lane = threadIdx.x & (kWave - 1);
+    for (uint32_t t0 = blockIdx.x * kTreeBlock; t0 < total; t0 += gridDim.x * kTreeBlock) {
+        const uint32_t t = t0 + threadIdx.x;
+        const bool valid = t < total;
+        const uint32_t v = b + (valid ? t / 8u : 0u);
+        const uint32_t d = t & 7u;
+        const uint32_t nc = valid ? nch[v - b] : 0u;
+        const uint4 md = valid ? meta[v] : make_uint4(0u, 0u, 0u, 0u);
+        const uint32_t c0 = e + (valid ? off[v - b] : 0u);
+        const bool fits = c0 + nc <= cap;
+        // every lane of the node reads meta[v] above before lane 0 rewrites it
+        const NodeRun r = node_split(keys, md, shift, nc > 0 && fits, d, lane);
         if (nc == 0) continue;
-        const uint4 md = meta[v];
-        const uint32_t c0 = e + off[v - b];
-        if (c0 + nc > cap) {
-            atomicOr(flag, 2u);
+        if (!fits) {
+            if (d == 0) atomicOr(flag, 2u);
             continue;
         }
-        uint32_t pos = md.x, k = 0;
-        const uint32_t end = md.x + md.y;
-        while (pos < end) {
-            const uint32_t d = digit_of(keys[pos], shift);
-            uint32_t lo = pos + 1, hi = end;
-            while (lo < hi) {
-                const uint32_t mid = lo + (hi - lo) / 2;
-                if (digit_of(keys[mid], shift) > d) hi = mid;
-                else lo = mid + 1;
-            }
-            meta[c0 + k] = make_uint4(pos, lo - pos, 0u,
+        // children in digit order: this run's rank among the non-empty digits
+        if (r.stop > r.start) {
+            const uint32_t k = (uint32_t)__popc(r.bits & ((1u << d) - 1u));
+            meta[c0 + k] = make_uint4(r.start, r.stop - r.start, 0u,
                                       (uint32_t)(L + 1) | (md.y > gsize ? kGroupRoot : 0u));
-            ++k;
-            pos = lo;
         }
-        meta[v] = make_uint4(md.x, md.y, c0, (md.w & kGroupRoot) | (nc << 8) | (uint32_t)L);
+        if (d == 0)
+            meta[v] = make_uint4(md.x, md.y, c0, (md.w & kGroupRoot) | (nc << 8) | (uint32_t)L);
     }
 }
 
