@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--bh-n-each", type=int, default=2097152)
     ap.add_argument("--bh-steps", type=int, default=20)
     ap.add_argument("--no-bh", action="store_true")
+    ap.add_argument("--bh-group", type=int, default=0,
+                    help="Barnes-Hut target group size (1..64) instead of autotuning "
+                         "(profiling runs)")
     ap.add_argument("--ens-steps", type=int, default=200)
     ap.add_argument("--ens-adj-steps", type=int, default=64)
     ap.add_argument("--no-ensemble", action="store_true")
@@ -236,9 +239,13 @@ def run_bh(args, g, torch, ctx, precision="mixed", steps=None):
     for f in ("ax", "ay", "az"):
         T[f] = torch.zeros(n, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
-    # group size 0: the context's, chosen by the autotuner on the mixed run
+    # group size 0: the context's, chosen by the autotuner (or pinned)
     bh = g.BarnesHut(0.01, 0.5, 16, 0, precision=precision, ctx=ctx)
-    log(f"bench: autotuned BH ({precision}): {ctx.autotune(bh, p)}")
+    if args.bh_group:
+        ctx.set_tuning("bh_group_size", args.bh_group)
+        log(f"bench: BH group size pinned to {args.bh_group}")
+    else:
+        log(f"bench: autotuned BH ({precision}): {ctx.autotune(bh, p)}")
     group = ctx.tuning()["bh_group_size"]
     L = ctx._L
     keys = ("x", "y", "z", "vx", "vy", "vz", "mass", "ax", "ay", "az")
@@ -251,25 +258,49 @@ def run_bh(args, g, torch, ctx, precision="mixed", steps=None):
     ctx.check(L.gravis_accel_device(ctx.handle, C.byref(d), n,
                                     *[C.c_void_p(T[k].data_ptr()) for k in
                                       ("x", "y", "z", "mass", "ax", "ay", "az")]))
-    kdk(2)  # warmup
-    torch.cuda.synchronize()
-    log(f"bench: timing {steps} BH KDK steps ({precision})")
-    t0 = time.perf_counter()
-    for _ in range(steps):
+
+    def measure(nsteps):
+        """Two warm-up steps, nsteps timed KDK steps, then one step with the
+        walk's HIP-event timer: step time, walk time, walk counters and the
+        walk's algorithmic bytes."""
+        kdk(2)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            kdk(1)
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / nsteps
+        ctx.set_profiling(True)
         kdk(1)
-    torch.cuda.synchronize()
-    t_step = (time.perf_counter() - t0) / steps
-    ctx.set_profiling(True)
-    kdk(1)
-    ctx.set_profiling(False)
-    ms, launches = ctx.kernel_time("bh_walk")
-    walk_s = ms / 1e3 / max(1, launches)
-    st = ctx.bh_stats()
-    # Algorithmic bytes of one walk: per frontier node test a 16 B node
-    # record + 32 B (com, mass); per accepted node and per leaf-particle
-    # source a 32 B record read by the wave; per target 32 B in + 24 B out.
-    walk_bytes = st["node_tests"] * 48 + (st["accepted_nodes"] + st["leaf_particles"]) * 32 \
-        + n * 56
+        ctx.set_profiling(False)
+        ms, launches = ctx.kernel_time("bh_walk")
+        ws = ms / 1e3 / max(1, launches)
+        st = ctx.bh_stats()
+        # Algorithmic bytes of one walk: per frontier node test a 16 B node
+        # record + 32 B (com, mass); per accepted node and per leaf-particle
+        # source a 32 B record read by the wave; per target 32 B in + 24 B
+        # out. A larger group shares one list among more targets, so it
+        # moves fewer bytes per step.
+        wb = st["node_tests"] * 48 + (st["accepted_nodes"] + st["leaf_particles"]) * 32 + n * 56
+        return t, ws, st, wb
+
+    log(f"bench: timing {steps} BH KDK steps ({precision})")
+    t_step, walk_s, st, walk_bytes = measure(steps)
+    # the other group sizes on the same run, for comparison (the autotuner's
+    # choice stays the reported one)
+    others = []
+    if precision == "mixed" and not args.bh_group:
+        for gs in (32, 64):
+            if gs == group:
+                continue
+            ctx.set_tuning("bh_group_size", gs)
+            log(f"bench: BH group size {gs} for comparison")
+            t2, w2, _, b2 = measure(max(5, steps // 2))
+            others.append({"group_size": gs, "ms_per_step": round(t2 * 1e3, 3),
+                           "walk_kernel_ms": round(w2 * 1e3, 3),
+                           "walk_algorithmic_GBps": round(b2 / w2 / 1e9, 1),
+                           "walk_frac_hbm_peak": round(b2 / w2 / 8.0e12, 4)})
+        ctx.set_tuning("bh_group_size", group)
     cpu = None
     if precision == "mixed" and not args.no_cpu_baseline:
         log(f"bench: CPU baseline, reference Barnes-Hut on {n} particles")
@@ -284,6 +315,7 @@ def run_bh(args, g, torch, ctx, precision="mixed", steps=None):
             "walk_achieved_tflops": round(FLOP_PER_INTERACTION * st["pairs"] / walk_s / 1e12, 3),
             "walk_algorithmic_GBps": round(walk_bytes / walk_s / 1e9, 1),
             "walk_frac_hbm_peak": round(walk_bytes / walk_s / 8.0e12, 4),
+            "other_group_sizes": others,
             "workload": "two Plummer spheres (seeds 42/43, a=1, M=1 each) 10a apart, parabolic "
                         "approach, KDK with Barnes-Hut theta=0.5, leaf 16"}
 
@@ -679,6 +711,31 @@ def main():
     ms64, l64 = ctx.kernel_time("bf_fp64")
     k_fp64_s = ms64 / 1e3 / max(1, l64)
 
+    # The vector-ALU kernel (no matrix cores) at its usual fastest shapes on
+    # the same state, beside the matrix-core kernel the autotuner picked.
+    valu = None
+    if tuning["bf_matrix"]:
+        log("bench: VALU-only mixed kernel shapes")
+        best = None
+        for key in ("2,128,256", "2,128,512", "1,128,256"):
+            set_shape(ctx, key)
+            accel(mixed, am)
+            torch.cuda.synchronize()
+            ctx.set_profiling(True)
+            for _ in range(3):
+                accel(mixed, am)
+            ctx.set_profiling(False)
+            ms, launches = ctx.kernel_time("bf_mixed")
+            k_s = ms / 1e3 / max(1, launches)
+            if best is None or k_s < best[1]:
+                best = (key, k_s)
+        set_shape(ctx, shape_key(tuning))
+        valu = {"kernel": "bf_mixed_kernel", "shape": best[0],
+                "kernel_ms": round(best[1] * 1e3, 4),
+                "G_interactions_per_s": round(inter / best[1] / 1e9, 3),
+                "frac_fp32_peak": round(FLOP_PER_INTERACTION * inter / best[1] / 1e12
+                                        / PEAK_FP32_TFLOPS, 4)}
+
     cpu = None
     if not args.no_cpu_baseline:
         log(f"bench: CPU baseline on {args.cpu_sample} particles")
@@ -739,6 +796,7 @@ def main():
                  "frac_fp64_peak": round(FLOP_PER_INTERACTION * inter / k_fp64_s / 1e12
                                          / PEAK_FP64_TFLOPS, 4)},
         "mixed_over_fp64": round(t_fp64 / t_mixed, 3),
+        "valu_kernel": valu,
         "force_error_vs_fp64": {"rms": err_rms, "max": err_max},
         "c1": c1,
         "barnes_hut": bh,

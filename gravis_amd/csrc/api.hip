@@ -658,16 +658,16 @@ gravis_status gravis_autotune(gravis_ctx* g, const gravis_solver* s, size_t n, c
     fa.ax = base + 4 * n; fa.ay = base + 5 * n; fa.az = base + 6 * n;
     GRAVIS_HIP(c, hipMemsetAsync(fa.ax, 0, 3 * n * sizeof(double), c->stream));
 
-    // one candidate: warm-up call, then the mean of two timed calls
-    auto time_it = [&](double* ms) -> gravis_status {
+    // one candidate: warm-up call, then the mean of `reps` timed calls
+    auto time_it = [&](double* ms, int reps) -> gravis_status {
         GRAVIS_TRY(force_device(c, s, n, dx, dy, dz, dm, fa));
         GRAVIS_HIP(c, hipEventRecord(c->ev0, c->stream));
-        for (int r = 0; r < 2; ++r) GRAVIS_TRY(force_device(c, s, n, dx, dy, dz, dm, fa));
+        for (int r = 0; r < reps; ++r) GRAVIS_TRY(force_device(c, s, n, dx, dy, dz, dm, fa));
         GRAVIS_HIP(c, hipEventRecord(c->ev1, c->stream));
         GRAVIS_HIP(c, hipEventSynchronize(c->ev1));
         float t = 0.f;
         GRAVIS_HIP(c, hipEventElapsedTime(&t, c->ev0, c->ev1));
-        *ms = 0.5 * (double)t;
+        *ms = (double)t / reps;
         return GRAVIS_OK;
     };
     const bool profiling = c->profiling;
@@ -703,7 +703,7 @@ gravis_status gravis_autotune(gravis_ctx* g, const gravis_solver* s, size_t n, c
                     }
                     c->bf_waves_per_cu = wpc;
                     double ms;
-                    st = time_it(&ms);
+                    st = time_it(&ms, 2);
                     if (st != GRAVIS_OK) break;
                     v.push_back({c->bf_ipt, c->bf_tile, wpc, c->bf_matrix, ms});
                 }
@@ -714,13 +714,17 @@ gravis_status gravis_autotune(gravis_ctx* g, const gravis_solver* s, size_t n, c
             best = b.ms;
         }
     } else if (s->algorithm == GRAVIS_BARNES_HUT && s->group_size == 0) {
-        // 32 first (measured fastest over many C3-class steps), then 64 and
-        // 16: another is kept only when it is more than 2 % faster
+        // 64 first (measured fastest over many C3-class steps), then 32 and
+        // 16: another is kept only when it is more than 2 % faster. The walk
+        // takes groups from a shared queue and a group whose frontier
+        // overflows is walked one target at a time, so one call's time varies
+        // with the order the groups are taken in (64-target groups: up to
+        // 25 % call to call); four timed calls per size.
         std::vector<Shape> v;
-        for (uint32_t gs : {32u, 64u, 16u}) {
+        for (uint32_t gs : {64u, 32u, 16u}) {
             c->bh_group = gs;
             double ms;
-            st = time_it(&ms);
+            st = time_it(&ms, 4);
             if (st != GRAVIS_OK) break;
             v.push_back({(int)gs, 0, 0, 0, ms});
         }
@@ -730,7 +734,7 @@ gravis_status gravis_autotune(gravis_ctx* g, const gravis_solver* s, size_t n, c
             best = b.ms;
         }
     } else {
-        st = time_it(&best);
+        st = time_it(&best, 2);
     }
     c->profiling = profiling;
     GRAVIS_TRY(st);

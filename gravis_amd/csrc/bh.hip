@@ -21,12 +21,14 @@
 // stages 64 source records in LDS (one load per lane) that all 64 lanes then
 // read back as broadcasts. A group of at most 32 (16) targets is held by 2
 // (4) lanes per target: copy r sums the list positions p with p % R == r,
-// in order, and the copies are added as (r0 + r1) [+ (r2 + r3)]. Pair
+// in order (the mixed walk: a contiguous block of each round of 64), and
+// the copies are added as (r0 + r1) [+ (r2 + r3)]. Pair
 // arithmetic is barnes_hut.rs:114-135 in FP64 (sqrt and divide correctly
 // rounded, no contraction); a source at r^2 = 0 is skipped. The oracle's
 // group walk restates the list positions (64-node chunks, 1024-entry list)
 // and the copies, so results are bit-for-bit the same with the same groups.
 #include <algorithm>
+#include <type_traits>
 
 #include "internal.h"
 
@@ -101,7 +103,8 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
         const int cnt = (int)gr.y;
         // A group of at most 32 (16) targets is held twice (four times)
         // across the wave; replica r sums the sources at list positions
-        // p % R == r, and the replicas are added in a fixed order at the end.
+        // p % R == r (FP64) or a block of each round (mixed), and the
+        // replicas are added in a fixed order at the end.
         const int R = cnt <= 16 ? 4 : (cnt <= 32 ? 2 : 1);
         const int tpr = kWave / R;          // lanes per replica
         const int tl = lane % tpr;          // this lane's target
@@ -111,6 +114,7 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
         // softening of this lane's target: one value, or one per target
         const double eps2 = eps2_t ? eps2_t[i0 + (active ? tl : 0)] : eps2_all;
         const float eps2f = in_vgpr((float)eps2);  // full-rate FMA operand
+        const bool soft = __ballot(!(eps2f > 0.f)) == 0ull;  // wave uniform
         double glo[3] = {me.x, me.y, me.z}, ghi[3] = {me.x, me.y, me.z};
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1)
@@ -146,23 +150,43 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
                                                     (float)(r.z - gcz), (float)r.w);
                     else
                         stage[lane] = r;
+                } else if (MIXED) {
+                    // past the round's end: zero mass (adds exactly zero)
+                    stage32[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
                 }
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
                 __builtin_amdgcn_wave_barrier();
                 const uint32_t m = min(nl - q0, (uint32_t)kWave);
                 if (MIXED) {
+                    // replica rep takes the records [rep per, (rep + 1) per)
+                    // in order, four per iteration, per = ceil(m / R) rounded
+                    // up to a multiple of 4: R (1, 2 or 4) divides 64/4, so
+                    // R per <= 64, and the records past m have zero mass
+                    const uint32_t per = __builtin_amdgcn_readfirstlane(
+                        (((m + (uint32_t)R - 1) / (uint32_t)R) + 3u) & ~3u);
+                    const float4* mine = stage32 + (uint32_t)rep * per;
                     float tx = 0.f, ty = 0.f, tz = 0.f;
-                    for (uint32_t k0 = 0; k0 < m; k0 += (uint32_t)R) {
-                        const uint32_t k = k0 + (uint32_t)rep;
-                        // past the round's end: a zero-mass copy of record 0
-                        float4 r = stage32[k < m ? k : 0];
-                        r.w = k < m ? r.w : 0.f;
-                        const float dx = r.x - fx, dy = r.y - fy, dz = r.z - fz;
-                        const float f = pair_f32<true>(dx, dy, dz, r.w, eps2f);
-                        tx = __builtin_fmaf(f, dx, tx);
-                        ty = __builtin_fmaf(f, dy, ty);
-                        tz = __builtin_fmaf(f, dz, tz);
-                    }
+                    auto sweep = [&](auto mask) {
+                        constexpr bool MK = decltype(mask)::value;
+#pragma unroll 1
+                        for (uint32_t i = 0; i < per; i += 4) {
+                            float4 r[4];
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) r[j] = mine[i + (uint32_t)j];
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                const float dx = r[j].x - fx, dy = r[j].y - fy,
+                                            dz = r[j].z - fz;
+                                const float f = pair_f32<MK>(dx, dy, dz, r[j].w, eps2f);
+                                tx = __builtin_fmaf(f, dx, tx);
+                                ty = __builtin_fmaf(f, dy, ty);
+                                tz = __builtin_fmaf(f, dz, tz);
+                            }
+                        }
+                    };
+                    // every softening positive: r^2 >= eps^2 > 0, no mask
+                    if (soft) sweep(std::false_type{});
+                    else sweep(std::true_type{});
                     if (keep) {
                         float y = tx - kx, t = sx + y; kx = (t - sx) - y; sx = t;
                         y = ty - ky; t = sy + y; ky = (t - sy) - y; sy = t;

@@ -350,9 +350,9 @@ gravis_status gravis_get_tuning(gravis_ctx* ctx, gravis_tuning* out);
  * shapes); FP64 brute force: bf_waves_per_cu 32..512; among brute-force
  * shapes within 2 % of the fastest, the first in that order (fewest j
  * chunks, i.e. least partial-sum memory, then the VALU kernels) is kept.
- * Barnes-Hut with group_size 0: group sizes 32, 64, 16, keeping 64 or 16
+ * Barnes-Hut with group_size 0: group sizes 64, 32, 16, keeping 32 or 16
  * only when it is more than 2 % faster than the ones before it.
- * Each shape costs three force calls. Shapes change the summation grouping,
+ * Each brute-force shape costs three force calls, each group size five. Shapes change the summation grouping,
  * so forces move in their last bits (brute force) or within the
  * opening-angle error (Barnes-Hut groups). out may be NULL. */
 gravis_status gravis_autotune(gravis_ctx* ctx, const gravis_solver* s, size_t n,
