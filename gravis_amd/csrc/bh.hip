@@ -153,6 +153,13 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
                 } else if (MIXED) {
                     // past the round's end: zero mass (adds exactly zero)
                     stage32[lane] = make_float4(0.f, 0.f, 0.f, 0.f);
+                } else {
+                    // past the round's end: a zero-mass copy of the round's
+                    // record 0 (adds exactly zero: that pair is evaluated
+                    // anyway, so r^2 is non-zero and finite or skipped)
+                    double4 r = src[list[q0]];
+                    r.w = 0.0;
+                    stage[lane] = r;
                 }
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
                 __builtin_amdgcn_wave_barrier();
@@ -193,14 +200,24 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
                         y = tz - kz; t = sz + y; kz = (t - sz) - y; sz = t;
                     }
                 } else {
+                    // replica rep takes records rep, rep + R, ... in order
+                    // (the oracle's order), per = ceil(m / R) rounded up to a
+                    // multiple of kU of them: R divides 64/kU, so the last
+                    // index stays below 64, and the records past m have
+                    // zero mass
+                    constexpr uint32_t kU = 2;
+                    const uint32_t per = __builtin_amdgcn_readfirstlane(
+                        (((m + (uint32_t)R - 1) / (uint32_t)R) + kU - 1) & ~(kU - 1));
                     double tx = ax, ty = ay, tz = az;
-                    for (uint32_t k0 = 0; k0 < m; k0 += (uint32_t)R) {
-                        const uint32_t k = k0 + (uint32_t)rep;
-                        // past the round's end: a zero-mass copy of record 0
-                        // (adds exactly zero)
-                        double4 r = stage[k < m ? k : 0];
-                        r.w = k < m ? r.w : 0.0;
-                        pair_ref(r, me.x, me.y, me.z, eps2, tx, ty, tz);
+#pragma unroll 1
+                    for (uint32_t i = 0; i < per; i += kU) {
+                        double4 r[kU];
+#pragma unroll
+                        for (uint32_t j = 0; j < kU; ++j)
+                            r[j] = stage[(i + j) * (uint32_t)R + (uint32_t)rep];
+#pragma unroll
+                        for (uint32_t j = 0; j < kU; ++j)
+                            pair_ref(r[j], me.x, me.y, me.z, eps2, tx, ty, tz);
                     }
                     if (keep) {
                         ax = tx;
