@@ -272,13 +272,8 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
                     // sources of this chunk in lane order: 1 per accepted
                     // node, count per opened leaf
                     const uint32_t add = accept ? 1u : (direct ? md.y : 0u);
-                    uint32_t sc = add;
-#pragma unroll
-                    for (int o = 1; o < kWave; o <<= 1) {
-                        const uint32_t t = __shfl_up(sc, o);
-                        if (lane >= o) sc += t;
-                    }
-                    const uint32_t total = __builtin_amdgcn_readfirstlane(__shfl(sc, kWave - 1));
+                    const uint32_t sc = wave_inclusive_scan(add);
+                    const uint32_t total = __builtin_amdgcn_readlane(sc, kWave - 1);
                     const uint64_t accb = __ballot(accept);
                     t_acc += (unsigned long long)__popcll(accb);
                     t_leafp += total - (uint32_t)__popcll(accb);
@@ -309,13 +304,8 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
                     }
                     // children of opened internal nodes -> next frontier
                     const uint32_t k = open ? nchild : 0u;
-                    uint32_t kc = k;
-#pragma unroll
-                    for (int o = 1; o < kWave; o <<= 1) {
-                        const uint32_t t = __shfl_up(kc, o);
-                        if (lane >= o) kc += t;
-                    }
-                    const uint32_t ktot = __builtin_amdgcn_readfirstlane(__shfl(kc, kWave - 1));
+                    const uint32_t kc = wave_inclusive_scan(k);
+                    const uint32_t ktot = __builtin_amdgcn_readlane(kc, kWave - 1);
                     if (nn + ktot > fcap) {
                         overflow = true;
                         break;

@@ -258,6 +258,21 @@ __device__ __forceinline__ bool tile_is_near(const double* __restrict__ tb, cons
     return gap2 < lim2;
 }
 
+// Inclusive scan of one u32 per lane over the wavefront, in six DPP adds
+// (no LDS round trips): row_shr 1, 2, 4, 8 scan each row of 16 lanes
+// (sources outside the row read 0), then row_bcast 15 adds row 0's total to
+// row 1 and row 2's to row 3, and row_bcast 31 adds lanes 0..31's total to
+// rows 2 and 3. Every lane of the wavefront must be active.
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x111, 0xF, 0xF, true);   // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x112, 0xF, 0xF, true);   // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x114, 0xF, 0xF, true);   // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x118, 0xF, 0xF, true);   // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x143, 0xC, 0xF, false);  // row_bcast:31
+    return x;
+}
+
 // Exclusive scan of one u32 per thread over a block of BLOCK threads (a
 // multiple of the wavefront); *total receives the block sum. Every thread of
 // the block must call it (it synchronises twice).
@@ -266,12 +281,7 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* t
     __shared__ uint32_t wsum[BLOCK / kWave];
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
-    uint32_t inc = v;
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        const uint32_t t = __shfl_up(inc, off);
-        if (lane >= off) inc += t;
-    }
+    const uint32_t inc = wave_inclusive_scan(v);
     if (lane == kWave - 1) wsum[w] = inc;
     __syncthreads();
     uint32_t before = 0, all = 0;

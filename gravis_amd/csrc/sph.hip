@@ -79,13 +79,9 @@ __device__ __forceinline__ void lds_sync() {
 }
 
 __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v, int lane, uint32_t* total) {
-    uint32_t s = v;
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        const uint32_t t = __shfl_up(s, o);
-        if (lane >= o) s += t;
-    }
-    *total = __builtin_amdgcn_readfirstlane(__shfl(s, kWave - 1));
+    (void)lane;
+    const uint32_t s = wave_inclusive_scan(v);
+    *total = __builtin_amdgcn_readlane(s, kWave - 1);
     return s - v;
 }
 
