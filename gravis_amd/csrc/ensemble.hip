@@ -217,15 +217,42 @@ __global__ __launch_bounds__(256) void ens_permute_kernel(EnsArrays a, const uin
     }
 }
 
+// Bounding box of every item's records [i0, i1): box[6 w .. 6 w + 5] =
+// (lo x, y, z, hi x, y, z) of item w. Items of a system cover it in
+// consecutive 64-record blocks from j0, so a source tile of kEnsTile
+// records is the union of kEnsTile / 64 consecutive items' boxes.
+__global__ __launch_bounds__(kWave) void ens_box_kernel(const EnsBlock* __restrict__ items,
+                                                        const double* __restrict__ x,
+                                                        const double* __restrict__ y,
+                                                        const double* __restrict__ z,
+                                                        double* __restrict__ box) {
+    const EnsBlock b = items[blockIdx.x];
+    const int i = b.i0 + (int)threadIdx.x;
+    const int ii = i < b.i1 ? i : b.i0;
+    double lo[3], hi[3];
+    const double xd[1] = {x[ii]}, yd[1] = {y[ii]}, zd[1] = {z[ii]};
+    wave_box<1>(xd, yd, zd, lo, hi);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            box[(size_t)6 * blockIdx.x + d] = lo[d];
+            box[(size_t)6 * blockIdx.x + 3 + d] = hi[d];
+        }
+    }
+}
+
 // One wavefront per item: targets [i0, i1) (at most 64) of one system,
 // sources [j0, j1) = that system; assigns a, optionally kicks v by h a.
+// box: ens_box_kernel's item boxes of the same positions.
 template <bool MASK>
 __global__ __launch_bounds__(kWave) void ens_wave_kernel(
-    const EnsBlock* __restrict__ items, const double* __restrict__ x,
+    const EnsBlock* __restrict__ items, const double* __restrict__ box,
+    const double* __restrict__ x,
     const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ m,
     float eps2, double near2, double h, int kick, double* __restrict__ ax,
     double* __restrict__ ay, double* __restrict__ az, double* __restrict__ vx,
     double* __restrict__ vy, double* __restrict__ vz) {
+    static_assert(kEnsTile % kWave == 0, "a tile is whole items");
     constexpr int LPT = kEnsTile / kWave;
     __shared__ double4 tile[kEnsTile];  // near: FP64 records; far: float4 [0, kEnsTile)
     const EnsBlock b = items[blockIdx.x];
@@ -233,10 +260,13 @@ __global__ __launch_bounds__(kWave) void ens_wave_kernel(
     const int i = b.i0 + lane;
     const bool active = i < b.i1;
     const int ii = active ? i : b.i0;
+    // the system's first item (items of a system are consecutive)
+    const int first = (int)blockIdx.x - (b.i0 - b.j0) / kWave;
     double lo[3], hi[3];
-    {
-        const double xd[1] = {x[ii]}, yd[1] = {y[ii]}, zd[1] = {z[ii]};
-        wave_box<1>(xd, yd, zd, lo, hi);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = box[(size_t)6 * blockIdx.x + d];
+        hi[d] = box[(size_t)6 * blockIdx.x + 3 + d];
     }
     const double px = x[ii], py = y[ii], pz = z[ii];
     const double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]), cz = 0.5 * (lo[2] + hi[2]);
@@ -261,27 +291,25 @@ __global__ __launch_bounds__(kWave) void ens_wave_kernel(
     if (b.j0 < b.j1) load(b.j0);
     for (int jb = b.j0; jb < b.j1; jb += kEnsTile) {
         double4 rec[LPT];
-        double tb[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
 #pragma unroll
-        for (int e = 0; e < LPT; ++e) {
-            const int j = jb + e * kWave + lane;
-            const bool ok = j < b.j1;
-            rec[e] = nxt[e];
-            if (ok) {
-                tb[0] = fmin(tb[0], rec[e].x); tb[3] = fmax(tb[3], rec[e].x);
-                tb[1] = fmin(tb[1], rec[e].y); tb[4] = fmax(tb[4], rec[e].y);
-                tb[2] = fmin(tb[2], rec[e].z); tb[5] = fmax(tb[5], rec[e].z);
-            }
-        }
+        for (int e = 0; e < LPT; ++e) rec[e] = nxt[e];
         bool near = false;
         if (lim2 > 0.0) {
+            // the tile's box: union of its items' boxes (records < j1 only)
+            const int t0 = first + (jb - b.j0) / kWave;
+            double tb[6];
 #pragma unroll
-            for (int off = 32; off > 0; off >>= 1)
+            for (int q = 0; q < 6; ++q) tb[q] = box[(size_t)6 * t0 + q];
 #pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    tb[d] = fmin(tb[d], __shfl_xor(tb[d], off));
-                    tb[3 + d] = fmax(tb[3 + d], __shfl_xor(tb[3 + d], off));
+            for (int e = 1; e < LPT; ++e) {
+                if (jb + e * kWave < b.j1) {
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        tb[d] = fmin(tb[d], box[(size_t)6 * (t0 + e) + d]);
+                        tb[3 + d] = fmax(tb[3 + d], box[(size_t)6 * (t0 + e) + 3 + d]);
+                    }
                 }
+            }
             near = tile_is_near(tb, lo, hi, lim2);
         }
         near = __builtin_amdgcn_readfirstlane((int)near) != 0;
@@ -525,25 +553,30 @@ gravis_status ens_step(Ctx* c, bool mixed, const EnsBlock* d_blocks, int nblocks
 }
 
 // Mixed force on state in per-system Morton order (one wavefront per item).
-gravis_status ens_wave_force(Ctx* c, const EnsBlock* d_waves, int nwaves, const EnsArgs& a,
-                             double eps2, double h, int kick) {
+// wbox: 6 nwaves doubles (allocated by the caller, outside any capture)
+gravis_status ens_wave_force(Ctx* c, const EnsBlock* d_waves, int nwaves, double* wbox,
+                             const EnsArgs& a, double eps2, double h, int kick) {
     if (nwaves == 0) return GRAVIS_OK;
     const float e2 = (float)eps2;
     const double q2 = c->bf_near_fraction * c->bf_near_fraction;
+    ens_box_kernel<<<nwaves, kWave, 0, c->stream>>>(d_waves, a.x, a.y, a.z, wbox);
+    GRAVIS_TRY(check_launch(c, "ens_box"));
     if (!(e2 >= 1.17549435e-38f))
         ens_wave_kernel<true><<<nwaves, kWave, 0, c->stream>>>(
-            d_waves, a.x, a.y, a.z, a.m, e2, q2, h, kick, a.ax, a.ay, a.az, a.vx, a.vy, a.vz);
+            d_waves, wbox, a.x, a.y, a.z, a.m, e2, q2, h, kick, a.ax, a.ay, a.az, a.vx, a.vy,
+            a.vz);
     else
         ens_wave_kernel<false><<<nwaves, kWave, 0, c->stream>>>(
-            d_waves, a.x, a.y, a.z, a.m, e2, q2, h, kick, a.ax, a.ay, a.az, a.vx, a.vy, a.vz);
+            d_waves, wbox, a.x, a.y, a.z, a.m, e2, q2, h, kick, a.ax, a.ay, a.az, a.vx, a.vy,
+            a.vz);
     return check_launch(c, "ens_wave_force");
 }
 
-gravis_status ens_wave_step(Ctx* c, const EnsBlock* d_waves, int nwaves, size_t n,
-                            const EnsArgs& a, double eps2, double dt) {
+gravis_status ens_wave_step(Ctx* c, const EnsBlock* d_waves, int nwaves, double* wbox,
+                            size_t n, const EnsArgs& a, double eps2, double dt) {
     const double h = 0.5 * dt;
     GRAVIS_TRY(kick_drift(c, n, a.x, a.y, a.z, a.vx, a.vy, a.vz, a.ax, a.ay, a.az, h, dt));
-    return ens_wave_force(c, d_waves, nwaves, a, eps2, h, 1);
+    return ens_wave_force(c, d_waves, nwaves, wbox, a, eps2, h, 1);
 }
 
 // Permutation that sorts every system's particles by Morton key within the
@@ -619,8 +652,10 @@ gravis_status ensemble_kdk(Ctx* c, const gravis_solver* s, int nsys, const uint6
                     sorted + 9 * n};
     }
     const int nw = (int)plan.waves.size();
+    double* wbox = nullptr;
+    if (sorted_run) GRAVIS_TRY(buf(c, "ens_wbox", (size_t)6 * std::max(nw, 1), &wbox));
     auto step = [&]() -> gravis_status {
-        return sorted_run ? ens_wave_step(c, d_waves, nw, n, a, eps2, dt)
+        return sorted_run ? ens_wave_step(c, d_waves, nw, wbox, n, a, eps2, dt)
                           : ens_step(c, mixed, d_blocks, nb, n, a, eps2, dt);
     };
     // back to the caller's order (positions, velocities, accelerations)
