@@ -260,9 +260,10 @@ def run_bh(args, g, torch, ctx, precision="mixed", steps=None):
                                       ("x", "y", "z", "mass", "ax", "ay", "az")]))
 
     def measure(nsteps):
-        """Two warm-up steps, nsteps timed KDK steps, then one step with the
-        walk's HIP-event timer: step time, walk time, walk counters and the
-        walk's algorithmic bytes."""
+        """Two warm-up steps, nsteps timed KDK steps, then three steps with
+        the walk's HIP-event timer (a walk's time varies call to call with
+        the order its queue hands out groups): step time, mean walk time,
+        walk counters of the last step and the walk's algorithmic bytes."""
         kdk(2)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -271,7 +272,7 @@ def run_bh(args, g, torch, ctx, precision="mixed", steps=None):
         torch.cuda.synchronize()
         t = (time.perf_counter() - t0) / nsteps
         ctx.set_profiling(True)
-        kdk(1)
+        kdk(3)
         ctx.set_profiling(False)
         ms, launches = ctx.kernel_time("bh_walk")
         ws = ms / 1e3 / max(1, launches)
