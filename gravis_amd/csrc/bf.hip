@@ -724,10 +724,53 @@ __global__ __launch_bounds__(256) void finish_kernel(const double* __restrict__ 
     }
 }
 
+// inv[perm[k]] = k
+__global__ __launch_bounds__(256) void invert_perm_kernel(const uint32_t* __restrict__ perm,
+                                                          int n, uint32_t* __restrict__ inv) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) inv[perm[k]] = (uint32_t)k;
+}
+
+// One partial array, delivered by caller index o: the sorted position
+// inv[o] is read at random and the caller's arrays are written (and, for
+// the kick, read) in order, where finish_kernel scatters six arrays.
+__global__ __launch_bounds__(256) void finish_inv_kernel(const double* __restrict__ part, int n,
+                                                         const uint32_t* __restrict__ inv,
+                                                         FinishArgs fa) {
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n) return;
+    const size_t k = inv[o];
+    // 0 + p, as finish_kernel's sum over one chunk forms it (-0 becomes +0)
+    const double a0 = 0.0 + part[k], a1 = 0.0 + part[(size_t)n + k],
+                 a2 = 0.0 + part[2 * (size_t)n + k];
+    if (fa.mode == kAccumulate) {
+        fa.ax[o] += a0;
+        fa.ay[o] += a1;
+        fa.az[o] += a2;
+    } else {
+        fa.ax[o] = a0;
+        fa.ay[o] = a1;
+        fa.az[o] = a2;
+        fa.vx[o] = fa.vx[o] + a0 * fa.half_dt;
+        fa.vy[o] = fa.vy[o] + a1 * fa.half_dt;
+        fa.vz[o] = fa.vz[o] + a2 * fa.half_dt;
+    }
+}
+
 }  // namespace
 
 gravis_status finish_forces(Ctx* c, const double* part, int nchunks, int n,
                             const uint32_t* perm, const FinishArgs& fa) {
+    if (perm && nchunks == 1 && n > 0) {
+        // one partial array: deliver in the caller's order through the
+        // inverse permutation (same values, same arithmetic per particle)
+        uint32_t* inv;
+        GRAVIS_TRY(buf(c, "finish_inv", (size_t)n, &inv));
+        invert_perm_kernel<<<grid_for((size_t)n, 256), 256, 0, c->stream>>>(perm, n, inv);
+        GRAVIS_TRY(check_launch(c, "invert_perm"));
+        finish_inv_kernel<<<grid_for((size_t)n, 256), 256, 0, c->stream>>>(part, n, inv, fa);
+        return check_launch(c, "finish_forces");
+    }
     finish_kernel<<<grid_for((size_t)n, 256), 256, 0, c->stream>>>(part, nchunks, n, perm, fa);
     return check_launch(c, "finish_forces");
 }
