@@ -61,7 +61,7 @@ typedef struct gravis_solver {
     double theta;        /* Barnes-Hut opening angle >= 0 */
     uint32_t leaf_size;  /* Barnes-Hut leaf capacity (0 = 16) */
     uint32_t group_size; /* Barnes-Hut target group, at most 64 (0 = the
-                            context's bh_group_size, default 64) */
+                            context's bh_group_size, default 32) */
 } gravis_solver;
 
 /* Diagnostics (crates/sim-core/src/diagnostics.rs:5-30). */
