@@ -66,8 +66,9 @@ __device__ __forceinline__ void pair_ref(const double4 s, double px, double py, 
 // centre (separations formed in FP64 first), pair forces are FP32 (v_rsq_f32)
 // and every round's FP32 sum is Kahan-added; the opening test, tree sums and
 // output stay FP64.
-// MIXED fits 96 VGPRs (5 wavefronts per SIMD; the FP64 walk needs more and
-// runs 4)
+// MIXED is capped at 96 VGPRs (5 wavefronts per SIMD; two 4-byte values
+// spill to scratch, touched once per group, outside the sweep loops); the
+// FP64 walk uses 127 and runs 4
 template <bool MIXED>
 __global__ __launch_bounds__(kWalkWaves* kWave)
 __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
