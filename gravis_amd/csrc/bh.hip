@@ -67,7 +67,7 @@ __device__ __forceinline__ void pair_ref(const double4 s, double px, double py, 
 // and every round's FP32 sum is Kahan-added; the opening test, tree sums and
 // output stay FP64.
 // MIXED is capped at 96 VGPRs (5 wavefronts per SIMD; two 4-byte values
-// spill to scratch, touched once per group, outside the sweep loops); the
+// spill to scratch, reloaded outside the pair sweep loops); the
 // FP64 walk uses 127 and runs 4
 template <bool MIXED>
 __global__ __launch_bounds__(kWalkWaves* kWave)
