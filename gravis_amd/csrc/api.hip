@@ -80,10 +80,6 @@ void timer_end(Ctx* c, const char* name) {
 using gravis::Ctx;
 using gravis::fail;
 
-struct gravis_ctx {
-    Ctx c;
-};
-
 namespace {
 
 gravis_status check_solver(Ctx* c, const gravis_solver* s) {
@@ -588,6 +584,12 @@ gravis_status gravis_set_tuning(gravis_ctx* g, const char* key, int64_t value) {
         c->sph_range_cap = (uint32_t)value;
         return GRAVIS_OK;
     }
+    if (std::strcmp(key, "sph_pool_init") == 0) {
+        if (value < 1 || value > 4096)
+            return fail(c, GRAVIS_INVALID_ARG, "sph_pool_init must be in [1, 4096]");
+        c->sph_pool_init = (uint32_t)value;
+        return GRAVIS_OK;
+    }
     if (std::strcmp(key, "bf_ipt") == 0) {
         if (value != 1 && value != 2)
             return fail(c, GRAVIS_INVALID_ARG, "bf_ipt must be 1 or 2");
@@ -672,6 +674,10 @@ gravis_status gravis_autotune(gravis_ctx* g, const gravis_solver* s, size_t n, c
     };
     const bool profiling = c->profiling;
     c->profiling = false;  // the timed calls must not touch the caller's kernel timers
+    // the shapes in force before the sweep, restored if a candidate fails
+    const int s_ipt = c->bf_ipt, s_tile = c->bf_tile, s_wpc = c->bf_waves_per_cu,
+              s_matrix = c->bf_matrix;
+    const uint32_t s_group = c->bh_group;
     double best = -1.0;
     gravis_status st = GRAVIS_OK;
     // Brute force: a finer j split balances better but writes more chunk
@@ -737,7 +743,11 @@ gravis_status gravis_autotune(gravis_ctx* g, const gravis_solver* s, size_t n, c
         st = time_it(&best, 2);
     }
     c->profiling = profiling;
-    GRAVIS_TRY(st);
+    if (st != GRAVIS_OK) {
+        c->bf_ipt = s_ipt; c->bf_tile = s_tile; c->bf_waves_per_cu = s_wpc;
+        c->bf_matrix = s_matrix; c->bh_group = s_group;
+        return st;
+    }
     if (out) {
         GRAVIS_TRY(gravis_get_tuning(g, out));
         out->best_ms = best;

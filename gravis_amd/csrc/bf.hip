@@ -31,12 +31,18 @@
 namespace gravis {
 namespace {
 
+// MASK gives 0 for d = 0 (whatever eps is); the unmasked form needs
+// eps^2 >= kMaskEps2D, where the self pair's m eps^-3 is finite.
 template <bool MASK>
 __device__ __forceinline__ double pair_f64(double dx, double dy, double dz, double m,
                                            double eps2, const Rinv3Consts& k3) {
+    if (MASK) {
+        const double d2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, dz * dz));
+        const double f = m_rinv3(d2 + eps2, m, k3);
+        return d2 > 0.0 ? f : 0.0;
+    }
     const double r2 = __builtin_fma(dx, dx, __builtin_fma(dy, dy, __builtin_fma(dz, dz, eps2)));
-    const double f = m_rinv3(r2, m, k3);
-    return MASK ? (r2 > 0.0 ? f : 0.0) : f;
+    return m_rinv3(r2, m, k3);
 }
 
 // Bounding box of each TJ-wide tile of the sorted records: tbox[kTileBox t
@@ -223,12 +229,13 @@ __global__ __launch_bounds__(kWave) void bf_mixed_kernel(const double4* __restri
                     const float dx = (float)(s.x - px[k]);
                     const float dy = (float)(s.y - py[k]);
                     const float dz = (float)(s.z - pz[k]);
-                    const float r2 =
-                        __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, e2)));
+                    const float d2 = MASK ? __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)) : 0.f;
+                    const float r2 = MASK ? d2 + e2
+                                          : __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, e2)));
                     float ri = __builtin_amdgcn_rsqf(r2);
                     const float h = (r2 * ri) * ri;
                     ri = ri * __builtin_fmaf(-0.5f, h, 1.5f);
-                    if (MASK) ri = r2 > 0.0f ? ri : 0.0f;
+                    if (MASK) ri = d2 > 0.0f ? ri : 0.0f;
                     const float f = (m * ri) * (ri * ri);
                     ax[k] = __builtin_fmaf(f, dx, ax[k]);
                     ay[k] = __builtin_fmaf(f, dy, ay[k]);
@@ -545,12 +552,13 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
                 const float dx = (float)(sj.x - p.x);
                 const float dy = (float)(sj.y - p.y);
                 const float dz = (float)(sj.z - p.z);
-                const float r2 =
-                    __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+                const float d2 = MASK ? __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)) : 0.f;
+                const float r2 = MASK ? d2 + eps2
+                                      : __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
                 float ri = __builtin_amdgcn_rsqf(r2);
                 const float h = (r2 * ri) * ri;
                 ri = ri * __builtin_fmaf(-0.5f, h, 1.5f);
-                if (MASK) ri = r2 > 0.0f ? ri : 0.0f;
+                if (MASK) ri = d2 > 0.0f ? ri : 0.0f;
                 const float f = (m * ri) * (ri * ri);
                 nx = __builtin_fmaf(f, dx, nx);
                 ny = __builtin_fmaf(f, dy, ny);
@@ -833,7 +841,7 @@ gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
     timer_begin(c);
     if (mixed) {
         const float eps2f = (float)eps2;
-        const bool mask = !(eps2f >= 1.17549435e-38f);
+        const bool mask = !(eps2f >= kMaskEps2F);
         const double q = c->bf_near_fraction;
         if (matrix) {
             // q' with (1 + 1/q')^2 = 2 (1 + 1/q): the same per-pair bound
@@ -865,7 +873,7 @@ gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
             launch_mixed<2, 256>(grid, c->stream, mask, pm, ni, jchunk, eps2f, f2, tbox, part);
         timer_end(c, "bf_mixed");
     } else {
-        const bool mask = !(eps2 > 0.0);
+        const bool mask = !(eps2 >= kMaskEps2D);
         launch_fp64<256, 1, 256>(grid, c->stream, mask, pm, ni, jchunk, eps2, part);
         timer_end(c, "bf_fp64");
     }

@@ -115,7 +115,7 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
         // softening of this lane's target: one value, or one per target
         const double eps2 = eps2_t ? eps2_t[i0 + (active ? tl : 0)] : eps2_all;
         const float eps2f = in_vgpr((float)eps2);  // full-rate FMA operand
-        const bool soft = __ballot(!(eps2f > 0.f)) == 0ull;  // wave uniform
+        const bool soft = __ballot(!(eps2f >= kMaskEps2F)) == 0ull;  // wave uniform
         double glo[3] = {me.x, me.y, me.z}, ghi[3] = {me.x, me.y, me.z};
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1)
@@ -192,7 +192,7 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
                             }
                         }
                     };
-                    // every softening positive: r^2 >= eps^2 > 0, no mask
+                    // every softening at least kMaskEps2F: the self pair is finite, no mask
                     if (soft) sweep(std::false_type{});
                     else sweep(std::true_type{});
                     if (keep) {

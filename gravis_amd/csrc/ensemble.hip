@@ -64,9 +64,11 @@ __global__ __launch_bounds__(kEnsBlock) void ens_force_kernel(
             for (int k = 0; k < cnt; ++k) {
                 const double4 s = tile[k];
                 const float dx = (float)(s.x - px), dy = (float)(s.y - py), dz = (float)(s.z - pz);
-                const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, e2)));
-                float ri = __builtin_amdgcn_rsqf(r2);
-                ri = r2 > 0.f ? ri : 0.f;
+                // zero separation (the self pair, coincident points) adds 0,
+                // also under a softening too small for m eps^-3 in FP32
+                const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+                float ri = __builtin_amdgcn_rsqf(d2 + e2);
+                ri = d2 > 0.f ? ri : 0.f;
                 const float f = ((float)s.w * ri) * (ri * ri);
                 tx = __builtin_fmaf(f, dx, tx);
                 ty = __builtin_fmaf(f, dy, ty);
@@ -344,12 +346,13 @@ __global__ __launch_bounds__(kWave) void ens_wave_kernel(
                 const double4 s4 = tile[k];
                 const float dx = (float)(s4.x - px), dy = (float)(s4.y - py),
                             dz = (float)(s4.z - pz);
-                const float r2 =
-                    __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+                const float d2 = MASK ? __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)) : 0.f;
+                const float r2 = MASK ? d2 + eps2
+                                      : __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
                 float ri = __builtin_amdgcn_rsqf(r2);
                 const float hh = (r2 * ri) * ri;
                 ri = ri * __builtin_fmaf(-0.5f, hh, 1.5f);
-                if (MASK) ri = r2 > 0.0f ? ri : 0.0f;
+                if (MASK) ri = d2 > 0.0f ? ri : 0.0f;
                 const float f = ((float)s4.w * ri) * (ri * ri);
                 tx = __builtin_fmaf(f, dx, tx);
                 ty = __builtin_fmaf(f, dy, ty);
@@ -561,7 +564,7 @@ gravis_status ens_wave_force(Ctx* c, const EnsBlock* d_waves, int nwaves, double
     const double q2 = c->bf_near_fraction * c->bf_near_fraction;
     ens_box_kernel<<<nwaves, kWave, 0, c->stream>>>(d_waves, a.x, a.y, a.z, wbox);
     GRAVIS_TRY(check_launch(c, "ens_box"));
-    if (!(e2 >= 1.17549435e-38f))
+    if (!(e2 >= kMaskEps2F))
         ens_wave_kernel<true><<<nwaves, kWave, 0, c->stream>>>(
             d_waves, wbox, a.x, a.y, a.z, a.m, e2, q2, h, kick, a.ax, a.ay, a.az, a.vx, a.vy,
             a.vz);

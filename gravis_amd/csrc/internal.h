@@ -53,6 +53,9 @@ struct Ctx {
     // SPH neighbour search: overlapping leaves one wave may hold in LDS
     // before its group is searched depth first (at most 512)
     uint32_t sph_range_cap = 512;
+    // initial neighbour-pool capacity in entries per particle (the pool
+    // grows past it, keeping what is written, when a query round runs out)
+    uint32_t sph_pool_init = 96;
     // Brute-force launch shape (gravis_set_tuning / gravis_autotune):
     // targets per lane of the mixed kernel (wavefront groups of 64 or 128),
     // j records per staged tile (128 or 256), and the wavefronts per CU the
@@ -198,12 +201,28 @@ __device__ __forceinline__ double m_rinv3(double r2, double m, const Rinv3Consts
     return __builtin_fma(z * e, p, z);
 }
 
-// Pair force factor in FP32: m * r^-3 with r^2 = d.d + eps^2.
+// Softening below which the FP32 pair loops mask pairs of zero separation
+// (the self pair, coincident points). At or above it the self pair's
+// m eps^-3 <= 1e18 m stays finite in FP32 for any mass below 3e20 and adds
+// m eps^-3 * 0 = 0, so the unmasked loops need no test.
+constexpr float kMaskEps2F = 1e-12f;
+// The same for the FP64 pair loops: m eps^-3 <= 1e150 m.
+constexpr double kMaskEps2D = 1e-100;
+
+// Pair force factor in FP32: m * r^-3 with r^2 = d.d + eps^2. MASK gives 0
+// for d = 0 (whatever eps is).
 template <bool MASK>
 __device__ __forceinline__ float pair_f32(float dx, float dy, float dz, float m, float eps2) {
-    const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
-    float ri = __builtin_amdgcn_rsqf(r2);
-    if (MASK) ri = r2 > 0.0f ? ri : 0.0f;
+    float ri;
+    if (MASK) {
+        const float d2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        ri = __builtin_amdgcn_rsqf(d2 + eps2);
+        ri = d2 > 0.0f ? ri : 0.0f;
+    } else {
+        const float r2 =
+            __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
+        ri = __builtin_amdgcn_rsqf(r2);
+    }
     const float mr = m * ri;
     const float ri2 = ri * ri;
     return mr * ri2;
@@ -455,3 +474,8 @@ inline unsigned grid_for(size_t n, unsigned block, unsigned cap = 1u << 30) {
 }
 
 }  // namespace gravis
+
+// The opaque context of the C ABI.
+struct gravis_ctx {
+    gravis::Ctx c;
+};

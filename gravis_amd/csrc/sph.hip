@@ -1197,7 +1197,8 @@ gravis_status sph_compute(Ctx* c, const gravis_sph_params* P, const gravis_sph_s
     GRAVIS_TRY(buf(c, "sph_dfs", n, &dfs_groups));
     GRAVIS_TRY(buf(c, "sph_counter", 2, &counter));
     // pool: grows (keeping what is written) when a round runs out
-    size_t pool_cap = std::max<size_t>(c->bufs["sph_pool"].bytes / sizeof(uint32_t), n * 96);
+    size_t pool_cap = std::max<size_t>(c->bufs["sph_pool"].bytes / sizeof(uint32_t),
+                                       n * c->sph_pool_init);
     pool_cap = std::min<size_t>(pool_cap, 0xFFFFFFFFull);
     void* pv = nullptr;
     GRAVIS_TRY(grow_keep(c, "sph_pool", pool_cap * sizeof(uint32_t), 0, &pv));

@@ -313,7 +313,9 @@ gravis_status gravis_kernel_time(gravis_ctx* ctx, const char* name, double* ms_t
  * 16384; a group whose frontier would exceed it is walked one target at a
  * time); "sph_query_leaves" = overlapping tree leaves one wave of the SPH
  * neighbour search holds (1..512, default 512; a group needing more is
- * searched one target at a time, depth first). */
+ * searched one target at a time, depth first); "sph_pool_init" = initial
+ * neighbour-pool capacity in entries per particle (1..4096, default 96; the
+ * pool grows past it, keeping the lists written, when a round runs out). */
 gravis_status gravis_set_tuning(gravis_ctx* ctx, const char* key, int64_t value);
 
 /* Launch shapes of the context (no reference counterpart). Brute force:
@@ -354,7 +356,9 @@ gravis_status gravis_get_tuning(gravis_ctx* ctx, gravis_tuning* out);
  * only when it is more than 2 % faster than the ones before it.
  * Each brute-force shape costs three force calls, each group size five. Shapes change the summation grouping,
  * so forces move in their last bits (brute force) or within the
- * opening-angle error (Barnes-Hut groups). out may be NULL. */
+ * opening-angle error (Barnes-Hut groups). out may be NULL. When a
+ * candidate fails the call returns its error and the context keeps the
+ * shapes it had before the call. */
 gravis_status gravis_autotune(gravis_ctx* ctx, const gravis_solver* s, size_t n,
                               const double* x, const double* y, const double* z,
                               const double* m, gravis_tuning* out);

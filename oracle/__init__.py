@@ -352,6 +352,7 @@ def _sph_lib():
             "orc_cs_grad_w": (None, [_d, _d, _d, _d, _d, _dp]),
             "orc_evrard_1r": (None, [_sz, _d, _d, _d, _d, _u64, sp]),
             "orc_cold_collapse": (None, [_sz, _d, _d, _u64, _i, sp]),
+            "orc_cold_collapse_ex": (None, [_sz, _d, _d, _u64, _i, _i, sp]),
             "orc_bf_accel_eps": (None, [_sz] + [_dp] * 8),
             "orc_bh_group_accel_eps": (_u64, [_sz, _dp, _d, _u32, _u32, _u32] + [_dp] * 7 + [_i]),
             "orc_w": (_d, [_d, _d]),
@@ -360,6 +361,10 @@ def _sph_lib():
             "orc_abs_grad_w": (_d, [_d, _d]),
             "orc_ball_query": (_sz, [sp, _sz, _d, _u32p, _sz]),
             "orc_sph_compute": (None, [sp, pp, _dp, _dp, _dp, _dp, _u32p]),
+            "orc_sph_compute_targets": (None, [sp, pp, _sz, _i64p, _dp, _dp, _dp, _dp, _u32p,
+                                               _i]),
+            "orc_bf_targets_eps": (None, [_sz, _dp] + [_dp] * 4 + [_sz, _i64p] + [_dp] * 3
+                                   + [_i]),
             "orc_sph_timestep": (_d, [sp, pp, _dp, _dp]),
             "orc_sph_step": (_d, [sp, pp, _i, _d, _d, _d]),
             "orc_evrard": (None, [_sz, _d, _d, _d, _u64, sp]),
@@ -432,11 +437,13 @@ def evrard_1r(n: int, seed: int = 42, total_mass: float = 1.0, radius: float = 1
 
 
 def cold_collapse(n: int, seed: int = 42, radius: float = 1.0, total_mass: float = 1.0,
-                  gas: bool = False) -> SphState:
-    """ColdCollapse::generate (cold_collapse.rs:56-102)."""
+                  gas: bool = False, com_shift: bool = True) -> SphState:
+    """ColdCollapse::generate (cold_collapse.rs:56-102); com_shift=False
+    returns the samples before shift_to_com_frame."""
     s = SphState(n)
     st = s.c_struct()
-    _sph_lib().orc_cold_collapse(n, radius, total_mass, seed, 1 if gas else 0, C.byref(st))
+    _sph_lib().orc_cold_collapse_ex(n, radius, total_mass, seed, 1 if gas else 0,
+                                    1 if com_shift else 0, C.byref(st))
     return s
 
 
@@ -485,6 +492,36 @@ def sph_compute(s: SphState, params: SphParams | None = None):
     return out
 
 
+def sph_compute_targets(s: SphState, targets, params: SphParams | None = None,
+                        nthreads: int = 0):
+    """SphSolver::compute restricted to `targets` (indices): the targets'
+    and their neighbours' density iteration, the targets' forces. Updates
+    rho/h/P/c of the particles visited and accumulates the targets' a and
+    du/dt in s; returns per-target dict(div_v, max_mu, dalpha_dt, omega,
+    nneigh). Each value equals what sph_compute gives for that particle."""
+    params = params or SphParams.default()
+    t = np.ascontiguousarray(targets, dtype=np.int64)
+    k = len(t)
+    out = {q: np.zeros(k) for q in ("div_v", "max_mu", "dalpha_dt", "omega")}
+    nn = np.zeros(k, dtype=np.uint32)
+    st = s.c_struct()
+    _sph_lib().orc_sph_compute_targets(C.byref(st), C.byref(params), k, t, out["div_v"],
+                                       out["max_mu"], out["dalpha_dt"], out["omega"], nn,
+                                       nthreads)
+    out["nneigh"] = nn
+    return out
+
+
+def bf_targets_eps(s, eps_t, targets, nthreads: int = 0):
+    """Brute-force accelerations at `targets` with softening eps_t[k] for
+    target k (sources in index order); returns (k, 3)."""
+    t = np.ascontiguousarray(targets, dtype=np.int64)
+    a = [np.zeros(len(t)) for _ in range(3)]
+    _sph_lib().orc_bf_targets_eps(s.n, np.ascontiguousarray(eps_t, dtype=np.float64), s.x,
+                                  s.y, s.z, s.m, len(t), t, *a, nthreads)
+    return np.stack(a, 1)
+
+
 def sph_timestep(s: SphState, inter: dict, params: SphParams | None = None) -> float:
     params = params or SphParams.default()
     st = s.c_struct()
@@ -512,6 +549,7 @@ def _adj_lib():
                                 + [_dp] * 7),
             "orc_ens_counts": (None, [_u64, _sz, _u32, _u32, _u32p]),
             "orc_ens_targets": (None, [_u64, _sz, _dp, _dp, _dp]),
+            "orc_ens_set_threads": (None, [_i]),
         }
         for name, (res, args) in spec.items():
             f = getattr(L, name)
@@ -519,6 +557,12 @@ def _adj_lib():
             f.argtypes = args
         L._adj_ready = True
     return L
+
+
+def ens_set_threads(n: int) -> None:
+    """Threads for ens_forward / ens_adjoint (default 1; results do not
+    depend on it)."""
+    _adj_lib().orc_ens_set_threads(int(n))
 
 
 def ens_forward(s: State, eps: float, dt: float, nsteps: int) -> None:

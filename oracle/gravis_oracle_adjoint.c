@@ -31,8 +31,15 @@
 
 #include "gravis_oracle.h"
 
+/* Threads for the per-particle sums (1 = serial, the timed CPU baseline).
+ * Each particle's sum runs in index order on one thread, so results do not
+ * depend on the count. */
+static int ens_threads = 1;
+void orc_ens_set_threads(int t) { ens_threads = t > 0 ? t : 1; }
+
 static void accel_gather(size_t n, double eps2, const double* x, const double* y,
                          const double* z, const double* m, double* ax, double* ay, double* az) {
+#pragma omp parallel for schedule(static) num_threads(ens_threads) if (ens_threads > 1 && n >= 256)
     for (size_t i = 0; i < n; ++i) {
         double a0 = 0, a1 = 0, a2 = 0;
         for (size_t j = 0; j < n; ++j) {
@@ -51,6 +58,7 @@ static void accel_gather(size_t n, double eps2, const double* x, const double* y
 static void vjp(size_t n, double eps2, const double* x, const double* y, const double* z,
                 const double* m, const double* ux, const double* uy, const double* uz,
                 double* lx, double* ly, double* lz, double* lm) {
+#pragma omp parallel for schedule(static) num_threads(ens_threads) if (ens_threads > 1 && n >= 256)
     for (size_t k = 0; k < n; ++k) {
         double g0 = 0, g1 = 0, g2 = 0, gm = 0;
         for (size_t j = 0; j < n; ++j) {

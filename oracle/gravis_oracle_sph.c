@@ -34,6 +34,9 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define SPH_PI 3.14159265358979323846
 #define NORM_3D (21.0 / (16.0 * SPH_PI))
@@ -369,135 +372,138 @@ static void density_dwdh(const orc_sph_state* s, const ulist* nb, size_t i, doub
     *dsum = d_;
 }
 
+/* One gas particle's list, density / smoothing-length iteration
+ * (density.rs:42-104) and equation of state (:111-119). It reads the
+ * positions, the masses and its own h only, so particles can be done in
+ * any order. lr_i is the radius the list nl was built with. */
+static void density_one(orc_sph_state* s, const stree* t, const orc_sph_params* pp, size_t i,
+                        double* omega_i, ulist* nl) {
+    *omega_i = 1.0;
+    if (s->type[i] != GAS) return;
+    double lr = 2.0 * s->h[i] * 1.1;
+    ball(t, s, i, lr, nl);
+    double h = s->h[i], mi = s->m[i];
+    for (int it = 0; it < pp->max_h_iter; ++it) {
+        double rho, dsum;
+        density_dwdh(s, nl, i, h, pp->kernel, &rho, &dsum);
+        double h_new;
+        if (pp->h_solver == 0) {
+            h_new = rho > 0.0 ? fmin(pp->eta * cbrt(mi / rho), pp->h_max) : fmin(h, pp->h_max);
+        } else {
+            double e = pp->eta / h;
+            double rt = mi * e * e * e;
+            double gp = dsum + 3.0 * rt / h;
+            double hn = h - (rho - rt) / gp;
+            if (!(gp > 0.0) || !(rho > 0.0)) hn = rho > 0.0 ? pp->eta * cbrt(mi / rho) : 2.0 * h;
+            if (hn < 0.5 * h) hn = 0.5 * h;
+            if (hn > 2.0 * h) hn = 2.0 * h;
+            h_new = fmin(hn, pp->h_max);
+        }
+        if (rho > 0.0) {
+            double dh = -h_new / (3.0 * rho);
+            *omega_i = 1.0 - dh * dsum;
+            if (*omega_i < 0.1) *omega_i = 0.1;
+        }
+        s->rho[i] = rho;
+        s->h[i] = h_new;
+        if (fabs(h_new - h) / fmax(h, 1e-30) < pp->h_tol) break;
+        /* sim-core compares with the current h (density.rs:96-97); the
+         * variant with the radius the list was built with */
+        double sr = pp->h_solver == 0 ? 2.0 * h * 1.1 : lr;
+        if (2.0 * h_new > sr) {
+            lr = 2.0 * h_new * 1.1;
+            ball(t, s, i, lr, nl);
+        }
+        h = h_new;
+    }
+    double rho = s->rho[i], u = s->u[i];
+    s->p[i] = (pp->gamma - 1.0) * rho * u;
+    s->c[i] = sqrt(pp->gamma * (pp->gamma - 1.0) * fmax(u, 0.0));
+}
+
 /* Computes density/h/omega/EOS; returns per-particle neighbour lists. */
 static ulist* compute_density(orc_sph_state* s, const stree* t, const orc_sph_params* pp,
                               double* omega) {
-    size_t n = s->n;
-    ulist* nl = (ulist*)calloc(n, sizeof(ulist));
-    double* lr = (double*)calloc(n, sizeof(double)); /* radius of each list */
-    for (size_t i = 0; i < n; ++i) {
-        omega[i] = 1.0;
-        if (s->type[i] != GAS) continue;
-        lr[i] = 2.0 * s->h[i] * 1.1;
-        ball(t, s, i, lr[i], &nl[i]);
-    }
-    for (size_t i = 0; i < n; ++i) {
-        if (s->type[i] != GAS) continue;
-        double h = s->h[i], mi = s->m[i];
-        for (int it = 0; it < pp->max_h_iter; ++it) {
-            double rho, dsum;
-            density_dwdh(s, &nl[i], i, h, pp->kernel, &rho, &dsum);
-            double h_new;
-            if (pp->h_solver == 0) {
-                h_new = rho > 0.0 ? fmin(pp->eta * cbrt(mi / rho), pp->h_max) : fmin(h, pp->h_max);
-            } else {
-                double e = pp->eta / h;
-                double rt = mi * e * e * e;
-                double gp = dsum + 3.0 * rt / h;
-                double hn = h - (rho - rt) / gp;
-                if (!(gp > 0.0) || !(rho > 0.0)) hn = rho > 0.0 ? pp->eta * cbrt(mi / rho) : 2.0 * h;
-                if (hn < 0.5 * h) hn = 0.5 * h;
-                if (hn > 2.0 * h) hn = 2.0 * h;
-                h_new = fmin(hn, pp->h_max);
-            }
-            if (rho > 0.0) {
-                double dh = -h_new / (3.0 * rho);
-                omega[i] = 1.0 - dh * dsum;
-                if (omega[i] < 0.1) omega[i] = 0.1;
-            }
-            s->rho[i] = rho;
-            s->h[i] = h_new;
-            if (fabs(h_new - h) / fmax(h, 1e-30) < pp->h_tol) break;
-            /* sim-core compares with the current h (density.rs:96-97); the
-             * variant with the radius the list was built with */
-            double sr = pp->h_solver == 0 ? 2.0 * h * 1.1 : lr[i];
-            if (2.0 * h_new > sr) {
-                lr[i] = 2.0 * h_new * 1.1;
-                ball(t, s, i, lr[i], &nl[i]);
-            }
-            h = h_new;
-        }
-    }
-    for (size_t i = 0; i < n; ++i) {
-        if (s->type[i] != GAS) continue;
-        double rho = s->rho[i], u = s->u[i];
-        s->p[i] = (pp->gamma - 1.0) * rho * u;
-        s->c[i] = sqrt(pp->gamma * (pp->gamma - 1.0) * fmax(u, 0.0));
-    }
-    free(lr);
+    ulist* nl = (ulist*)calloc(s->n, sizeof(ulist));
+    for (size_t i = 0; i < s->n; ++i) density_one(s, t, pp, i, &omega[i], &nl[i]);
     return nl;
 }
 
 /* ---- forces.rs:69-232 -------------------------------------------------- */
+/* One gas particle's force sums over its final list; ADDS to a and du/dt. */
+static void force_one(orc_sph_state* s, const ulist* nli, const double* omega,
+                      const orc_sph_params* pp, size_t i, double* div_v, double* max_mu,
+                      double* dalpha) {
+    *div_v = 0.0; *max_mu = 0.0; *dalpha = 0.0;
+    if (s->type[i] != GAS) return;
+    double xi = s->x[i], yi = s->y[i], zi = s->z[i];
+    double vxi = s->vx[i], vyi = s->vy[i], vzi = s->vz[i];
+    double hi = s->h[i];
+    double rho_i = fmax(s->rho[i], DENSITY_FLOOR);
+    double p_i = s->p[i], u_i = s->u[i], c_i = s->c[i], a_i = s->alpha[i];
+    double om_i = omega[i];
+    double pr_i = rho_i > 0.0 ? p_i / (om_i * rho_i * rho_i) : 0.0;
+    double ax = 0, ay = 0, az = 0, du = 0, dv = 0, mmu = 0;
+    for (size_t k = 0; k < nli->len; ++k) {
+        uint32_t j = nli->v[k];
+        if (j == i) continue;
+        double dx = xi - s->x[j], dy = yi - s->y[j], dz = zi - s->z[j];
+        double r = sqrt(dx * dx + dy * dy + dz * dz);
+        double hj = s->h[j];
+        double havg = 0.5 * (hi + hj);
+        if (r >= 2.0 * hi && r >= 2.0 * hj) continue;
+        double mj = s->m[j];
+        double rho_j = fmax(s->rho[j], DENSITY_FLOOR);
+        double p_j = s->p[j], c_j = s->c[j], a_j = s->alpha[j], om_j = omega[j];
+        double dvx = vxi - s->vx[j], dvy = vyi - s->vy[j], dvz = vzi - s->vz[j];
+        double vr = dvx * dx + dvy * dy + dvz * dz;
+        double gi[3], gj[3], ga[3];
+        k_grad(pp->kernel, dx, dy, dz, r, hi, gi);
+        k_grad(pp->kernel, dx, dy, dz, r, hj, gj);
+        k_grad(pp->kernel, dx, dy, dz, r, havg, ga);
+        double pr_j = rho_j > 0.0 ? p_j / (om_j * rho_j * rho_j) : 0.0;
+        double pij = 0.0;
+        if (vr < 0.0) {
+            double aavg = 0.5 * (a_i + a_j), cavg = 0.5 * (c_i + c_j),
+                   ravg = 0.5 * (rho_i + rho_j);
+            double eta2 = 0.01 * havg * havg;
+            double mu = havg * vr / (r * r + eta2);
+            mmu = fmax(mmu, fabs(mu));
+            pij = ravg > 0.0 ? (-aavg * cavg * mu + pp->beta_visc * mu * mu) / ravg : 0.0;
+        }
+        ax -= mj * (pr_i * gi[0] + pr_j * gj[0] + pij * ga[0]);
+        ay -= mj * (pr_i * gi[1] + pr_j * gj[1] + pij * ga[1]);
+        az -= mj * (pr_i * gi[2] + pr_j * gj[2] + pij * ga[2]);
+        double vgi = dvx * gi[0] + dvy * gi[1] + dvz * gi[2];
+        double vga = dvx * ga[0] + dvy * ga[1] + dvz * ga[2];
+        du += mj * (pr_i * vgi + 0.5 * pij * vga);
+        dv -= mj * vgi;
+        if (pp->alpha_cond > 0.0) {
+            double u_j = s->u[j];
+            double rs = rho_i + rho_j;
+            if (rs > 0.0) {
+                double ravg = 0.5 * rs;
+                double dp = fabs(p_i - p_j);
+                double vsig = sqrt(dp / ravg);
+                double agw = k_absgrad(pp->kernel, r, havg);
+                du += mj * pp->alpha_cond * vsig * (u_i - u_j) / rs * agw;
+            }
+        }
+    }
+    s->ax[i] += ax; s->ay[i] += ay; s->az[i] += az;
+    s->dudt[i] += du;
+    if (rho_i > 0.0) *div_v = dv / rho_i;
+    *max_mu = mmu;
+    double tau = c_i > 0.0 ? hi / (2.0 * pp->sigma_visc * c_i) : 1e10;
+    double src = fmax(-*div_v, 0.0) * (pp->alpha_max - a_i);
+    *dalpha = pp->alpha_switch ? -(a_i - pp->alpha_min) / tau + src : 0.0;
+}
+
 static void compute_forces(orc_sph_state* s, const ulist* nl, const double* omega,
                            const orc_sph_params* pp, double* div_v, double* max_mu,
                            double* dalpha) {
-    size_t n = s->n;
-    for (size_t i = 0; i < n; ++i) {
-        div_v[i] = 0.0; max_mu[i] = 0.0; dalpha[i] = 0.0;
-        if (s->type[i] != GAS) continue;
-        double xi = s->x[i], yi = s->y[i], zi = s->z[i];
-        double vxi = s->vx[i], vyi = s->vy[i], vzi = s->vz[i];
-        double hi = s->h[i];
-        double rho_i = fmax(s->rho[i], DENSITY_FLOOR);
-        double p_i = s->p[i], u_i = s->u[i], c_i = s->c[i], a_i = s->alpha[i];
-        double om_i = omega[i];
-        double pr_i = rho_i > 0.0 ? p_i / (om_i * rho_i * rho_i) : 0.0;
-        double ax = 0, ay = 0, az = 0, du = 0, dv = 0, mmu = 0;
-        for (size_t k = 0; k < nl[i].len; ++k) {
-            uint32_t j = nl[i].v[k];
-            if (j == i) continue;
-            double dx = xi - s->x[j], dy = yi - s->y[j], dz = zi - s->z[j];
-            double r = sqrt(dx * dx + dy * dy + dz * dz);
-            double hj = s->h[j];
-            double havg = 0.5 * (hi + hj);
-            if (r >= 2.0 * hi && r >= 2.0 * hj) continue;
-            double mj = s->m[j];
-            double rho_j = fmax(s->rho[j], DENSITY_FLOOR);
-            double p_j = s->p[j], c_j = s->c[j], a_j = s->alpha[j], om_j = omega[j];
-            double dvx = vxi - s->vx[j], dvy = vyi - s->vy[j], dvz = vzi - s->vz[j];
-            double vr = dvx * dx + dvy * dy + dvz * dz;
-            double gi[3], gj[3], ga[3];
-            k_grad(pp->kernel, dx, dy, dz, r, hi, gi);
-            k_grad(pp->kernel, dx, dy, dz, r, hj, gj);
-            k_grad(pp->kernel, dx, dy, dz, r, havg, ga);
-            double pr_j = rho_j > 0.0 ? p_j / (om_j * rho_j * rho_j) : 0.0;
-            double pij = 0.0;
-            if (vr < 0.0) {
-                double aavg = 0.5 * (a_i + a_j), cavg = 0.5 * (c_i + c_j),
-                       ravg = 0.5 * (rho_i + rho_j);
-                double eta2 = 0.01 * havg * havg;
-                double mu = havg * vr / (r * r + eta2);
-                mmu = fmax(mmu, fabs(mu));
-                pij = ravg > 0.0 ? (-aavg * cavg * mu + pp->beta_visc * mu * mu) / ravg : 0.0;
-            }
-            ax -= mj * (pr_i * gi[0] + pr_j * gj[0] + pij * ga[0]);
-            ay -= mj * (pr_i * gi[1] + pr_j * gj[1] + pij * ga[1]);
-            az -= mj * (pr_i * gi[2] + pr_j * gj[2] + pij * ga[2]);
-            double vgi = dvx * gi[0] + dvy * gi[1] + dvz * gi[2];
-            double vga = dvx * ga[0] + dvy * ga[1] + dvz * ga[2];
-            du += mj * (pr_i * vgi + 0.5 * pij * vga);
-            dv -= mj * vgi;
-            if (pp->alpha_cond > 0.0) {
-                double u_j = s->u[j];
-                double rs = rho_i + rho_j;
-                if (rs > 0.0) {
-                    double ravg = 0.5 * rs;
-                    double dp = fabs(p_i - p_j);
-                    double vsig = sqrt(dp / ravg);
-                    double agw = k_absgrad(pp->kernel, r, havg);
-                    du += mj * pp->alpha_cond * vsig * (u_i - u_j) / rs * agw;
-                }
-            }
-        }
-        s->ax[i] += ax; s->ay[i] += ay; s->az[i] += az;
-        s->dudt[i] += du;
-        if (rho_i > 0.0) div_v[i] = dv / rho_i;
-        max_mu[i] = mmu;
-        double tau = c_i > 0.0 ? hi / (2.0 * pp->sigma_visc * c_i) : 1e10;
-        double src = fmax(-div_v[i], 0.0) * (pp->alpha_max - a_i);
-        dalpha[i] = pp->alpha_switch ? -(a_i - pp->alpha_min) / tau + src : 0.0;
-    }
+    for (size_t i = 0; i < s->n; ++i)
+        force_one(s, &nl[i], omega, pp, i, &div_v[i], &max_mu[i], &dalpha[i]);
 }
 
 /* SphSolver::compute (solver.rs:61-80): density + forces; intermediates out. */
@@ -517,6 +523,100 @@ void orc_sph_compute(orc_sph_state* s, const orc_sph_params* pp, double* div_v, 
     free(nl);
     free(omega);
     free(t.v);
+}
+
+/* SphSolver::compute restricted to nt target particles (a checker for
+ * large states): the density iteration of each target, then of every
+ * particle on a target's final list, then the forces of the targets only.
+ * Each particle's density depends on the positions, the masses and its own
+ * h alone, so every value written equals what orc_sph_compute writes for
+ * that particle. Writes rho, h, p, c of the particles it visits, ADDS the
+ * targets' a and du/dt, and fills the per-target arrays (k-th entry for
+ * targets[k]; any may be NULL). Parallel over particles (OpenMP). */
+void orc_sph_compute_targets(orc_sph_state* s, const orc_sph_params* pp, size_t nt,
+                             const int64_t* targets, double* div_v, double* max_mu,
+                             double* dalpha, double* omega_out, uint32_t* nneigh_out,
+                             int nthreads) {
+    size_t n = s->n;
+    stree t = {0};
+    s_build(&t, s);
+    double* omega = (double*)malloc(n * sizeof(double));
+    ulist* nl = (ulist*)calloc(n, sizeof(ulist));
+    uint8_t* done = (uint8_t*)calloc(n, 1);
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+    for (size_t k = 0; k < nt; ++k) done[targets[k]] = 1;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+    for (long long k = 0; k < (long long)nt; ++k) {
+        size_t i = (size_t)targets[k];
+        density_one(s, &t, pp, i, &omega[i], &nl[i]);
+    }
+    /* the particles the targets' forces read */
+    size_t cap = 1024, cnt = 0;
+    uint32_t* todo = (uint32_t*)malloc(cap * sizeof(uint32_t));
+    for (size_t k = 0; k < nt; ++k) {
+        const ulist* l = &nl[targets[k]];
+        for (size_t q = 0; q < l->len; ++q) {
+            uint32_t j = l->v[q];
+            if (done[j]) continue;
+            done[j] = 1;
+            if (cnt == cap) { cap *= 2; todo = (uint32_t*)realloc(todo, cap * sizeof(uint32_t)); }
+            todo[cnt++] = j;
+        }
+    }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+    for (long long k = 0; k < (long long)cnt; ++k)
+        density_one(s, &t, pp, todo[k], &omega[todo[k]], &nl[todo[k]]);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16)
+#endif
+    for (long long k = 0; k < (long long)nt; ++k) {
+        size_t i = (size_t)targets[k];
+        double dv, mm, da;
+        force_one(s, &nl[i], omega, pp, i, &dv, &mm, &da);
+        if (div_v) div_v[k] = dv;
+        if (max_mu) max_mu[k] = mm;
+        if (dalpha) dalpha[k] = da;
+        if (omega_out) omega_out[k] = omega[i];
+        if (nneigh_out) nneigh_out[k] = (uint32_t)nl[i].len;
+    }
+    for (size_t i = 0; i < n; ++i) free(nl[i].v);
+    free(todo);
+    free(done);
+    free(nl);
+    free(omega);
+    free(t.v);
+}
+
+/* Brute force at nt targets with the softening eps_i[k] of target k
+ * (the variant's eps = h): a_k = sum_{j != t_k} m_j d / (r^2 + eps_k^2)^{3/2},
+ * sources in index order, as orc_bf_accel_eps. Overwrites ax/ay/az[k]. */
+void orc_bf_targets_eps(size_t n, const double* eps_t, const double* x, const double* y,
+                        const double* z, const double* m, size_t nt, const int64_t* targets,
+                        double* ax, double* ay, double* az, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
+    for (long long k = 0; k < (long long)nt; ++k) {
+        size_t i = (size_t)targets[k];
+        double e2 = eps_t[k] * eps_t[k];
+        double a0 = 0, a1 = 0, a2 = 0;
+        for (size_t j = 0; j < n; ++j) {
+            if (j == i) continue;
+            double dx = x[j] - x[i], dy = y[j] - y[i], dz = z[j] - z[i];
+            double r2 = dx * dx + dy * dy + dz * dz + e2;
+            double w = 1.0 / (r2 * sqrt(r2));
+            double f = m[j] * w;
+            a0 += f * dx; a1 += f * dy; a2 += f * dz;
+        }
+        ax[k] = a0; ay[k] = a1; az[k] = a2;
+    }
 }
 
 /* SphSolver::compute_timestep (solver.rs:90-131). */
@@ -667,9 +767,10 @@ void orc_evrard_1r(size_t n, double total_mass, double radius, double u0, double
 /* ColdCollapse::generate (cold_collapse.rs:56-102): gas = 0 rejection with
  * random_range(-1..1) per axis scaled by R (type 0); gas = 1
  * fill_uniform_gas_sphere with u = 0.05, h = 1.5 spacing (type 4); then
- * shift_to_com_frame. Every other field 0. */
-void orc_cold_collapse(size_t n, double radius, double total_mass, uint64_t seed, int gas,
-                       orc_sph_state* s) {
+ * shift_to_com_frame (skipped when com_shift = 0, so tests can check the
+ * samples themselves). Every other field 0. */
+void orc_cold_collapse_ex(size_t n, double radius, double total_mass, uint64_t seed, int gas,
+                          int com_shift, orc_sph_state* s) {
     orc_rng rng;
     orc_rng_seed_u64(&rng, seed);
     double m = total_mass / (double)n;
@@ -704,7 +805,12 @@ void orc_cold_collapse(size_t n, double radius, double total_mass, uint64_t seed
             s->type[i] = 0;
         }
     }
-    orc_shift_to_com(n, s->x, s->y, s->z, s->vx, s->vy, s->vz, s->m);
+    if (com_shift) orc_shift_to_com(n, s->x, s->y, s->z, s->vx, s->vy, s->vz, s->m);
+}
+
+void orc_cold_collapse(size_t n, double radius, double total_mass, uint64_t seed, int gas,
+                       orc_sph_state* s) {
+    orc_cold_collapse_ex(n, radius, total_mass, seed, gas, 1, s);
 }
 
 double orc_evrard_softening(size_t n, double radius) {

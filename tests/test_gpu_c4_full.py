@@ -1,0 +1,169 @@
+"""Grad-h SPH with Barnes-Hut self-gravity at its configured size: 1,048,576
+gas particles in the rho ~ 1/r sphere (R = 1, M = 1, u = 0.05, seed 42, at
+rest), cubic spline, eta 1.2, Newton h to 1e-4, alpha 1 / beta 2 fixed,
+gravity softening eps = h, FP32 pair loops with FP64 sums, the mixed
+Barnes-Hut walk at theta 0.5.
+
+At this size the step takes paths the small parity tests never reach: the
+depth-first fallback for overflowing target groups, the sub-range re-search,
+and a neighbour pool that runs out and grows during a compute (forced here
+by starting a fresh context's pool at 40 entries per particle, about half
+of what the sphere needs).
+
+The oracle is evaluated on the same state as the GPU, restricted to 1,024
+random targets (oracle.sph_compute_targets: the targets' and their
+neighbours' density iteration, then the targets' forces; each value equals
+the full oracle compute's, see tests/test_oracle_sph_targets.py).
+
+Bars (the FP32-pair bars of the small variant tests):
+  * rho, h, P within 2e-6 relative per target; omega within 1e-5; c within
+    1e-14 (it depends on u only);
+  * SPH a and du/dt within 1e-5 of the field's largest magnitude over the
+    targets;
+  * gravity (mixed walk, eps = h) against the oracle's f64 brute force with
+    the same per-target softening: RMS and max relative error within the
+    reference's own theta = 0.5 Barnes-Hut error (m3_accuracy.csv: RMS
+    4.942351e-3, max 3.588391e-2);
+  * three gravis_sph_run steps: energy (K + U + E_th, U at the median h)
+    changes by less than 1e-4 relative, and a compute on the stepped state
+    meets the same bars.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+N = 1 << 20
+NT = 1024
+BH_RMS, BH_MAX = 4.942351e-3, 3.588391e-2
+
+PAIRS = (("x", "x"), ("y", "y"), ("z", "z"), ("vx", "vx"), ("vy", "vy"), ("vz", "vz"),
+         ("ax", "ax"), ("ay", "ay"), ("az", "az"), ("mass", "m"), ("density", "rho"),
+         ("pressure", "p"), ("internal_energy", "u"), ("du_dt", "dudt"),
+         ("smoothing_length", "h"), ("sound_speed", "c"), ("alpha_visc", "alpha"))
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def to_oracle(p):
+    s = O.SphState(p.count)
+    for a, b in PAIRS:
+        getattr(s, b)[:] = getattr(p, a)
+    s.type[:] = p.particle_type
+    return s
+
+
+@pytest.fixture(scope="module")
+def C4():
+    import gravis_amd as g
+    import gravis_amd.sph as sph
+    ctx = g.Context(0)
+    ctx.set_tuning("sph_pool_init", 40)
+    solver = sph.SphSolver(ctx, variant=True)
+    bh = g.BarnesHut(0.0, 0.5, 16, 64, precision="mixed", ctx=ctx)
+    say(f"C4: generating rho~1/r sphere N={N}")
+    p = g.scenarios.evrard_1r(N)
+    targets = np.random.default_rng(2024).choice(N, NT, replace=False)
+    yield dict(g=g, sph=sph, ctx=ctx, solver=solver, bh=bh, p=p, targets=targets)
+    ctx.close()
+
+
+def check_compute(c, p, label):
+    """GPU compute on a copy of p (a, du/dt zeroed) against the oracle on
+    the same state; returns the GPU's compute stats."""
+    sph, solver, tg = c["sph"], c["solver"], c["targets"]
+    q = p.copy()
+    q.clear_accelerations()
+    q.clear_sph_rates()
+    s = to_oracle(q)
+    say(f"C4 {label}: GPU compute")
+    gi = solver.compute(q, hooks=True)
+    st = solver.stats()
+    say(f"C4 {label}: oracle on {NT} targets")
+    oi = O.sph_compute_targets(s, tg, O.SphParams.variant(), nthreads=16)
+    e = {}
+    for a, b in (("density", "rho"), ("smoothing_length", "h"), ("pressure", "p"),
+                 ("sound_speed", "c")):
+        ga, ob = getattr(q, a)[tg], getattr(s, b)[tg]
+        e[b] = (np.abs(ga - ob) / np.abs(ob)).max()
+    e["omega"] = (np.abs(gi.omega[tg] - oi["omega"]) / np.abs(oi["omega"])).max()
+    for a, b in (("ax", "ax"), ("ay", "ay"), ("az", "az"), ("du_dt", "dudt")):
+        ga, ob = getattr(q, a)[tg], getattr(s, b)[tg]
+        # at rest du/dt is exactly zero (every term carries v_i - v_j)
+        e[b] = np.abs(ga - ob).max() / np.abs(ob).max() if np.abs(ob).max() > 0 else \
+            np.abs(ga).max()
+    nd = int((gi.nneigh[tg] != oi["nneigh"]).sum())
+    say(f"C4 {label}: stats {st}; lists of different length {nd}/{NT}; "
+        + ", ".join(f"{k} {v:.1e}" for k, v in e.items()))
+    assert e["rho"] <= 2e-6 and e["h"] <= 2e-6 and e["p"] <= 2e-6
+    assert e["c"] <= 1e-14
+    assert e["omega"] <= 1e-5
+    for k in ("ax", "ay", "az", "dudt"):
+        assert e[k] <= 1e-5, k
+    assert nd <= NT // 100
+    assert np.all(gi.dalpha_dt == 0.0)
+    return st
+
+
+def check_gravity(c, p, label, precision="mixed"):
+    g, sph, solver, tg = c["g"], c["sph"], c["solver"], c["targets"]
+    q = p.copy()
+    q.clear_accelerations()
+    bh = c["bh"] if precision == "mixed" else g.BarnesHut(0.0, 0.5, 16, 64, ctx=c["ctx"])
+    say(f"C4 {label}: GPU gravity ({precision} walk, eps = h)")
+    sph.accelerations(q, bh, solver)
+    s = to_oracle(q)
+    say(f"C4 {label}: oracle brute force on {NT} targets")
+    ref = O.bf_targets_eps(s, q.smoothing_length[tg], tg, nthreads=16)
+    got = q.acc()[tg]
+    rel = np.sqrt(((got - ref) ** 2).sum(1) / (ref ** 2).sum(1))
+    rms = np.sqrt((rel ** 2).mean())
+    say(f"C4 {label}: gravity rms {rms:.3e} max {rel.max():.3e}")
+    assert rms <= BH_RMS and rel.max() <= BH_MAX
+
+
+def test_c4_initial_compute_matches_oracle(C4):
+    st = check_compute(C4, C4["p"], "t=0")
+    # the paths the benchmark hits: overflowing groups searched depth first,
+    # and a pool that outgrew its initial 40 entries per particle
+    assert st["dfs_groups"] > 0
+    assert st["pool_entries"] > 40 * N
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_c4_gravity_eps_h(C4, precision):
+    check_gravity(C4, C4["p"], "t=0", precision)
+
+
+def test_c4_three_steps(C4):
+    g, sph, solver, bh, p = C4["g"], C4["sph"], C4["solver"], C4["bh"], C4["p"]
+    ctx = C4["ctx"]
+    p.clear_accelerations()
+    p.clear_sph_rates()
+    sph.accelerations(p, bh, solver)
+    solver.compute(p)
+    hmed = float(np.median(p.smoothing_length))
+
+    def energy():
+        d = g.diagnostics.compute(p, hmed, ctx=ctx)
+        return d.total_energy, d.kinetic_energy, d.thermal_energy
+
+    e0, k0, t0 = energy()
+    say(f"C4: E0 {e0:.9e} (K {k0:.3e}, E_th {t0:.6e}); 3 steps")
+    dt, tsim = sph.run(p, bh, solver, 1e-4, 3)
+    e1, k1, t1 = energy()
+    de = abs(e1 - e0) / abs(e0)
+    say(f"C4: after 3 steps t={tsim:.3e} dt_next={dt:.3e}: E1 {e1:.9e} (K {k1:.3e}, "
+        f"E_th {t1:.6e}), |dE/E| {de:.2e}")
+    assert np.isfinite(p.x).all() and np.isfinite(p.ax).all() and np.isfinite(p.du_dt).all()
+    assert k1 > k0 and de <= 1e-4 and dt > 0
+    st = check_compute(C4, p, "after 3 steps")
+    assert st["dfs_groups"] > 0
+    check_gravity(C4, p, "after 3 steps")
