@@ -14,9 +14,9 @@
 // s^2 < theta^2 d_min^2 with d_min the distance from its COM to the group's
 // bounding box (so never less accurate than the reference's per-particle
 // test); a rejected leaf contributes its particles directly; a rejected
-// internal node pushes its children. If a group's frontier would exceed the
-// per-wave capacity, the group is walked again one target at a time (point
-// boxes). Accepted nodes and the particles of opened leaves are appended, in
+// internal node pushes its children. If a group's frontier would exceed
+// kGroupFrontierCap (or the per-wave capacity, if smaller), the group is
+// walked again one target at a time (point boxes). Accepted nodes and the particles of opened leaves are appended, in
 // frontier order, as source indices to a per-wave LDS list; each flush round
 // stages 64 source records in LDS (one load per lane) that all 64 lanes then
 // read back as broadcasts. A group of at most 32 (16) targets is held by 2
@@ -45,6 +45,11 @@ __global__ void walk_reset_kernel(uint32_t* __restrict__ counter,
 
 constexpr int kWalkWaves = 4;
 constexpr uint32_t kListCap = 1024;  // source indices per wave (LDS)
+// Frontier at which a group's breadth-first walk stops and its targets are
+// walked one at a time (measured: 2048-8192 all give the shortest C3 and
+// single-sphere walks; 16384 lets a few groups whose box spans a sphere's
+// core run breadth first through most of the tree and set the kernel's tail).
+constexpr uint32_t kGroupFrontierCap = 4096;
 
 __device__ __forceinline__ void pair_ref(const double4 s, double px, double py, double pz,
                                          double eps2, double& ax, double& ay, double& az) {
@@ -90,6 +95,10 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
     uint32_t* const fa0 = frontier + slot * 2 * (size_t)fcap;
     uint32_t* const fb0 = fa0 + fcap;
     const double side0 = 2.0 * box[3];  // root side, octree.rs:81-85
+    // a group walk gives up at a smaller frontier than a one-target walk:
+    // the groups that reach it (outlying targets whose box spans the core)
+    // are cheaper walked one target at a time than breadth first
+    const uint32_t gcap = min(fcap, kGroupFrontierCap);
 
     const uint32_t ng = __builtin_amdgcn_readfirstlane(ngroups[0]);
     // groups are Morton ordered: each XCD drains its own contiguous range
@@ -243,8 +252,9 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
             }
         };
         // Breadth-first walk with opening box (lo, hi) for `ntargets`
-        // targets; returns true if a next frontier would exceed fcap.
-        auto walk = [&](const double (&lo)[3], const double (&hi)[3], int ntargets) -> bool {
+        // targets; returns true if a next frontier would exceed cap.
+        auto walk = [&](const double (&lo)[3], const double (&hi)[3], int ntargets,
+                        uint32_t cap) -> bool {
             uint32_t* fa = fa0;
             uint32_t* fb = fb0;
             if (lane == 0) fa[0] = 0u;
@@ -307,7 +317,7 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
                     const uint32_t k = open ? nchild : 0u;
                     const uint32_t kc = wave_inclusive_scan(k);
                     const uint32_t ktot = __builtin_amdgcn_readlane(kc, kWave - 1);
-                    if (nn + ktot > fcap) {
+                    if (nn + ktot > cap) {
                         overflow = true;
                         break;
                     }
@@ -326,7 +336,7 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
             return overflow;
         };
 
-        if (walk(glo, ghi, cnt)) {
+        if (walk(glo, ghi, cnt, gcap)) {
             // the group box opens too much: walk every target alone
             ax = ay = az = 0.0;
             sx = sy = sz = kx = ky = kz = 0.f;
@@ -334,7 +344,7 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
                 const double px = __shfl(me.x, l), py = __shfl(me.y, l), pz = __shfl(me.z, l);
                 const double p[3] = {px, py, pz};
                 keep = tl == l;
-                if (walk(p, p, 1) && lane == 0) atomicOr(flag, 1u);
+                if (walk(p, p, 1, fcap) && lane == 0) atomicOr(flag, 1u);
             }
             keep = true;
         }

@@ -310,8 +310,9 @@ gravis_status gravis_kernel_time(gravis_ctx* ctx, const char* name, double* ms_t
 
 /* Tuning knobs (no reference counterpart): "bh_frontier_cap" = per-wave
  * breadth-first frontier capacity of the Barnes-Hut walk in nodes (default
- * 16384; a group whose frontier would exceed it is walked one target at a
- * time); "sph_query_leaves" = overlapping tree leaves one wave of the SPH
+ * 16384; a group whose frontier would exceed 4096 nodes, or this capacity if
+ * smaller, is walked one target at a time, and a single target's frontier
+ * must fit it); "sph_query_leaves" = overlapping tree leaves one wave of the SPH
  * neighbour search holds (1..512, default 512; a group needing more is
  * searched one target at a time, depth first); "sph_pool_init" = initial
  * neighbour-pool capacity in entries per particle (1..4096, default 96; the

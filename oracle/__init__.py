@@ -196,7 +196,7 @@ def bh_accel(s: State, eps: float, theta: float, nthreads: int = 0) -> None:
     lib().orc_bh_accel(s.n, eps, theta, s.x, s.y, s.z, s.m, s.ax, s.ay, s.az, nthreads)
 
 
-FRONTIER_CAP = 16384  # the device walk's per-wave frontier capacity
+FRONTIER_CAP = 4096  # the device's group-walk frontier capacity (kGroupFrontierCap)
 
 
 def bh_group_accel(s: State, eps: float, theta: float, leaf: int = 16, group: int = 64,

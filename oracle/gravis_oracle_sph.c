@@ -652,12 +652,12 @@ double orc_sph_step(orc_sph_state* s, const orc_sph_params* pp, int gravity, dou
     for (size_t i = 0; i < n; ++i) { s->ax[i] = 0.0; s->ay[i] = 0.0; s->az[i] = 0.0; s->dudt[i] = 0.0; }
     if (pp->soft_h && gravity == 1) orc_bf_accel_eps(n, s->h, s->x, s->y, s->z, s->m, s->ax, s->ay, s->az);
     else if (pp->soft_h && gravity == 3)
-        orc_bh_group_accel_eps(n, s->h, theta, 16, 64, 16384, s->x, s->y, s->z, s->m, s->ax,
+        orc_bh_group_accel_eps(n, s->h, theta, 16, 64, 4096, s->x, s->y, s->z, s->m, s->ax,
                                s->ay, s->az, 0);
     else if (gravity == 1) orc_bf_accel(n, eps, s->x, s->y, s->z, s->m, s->ax, s->ay, s->az);
     else if (gravity == 2) orc_bh_accel(n, eps, theta, s->x, s->y, s->z, s->m, s->ax, s->ay, s->az, 0);
     else if (gravity == 3)
-        orc_bh_group_accel(n, eps, theta, 16, 64, 16384, s->x, s->y, s->z, s->m, s->ax, s->ay,
+        orc_bh_group_accel(n, eps, theta, 16, 64, 4096, s->x, s->y, s->z, s->m, s->ax, s->ay,
                            s->az, 0);
     double* tmp = (double*)malloc(3 * n * sizeof(double));
     orc_sph_compute(s, pp, tmp, tmp + n, tmp + 2 * n, NULL, NULL);

@@ -176,7 +176,7 @@ def test_barnes_hut_scaling_is_nlogn(g):
     # warm-up; the walk kernel's own time beside the whole call (sort, tree
     # build and walk)
     import torch
-    big = [524288, 1048576, 2097152, 4194304]
+    big = [1048576, 2097152, 4194304, 8388608]
     calls, walks = [], []
     stream = torch.cuda.Stream()
     ctx.set_stream(stream.cuda_stream)
