@@ -36,6 +36,7 @@ EXPORTS = (
     "gravis_snapshot_read", "gravis_scenario_cold_collapse", "gravis_galaxy_default_params",
     "gravis_scenario_galaxy_collision", "gravis_galaxy_suggested_dt",
     "gravis_galaxy_suggested_softening", "gravis_get_tuning", "gravis_autotune",
+    "gravis_accel_device_noalloc",
 )
 
 
@@ -124,6 +125,7 @@ def load() -> C.CDLL:
         "gravis_kernel_time": (st, [vp, C.c_char_p, dp, C.POINTER(C.c_uint64)]),
         "gravis_accel": (st, [vp, C.POINTER(Solver), sz] + [vp] * 7),
         "gravis_accel_device": (st, [vp, C.POINTER(Solver), sz] + [vp] * 7),
+        "gravis_accel_device_noalloc": (st, [vp, C.POINTER(Solver), sz] + [vp] * 7),
         "gravis_kdk": (st, [vp, C.POINTER(Solver), sz] + [vp] * 10 + [d, C.c_uint64]),
         "gravis_kdk_device": (st, [vp, C.POINTER(Solver), sz] + [vp] * 10 + [d, C.c_uint64]),
         "gravis_diagnostics": (st, [vp, sz, d, i32] + [vp] * 9 + [d, C.c_uint64,

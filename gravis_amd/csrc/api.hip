@@ -26,6 +26,26 @@ gravis_status fail(Ctx* c, gravis_status s, const char* fmt, ...) {
 
 gravis_status buffer(Ctx* c, const char* name, size_t bytes, void** out) {
     if (bytes == 0) bytes = 256;
+    if (c->arena) {
+        for (int k = 0; k < c->arena_n; ++k) {
+            const Ctx::ArenaEntry& e = c->arena_tab[k];
+            if (std::strcmp(e.name, name) != 0) continue;
+            if (e.bytes < bytes)
+                return fail(c, GRAVIS_INTERNAL, "scratch range %s: %zu bytes asked, %zu held",
+                            name, bytes, e.bytes);
+            *out = c->arena + e.off;
+            return GRAVIS_OK;
+        }
+        const size_t need = (bytes + 255) / 256 * 256;
+        if (c->arena_n == Ctx::kArenaEntries || c->arena_used + need > c->arena_cap)
+            return fail(c, GRAVIS_INVALID_ARG,
+                        "the no-allocation scratch (%zu bytes) cannot hold %s (%zu more bytes)",
+                        c->arena_cap, name, need);
+        c->arena_tab[c->arena_n++] = {name, c->arena_used, need};
+        *out = c->arena + c->arena_used;
+        c->arena_used += need;
+        return GRAVIS_OK;
+    }
     DevBuf& b = c->bufs[name];
     if (b.bytes < bytes) {
         if (b.p) {
@@ -247,6 +267,7 @@ gravis_status gravis_create(int device, gravis_ctx** out) {
         return GRAVIS_HIP_ERROR;
     }
     c->stream = c->own_stream;
+    gravis::noalloc_setup(c);
     *out = g;
     return GRAVIS_OK;
 }

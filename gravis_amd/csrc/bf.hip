@@ -819,6 +819,7 @@ gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
     int S = (target + gx - 1) / gx;
     const int max_s = (ni + tj - 1) / tj;
     if (S > max_s) S = max_s;
+    if (c->arena_max_chunks > 0 && S > c->arena_max_chunks) S = c->arena_max_chunks;
     if (S < 1) S = 1;
     int jchunk = (ni + S - 1) / S;
     jchunk = (jchunk + tj - 1) / tj * tj;

@@ -74,7 +74,30 @@ struct Ctx {
     // fraction q of the box diagonal gets FP64 pair separations, unless the
     // softening exceeds q times the diagonal.
     double bf_near_fraction = 1.0 / 64.0;
+    // No-allocation mode (gravis_accel_device_noalloc): while `arena` is
+    // set, buffer() hands out 256-byte-aligned ranges of it by name (the
+    // same name gives the same range within one call) instead of
+    // allocating, and the brute-force j split is capped at
+    // arena_max_chunks chunks.
+    struct ArenaEntry {
+        const char* name;
+        size_t off, bytes;
+    };
+    static constexpr int kArenaEntries = 32;
+    char* arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0;
+    int arena_n = 0;
+    ArenaEntry arena_tab[kArenaEntries];
+    int arena_max_chunks = 0;
+    // this device's copy of the library's static scratch array (set by
+    // gravis_create; null with scratch_err set if its address was not found)
+    void* scratch = nullptr;
+    hipError_t scratch_err = hipSuccess;
 };
+
+// Looks up (gravis_create) this device's address of the static scratch
+// array of the no-allocation entry point.
+void noalloc_setup(Ctx* c);
 
 // Status helpers ----------------------------------------------------------
 gravis_status fail(Ctx* c, gravis_status s, const char* fmt, ...);

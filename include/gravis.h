@@ -105,6 +105,21 @@ gravis_status gravis_accel_device(gravis_ctx* ctx, const gravis_solver* solver, 
                                   const double* x, const double* y, const double* z,
                                   const double* m, double* ax, double* ay, double* az);
 
+/* No reference counterpart: gravis_accel_device for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Brute force only (FP64 or
+ * MIXED; GRAVIS_NO_GRAVITY adds nothing; Barnes-Hut gives
+ * GRAVIS_UNSUPPORTED), n <= 65,536 (larger n gives GRAVIS_INVALID_ARG), at
+ * most 8 j chunks. Same kernels and accumulate contract as
+ * gravis_accel_device; every scratch buffer lives in one static 20 MiB
+ * device array per GPU, held until the process ends and shared by all
+ * contexts on that GPU, so calls that overlap on one GPU race. Device
+ * pointers, asynchronous on the context stream; no allocation. */
+gravis_status gravis_accel_device_noalloc(gravis_ctx* ctx, const gravis_solver* solver,
+                                          size_t n, const double* x, const double* y,
+                                          const double* z, const double* m, double* ax,
+                                          double* ay, double* az);
+
 /* nsteps of LeapfrogKDK::step (integrator.rs:41-77) with the given solver:
  * v += a dt/2; x += v dt; a = F(x); v += a dt/2. On entry a must hold
  * a(t) (as after sim-core's initial clear + compute_accelerations,
