@@ -31,6 +31,7 @@ METRIC = "G-interactions/s at N=256K (mixed vs FP64, err<=1e-6); BH steps/s at N
 FLOP_PER_INTERACTION = 20.0  # one softened pair force (GPU Gems 3 ch. 31 convention)
 PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector/matrix dense peak (datasheet)
 PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_BPS = 8.0e12         # MI355X HBM3E (datasheet)
 
 
 def log(msg):
@@ -95,6 +96,90 @@ def traffic_from_profiles(name, key):
         return float(d["shapes"][key]["hbm_bytes_per_launch"])
     except Exception:
         return None
+
+
+def walk_pmc_traffic(precision, group):
+    """HBM bytes per Barnes-Hut walk on the two-sphere workload from the
+    committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE scaled by a
+    copy of known size); None when that walk was not profiled."""
+    path = os.path.join(ROOT, "profiles", "bh_c3_n4194304_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        d = json.load(open(path))["walks"][f"{precision},{group}"]
+        return {"fetch_bytes": d["fetch_bytes"], "write_bytes": d["write_bytes"],
+                "bytes_per_walk": d["fetch_bytes"] + d["write_bytes"],
+                "source": "profiles/bh_c3_n4194304_traffic.json"}
+    except Exception:
+        return None
+
+
+def phase(bytes_, seconds, note):
+    """Compulsory bytes of one phase against its measured time."""
+    if not seconds:
+        return None
+    return {"ms": round(seconds * 1e3, 4), "compulsory_bytes": int(bytes_),
+            "GBps": round(bytes_ / seconds / 1e9, 1),
+            "frac_hbm_peak": round(bytes_ / seconds / PEAK_HBM_BPS, 4), "bytes": note}
+
+
+def tree_phase_rooflines(n, nodes, t):
+    """The octree build's phases (HIP events around each, launch gaps
+    included) against their compulsory bytes: Morton keys and an 8-pass
+    8-bit radix sort of 12-byte (key, index) pairs; the gather into Morton
+    order, one read of the records for the sums and the node arrays."""
+    return {
+        "morton_sort": phase(n * (24 + 12) + 8 * n * 24, t.get("morton_sort"),
+                             "N*(24 pos in + 12 key/index out) + 8 passes * N*24 (12 B read + "
+                             "12 B written per key)"),
+        "tree_build": phase(n * 68 + n * 32 + nodes * 112, t.get("tree_build"),
+                            "N*68 gather (32 B in, 4 B index, 32 B record out) + N*32 record "
+                            "read for the sums + nodes*112 (16 B meta, 32 B COM/mass, 64 B "
+                            "bounds)"),
+    }
+
+
+def sph_phase_rooflines(n, stats, nodes, kms):
+    """One SPH step's phases (milliseconds from the library's event timers)
+    against their compulsory bytes: the query reads each target's position
+    once and writes its list; a density pass reads position, mass and h and
+    writes rho, h, omega, P, c once per particle; the force pass reads the
+    112-byte record and writes 56 bytes per particle."""
+    sec = {k: (v / 1e3 if v else None) for k, v in kms.items()}
+    out = tree_phase_rooflines(n, nodes, sec)
+    out["sph_query"] = phase(n * 40 + stats["pool_entries"] * 4, sec.get("sph_query"),
+                             "N*32 positions + N*8 list heads + pool entries*4 written")
+    out["sph_density"] = phase(n * 80, sec.get("sph_density"),
+                               "N*(40 in: position, mass, h; 40 out: rho, h, omega, P, c)")
+    out["sph_force"] = phase(n * 168, sec.get("sph_force"),
+                             "N*(112 in: the force record; 56 out: a, du/dt, div v, max mu, "
+                             "dalpha/dt)")
+    return out
+
+
+def host_info():
+    """The host the CPU baselines ran on: logical CPUs the process sees
+    (nproc) and the CPU model from /proc/cpuinfo."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count()
+    return {"nproc": nproc, "cpu_model": model}
+
+
+def with_host(cpu):
+    """A cpu_baseline record with the host's nproc and CPU model added."""
+    if cpu is not None:
+        cpu.update(host_info())
+    return cpu
 
 
 def cpu_baseline(n_sample, eps):
@@ -187,10 +272,10 @@ def run_c1(args, g, ctx):
     return {"n": n, "steps": T, "steps_per_s": round(T / t_gpu, 1),
             "ms_per_step": round(t_gpu / T * 1e3, 4),
             "energy_drift": (d.total_energy - e0) / abs(e0), "virial_ratio": d.virial_ratio,
-            "cpu_baseline": {"value": round(1.0 / t_cpu, 1), "unit": "steps/s", "cores": 1,
-                             "kind": "port",
-                             "sample": f"{cpu_steps} steps of the oracle's sim-core loop "
-                                       f"(i<j brute force, f64, 1 thread)"},
+            "cpu_baseline": with_host({"value": round(1.0 / t_cpu, 1), "unit": "steps/s",
+                                       "cores": 1, "kind": "port",
+                                       "sample": f"{cpu_steps} steps of the oracle's sim-core "
+                                                 f"loop (i<j brute force, f64, 1 thread)"}),
             "workload": "Plummer N=1000, seed 42, eps 0.05, dt 1e-3, FP64 brute force, "
                         "leapfrog KDK fused on the device"}
 
@@ -277,16 +362,22 @@ def run_bh(args, g, torch, ctx, precision="mixed", steps=None):
         ms, launches = ctx.kernel_time("bh_walk")
         ws = ms / 1e3 / max(1, launches)
         st = ctx.bh_stats()
-        # Algorithmic bytes of one walk: per frontier node test a 16 B node
-        # record + 32 B (com, mass); per accepted node and per leaf-particle
-        # source a 32 B record read by the wave; per target 32 B in + 24 B
-        # out. A larger group shares one list among more targets, so it
-        # moves fewer bytes per step.
-        wb = st["node_tests"] * 48 + (st["accepted_nodes"] + st["leaf_particles"]) * 32 + n * 56
-        return t, ws, st, wb
+        # Algorithmic bytes of one walk (per group, every node it touches
+        # once and every leaf particle it takes as a source once, plus the
+        # targets in and out): node_tests * B_node + leaf_particles * B_part
+        # + N * 48, with B_node = 20 B (FP32 COM and mass, node size) or
+        # 40 B (FP64) and B_part = 16 B or 32 B. A larger group shares one
+        # list among more targets, so it moves fewer bytes per step.
+        b_node, b_part = (20, 16) if precision == "mixed" else (40, 32)
+        wb = st["node_tests"] * b_node + st["leaf_particles"] * b_part + n * 48
+        phases = {}
+        for k in ("morton_sort", "tree_build"):
+            ms, launches = ctx.kernel_time(k)
+            phases[k] = ms / 1e3 / max(1, launches)
+        return t, ws, st, wb, phases
 
     log(f"bench: timing {steps} BH KDK steps ({precision})")
-    t_step, walk_s, st, walk_bytes = measure(steps)
+    t_step, walk_s, st, walk_bytes, phases = measure(steps)
     # the other group sizes on the same run, for comparison (the autotuner's
     # choice stays the reported one)
     others = []
@@ -296,16 +387,20 @@ def run_bh(args, g, torch, ctx, precision="mixed", steps=None):
                 continue
             ctx.set_tuning("bh_group_size", gs)
             log(f"bench: BH group size {gs} for comparison")
-            t2, w2, _, b2 = measure(max(5, steps // 2))
+            t2, w2, _, b2, _ = measure(max(5, steps // 2))
             others.append({"group_size": gs, "ms_per_step": round(t2 * 1e3, 3),
                            "walk_kernel_ms": round(w2 * 1e3, 3),
                            "walk_algorithmic_GBps": round(b2 / w2 / 1e9, 1),
-                           "walk_frac_hbm_peak": round(b2 / w2 / 8.0e12, 4)})
+                           "walk_frac_hbm_peak": round(b2 / w2 / PEAK_HBM_BPS, 4)})
         ctx.set_tuning("bh_group_size", group)
     cpu = None
     if precision == "mixed" and not args.no_cpu_baseline:
         log(f"bench: CPU baseline, reference Barnes-Hut on {n} particles")
-        cpu = bh_cpu_baseline(p, 0.01, 0.5)
+        cpu = with_host(bh_cpu_baseline(p, 0.01, 0.5))
+    pmc = walk_pmc_traffic(precision, group)
+    if pmc:
+        pmc["GBps"] = round(pmc["bytes_per_walk"] / walk_s / 1e9, 1)
+        pmc["frac_hbm_peak"] = round(pmc["bytes_per_walk"] / walk_s / PEAK_HBM_BPS, 4)
     return {"n": n, "precision": precision, "steps_per_s": round(1.0 / t_step, 3),
             "cpu_baseline": cpu,
             "ms_per_step": round(t_step * 1e3, 3), "steps": steps, "theta": 0.5, "leaf_size": 16, "group_size": group,
@@ -314,8 +409,14 @@ def run_bh(args, g, torch, ctx, precision="mixed", steps=None):
             "walk_stats": st,
             "walk_pairs_per_s": st["pairs"] / walk_s,
             "walk_achieved_tflops": round(FLOP_PER_INTERACTION * st["pairs"] / walk_s / 1e12, 3),
+            "walk_algorithmic_bytes": walk_bytes,
             "walk_algorithmic_GBps": round(walk_bytes / walk_s / 1e9, 1),
-            "walk_frac_hbm_peak": round(walk_bytes / walk_s / 8.0e12, 4),
+            "walk_frac_hbm_peak": round(walk_bytes / walk_s / PEAK_HBM_BPS, 4),
+            "walk_bytes_definition": "node_tests*%d + leaf_particles*%d + N*48 (each node and "
+                                     "leaf particle counted once per target group)"
+                                     % ((20, 16) if precision == "mixed" else (40, 32)),
+            "walk_pmc_traffic": pmc,
+            "tree_phases": tree_phase_rooflines(n, st["nodes"], phases),
             "other_group_sizes": others,
             "workload": "two Plummer spheres (seeds 42/43, a=1, M=1 each) 10a apart, parabolic "
                         "approach, KDK with Barnes-Hut theta=0.5, leaf 16"}
@@ -400,7 +501,7 @@ def run_ensemble(args, g, torch, ctx):
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: CPU baseline, oracle ensemble forward and adjoint on 4 systems")
-        cpu = ens_cpu_baseline(cnts)
+        cpu = with_host(ens_cpu_baseline(cnts))
     return {"systems": len(cnts), "n_total": n, "interactions_per_step": inter,
             "steps": args.ens_steps, "ms_per_step": round(t_step * 1e3, 4),
             "G_interactions_per_s": round(inter / t_step / 1e9, 3),
@@ -473,10 +574,11 @@ def run_sph(args, g, torch, ctx):
     run(1)
     ctx.set_profiling(False)
     kms = {}
-    for k in ("sph_query", "sph_density", "sph_force", "bh_walk"):
+    for k in ("sph_query", "sph_density", "sph_force", "bh_walk", "morton_sort", "tree_build"):
         ms, launches = ctx.kernel_time(k)
         kms[k] = round(ms, 3)
     stats = solver.stats()
+    roofs = sph_phase_rooflines(n, stats, ctx.bh_stats()["nodes"], kms)
     cpu = None
     if not args.no_cpu_baseline:
         import oracle as O
@@ -496,7 +598,8 @@ def run_sph(args, g, torch, ctx):
             "particle_steps_per_s": round(n / t_step, 1),
             "steps": args.sph_steps, "warmup": args.sph_warmup, "sim_time_span": sim_span,
             "dt_last": dt.value, "softening": eps, "theta": 0.5,
-            "kernel_ms_one_step": kms, "last_compute_stats": stats, "cpu_baseline": cpu,
+            "kernel_ms_one_step": kms, "last_compute_stats": stats,
+            "phase_rooflines": roofs, "cpu_baseline": with_host(cpu),
             "workload": "Evrard collapse (uniform sphere R=1, M=1, u0=0.05, seed 42), sim-core "
                         "SPH (Wendland C2, fixed-point h, Monaghan AV + switch, conductivity), "
                         "FP64 Barnes-Hut gravity theta=0.5, headless dt rule"}
@@ -552,10 +655,11 @@ def run_sph_c4(args, g, torch, ctx):
                                       C.byref(dt), C.byref(tsim)))
     ctx.set_profiling(False)
     kms = {}
-    for k in ("sph_query", "sph_density", "sph_force", "bh_walk"):
+    for k in ("sph_query", "sph_density", "sph_force", "bh_walk", "morton_sort", "tree_build"):
         ms, _ = ctx.kernel_time(k)
         kms[k] = round(ms, 3)
     stats = solver.stats()
+    roofs = sph_phase_rooflines(n, stats, ctx.bh_stats()["nodes"], kms)
     ke = float(0.5 * (T["m"] * (T["vx"] ** 2 + T["vy"] ** 2 + T["vz"] ** 2)).sum())
     eth = float((T["m"] * T["u"]).sum())
     cpu = None
@@ -576,7 +680,7 @@ def run_sph_c4(args, g, torch, ctx):
             "ms_per_step": round(t_step * 1e3, 3), "particle_steps_per_s": round(n / t_step, 1),
             "sim_time": tsim.value, "dt_last": dt.value, "kinetic_energy": ke,
             "thermal_energy": eth, "kernel_ms_one_step": kms, "last_compute_stats": stats,
-            "cpu_baseline": cpu,
+            "phase_rooflines": roofs, "cpu_baseline": with_host(cpu),
             "workload": "rho~1/r gas sphere (R=1, M=1, u=0.05, seed 42), cubic spline, "
                         "eta=1.2, Newton h to 1e-4, alpha=1 beta=2, eps=h Barnes-Hut "
                         "theta=0.5 (mixed walk), FP32 pairs / FP64 sums, CFL 0.3"}
@@ -740,7 +844,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log(f"bench: CPU baseline on {args.cpu_sample} particles")
-        cpu = cpu_baseline(args.cpu_sample, eps)
+        cpu = with_host(cpu_baseline(args.cpu_sample, eps))
 
     # C1: the reference's own CPU case (Plummer N=1000, eps 0.05, dt 1e-3,
     # 10,000 FP64 KDK steps) on the GPU, the oracle beside it

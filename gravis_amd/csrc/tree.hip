@@ -356,10 +356,12 @@ gravis_status build_octree(Ctx* c, size_t n, const double* x, const double* y, c
         return fail(c, GRAVIS_INVALID_ARG, "group_size must be 1..64");
     double* box;
     GRAVIS_TRY(buf(c, "tree_box", 4, &box));
+    timer_begin(c);
     GRAVIS_TRY(root_box(c, n, x, y, z, box));
     uint64_t* keys;
     uint32_t* perm;
     GRAVIS_TRY(morton_sort(c, n, x, y, z, box, &keys, &perm));
+    timer_end(c, "morton_sort");
 
     // Internal nodes of one level are disjoint runs of more than `leaf`
     // particles and have at most 8 children each.
@@ -381,6 +383,7 @@ gravis_status build_octree(Ctx* c, size_t n, const double* x, const double* y, c
     GRAVIS_TRY(buf(c, "tree_flags", 4, &flags));
     GRAVIS_TRY(buf(c, "tree_ngroups", 1, &ngroups));
     GRAVIS_TRY(buf(c, "tree_groups", n, &groups));
+    timer_begin(c);
     GRAVIS_TRY(gather_sorted(c, n, x, y, z, m, perm, src));
 
     hipStream_t st = c->stream;
@@ -403,6 +406,7 @@ gravis_status build_octree(Ctx* c, size_t n, const double* x, const double* y, c
         tree_sums_kernel<<<tg, kTreeBlock, 0, st>>>(src, meta, lvl, L, sums, bmin, bmax);
     tree_com_kernel<<<tg, kTreeBlock, 0, st>>>(src, (int)n, meta, lvl, sums);
     group_emit_kernel<<<tg, kTreeBlock, 0, st>>>(meta, lvl, bmin, bmax, group, groups, ngroups);
+    timer_end(c, "tree_build");
     GRAVIS_TRY(check_launch(c, "tree sums and groups"));
 
     t->n = n;

@@ -318,7 +318,10 @@ gravis_status gravis_sph_run_device(gravis_ctx* ctx, const gravis_sph_params* pa
 /* Kernel timing: when enabled, the dominant force kernel of every call is
  * bracketed by HIP events on the stream it runs on. gravis_kernel_time
  * returns the summed milliseconds and launch count for `name`
- * ("bf_fp64", "bf_mixed", "bh_walk", ...) and resets them. */
+ * ("bf_fp64", "bf_mixed", "bf_matrix", "bh_walk", "sph_query",
+ * "sph_density", "sph_force", and the octree phases "morton_sort" (root
+ * box, keys, radix sort) and "tree_build" (gather, levels, sums, groups),
+ * which include the gaps between their launches) and resets them. */
 gravis_status gravis_set_profiling(gravis_ctx* ctx, int enable);
 gravis_status gravis_kernel_time(gravis_ctx* ctx, const char* name, double* ms_total,
                                  uint64_t* launches);
