@@ -10,7 +10,8 @@
 // Here one wavefront walks one target group (the first tree node on each
 // root-to-leaf path holding at most `group` (<= 64) particles, siblings
 // packed). The frontier is processed breadth first, 64 nodes at a time (one
-// per lane). A node is accepted for the whole group when
+// per lane), from LDS (levels of up to 512 nodes; larger levels spill their
+// tail to a per-wave global buffer). A node is accepted for the whole group when
 // s^2 < theta^2 d_min^2 with d_min the distance from its COM to the group's
 // bounding box (so never less accurate than the reference's per-particle
 // test); a rejected leaf contributes its particles directly; a rejected
@@ -44,7 +45,13 @@ __global__ void walk_reset_kernel(uint32_t* __restrict__ counter,
 }
 
 constexpr int kWalkWaves = 4;
-constexpr uint32_t kListCap = 1024;  // source indices per wave (LDS)
+constexpr uint32_t kListCap = 1024;  // source indices per wave (LDS), FP64 walk
+constexpr uint32_t kListCapMixed = 768;  // the same, mixed walk
+// Frontier entries per level held in LDS; a level's entries past this spill
+// to the wave's global frontier buffer. On the two-sphere 4M workload 91 %
+// of the node tests come from levels of at most 512 nodes.
+constexpr uint32_t kLdsFront = 512;
+static_assert(kLdsFront % 64 == 0, "a 64-node chunk is wholly in LDS or wholly in memory");
 // Frontier at which a group's breadth-first walk stops and its targets are
 // walked one at a time (measured: 2048-8192 all give the shortest C3 and
 // single-sphere walks; 16384 lets a few groups whose box spans a sphere's
@@ -71,9 +78,10 @@ __device__ __forceinline__ void pair_ref(const double4 s, double px, double py, 
 // centre (separations formed in FP64 first), pair forces are FP32 (v_rsq_f32)
 // and every round's FP32 sum is Kahan-added; the opening test, tree sums and
 // output stay FP64.
-// MIXED is capped at 96 VGPRs (5 wavefronts per SIMD; two 4-byte values
-// spill to scratch, reloaded outside the pair sweep loops); the
-// FP64 walk uses 127 and runs 4
+// The breadth-first frontier of each level lives in LDS up to kLdsFront
+// entries (the rest in the wave's global buffer). MIXED: 96 VGPRs, no spill,
+// 32 KB of LDS per block (768-entry lists, FP32 stages): 5 wavefronts per
+// SIMD. FP64: about 121 VGPRs and 40 KB: 4 wavefronts per SIMD.
 template <bool MIXED>
 __global__ __launch_bounds__(kWalkWaves* kWave)
 __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
@@ -83,17 +91,23 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
     const double* __restrict__ eps2_t, uint32_t* __restrict__ frontier,
     uint32_t fcap, uint32_t* __restrict__ counter, uint32_t* __restrict__ flag,
     double* __restrict__ part, unsigned long long* __restrict__ stats) {
-    __shared__ uint32_t lists[kWalkWaves][kListCap];
-    __shared__ double4 stages[kWalkWaves][kWave];
+    // LDS per block: 32 KB (mixed: 768-entry lists, FP32 stages) for five
+    // blocks per CU, 40 KB (FP64) for four
+    constexpr uint32_t kList = MIXED ? kListCapMixed : kListCap;
+    __shared__ uint32_t lists[kWalkWaves][kList];
+    __shared__ uint32_t fronts[kWalkWaves][2][kLdsFront];
+    __shared__ __align__(16) unsigned char stage_mem[kWalkWaves][kWave * (MIXED ? 16 : 32)];
     unsigned long long t_nodes = 0, t_acc = 0, t_leafp = 0, t_pairs = 0;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
     uint32_t* list = lists[w];
-    double4* stage = stages[w];
-    float4* stage32 = reinterpret_cast<float4*>(stage);
+    double4* stage = reinterpret_cast<double4*>(stage_mem[w]);
+    float4* stage32 = reinterpret_cast<float4*>(stage_mem[w]);
     const size_t slot = (size_t)blockIdx.x * kWalkWaves + w;
-    uint32_t* const fa0 = frontier + slot * 2 * (size_t)fcap;
-    uint32_t* const fb0 = fa0 + fcap;
+    // level buffer b: entry p < kLdsFront in fronts[w][b], the rest in
+    // the wave's global buffer at the same index
+    uint32_t* const fg0 = frontier + slot * 2 * (size_t)fcap;
+    uint32_t* const fg1 = fg0 + fcap;
     const double side0 = 2.0 * box[3];  // root side, octree.rs:81-85
     // a group walk gives up at a smaller frontier than a one-target walk:
     // the groups that reach it (outlying targets whose box spans the core)
@@ -243,8 +257,8 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
         // appends the uniform index range [first, first + count) in order
         auto append_range = [&](uint32_t first, uint32_t count) {
             while (count > 0) {
-                if (nl == kListCap) flush();
-                const uint32_t k = min(count, kListCap - nl);
+                if (nl == kList) flush();
+                const uint32_t k = min(count, kList - nl);
                 for (uint32_t r = lane; r < k; r += kWave) list[nl + r] = first + r;
                 nl += k;
                 first += k;
@@ -255,9 +269,8 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
         // targets; returns true if a next frontier would exceed cap.
         auto walk = [&](const double (&lo)[3], const double (&hi)[3], int ntargets,
                         uint32_t cap) -> bool {
-            uint32_t* fa = fa0;
-            uint32_t* fb = fb0;
-            if (lane == 0) fa[0] = 0u;
+            int cur = 0;  // wave-uniform: level buffer being read
+            if (lane == 0) fronts[w][0][0] = 0u;
             uint32_t nf = 1;
             bool overflow = false;
             __threadfence_block();
@@ -267,7 +280,10 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
                     const uint32_t idx = base + lane;
                     const bool valid = idx < nf;
                     t_nodes += min(nf - base, (uint32_t)kWave);
-                    const uint32_t v = valid ? fa[idx] : 0u;
+                    const uint32_t* fl = fronts[w][cur];
+                    const uint32_t* fgr = cur ? fg1 : fg0;
+                    // base is a multiple of 64: the chunk is in one place
+                    const uint32_t v = !valid ? 0u : (base < kLdsFront ? fl[idx] : fgr[idx]);
                     const uint4 md = meta[v];
                     const double4 c = src[(size_t)n + v];
                     const uint32_t level = md.w & 255u;
@@ -290,8 +306,8 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
                     t_leafp += total - (uint32_t)__popcll(accb);
                     t_pairs += (unsigned long long)total * (unsigned long long)ntargets;
                     if (total > 0) {
-                        if (nl + total > kListCap) flush();
-                        if (total <= kListCap) {
+                        if (nl + total > kList) flush();
+                        if (total <= kList) {
                             const uint32_t pos = nl + sc - add;
                             if (accept) list[pos] = (uint32_t)n + v;
                             else if (direct)
@@ -322,14 +338,18 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
                         break;
                     }
                     const uint32_t ex0 = nn + kc - k;
-                    for (uint32_t q = 0; q < k; ++q) fb[ex0 + q] = md.z + q;
+                    uint32_t* flw = fronts[w][cur ^ 1];
+                    uint32_t* fgw = cur ? fg0 : fg1;
+                    for (uint32_t q = 0; q < k; ++q) {
+                        const uint32_t pos = ex0 + q;
+                        if (pos < kLdsFront) flw[pos] = md.z + q;
+                        else fgw[pos] = md.z + q;
+                    }
                     nn += ktot;
                 }
                 __threadfence_block();
                 __builtin_amdgcn_wave_barrier();
-                uint32_t* t = fa;
-                fa = fb;
-                fb = t;
+                cur ^= 1;
                 nf = nn;
             }
             flush();
@@ -382,7 +402,8 @@ gravis_status bh_walk(Ctx* c, const gravis_solver* s, const Octree& t, const Fin
     const size_t n = t.n;
     const int ni = (int)n;
     hipStream_t st = c->stream;
-    const int per_cu = s->precision == GRAVIS_MIXED ? 5 : 4;  // resident blocks per CU
+    // resident blocks per CU: 32 KB of LDS each (mixed), 40 KB (FP64)
+    const int per_cu = s->precision == GRAVIS_MIXED ? 5 : 4;
     const int blocks = (int)std::min<long>((long)(n / 64 / kWalkWaves) + 1,
                                            (long)c->num_cus * per_cu);
     const uint32_t fcap = (uint32_t)std::min<size_t>(t.cap, c->bh_frontier_cap);
