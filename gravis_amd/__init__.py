@@ -25,7 +25,8 @@ from . import _lib
 from ._lib import GravisError
 
 __all__ = ["scenarios", "Particles", "Context", "GravitySolver", "NoGravity", "BruteForce", "BarnesHut",
-           "LeapfrogKDK", "diagnostics", "GravisError", "default_context"]
+           "LeapfrogKDK", "diagnostics", "GravisError", "default_context", "render_pack",
+           "GPU_PARTICLE"]
 
 
 class Context:
@@ -310,6 +311,25 @@ class _Diagnostics:
 
 
 diagnostics = _Diagnostics()
+
+# GpuParticle (crates/render-core/src/gpu_types.rs:5-9): 32 bytes
+GPU_PARTICLE = np.dtype([("position", np.float32, 3), ("mass", np.float32),
+                         ("color", np.float32, 4)])
+
+
+def render_pack(p: Particles, ctx: Context | None = None) -> np.ndarray:
+    """The render view of p: one GpuParticle per particle (positions, masses
+    and internal energies rounded to f32 as the sim thread does,
+    crates/native-app/src/sim_thread.rs:250-258; colours by particle_color,
+    crates/render-core/src/color.rs:9-52), packed on the GPU
+    (gravis_render_pack). Returns a structured array of GPU_PARTICLE."""
+    ctx = ctx or default_context()
+    out = np.zeros(p.count, dtype=GPU_PARTICLE)
+    ctx.check(ctx._L.gravis_render_pack(ctx.handle, p.count,
+                                        *[_ptr(getattr(p, f)) for f in ("x", "y", "z", "mass")],
+                                        _ptr(p.internal_energy), _u8(p.particle_type),
+                                        out.ctypes.data_as(C.c_void_p)))
+    return out
 
 # Particles field -> gravis_sph_state member
 _SPH_MAP = dict(x="x", y="y", z="z", vx="vx", vy="vy", vz="vz", ax="ax", ay="ay", az="az",

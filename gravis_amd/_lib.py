@@ -36,7 +36,8 @@ EXPORTS = (
     "gravis_snapshot_read", "gravis_scenario_cold_collapse", "gravis_galaxy_default_params",
     "gravis_scenario_galaxy_collision", "gravis_galaxy_suggested_dt",
     "gravis_galaxy_suggested_softening", "gravis_get_tuning", "gravis_autotune",
-    "gravis_accel_device_noalloc",
+    "gravis_accel_device_noalloc", "gravis_render_pack", "gravis_render_pack_device",
+    "gravis_ensemble_accel",
 )
 
 
@@ -126,6 +127,8 @@ def load() -> C.CDLL:
         "gravis_accel": (st, [vp, C.POINTER(Solver), sz] + [vp] * 7),
         "gravis_accel_device": (st, [vp, C.POINTER(Solver), sz] + [vp] * 7),
         "gravis_accel_device_noalloc": (st, [vp, C.POINTER(Solver), sz] + [vp] * 7),
+        "gravis_render_pack": (st, [vp, sz] + [vp] * 7),
+        "gravis_render_pack_device": (st, [vp, sz] + [vp] * 7),
         "gravis_kdk": (st, [vp, C.POINTER(Solver), sz] + [vp] * 10 + [d, C.c_uint64]),
         "gravis_kdk_device": (st, [vp, C.POINTER(Solver), sz] + [vp] * 10 + [d, C.c_uint64]),
         "gravis_diagnostics": (st, [vp, sz, d, i32] + [vp] * 9 + [d, C.c_uint64,
@@ -146,6 +149,7 @@ def load() -> C.CDLL:
         "gravis_ensemble_kdk": (st, [vp, C.POINTER(Solver), i32, vp] + [vp] * 10
                                 + [d, C.c_uint64, i32]),
         "gravis_ensemble_accel_device": (st, [vp, C.POINTER(Solver), i32, vp] + [vp] * 7),
+        "gravis_ensemble_accel": (st, [vp, C.POINTER(Solver), i32, vp] + [vp] * 7),
         "gravis_ensemble_kdk_device": (st, [vp, C.POINTER(Solver), i32, vp] + [vp] * 10
                                        + [d, C.c_uint64, i32]),
         "gravis_ensemble_adjoint": (st, [vp, C.POINTER(Solver), i32, vp] + [vp] * 7

@@ -378,6 +378,10 @@ gravis_status gravis_kdk_device(gravis_ctx* g, const gravis_solver* s, size_t n,
     fa.ax = ax; fa.ay = ay; fa.az = az;
     fa.vx = vx; fa.vy = vy; fa.vz = vz;
     fa.half_dt = h;
+    if (s->algorithm == GRAVIS_BRUTE_FORCE && s->precision == GRAVIS_FP64 &&
+        n <= c->kdk_fused_max_n)
+        return gravis::kdk_small(c, n, x, y, z, vx, vy, vz, m, ax, ay, az, s->softening, dt,
+                                 nsteps);
     for (uint64_t step = 0; step < nsteps; ++step) {
         GRAVIS_TRY(gravis::kick_drift(c, n, x, y, z, vx, vy, vz, ax, ay, az, h, dt));
         GRAVIS_TRY(force_device(c, s, n, x, y, z, m, fa));
@@ -430,6 +434,44 @@ gravis_status gravis_diagnostics_device(gravis_ctx* g, size_t n, double softenin
     return GRAVIS_OK;
 }
 
+gravis_status gravis_render_pack_device(gravis_ctx* g, size_t n, const double* x,
+                                       const double* y, const double* z, const double* m,
+                                       const double* u, const uint8_t* ptype, void* out) {
+    if (!g) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    if (n == 0) return GRAVIS_OK;
+    if (any_null({x, y, z, m, out})) return fail(c, GRAVIS_INVALID_ARG, "null array");
+    GRAVIS_HIP(c, hipSetDevice(c->device));
+    return gravis::render_pack(c, n, x, y, z, m, u, ptype, out);
+}
+
+gravis_status gravis_render_pack(gravis_ctx* g, size_t n, const double* x, const double* y,
+                                const double* z, const double* m, const double* u,
+                                const uint8_t* ptype, void* out) {
+    if (!g) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    if (n == 0) return GRAVIS_OK;
+    if (any_null({x, y, z, m, out})) return fail(c, GRAVIS_INVALID_ARG, "null array");
+    GRAVIS_HIP(c, hipSetDevice(c->device));
+    double* base;
+    GRAVIS_TRY(gravis::buf(c, "render_in", 5 * n, &base));
+    uint8_t* dt = nullptr;
+    float* dout;
+    GRAVIS_TRY(gravis::buf(c, "render_out", 8 * n, &dout));
+    GRAVIS_TRY(h2d(c, base, x, n)); GRAVIS_TRY(h2d(c, base + n, y, n));
+    GRAVIS_TRY(h2d(c, base + 2 * n, z, n)); GRAVIS_TRY(h2d(c, base + 3 * n, m, n));
+    if (u) GRAVIS_TRY(h2d(c, base + 4 * n, u, n));
+    if (ptype) {
+        GRAVIS_TRY(gravis::buf(c, "render_type", n, &dt));
+        GRAVIS_HIP(c, hipMemcpyAsync(dt, ptype, n, hipMemcpyHostToDevice, c->stream));
+    }
+    GRAVIS_TRY(gravis::render_pack(c, n, base, base + n, base + 2 * n, base + 3 * n,
+                                   u ? base + 4 * n : nullptr, dt, dout));
+    GRAVIS_HIP(c, hipMemcpyAsync(out, dout, 32 * n, hipMemcpyDeviceToHost, c->stream));
+    GRAVIS_HIP(c, hipStreamSynchronize(c->stream));
+    return GRAVIS_OK;
+}
+
 gravis_status gravis_diagnostics(gravis_ctx* g, size_t n, double softening,
                                  int compute_potential, const double* x, const double* y,
                                  const double* z, const double* vx, const double* vy,
@@ -476,6 +518,28 @@ gravis_status gravis_ensemble_accel_device(gravis_ctx* g, const gravis_solver* s
         return fail(c, GRAVIS_INVALID_ARG, "null array");
     GRAVIS_HIP(c, hipSetDevice(c->device));
     return gravis::ensemble_accel(c, s, nsys, offsets, x, y, z, m, ax, ay, az);
+}
+
+gravis_status gravis_ensemble_accel(gravis_ctx* g, const gravis_solver* s, int nsys,
+                                    const uint64_t* offsets, const double* x, const double* y,
+                                    const double* z, const double* m, double* ax, double* ay,
+                                    double* az) {
+    if (!g) return GRAVIS_INVALID_ARG;
+    Ctx* c = &g->c;
+    if (any_null({offsets, x, y, z, m, ax, ay, az}) || nsys <= 0)
+        return fail(c, GRAVIS_INVALID_ARG, "null array or empty ensemble");
+    const size_t n = offsets[nsys];
+    if (n == 0) return GRAVIS_OK;
+    GRAVIS_HIP(c, hipSetDevice(c->device));
+    HostState d;
+    GRAVIS_TRY(stage(c, n, &d));
+    GRAVIS_TRY(h2d(c, d.x, x, n)); GRAVIS_TRY(h2d(c, d.y, y, n)); GRAVIS_TRY(h2d(c, d.z, z, n));
+    GRAVIS_TRY(h2d(c, d.m, m, n));
+    GRAVIS_TRY(gravis::ensemble_accel(c, s, nsys, offsets, d.x, d.y, d.z, d.m, d.ax, d.ay,
+                                      d.az));
+    GRAVIS_TRY(d2h(c, ax, d.ax, n)); GRAVIS_TRY(d2h(c, ay, d.ay, n)); GRAVIS_TRY(d2h(c, az, d.az, n));
+    GRAVIS_HIP(c, hipStreamSynchronize(c->stream));
+    return GRAVIS_OK;
 }
 
 gravis_status gravis_ensemble_kdk_device(gravis_ctx* g, const gravis_solver* s, int nsys,
@@ -609,6 +673,13 @@ gravis_status gravis_set_tuning(gravis_ctx* g, const char* key, int64_t value) {
         if (value < 1 || value > 4096)
             return fail(c, GRAVIS_INVALID_ARG, "sph_pool_init must be in [1, 4096]");
         c->sph_pool_init = (uint32_t)value;
+        return GRAVIS_OK;
+    }
+    if (std::strcmp(key, "kdk_fused_max_n") == 0) {
+        if (value < 0 || value > gravis::kSmallN)
+            return fail(c, GRAVIS_INVALID_ARG, "kdk_fused_max_n must be in [0, %d]",
+                        gravis::kSmallN);
+        c->kdk_fused_max_n = (uint32_t)value;
         return GRAVIS_OK;
     }
     if (std::strcmp(key, "bf_ipt") == 0) {

@@ -666,6 +666,70 @@ __global__ __launch_bounds__(BLOCK) void bf_fp64_kernel(const double4* __restric
     }
 }
 
+// Leapfrog KDK of a small system (n <= kSmallN) with FP64 brute force, all
+// nsteps in one launch of one workgroup: thread i keeps particle i's state
+// in registers, the positions and masses of the step go through LDS. Every
+// operation is the multi-launch path's (kick_drift_kernel, bf_fp64_kernel
+// with its single 256-record tile and zero-mass padding, finish_kernel's
+// 0 + partial and kick), in the same order, so the results are
+// bit-identical to it; only the launches between the steps are gone.
+template <bool MASK>
+__global__ __launch_bounds__(kSmallN) void kdk_small_kernel(
+    int n, double* __restrict__ x, double* __restrict__ y, double* __restrict__ z,
+    double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
+    const double* __restrict__ m, double* __restrict__ ax, double* __restrict__ ay,
+    double* __restrict__ az, double eps2, double h, double dt, uint32_t nsteps) {
+    __shared__ double4 tile[kSmallN];
+    const int i = threadIdx.x;
+    const bool live = i < n;
+    double px = 0, py = 0, pz = 0, pvx = 0, pvy = 0, pvz = 0, pm = 0, a0 = 0, a1 = 0, a2 = 0;
+    if (live) {
+        px = x[i]; py = y[i]; pz = z[i];
+        pvx = vx[i]; pvy = vy[i]; pvz = vz[i];
+        pm = m[i];
+        a0 = ax[i]; a1 = ay[i]; a2 = az[i];
+    }
+    const double e2 = in_vgpr(eps2);
+    const Rinv3Consts k3 = rinv3_consts();
+    for (uint32_t step = 0; step < nsteps; ++step) {
+        // v += a h; x += v dt (integrator.rs:53-64)
+        pvx = pvx + a0 * h;
+        pvy = pvy + a1 * h;
+        pvz = pvz + a2 * h;
+        px = px + pvx * dt;
+        py = py + pvy * dt;
+        pz = pz + pvz * dt;
+        if (live) tile[i] = make_double4(px, py, pz, pm);
+        __syncthreads();
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        // the records past n (zero-mass padding in bf_fp64_kernel's tile)
+        // add m r^-3 d = 0 to a non-zero or +0 sum: skipping them changes
+        // no bit
+#pragma unroll 4
+        for (int j = 0; j < n; ++j) {
+            const double4 s = tile[j];
+            const double dx = s.x - px, dy = s.y - py, dz = s.z - pz;
+            const double f = pair_f64<MASK>(dx, dy, dz, s.w, e2, k3);
+            sx = __builtin_fma(f, dx, sx);
+            sy = __builtin_fma(f, dy, sy);
+            sz = __builtin_fma(f, dz, sz);
+        }
+        // the partial of one chunk (0 + tile sum), then v += a h
+        a0 = 0.0 + (0.0 + sx);
+        a1 = 0.0 + (0.0 + sy);
+        a2 = 0.0 + (0.0 + sz);
+        pvx = pvx + a0 * h;
+        pvy = pvy + a1 * h;
+        pvz = pvz + a2 * h;
+        __syncthreads();  // the tile is rewritten by the next step
+    }
+    if (live) {
+        x[i] = px; y[i] = py; z[i] = pz;
+        vx[i] = pvx; vy[i] = pvy; vz[i] = pvz;
+        ax[i] = a0; ay[i] = a1; az[i] = a2;
+    }
+}
+
 constexpr size_t kSortMinN = 2048;
 
 template <int IPT, int TJ>
@@ -880,6 +944,30 @@ gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
     }
     GRAVIS_TRY(check_launch(c, "bf kernel"));
     return finish_forces(c, part, S, ni, perm, fa);
+}
+
+
+gravis_status kdk_small(Ctx* c, size_t n, double* x, double* y, double* z, double* vx,
+                        double* vy, double* vz, const double* m, double* ax, double* ay,
+                        double* az, double softening, double dt, uint64_t nsteps) {
+    if (n == 0 || n > (size_t)kSmallN) return fail(c, GRAVIS_INTERNAL, "kdk_small: n = %zu", n);
+    const double eps2 = softening * softening;
+    const double h = 0.5 * dt;
+    const bool mask = !(eps2 >= kMaskEps2D);
+    const unsigned threads = (unsigned)((n + kWave - 1) / kWave * kWave);
+    // launches of at most 2^20 steps each
+    while (nsteps > 0) {
+        const uint32_t k = (uint32_t)std::min<uint64_t>(nsteps, 1u << 20);
+        if (mask)
+            kdk_small_kernel<true><<<1, threads, 0, c->stream>>>((int)n, x, y, z, vx, vy, vz, m,
+                                                                 ax, ay, az, eps2, h, dt, k);
+        else
+            kdk_small_kernel<false><<<1, threads, 0, c->stream>>>((int)n, x, y, z, vx, vy, vz, m,
+                                                                  ax, ay, az, eps2, h, dt, k);
+        GRAVIS_TRY(check_launch(c, "kdk_small"));
+        nsteps -= k;
+    }
+    return GRAVIS_OK;
 }
 
 }  // namespace gravis

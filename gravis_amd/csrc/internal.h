@@ -56,6 +56,9 @@ struct Ctx {
     // initial neighbour-pool capacity in entries per particle (the pool
     // grows past it, keeping what is written, when a query round runs out)
     uint32_t sph_pool_init = 96;
+    // FP64 brute-force KDK of at most this many particles runs fused in one
+    // workgroup (0 disables; at most kSmallN)
+    uint32_t kdk_fused_max_n = 256;
     // Brute-force launch shape (gravis_set_tuning / gravis_autotune):
     // targets per lane of the mixed kernel (wavefront groups of 64 or 128),
     // j records per staged tile (128 or 256), and the wavefronts per CU the
@@ -431,6 +434,13 @@ gravis_status finish_forces(Ctx* c, const double* part, int nchunks, int n,
                             const uint32_t* perm, const FinishArgs& fa);
 
 // Brute-force force on device SoA arrays, result delivered per FinishArgs.
+// Particles up to which a FP64 brute-force KDK runs all its steps in one
+// workgroup launch (kdk_small, bf.hip).
+constexpr int kSmallN = 256;
+gravis_status kdk_small(Ctx* c, size_t n, double* x, double* y, double* z, double* vx,
+                        double* vy, double* vz, const double* m, double* ax, double* ay,
+                        double* az, double softening, double dt, uint64_t nsteps);
+
 gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x,
                        const double* y, const double* z, const double* m, const FinishArgs& fa);
 
@@ -465,6 +475,11 @@ gravis_status sph_run(Ctx* c, const gravis_sph_params* P, const gravis_solver* g
 gravis_status kick_drift(Ctx* c, size_t n, double* x, double* y, double* z, double* vx,
                          double* vy, double* vz, const double* ax, const double* ay,
                          const double* az, double half_dt, double dt);
+
+// GpuParticle records (32 B: f32 position, mass, colour) of n particles
+// into out (render.hip).
+gravis_status render_pack(Ctx* c, size_t n, const double* x, const double* y, const double* z,
+                          const double* m, const double* u, const uint8_t* type, void* out);
 
 gravis_status diag_device(Ctx* c, size_t n, double eps, int with_potential, const double* x,
                           const double* y, const double* z, const double* vx, const double* vy,

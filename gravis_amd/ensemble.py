@@ -60,6 +60,18 @@ def plummer_ensemble(cnts, seed0: int = 42) -> dict:
     return st
 
 
+def accel(state: dict, counts, eps: float, precision: str = "fp64",
+          ctx: Context | None = None) -> None:
+    """a = F(x) for every system into state["ax"/"ay"/"az"] (assigns;
+    gravis_ensemble_accel)."""
+    ctx = ctx or default_context()
+    off = _offsets(counts)
+    s = _solver(eps, precision)
+    ctx.check(ctx._L.gravis_ensemble_accel(
+        ctx.handle, C.byref(s), len(counts), _p(off),
+        *[_p(state[k]) for k in ("x", "y", "z", "m", "ax", "ay", "az")]))
+
+
 def run(state: dict, counts, eps: float, dt: float, nsteps: int, precision: str = "fp64",
         use_graph: bool = True, ctx: Context | None = None) -> None:
     """nsteps of KDK for every system; state holds numpy arrays x..vz, m,

@@ -155,6 +155,29 @@ gravis_status gravis_diagnostics_device(gravis_ctx* ctx, size_t n, double soften
                                         const double* u, const uint8_t* ptype, double time,
                                         uint64_t step, gravis_diag* out);
 
+/* ---- render view (crates/render-core/src/gpu_types.rs:5-9) --------------
+ * GpuParticle: 32 bytes per particle, position f32 x3, mass f32, colour
+ * f32 x4 (RGBA). Positions, masses and internal energies are rounded to
+ * f32 as the sim thread's `as f32` does (crates/native-app/src/
+ * sim_thread.rs:250-258); the colour is particle_color (crates/render-core/
+ * src/color.rs:9-52): gas (type 4) by log10 of its internal energy, other
+ * types from the fixed palette. u and ptype may be NULL (u = 0, type 0).
+ * The _device form takes device pointers (out: 32 n bytes of device
+ * memory a render thread can map) and runs asynchronously on the context
+ * stream; gravis_render_pack takes host pointers and synchronises. */
+typedef struct gravis_gpu_particle {
+    float position[3];
+    float mass;
+    float color[4];
+} gravis_gpu_particle;
+
+gravis_status gravis_render_pack_device(gravis_ctx* ctx, size_t n, const double* x,
+                                        const double* y, const double* z, const double* m,
+                                        const double* u, const uint8_t* ptype, void* out);
+gravis_status gravis_render_pack(gravis_ctx* ctx, size_t n, const double* x, const double* y,
+                                 const double* z, const double* m, const double* u,
+                                 const uint8_t* ptype, void* out);
+
 /* ---- batched ensemble and reverse mode (no reference counterpart) -------
  * nsys independent systems stored back to back: system s owns particles
  * [offsets[s], offsets[s+1]) of the SoA arrays (offsets is a HOST array of
@@ -162,8 +185,13 @@ gravis_status gravis_diagnostics_device(gravis_ctx* ctx, size_t n, double soften
  * (integrator.rs:41-77) with brute-force gravity among its own particles
  * (solver->algorithm must be GRAVIS_BRUTE_FORCE; FP64 or MIXED). */
 
-/* a = F(x) for every system (assigns, unlike gravis_accel). Device
- * pointers, asynchronous. */
+/* a = F(x) for every system (assigns, unlike gravis_accel). Host
+ * pointers. */
+gravis_status gravis_ensemble_accel(gravis_ctx* ctx, const gravis_solver* solver, int nsys,
+                                    const uint64_t* offsets, const double* x, const double* y,
+                                    const double* z, const double* m, double* ax, double* ay,
+                                    double* az);
+/* Same with device pointers, asynchronous. */
 gravis_status gravis_ensemble_accel_device(gravis_ctx* ctx, const gravis_solver* solver,
                                            int nsys, const uint64_t* offsets, const double* x,
                                            const double* y, const double* z, const double* m,

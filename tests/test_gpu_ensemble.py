@@ -259,3 +259,16 @@ def test_autograd_function_matches_adjoint(E):
     assert np.abs(gx[0] - ref["x"]).max() <= 1e-12 * np.abs(ref["x"]).max()
     assert np.abs(v0.grad.cpu().numpy()[2] - ref["vz"]).max() <= 1e-12 * np.abs(ref["vz"]).max()
     assert np.abs(m.grad.cpu().numpy() - ref["m"]).max() <= 1e-12 * np.abs(ref["m"]).max()
+
+
+def test_host_ensemble_accel_matches_device(E):
+    """gravis_ensemble_accel (host arrays, what a Rust host calls before
+    gravis_ensemble_kdk) gives the device entry point's bits."""
+    counts = [300, 1, 0, 517]
+    for precision in ("fp64", "mixed"):
+        st, _ = make_ensemble(counts, 5)
+        dev = {k: v.copy() for k, v in st.items()}
+        init_accel(E, dev, counts, 0.05, precision)
+        E.accel(st, counts, 0.05, precision)
+        for k in ("ax", "ay", "az"):
+            assert np.array_equal(st[k], dev[k]), (precision, k)
