@@ -575,9 +575,45 @@ __global__ __launch_bounds__(256) void sph_pack_kernel(int n, const double4* __r
     rec[s] = ForceRec{pos[s], vh[s], aux[s], make_double4(b.x, b.y, 0.0, 0.0)};
 }
 
+// The FP32-pair force pass reads a 64-byte record per neighbour: the FP64
+// position (separations are formed in FP64) and the FP32 values its pair
+// terms use, with the pressure term P / (Omega rho^2) formed once per
+// particle (the expression the pair loop used to evaluate per pair, in the
+// same FP32 operations). P and u, needed by conduction only, sit beside.
+struct __align__(16) ForceRec32 {
+    double x, y, z;
+    float m, vx, vy, vz, h, rho, pr, c, alpha, spare;
+};
+static_assert(sizeof(ForceRec32) == 64, "one 64-byte record per particle");
+
+__global__ __launch_bounds__(256) void sph_pack32_kernel(
+    int n, const double4* __restrict__ pos, const double4* __restrict__ vh,
+    const double4* __restrict__ aux, const double2* __restrict__ aux2,
+    ForceRec32* __restrict__ rec, float2* __restrict__ cond) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const double4 p = pos[s], v = vh[s], a = aux[s];
+    const double2 b = aux2[s];
+    const float rho = fmaxf((float)a.x, (float)kDensityFloor);
+    const float pf = (float)a.y, om = (float)b.x;
+    ForceRec32 r;
+    r.x = p.x; r.y = p.y; r.z = p.z;
+    r.m = (float)p.w;
+    r.vx = (float)v.x; r.vy = (float)v.y; r.vz = (float)v.z;
+    r.h = (float)v.w;
+    r.rho = rho;
+    r.pr = pf / (om * rho * rho);
+    r.c = (float)a.z;
+    r.alpha = (float)a.w;
+    r.spare = 0.0f;
+    rec[s] = r;
+    cond[s] = make_float2(pf, (float)b.y);
+}
+
 // As gather_batch for the force pass: kFGather members' index and packed
-// record.
+// record (kFGather32 of the 64-byte records).
 constexpr uint32_t kFGather = 2;
+constexpr uint32_t kFGather32 = 4;
 
 __device__ __forceinline__ void gather_force_batch(
     const ForceRec* __restrict__ rec, const uint32_t* __restrict__ pool, uint2 nb, uint32_t k0,
@@ -825,9 +861,17 @@ __device__ __forceinline__ float grad_factor_f(float r, float ir, float ih) {
 // compute_forces (forces.rs:80-230), one lane per particle over its list.
 // F32: separations formed in FP64 and rounded, pair terms in FP32, sums in
 // FP64. alpha_switch 0: dalpha/dt = 0.
+struct ForceIn {
+    const ForceRec* rec;      // FP64 pairs
+    const ForceRec32* rec32;  // FP32 pairs
+    const float2* cond;       // FP32 pairs: (P, u)
+    const double4 *pos, *vh, *aux;
+    const double2* aux2;
+};
+
 template <int K, bool F32>
 __global__ __launch_bounds__(256) void sph_force_kernel(
-    int n, const uint32_t* __restrict__ perm, const ForceRec* __restrict__ rec,
+    int n, const uint32_t* __restrict__ perm, ForceIn in,
     const uint8_t* __restrict__ qstate, const uint32_t* __restrict__ pool,
     const uint2* __restrict__ nbr, gravis_sph_params P, ForceOut o) {
     const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -841,10 +885,11 @@ __global__ __launch_bounds__(256) void sph_force_kernel(
         if (o.nneigh) o.nneigh[pi] = 0u;
         return;
     }
-    const double4 me = rec[s].p;
-    const double4 vi = rec[s].v;
-    const double4 ai = rec[s].a;
-    const double2 bi = make_double2(rec[s].b.x, rec[s].b.y);
+    const ForceRec* __restrict__ rec = in.rec;
+    const double4 me = F32 ? in.pos[s] : rec[s].p;
+    const double4 vi = F32 ? in.vh[s] : rec[s].v;
+    const double4 ai = F32 ? in.aux[s] : rec[s].a;
+    const double2 bi = F32 ? in.aux2[s] : make_double2(rec[s].b.x, rec[s].b.y);
     const double hi = vi.w;
     const double rho_i = fmax(ai.x, kDensityFloor);
     const double p_i = ai.y, c_i = ai.z, alpha_i = ai.w;
@@ -920,45 +965,47 @@ __global__ __launch_bounds__(256) void sph_force_kernel(
         const float beta = (float)P.beta_visc, acond = (float)P.alpha_cond;
         const float vxi = (float)vi.x, vyi = (float)vi.y, vzi = (float)vi.z;
         float mmuf = 0.0f;
-        for (uint32_t k0 = 0; k0 < nb.y; k0 += kFGather) {
-            uint32_t jb[kFGather];
-            double4 pb[kFGather], vb[kFGather], ab[kFGather];
-            double2 bb[kFGather];
-            gather_force_batch(rec, pool, nb, k0, jb, pb, vb, ab, bb);
+        const ForceRec32* __restrict__ r32 = in.rec32;
+        for (uint32_t k0 = 0; k0 < nb.y; k0 += kFGather32) {
+            // the batch's indices, then its records, then the pairs in order
+            uint32_t jb[kFGather32];
+            ForceRec32 rb[kFGather32];
 #pragma unroll
-            for (uint32_t e = 0; e < kFGather; ++e) {
+            for (uint32_t e = 0; e < kFGather32; ++e)
+                jb[e] = k0 + e < nb.y ? pool[nb.x + k0 + e] : 0u;
+#pragma unroll
+            for (uint32_t e = 0; e < kFGather32; ++e) rb[e] = r32[jb[e]];
+#pragma unroll
+            for (uint32_t e = 0; e < kFGather32; ++e) {
             if (k0 + e >= nb.y) break;
             const uint32_t j = jb[e];
             if (j == (uint32_t)s) continue;
-            const double4 pj = pb[e];
-            const float dx = (float)(me.x - pj.x), dy = (float)(me.y - pj.y),
-                        dz = (float)(me.z - pj.z);
+            const ForceRec32& q = rb[e];
+            const float dx = (float)(me.x - q.x), dy = (float)(me.y - q.y),
+                        dz = (float)(me.z - q.z);
             const float r2 = dx * dx + dy * dy + dz * dz;
-            const float r = __builtin_sqrtf(r2);
-            const double4 vj = vb[e];
-            const float hj = (float)vj.w;
+            // r and 1/r from one v_rsq_f32 (0 for coincident points)
+            const float rr = __builtin_amdgcn_rsqf(r2);
+            const float ir = r2 > 0.0f ? rr : 0.0f;
+            const float r = r2 * ir;
+            const float hj = q.h;
             if (r >= 2.0f * hif && r >= 2.0f * hj) continue;
             const float havg = 0.5f * (hif + hj);
-            const float mj = (float)pj.w;
-            const double4 aj = ab[e];
-            const double2 bj = bb[e];
-            const float rhoj = fmaxf((float)aj.x, (float)kDensityFloor);
-            const float pj_ = (float)aj.y, cj = (float)aj.z, alj = (float)aj.w;
-            const float omj = (float)bj.x;
-            const float dvx = vxi - (float)vj.x, dvy = vyi - (float)vj.y, dvz = vzi - (float)vj.z;
+            const float mj = q.m;
+            const float rhoj = q.rho;
+            const float dvx = vxi - q.vx, dvy = vyi - q.vy, dvz = vzi - q.vz;
             const float vr = dvx * dx + dvy * dy + dvz * dz;
-            const float ir = r > 0.0f ? 1.0f / r : 0.0f;
-            const float iha = 1.0f / havg;
+            const float iha = __builtin_amdgcn_rcpf(havg);
             const float fi = grad_factor_f<K>(r, ir, ihi);
-            const float fj = grad_factor_f<K>(r, ir, 1.0f / hj);
+            const float fj = grad_factor_f<K>(r, ir, __builtin_amdgcn_rcpf(hj));
             const float fa = grad_factor_f<K>(r, ir, iha);
-            const float prj = pj_ / (omj * rhoj * rhoj);
+            const float prj = q.pr;
             float pij = 0.0f;
             if (vr < 0.0f) {
-                const float mu = havg * vr / (r2 + 0.01f * havg * havg);
+                const float mu = havg * vr * __builtin_amdgcn_rcpf(r2 + 0.01f * havg * havg);
                 mmuf = fmaxf(mmuf, fabsf(mu));
-                pij = (-0.5f * (ali + alj) * 0.5f * (ci + cj) * mu + beta * mu * mu) /
-                      (0.5f * (rhoi + rhoj));
+                pij = (-0.5f * (ali + q.alpha) * 0.5f * (ci + q.c) * mu + beta * mu * mu) *
+                      __builtin_amdgcn_rcpf(0.5f * (rhoi + rhoj));
             }
             const float fsum = pri * fi + prj * fj + pij * fa;
             axs -= (double)(mj * fsum * dx);
@@ -968,10 +1015,11 @@ __global__ __launch_bounds__(256) void sph_force_kernel(
             du += (double)(mj * (pri * vgi + 0.5f * pij * fa * vr));
             dv -= (double)(mj * vgi);
             if (acond > 0.0f) {
+                const float2 cj = in.cond[j];
                 const float rsum = rhoi + rhoj;
-                const float vsig = __builtin_sqrtf(fabsf(pif - pj_) / (0.5f * rsum));
+                const float vsig = __builtin_sqrtf(fabsf(pif - cj.x) / (0.5f * rsum));
                 const float agw = -fa * r;  // |grad W(h_avg)|
-                du += (double)(mj * acond * vsig * (ui - (float)bj.y) / rsum * agw);
+                du += (double)(mj * acond * vsig * (ui - cj.y) / rsum * agw);
             }
             }
         }
@@ -1265,12 +1313,24 @@ gravis_status sph_compute(Ctx* c, const gravis_sph_params* P, const gravis_sph_s
     }
     ForceOut o{S->ax, S->ay, S->az, S->dudt, S->rho, S->h, S->p, S->c,
                div_v, max_mu, dalpha, omega, nneigh};
-    ForceRec* frec;
-    GRAVIS_TRY(buf(c, "sph_frec", n, &frec));
+    ForceIn in{nullptr, nullptr, nullptr, t.src, vh, aux, aux2};
     timer_begin(c);
-    sph_pack_kernel<<<g256, 256, 0, st>>>(ni, t.src, vh, aux, aux2, frec);
+    if (P->pair_fp32) {
+        ForceRec32* r32;
+        float2* cond;
+        GRAVIS_TRY(buf(c, "sph_frec32", n, &r32));
+        GRAVIS_TRY(buf(c, "sph_cond", n, &cond));
+        sph_pack32_kernel<<<g256, 256, 0, st>>>(ni, t.src, vh, aux, aux2, r32, cond);
+        in.rec32 = r32;
+        in.cond = cond;
+    } else {
+        ForceRec* frec;
+        GRAVIS_TRY(buf(c, "sph_frec", n, &frec));
+        sph_pack_kernel<<<g256, 256, 0, st>>>(ni, t.src, vh, aux, aux2, frec);
+        in.rec = frec;
+    }
 #define GRAVIS_FORCE(K, F)                                                               \
-    sph_force_kernel<K, F><<<g256, 256, 0, st>>>(ni, t.perm, frec, qstate, pool, nbr, *P, o)
+    sph_force_kernel<K, F><<<g256, 256, 0, st>>>(ni, t.perm, in, qstate, pool, nbr, *P, o)
     if (P->kernel == 0 && !P->pair_fp32) GRAVIS_FORCE(0, false);
     else if (P->kernel == 0) GRAVIS_FORCE(0, true);
     else if (!P->pair_fp32) GRAVIS_FORCE(1, false);
