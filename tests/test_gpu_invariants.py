@@ -13,9 +13,9 @@ backend with the reference's scenarios, step counts and bounds:
   * barnes_hut.rs:141-194 -- Plummer N=500 with theta = 0.5 over 10
     dynamical times: virial deviation < 0.5 and dE/E < 0.01;
   * scaling.rs:14-76 -- Barnes-Hut is O(N log N): the log-log slope of the
-    walk's pair count over N = 1,000..20,000 (the reference's sizes) and of
-    the force time over N = 262,144..4,194,304 (sizes where the GPU is not
-    launch-bound) lie in [0.8, 1.8].
+    walk's pair count over N = 2,000..20,000 (the reference's sizes), and
+    of the device-resident force call and of its walk kernel over
+    N = 1,048,576..8,388,608 (sizes that fill the GPU), lie in [0.8, 1.8].
 """
 import sys
 import time
