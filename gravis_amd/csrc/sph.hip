@@ -159,6 +159,42 @@ __global__ __launch_bounds__(256) void sph_prep_kernel(
     radius[s] = 2.0 * hp * kPad;
 }
 
+// Stages candidates [c0, c0 + 64) of the group's leaf ranges (rg: first
+// particle, exclusive offset; nr ranges, total candidates), keeping only
+// those within sqrt(R2g) of the group box [lo, hi] (no target's exact test
+// can accept the others: every rounding in the box distance is monotone and
+// R2g is the largest R^2), compacted in order: stage[k] = (x, y, z, index)
+// for k < the returned count.
+__device__ __forceinline__ uint32_t stage_round(uint32_t c0, uint32_t total, uint32_t nr,
+                                                const uint2* rg,
+                                                const double4* __restrict__ pos,
+                                                const double (&lo)[3], const double (&hi)[3],
+                                                double R2g, int lane, uint64_t below,
+                                                double4* stage) {
+    const uint32_t c = c0 + (uint32_t)lane;
+    bool keep = false;
+    double4 pj = make_double4(0.0, 0.0, 0.0, 0.0);
+    uint32_t j = 0;
+    if (c < total) {
+        uint32_t a = 0, b = nr;  // last leaf whose offset <= c
+        while (b - a > 1) {
+            const uint32_t mid = (a + b) >> 1;
+            if (rg[mid].y <= c) a = mid;
+            else b = mid;
+        }
+        j = rg[a].x + (c - rg[a].y);
+        pj = pos[j];
+        const double ex = fmax(fmax(lo[0] - pj.x, pj.x - hi[0]), 0.0);
+        const double ey = fmax(fmax(lo[1] - pj.y, pj.y - hi[1]), 0.0);
+        const double ez = fmax(fmax(lo[2] - pj.z, pj.z - hi[2]), 0.0);
+        keep = ex * ex + ey * ey + ez * ez <= R2g;
+    }
+    const uint64_t kb = __ballot(keep);
+    if (keep) stage[__popcll(kb & below)] = make_double4(pj.x, pj.y, pj.z, (double)j);
+    lds_sync();
+    return (uint32_t)__popcll(kb);
+}
+
 // Breadth-first group query of one target group (see the file comment).
 // Targets with qstate == kQQuery get a list {start, count} in nbr and move
 // on to kQDensity (kQDone if their iteration already ended); a group that
@@ -304,21 +340,9 @@ __device__ __forceinline__ bool query_range(
             hit[q] = 0ull;
             cj[q] = 0u;
             if (q < nrounds) {
-                const uint32_t c = q * kWave + (uint32_t)lane;
-                if (c < total) {
-                    uint32_t a = 0, b = nr;  // last leaf whose offset <= c
-                    while (b - a > 1) {
-                        const uint32_t mid = (a + b) >> 1;
-                        if (rg[mid].y <= c) a = mid;
-                        else b = mid;
-                    }
-                    const uint32_t j = rg[a].x + (c - rg[a].y);
-                    const double4 pj = pos[j];
-                    stage[lane] = make_double4(pj.x, pj.y, pj.z, 0.0);
-                    cj[q] = j;
-                }
-                lds_sync();
-                const uint32_t m = min(total - q * kWave, (uint32_t)kWave);
+                const uint32_t m = stage_round(q * kWave, total, nr, rg, pos, lo, hi, R2g, lane,
+                                               below, stage);
+                cj[q] = (uint32_t)lane < m ? (uint32_t)stage[lane].w : 0u;
                 if (act) {
                     for (uint32_t k = 0; k < m; ++k) {
                         const double4 c4 = stage[k];
@@ -372,20 +396,7 @@ __device__ __forceinline__ bool query_range(
             if (act) nbr[s] = make_uint2(wpos, mine);
         }
         for (uint32_t q0 = 0; q0 < total; q0 += kWave) {
-            const uint32_t q = q0 + (uint32_t)lane;
-            if (q < total) {
-                uint32_t a = 0, b = nr;  // last leaf whose offset <= q
-                while (b - a > 1) {
-                    const uint32_t mid = (a + b) >> 1;
-                    if (rg[mid].y <= q) a = mid;
-                    else b = mid;
-                }
-                const uint32_t j = rg[a].x + (q - rg[a].y);
-                const double4 pj = pos[j];
-                stage[lane] = make_double4(pj.x, pj.y, pj.z, (double)j);
-            }
-            lds_sync();
-            const uint32_t m = min(total - q0, (uint32_t)kWave);
+            const uint32_t m = stage_round(q0, total, nr, rg, pos, lo, hi, R2g, lane, below, stage);
             if (act) {
                 for (uint32_t k = 0; k < m; ++k) {
                     const double4 c = stage[k];
