@@ -418,7 +418,7 @@ __device__ __forceinline__ bool query_range(
 
 // 128 VGPRs at most: four wavefronts per SIMD, as many as the 40 KB of LDS
 // per block allows (the search is bound by the latency of its node and
-// candidate loads; three per SIMD measured 12 % slower on C4)
+// candidate loads; three per SIMD measured about 10 % slower on C4)
 __global__ __launch_bounds__(kQWaves* kWave) __attribute__((amdgpu_waves_per_eu(4, 8))) void sph_query_kernel(
     const double4* __restrict__ pos, const uint4* __restrict__ meta,
     const double4* __restrict__ bmin, const double4* __restrict__ bmax,
