@@ -252,8 +252,26 @@ __device__ __forceinline__ bool query_range(
             const double d2 = ex * ex + ey * ey + ez * ez;
             const bool hit = valid && d2 <= R2g;
             const uint32_t nchild = (md.w >> 8) & 255u;
-            const bool leafhit = hit && nchild == 0;
-            const uint64_t lb = __ballot(leafhit);
+            // a leaf is kept only if some target's own ball reaches its box
+            // (the box test is conservative, as above): a leaf that only the
+            // grown group box reaches holds no neighbour of any target
+            uint64_t lb = __ballot(hit && nchild == 0);
+            {
+                uint64_t todo = lb, keep = 0ull;
+                while (todo) {
+                    const int l = __ffsll((long long)todo) - 1;
+                    todo &= todo - 1ull;
+                    const double ax = __shfl(a.x, l), ay = __shfl(a.y, l), az = __shfl(a.z, l);
+                    const double bx = __shfl(b.x, l), by = __shfl(b.y, l), bz = __shfl(b.z, l);
+                    const double fx = fmax(fmax(ax - me.x, me.x - bx), 0.0);
+                    const double fy = fmax(fmax(ay - me.y, me.y - by), 0.0);
+                    const double fz = fmax(fmax(az - me.z, me.z - bz), 0.0);
+                    if (__ballot(act && fx * fx + fy * fy + fz * fz <= R2) != 0ull)
+                        keep |= 1ull << l;
+                }
+                lb = keep;
+            }
+            const bool leafhit = ((lb >> lane) & 1ull) != 0ull;
             const uint32_t nl = (uint32_t)__popcll(lb);
             if (nr + nl > rcap) {
                 ovf = true;
