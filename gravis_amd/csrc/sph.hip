@@ -1292,7 +1292,9 @@ gravis_status sph_compute(Ctx* c, const gravis_sph_params* P, const gravis_sph_s
                                           iter, hcur, radius, ctl);
     GRAVIS_TRY(check_launch(c, "sph_prep"));
     const int qblocks = (int)std::min<long>((long)(n / 64 / kQWaves) + 1, (long)c->num_cus * 8);
-    const int dblocks = c->num_cus * 4;
+    // one wavefront per target range of the overflowed groups (8 per group),
+    // 16 resident per CU
+    const int dblocks = c->num_cus * 16;
     for (int round = 0;; ++round) {
         if (round > 64) return fail(c, GRAVIS_INTERNAL, "SPH neighbour rounds did not settle");
         sph_query_reset_kernel<<<1, 64, 0, st>>>(counter, ctl);
