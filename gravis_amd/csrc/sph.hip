@@ -252,12 +252,12 @@ __device__ __forceinline__ bool query_range(
             const double d2 = ex * ex + ey * ey + ez * ez;
             const bool hit = valid && d2 <= R2g;
             const uint32_t nchild = (md.w >> 8) & 255u;
-            // a leaf is kept only if some target's own ball reaches its box
-            // (the box test is conservative, as above): a leaf that only the
+            // a node is kept only if some target's own ball reaches its box
+            // (the box test is conservative, as above): a node that only the
             // grown group box reaches holds no neighbour of any target
-            uint64_t lb = __ballot(hit && nchild == 0);
+            uint64_t hb = __ballot(hit);
             {
-                uint64_t todo = lb, keep = 0ull;
+                uint64_t todo = hb, keep = 0ull;
                 while (todo) {
                     const int l = __ffsll((long long)todo) - 1;
                     todo &= todo - 1ull;
@@ -269,9 +269,11 @@ __device__ __forceinline__ bool query_range(
                     if (__ballot(act && fx * fx + fy * fy + fz * fz <= R2) != 0ull)
                         keep |= 1ull << l;
                 }
-                lb = keep;
+                hb = keep;
             }
-            const bool leafhit = ((lb >> lane) & 1ull) != 0ull;
+            const bool kept = ((hb >> lane) & 1ull) != 0ull;
+            const bool leafhit = kept && nchild == 0;
+            const uint64_t lb = __ballot(leafhit);
             const uint32_t nl = (uint32_t)__popcll(lb);
             if (nr + nl > rcap) {
                 ovf = true;
@@ -279,7 +281,7 @@ __device__ __forceinline__ bool query_range(
             }
             if (leafhit) rg[nr + (uint32_t)__popcll(lb & below)] = make_uint2(md.x, md.y);
             nr += nl;
-            const uint32_t k = (hit && nchild > 0) ? nchild : 0u;
+            const uint32_t k = (kept && nchild > 0) ? nchild : 0u;
             uint32_t ktot;
             const uint32_t kx = wave_excl_scan(k, lane, &ktot);
             if (nn + ktot > kQFront) {
