@@ -797,8 +797,8 @@ __global__ __launch_bounds__(256) void finish_kernel(const double* __restrict__ 
 }
 
 // inv[perm[k]] = k
-__global__ __launch_bounds__(256) void invert_perm_kernel(const uint32_t* __restrict__ perm,
-                                                          int n, uint32_t* __restrict__ inv) {
+__global__ __launch_bounds__(256) void finish_invert_kernel(const uint32_t* __restrict__ perm,
+                                                            int n, uint32_t* __restrict__ inv) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k < n) inv[perm[k]] = (uint32_t)k;
 }
@@ -832,14 +832,17 @@ __global__ __launch_bounds__(256) void finish_inv_kernel(const double* __restric
 }  // namespace
 
 gravis_status finish_forces(Ctx* c, const double* part, int nchunks, int n,
-                            const uint32_t* perm, const FinishArgs& fa) {
+                            const uint32_t* perm, const FinishArgs& fa, const uint32_t* inv) {
     if (perm && nchunks == 1 && n > 0) {
         // one partial array: deliver in the caller's order through the
         // inverse permutation (same values, same arithmetic per particle)
-        uint32_t* inv;
-        GRAVIS_TRY(buf(c, "finish_inv", (size_t)n, &inv));
-        invert_perm_kernel<<<grid_for((size_t)n, 256), 256, 0, c->stream>>>(perm, n, inv);
-        GRAVIS_TRY(check_launch(c, "invert_perm"));
+        if (!inv) {
+            uint32_t* w;
+            GRAVIS_TRY(buf(c, "finish_inv", (size_t)n, &w));
+            finish_invert_kernel<<<grid_for((size_t)n, 256), 256, 0, c->stream>>>(perm, n, w);
+            GRAVIS_TRY(check_launch(c, "invert_perm"));
+            inv = w;
+        }
         finish_inv_kernel<<<grid_for((size_t)n, 256), 256, 0, c->stream>>>(part, n, inv, fa);
         return check_launch(c, "finish_forces");
     }
@@ -866,7 +869,8 @@ gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
     }
     double4* pm;
     GRAVIS_TRY(buf(c, "bf_pm", n, &pm));
-    GRAVIS_TRY(gather_sorted(c, n, x, y, z, m, perm, pm));
+    const uint32_t* inv = nullptr;
+    GRAVIS_TRY(gather_sorted(c, n, x, y, z, m, perm, pm, &inv));
 
     // FP64: 256-thread workgroups of 256 targets. Mixed: one wavefront per
     // workgroup with bf_ipt * 64 targets (64 for the matrix-core kernel) and
@@ -925,7 +929,7 @@ gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
                                           tbox, part);
             timer_end(c, "bf_matrix");
             GRAVIS_TRY(check_launch(c, "bf kernel"));
-            return finish_forces(c, part, S, ni, perm, fa);
+            return finish_forces(c, part, S, ni, perm, fa, inv);
         }
         const double f2 = q * q;
         if (ipt == 1 && tj == 128)
@@ -943,7 +947,7 @@ gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
         timer_end(c, "bf_fp64");
     }
     GRAVIS_TRY(check_launch(c, "bf kernel"));
-    return finish_forces(c, part, S, ni, perm, fa);
+    return finish_forces(c, part, S, ni, perm, fa, inv);
 }
 
 

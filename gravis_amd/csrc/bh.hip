@@ -443,7 +443,7 @@ gravis_status bh_walk(Ctx* c, const gravis_solver* s, const Octree& t, const Fin
         return fail(c, GRAVIS_INTERNAL,
                     "Barnes-Hut walk frontier of a single target exceeded %u nodes (theta %g)",
                     fcap, s->theta);
-    return finish_forces(c, part, 1, ni, t.perm, fa);
+    return finish_forces(c, part, 1, ni, t.perm, fa, t.inv);
 }
 
 gravis_status bh_force(Ctx* c, const gravis_solver* s, size_t n, const double* x,

@@ -370,10 +370,14 @@ __device__ __forceinline__ uint64_t morton_spread3(uint32_t v) {
     x = (x | (x << 2)) & 0x1249249249249249ull;
     return x;
 }
-// Gather x,y,z,m into double4 records in d_perm order (identity if null).
+// x,y,z,m as double4 records in d_perm order (d_perm: sorted -> caller
+// index; identity if null). With a permutation the inverse (caller ->
+// sorted) is formed first and the records are scattered through it (reads
+// in the caller's order, one 32-byte write per particle, instead of four
+// line-sized random reads); *inv_out (if given) receives it, or null.
 gravis_status gather_sorted(Ctx* c, size_t n, const double* x, const double* y,
                             const double* z, const double* m, const uint32_t* d_perm,
-                            double4* d_out);
+                            double4* d_out, const uint32_t** inv_out = nullptr);
 
 // ---- linear octree (tree.hip) -------------------------------------------
 constexpr int kTreeMaxLevel = 21;  // 63-bit keys: 21 levels below the root
@@ -393,6 +397,7 @@ struct Octree {
     const double* box = nullptr;     // root cube (cx, cy, cz, half_width)
     const uint64_t* keys = nullptr;  // sorted Morton keys
     const uint32_t* perm = nullptr;  // sorted position -> caller index
+    const uint32_t* inv = nullptr;   // caller index -> sorted position
     double4* src = nullptr;          // [n sorted (x, y, z, m) | cap node (com, mass)]
     const uint4* meta = nullptr;     // {first, count, first child, flags}
     const double4* bmin = nullptr;   // particle bounding box of each node
@@ -429,9 +434,11 @@ struct FinishArgs {
 
 // Sums nchunks partial force arrays part[q][3][n] (sorted order, chunk order
 // fixed) and delivers them in the caller's order (perm: sorted -> caller
-// index, identity if null) per FinishArgs.
+// index, identity if null; inv: its inverse if already formed) per
+// FinishArgs.
 gravis_status finish_forces(Ctx* c, const double* part, int nchunks, int n,
-                            const uint32_t* perm, const FinishArgs& fa);
+                            const uint32_t* perm, const FinishArgs& fa,
+                            const uint32_t* inv = nullptr);
 
 // Brute-force force on device SoA arrays, result delivered per FinishArgs.
 // Particles up to which a FP64 brute-force KDK runs all its steps in one

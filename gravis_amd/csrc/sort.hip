@@ -267,6 +267,24 @@ __global__ __launch_bounds__(kRsBlock) void radix_scatter_kernel(
     }
 }
 
+// inv[perm[k]] = k
+__global__ __launch_bounds__(256) void invert_perm_kernel(const uint32_t* __restrict__ perm,
+                                                          size_t n, uint32_t* __restrict__ inv) {
+    for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+         k += (size_t)gridDim.x * blockDim.x)
+        inv[perm[k]] = (uint32_t)k;
+}
+
+// out[inv[i]] = (x_i, y_i, z_i, m_i)
+__global__ __launch_bounds__(256) void scatter_sorted_kernel(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ z,
+    const double* __restrict__ m, const uint32_t* __restrict__ inv, size_t n,
+    double4* __restrict__ out) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        out[inv[i]] = make_double4(x[i], y[i], z[i], m[i]);
+}
+
 __global__ __launch_bounds__(256) void gather_kernel(const double* __restrict__ x,
                                                      const double* __restrict__ y,
                                                      const double* __restrict__ z,
@@ -340,8 +358,19 @@ gravis_status morton_sort(Ctx* c, size_t n, const double* x, const double* y, co
 
 gravis_status gather_sorted(Ctx* c, size_t n, const double* x, const double* y,
                             const double* z, const double* m, const uint32_t* d_perm,
-                            double4* d_out) {
-    gather_kernel<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(x, y, z, m, d_perm, n, d_out);
+                            double4* d_out, const uint32_t** inv_out) {
+    if (inv_out) *inv_out = nullptr;
+    if (!d_perm || n == 0) {
+        gather_kernel<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(x, y, z, m, d_perm, n,
+                                                                     d_out);
+        return check_launch(c, "gather_sorted");
+    }
+    uint32_t* inv;
+    GRAVIS_TRY(buf(c, "sort_inv", n, &inv));
+    invert_perm_kernel<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(d_perm, n, inv);
+    scatter_sorted_kernel<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(x, y, z, m, inv, n,
+                                                                         d_out);
+    if (inv_out) *inv_out = inv;
     return check_launch(c, "gather_sorted");
 }
 

@@ -137,18 +137,19 @@ __device__ __forceinline__ void cs_f_df(T q, T& f, T& df) {
 // radius 2 h 1.1 of every gas particle (neighbors.rs:67-70 with
 // density.rs:52).
 __global__ __launch_bounds__(256) void sph_prep_kernel(
-    int n, const uint32_t* __restrict__ perm, const double* __restrict__ vx,
+    int n, const uint32_t* __restrict__ inv, const double* __restrict__ vx,
     const double* __restrict__ vy, const double* __restrict__ vz, const double* __restrict__ h,
     const double* __restrict__ rho, const double* __restrict__ pr, const double* __restrict__ cs,
     const double* __restrict__ alpha, const double* __restrict__ u,
     const uint8_t* __restrict__ type, double4* __restrict__ vh, double4* __restrict__ aux,
     double2* __restrict__ aux2, uint8_t* __restrict__ qstate, uint8_t* __restrict__ iter,
     double* __restrict__ hcur, double* __restrict__ radius, uint32_t* __restrict__ ctl) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s == 0)
+    // caller particle p (reads in the caller's order) -> sorted slot s
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0)
         for (int k = 0; k < kCtlWords; ++k) ctl[k] = 0u;
-    if (s >= n) return;
-    const uint32_t p = perm[s];
+    if (p >= n) return;
+    const uint32_t s = inv[p];
     const double hp = h[p];
     vh[s] = make_double4(vx[p], vy[p], vz[p], hp);
     aux[s] = make_double4(rho[p], pr[p], cs[p], alpha[p]);
@@ -1167,13 +1168,14 @@ __global__ __launch_bounds__(256) void sph_kick_kernel(
 }
 
 // Squared softening h^2 of every particle in the tree's order (eps = h).
-__global__ __launch_bounds__(256) void sph_eps2_kernel(int n, const uint32_t* __restrict__ perm,
+// e2[sorted slot] = h^2 (caller particle p -> slot inv[p])
+__global__ __launch_bounds__(256) void sph_eps2_kernel(int n, const uint32_t* __restrict__ inv,
                                                        const double* __restrict__ h,
                                                        double* __restrict__ e2) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s < n) {
-        const double hs = h[perm[s]];
-        e2[s] = hs * hs;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) {
+        const double hp = h[p];
+        e2[inv[p]] = hp * hp;
     }
 }
 
@@ -1289,7 +1291,7 @@ gravis_status sph_compute(Ctx* c, const gravis_sph_params* P, const gravis_sph_s
     uint32_t* pool = static_cast<uint32_t*>(pv);
 
     const unsigned g256 = grid_for(n, 256);
-    sph_prep_kernel<<<g256, 256, 0, st>>>(ni, t.perm, S->vx, S->vy, S->vz, S->h, S->rho, S->p,
+    sph_prep_kernel<<<g256, 256, 0, st>>>(ni, t.inv, S->vx, S->vy, S->vz, S->h, S->rho, S->p,
                                           S->c, S->alpha, S->u, S->type, vh, aux, aux2, qstate,
                                           iter, hcur, radius, ctl);
     GRAVIS_TRY(check_launch(c, "sph_prep"));
@@ -1393,7 +1395,7 @@ gravis_status sph_gravity(Ctx* c, const gravis_sph_params* P, const gravis_solve
         double* e2 = nullptr;
         if (P->soft_h) {
             GRAVIS_TRY(buf(c, "sph_eps2", n, &e2));
-            sph_eps2_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>((int)n, t->perm, S->h, e2);
+            sph_eps2_kernel<<<grid_for(n, 256), 256, 0, c->stream>>>((int)n, t->inv, S->h, e2);
             GRAVIS_TRY(check_launch(c, "sph_eps2"));
         }
         GRAVIS_TRY(bh_walk(c, g, *t, fa, e2));

@@ -384,7 +384,8 @@ gravis_status build_octree(Ctx* c, size_t n, const double* x, const double* y, c
     GRAVIS_TRY(buf(c, "tree_ngroups", 1, &ngroups));
     GRAVIS_TRY(buf(c, "tree_groups", n, &groups));
     timer_begin(c);
-    GRAVIS_TRY(gather_sorted(c, n, x, y, z, m, perm, src));
+    const uint32_t* inv = nullptr;
+    GRAVIS_TRY(gather_sorted(c, n, x, y, z, m, perm, src, &inv));
 
     hipStream_t st = c->stream;
     tree_init_kernel<<<1, 64, 0, st>>>(meta, lvl, (uint32_t)n, flags, ngroups);
@@ -416,6 +417,7 @@ gravis_status build_octree(Ctx* c, size_t n, const double* x, const double* y, c
     t->box = box;
     t->keys = keys;
     t->perm = perm;
+    t->inv = inv;
     t->src = src;
     t->meta = meta;
     t->bmin = bmin;
