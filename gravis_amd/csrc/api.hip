@@ -772,17 +772,47 @@ gravis_status gravis_autotune(gravis_ctx* g, const gravis_solver* s, size_t n, c
     const uint32_t s_group = c->bh_group;
     double best = -1.0;
     gravis_status st = GRAVIS_OK;
-    // Brute force: a finer j split balances better but writes more chunk
-    // partials (3 doubles per particle per chunk), so among the shapes within
-    // 2 % of the fastest the one with the fewest chunks wins; candidates are
-    // listed in that order of preference.
     struct Shape { int ipt, tile, wpc, matrix; double ms; };
+    // Barnes-Hut: among the shapes within 2 % of the fastest, the first
+    // listed wins
     auto pick = [&](std::vector<Shape>& v) {
         double fastest = v[0].ms;
         for (const Shape& x : v) fastest = std::min(fastest, x.ms);
         for (const Shape& x : v)
             if (x.ms <= 1.02 * fastest) return x;
         return v[0];
+    };
+    // Brute force: every shape is timed once (two calls); the four fastest
+    // are then timed again in three interleaved rounds of three calls each,
+    // so a drift of the clock between them cancels, and the fastest total
+    // wins. Shapes are listed with the fewest j chunks first (a finer split
+    // balances better but writes 3 doubles per particle per chunk), and a
+    // later one must be more than 0.5 % faster to be taken.
+    auto set_shape = [&](const Shape& b) {
+        c->bf_ipt = b.ipt; c->bf_tile = b.tile; c->bf_waves_per_cu = b.wpc;
+        c->bf_matrix = b.matrix;
+    };
+    auto refine = [&](std::vector<Shape>& v, Shape* out) -> gravis_status {
+        std::vector<size_t> idx(v.size());
+        for (size_t i = 0; i < v.size(); ++i) idx[i] = i;
+        std::stable_sort(idx.begin(), idx.end(),
+                         [&](size_t a, size_t b) { return v[a].ms < v[b].ms; });
+        idx.resize(std::min<size_t>(4, idx.size()));
+        std::sort(idx.begin(), idx.end());  // back to the order of preference
+        std::vector<double> tot(idx.size(), 0.0);
+        for (int round = 0; round < 3; ++round)
+            for (size_t k = 0; k < idx.size(); ++k) {
+                set_shape(v[idx[k]]);
+                double ms;
+                GRAVIS_TRY(time_it(&ms, 3));
+                tot[k] += ms / 3.0;
+            }
+        size_t b = 0;
+        for (size_t k = 1; k < idx.size(); ++k)
+            if (tot[k] < tot[b] / 1.005) b = k;
+        *out = v[idx[b]];
+        out->ms = tot[b];
+        return GRAVIS_OK;
     };
     if (s->algorithm == GRAVIS_BRUTE_FORCE) {
         const bool mx = s->precision == GRAVIS_MIXED;
@@ -805,10 +835,10 @@ gravis_status gravis_autotune(gravis_ctx* g, const gravis_solver* s, size_t n, c
                     if (st != GRAVIS_OK) break;
                     v.push_back({c->bf_ipt, c->bf_tile, wpc, c->bf_matrix, ms});
                 }
+        Shape b{};
+        if (st == GRAVIS_OK) st = refine(v, &b);
         if (st == GRAVIS_OK) {
-            const Shape b = pick(v);
-            c->bf_ipt = b.ipt; c->bf_tile = b.tile; c->bf_waves_per_cu = b.wpc;
-            c->bf_matrix = b.matrix;
+            set_shape(b);
             best = b.ms;
         }
     } else if (s->algorithm == GRAVIS_BARNES_HUT && s->group_size == 0) {

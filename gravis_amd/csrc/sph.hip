@@ -608,7 +608,11 @@ __global__ __launch_bounds__(256) void sph_pack_kernel(int n, const double4* __r
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const double2 b = aux2[s];
-    rec[s] = ForceRec{pos[s], vh[s], aux[s], make_double4(b.x, b.y, 0.0, 0.0)};
+    ForceRec& r = rec[s];  // member stores: an aggregate temporary went to scratch
+    r.p = pos[s];
+    r.v = vh[s];
+    r.a = aux[s];
+    r.b = make_double4(b.x, b.y, 0.0, 0.0);
 }
 
 // The FP32-pair force pass reads a 64-byte record per neighbour: the FP64

@@ -396,12 +396,14 @@ gravis_status gravis_get_tuning(gravis_ctx* ctx, gravis_tuning* out);
  * for solver s on the given particles (host arrays, copied once) and keeps
  * the chosen one in the context. Mixed brute force: bf_waves_per_cu
  * 64..512 x (bf_ipt 2, bf_ipt 1, bf_matrix 1, bf_matrix 2) x bf_tile (32
- * shapes); FP64 brute force: bf_waves_per_cu 32..512; among brute-force
- * shapes within 2 % of the fastest, the first in that order (fewest j
- * chunks, i.e. least partial-sum memory, then the VALU kernels) is kept.
+ * shapes); FP64 brute force: bf_waves_per_cu 32..512. Each brute-force
+ * shape is timed once (a warm-up and two calls); the four fastest are timed
+ * again in three interleaved rounds of four calls and the fastest total is
+ * kept, a shape later in that order (more j chunks, i.e. more partial-sum
+ * memory) only when it is more than 0.5 % faster.
  * Barnes-Hut with group_size 0: group sizes 64, 32, 16, keeping 32 or 16
- * only when it is more than 2 % faster than the ones before it.
- * Each brute-force shape costs three force calls, each group size five. Shapes change the summation grouping,
+ * only when it is more than 2 % faster than the ones before it; each group
+ * size costs five force calls. Shapes change the summation grouping,
  * so forces move in their last bits (brute force) or within the
  * opening-angle error (Barnes-Hut groups). out may be NULL. When a
  * candidate fails the call returns its error and the context keeps the
