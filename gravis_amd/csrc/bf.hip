@@ -27,6 +27,7 @@
 #include <type_traits>
 
 #include "internal.h"
+#include "matrix.h"
 
 namespace gravis {
 namespace {
@@ -264,128 +265,22 @@ __global__ __launch_bounds__(kWave) void bf_mixed_kernel(const double4* __restri
     }
 }
 
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-
-// The r^2 product of one 16 x 16 block: FP32 operands on
-// v_mfma_f32_16x16x4f32 (K = 4), or bf16 operands on
-// v_mfma_f32_16x16x32_bf16 (K = 32, each FP32 value split in three exact
-// bf16 pieces; half the matrix-core cycles).
-__device__ __forceinline__ f32x4 mx_mfma(float a, float b, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-}
-__device__ __forceinline__ f32x4 mx_mfma(bf16x8 a, bf16x8 b, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
-}
-
-// x = hi + mid + lo exactly for FP32 x (8 significant bits each; the
-// residuals are exact in FP32).
-__device__ __forceinline__ void split_bf16x3(float x, __bf16& hi, __bf16& mid, __bf16& lo) {
-    hi = (__bf16)x;
-    const float r1 = x - (float)hi;
-    mid = (__bf16)r1;
-    lo = (__bf16)(r1 - (float)mid);
-}
-
-// K slots of the bf16 form: for component c, slots 6c..6c+5 hold the
-// products (b, a') pieces (hi,hi) (hi,mid) (mid,hi) (hi,lo) (mid,mid)
-// (lo,hi) -- every product above 2^-24 of |b||a'| -- slots 18..20 hold
-// 1 x (|a|^2 + eps^2) in three pieces, slots 21..31 are zero.
-__device__ constexpr int kSplitB[6] = {0, 0, 1, 0, 1, 2};
-__device__ constexpr int kSplitA[6] = {0, 1, 0, 2, 1, 0};
-
-// One source block of the matrix-core far loop: the pair forces of block
-// sbc (products dc; its source terms are read from LDS here rather than a
-// block ahead, so the split-bf16 kernel fits 128 VGPRs, four wavefronts per
-// SIMD: 0.5-1.5 % faster than three with the early read) are summed into acc
-// while the products of block sbn are issued (MORE), one product per
-// 16-target block between the VALU work of the previous one. T is the
-// operand type (float or bf16x8).
-template <bool MORE, bool MASK, typename T>
-__device__ __forceinline__ void mx_block(const T* __restrict__ aop,
-                                         const float4* __restrict__ qop,
-                                         const float* __restrict__ bbq, int sbc, int sbn,
-                                         int lane, int grp, const T (&bop)[4],
-                                         const f32x4 (&dc)[4], f32x4 (&dn)[4],
-                                         float (&acc)[4][4]) {
-    T avn = {};
-    f32x4 cbn = {0.f, 0.f, 0.f, 0.f};
-    float4 qc[4];
-#pragma unroll
-    for (int v = 0; v < 4; ++v) qc[v] = qop[sbc * 16 + 4 * grp + v];
-    if (MORE) {
-        avn = aop[sbn * 64 + lane];
-        const float4 c4 = *reinterpret_cast<const float4*>(bbq + sbn * 16 + 4 * grp);
-        cbn = f32x4{c4.x, c4.y, c4.z, c4.w};
-    }
-#pragma unroll
-    for (int tb = 0; tb < 4; ++tb) {
-        if (MORE) dn[tb] = mx_mfma(avn, bop[tb], cbn);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const float r2 = dc[tb][v];
-            float ri = __builtin_amdgcn_rsqf(r2);
-            if (MASK) ri = r2 > 0.0f ? ri : 0.0f;
-            const float f = (ri * ri) * ri;
-            acc[tb][0] = __builtin_fmaf(f, qc[v].x, acc[tb][0]);
-            acc[tb][1] = __builtin_fmaf(f, qc[v].y, acc[tb][1]);
-            acc[tb][2] = __builtin_fmaf(f, qc[v].z, acc[tb][2]);
-            acc[tb][3] = __builtin_fmaf(f, qc[v].w, acc[tb][3]);
-        }
-    }
-    if (MORE) {
-#pragma unroll
-        for (int tb = 0; tb < 4; ++tb) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-            __builtin_amdgcn_sched_group_barrier(0x002, MASK ? 32 : 28, 0);
-        }
-    }
-}
-
-// Matrix-core variant of the mixed kernel. With a = x_i - c and b = x_j - c
-// (FP32, c the wavefront's target-box centre),
-//   |x_j - x_i|^2 + eps^2 = (b, 1) . (-2 a, |a|^2 + eps^2) + |b|^2,
-// one v_mfma_f32_16x16x4f32 per 16 sources x 16 targets: A rows are sources
-// (b_x, b_y, b_z, 1), B columns are targets (-2a_x, -2a_y, -2a_z, |a|^2 +
-// eps^2), C is |b|^2 of the row's source. The acceleration is then
-//   sum_j f_ij m_j (b_j - a_i) = sum_j f_ij (m_j b_j) - a_i sum_j f_ij m_j,
-// f = r^-3, so each pair costs one v_rsq_f32, two multiplies and four FMAs
-// on the VALU (six operations instead of twelve). Lane l of the product holds
-// rows 4 (l / 16) + v, v = 0..3, of target column l % 16: each lane sums for
-// one target per 16-target block (64 targets, four blocks) over four sources
-// per block; the four lane groups are added at the end of every tile and the
-// lane of target l keeps its Kahan-compensated sums. The products of the next
-// source block are issued between the VALU work of the current one, so the
-// matrix core runs beside the VALU.
-// Error: the expansion rounds r^2 + eps^2 with an absolute error of about
-// u (|a| + |b|)^2 <= u (diag + |d|)^2, i.e. a relative error of
-// u (1 + 1 / q')^2 when max(|d|, eps) >= q' diag. The caller passes q' with
-// (1 + 1 / q')^2 = 2 (1 + 1 / q), which keeps the per-pair bound of the VALU
-// kernel for the context's q; nearer tiles take the FP64-separation path.
+// Matrix-core variant of the mixed kernel (the sweep, its tiers and error
+// bound: matrix.h). One wavefront per 64 targets and j chunk. near2 = q'^2
+// with (1 + 1 / q')^2 = 2 (1 + 1 / q), mid2 = q^2 for the context's q: the
+// per-pair bound of the VALU kernel.
 template <int TJ, bool MASK, bool BF, int WPE = (TJ == 128 ? 4 : 3)>
 __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void bf_matrix_kernel(const double4* __restrict__ pm, int n,
                                                           int jchunk, float eps2, double near2,
+                                                          double mid2,
                                                           const double* __restrict__ tbox,
                                                           double* __restrict__ part) {
-    static_assert(TJ % kWave == 0, "tile must be a multiple of the wavefront");
-    constexpr int LPT = TJ / kWave;  // records each lane stages per tile
-    constexpr int SB = TJ / 16;      // 16-source blocks per tile (even)
-    using T = typename std::conditional<BF, bf16x8, float>::type;
-    // near: TJ FP64 records; far: A operands [SB][4][16] of type T, then
-    // (m b, m) and |b|^2 per source (36 or 84 TJ bytes)
-    constexpr int kA = BF ? 4 * TJ : TJ;  // float4 units of the A operands
-    __shared__ float4 lds[kA + TJ + TJ / 4];
-    double4* tile = reinterpret_cast<double4*>(&lds[0]);
-    T* aop = reinterpret_cast<T*>(&lds[0]);
-    float4* qop = &lds[kA];
-    float* bbq = reinterpret_cast<float*>(&lds[kA + TJ]);
-
+    __shared__ float4 lds[mx_lds_f4<TJ, BF>()];
     const int lane = threadIdx.x;
-    const int col = lane & 15, grp = lane >> 4;
     const int ibase = blockIdx.x * kWave;
     const int iself = ibase + lane < n ? ibase + lane : ibase;
     double lo[3], hi[3];
-    double cx, cy, cz, lim2;
+    double cx, cy, cz, lim2, limv2;
     float ax, ay, az;  // this lane's target relative to c (set-up only)
     {
         const double4 p = pm[iself];
@@ -399,185 +294,26 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
         for (int d = 0; d < 3; ++d) di2 = __builtin_fma(hi[d] - lo[d], hi[d] - lo[d], di2);
         lim2 = near2 * di2;
         if ((double)eps2 >= lim2) lim2 = 0.0;
+        limv2 = mid2 * di2;
+        if ((double)eps2 >= limv2) limv2 = 0.0;
         ax = (float)(p.x - cx);
         ay = (float)(p.y - cy);
         az = (float)(p.z - cz);
     }
-    // B operand of each 16-target block
-    T bop[4];
-#pragma unroll
-    for (int tb = 0; tb < 4; ++tb) {
-        const float tx = __shfl(ax, 16 * tb + col);
-        const float ty = __shfl(ay, 16 * tb + col);
-        const float tz = __shfl(az, 16 * tb + col);
-        const float aa = __builtin_fmaf(tx, tx, __builtin_fmaf(ty, ty, __builtin_fmaf(tz, tz, eps2)));
-        if constexpr (BF) {
-            // slots 8 grp .. 8 grp + 7 of this lane's target column
-            __bf16 pc[4][3];
-            split_bf16x3(-2.0f * tx, pc[0][0], pc[0][1], pc[0][2]);
-            split_bf16x3(-2.0f * ty, pc[1][0], pc[1][1], pc[1][2]);
-            split_bf16x3(-2.0f * tz, pc[2][0], pc[2][1], pc[2][2]);
-            split_bf16x3(aa, pc[3][0], pc[3][1], pc[3][2]);
-            // every slot index is a compile-time constant (a runtime index
-            // into pc would place it in scratch memory); grp selects
-            bf16x8 v;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                __bf16 c4[4];
-#pragma unroll
-                for (int gs = 0; gs < 4; ++gs) {
-                    const int kk = 8 * gs + j;
-                    c4[gs] = kk < 18 ? pc[kk / 6][kSplitA[kk % 6]]
-                                     : (kk < 21 ? pc[3][kk - 18] : (__bf16)0.0f);
-                }
-                v[j] = grp == 0 ? c4[0] : grp == 1 ? c4[1] : grp == 2 ? c4[2] : c4[3];
-            }
-            bop[tb] = v;
-        } else {
-            bop[tb] = grp == 0 ? -2.0f * tx : grp == 1 ? -2.0f * ty : grp == 2 ? -2.0f * tz : aa;
-        }
-    }
-    float s[4], k[4];  // target ibase + lane: x, y, z, sum f m
-#pragma unroll
-    for (int q = 0; q < 4; ++q) s[q] = k[q] = 0.f;
-
     const int j0 = blockIdx.y * jchunk;
     const int j1 = min(n, j0 + jchunk);
-    const int ntiles = (j1 - j0 + TJ - 1) / TJ;
     const int tile0 = j0 / TJ;
-
-    double4 stage[LPT];
-    auto load = [&](int t) {
-#pragma unroll
-        for (int e = 0; e < LPT; ++e) {
-            const int j = j0 + t * TJ + e * kWave + lane;
-            stage[e] = j < j1 ? pm[j] : make_double4(cx, cy, cz, 0.0);
-        }
-    };
-    auto store = [&](bool near) {
-        if (near) {
-#pragma unroll
-            for (int e = 0; e < LPT; ++e) tile[e * kWave + lane] = stage[e];
-        } else {
-#pragma unroll
-            for (int e = 0; e < LPT; ++e) {
-                const int sj = e * kWave + lane;
-                const float bx = (float)(stage[e].x - cx);
-                const float by = (float)(stage[e].y - cy);
-                const float bz = (float)(stage[e].z - cz);
-                const float m = (float)stage[e].w;
-                if constexpr (BF) {
-                    // row sj & 15 of block sj >> 4, slot groups 0..3
-                    __bf16 pc[3][3];
-                    split_bf16x3(bx, pc[0][0], pc[0][1], pc[0][2]);
-                    split_bf16x3(by, pc[1][0], pc[1][1], pc[1][2]);
-                    split_bf16x3(bz, pc[2][0], pc[2][1], pc[2][2]);
-                    T* a = aop + (sj >> 4) * 64 + (sj & 15);
-#pragma unroll
-                    for (int kg = 0; kg < 4; ++kg) {
-                        bf16x8 v;
-#pragma unroll
-                        for (int j = 0; j < 8; ++j) {
-                            const int kk = 8 * kg + j;
-                            v[j] = kk < 18 ? pc[kk / 6][kSplitB[kk % 6]]
-                                           : (kk < 21 ? (__bf16)1.0f : (__bf16)0.0f);
-                        }
-                        a[16 * kg] = v;
-                    }
-                } else {
-                    float* a = aop + (sj >> 4) * 64 + (sj & 15);
-                    a[0] = bx;
-                    a[16] = by;
-                    a[32] = bz;
-                    a[48] = 1.0f;
-                }
-                qop[sj] = make_float4(m * bx, m * by, m * bz, m);
-                bbq[sj] = __builtin_fmaf(bx, bx, __builtin_fmaf(by, by, bz * bz));
-            }
-        }
-    };
-
-    // records are loaded at the top of each tile (no register prefetch: the
-    // registers go to occupancy, and other wavefronts cover the latency)
-    for (int t = 0; t < ntiles; ++t) {
-        const bool near_cur = tile_is_near(tbox + (size_t)(tile0 + t) * kTileBox, lo, hi, lim2);
-        load(t);
-        store(near_cur);
-        __syncthreads();
-        float own[4];
-        if (!near_cur) {
-            float acc[4][4];
-#pragma unroll
-            for (int tb = 0; tb < 4; ++tb)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) acc[tb][q] = 0.f;
-            f32x4 dA[4], dB[4];
-            {
-                const T av = aop[lane];
-                const float4 c4 = *reinterpret_cast<const float4*>(bbq + 4 * grp);
-                const f32x4 cb = {c4.x, c4.y, c4.z, c4.w};
-#pragma unroll
-                for (int tb = 0; tb < 4; ++tb)
-                    dA[tb] = mx_mfma(av, bop[tb], cb);
-            }
-            // blocks in pairs (A current, B next; then B current, A next)
-            int sb = 0;
-#pragma unroll 1
-            for (; sb + 2 < SB; sb += 2) {
-                mx_block<true, MASK>(aop, qop, bbq, sb, sb + 1, lane, grp, bop, dA, dB, acc);
-                mx_block<true, MASK>(aop, qop, bbq, sb + 1, sb + 2, lane, grp, bop, dB, dA, acc);
-            }
-            mx_block<true, MASK>(aop, qop, bbq, SB - 2, SB - 1, lane, grp, bop, dA, dB, acc);
-            mx_block<false, MASK>(aop, qop, bbq, SB - 1, 0, lane, grp, bop, dB, dA, acc);
-            // add the four lane groups; lane l keeps block grp (target l)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-#pragma unroll
-                for (int tb = 0; tb < 4; ++tb) {
-                    float v = acc[tb][q];
-                    v += __shfl_xor(v, 16);
-                    v += __shfl_xor(v, 32);
-                    acc[tb][q] = v;
-                }
-                own[q] = grp == 0 ? acc[0][q] : grp == 1 ? acc[1][q] : grp == 2 ? acc[2][q] : acc[3][q];
-            }
-        } else {
-            // FP64 separations, one target per lane (target ibase + lane)
-            const double4 p = pm[iself];
-            float nx = 0.f, ny = 0.f, nz = 0.f;
-#pragma unroll 1
-            for (int j = 0; j < TJ; ++j) {
-                const double4 sj = tile[j];
-                const float m = (float)sj.w;
-                const float dx = (float)(sj.x - p.x);
-                const float dy = (float)(sj.y - p.y);
-                const float dz = (float)(sj.z - p.z);
-                const float d2 = MASK ? __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)) : 0.f;
-                const float r2 = MASK ? d2 + eps2
-                                      : __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, eps2)));
-                float ri = __builtin_amdgcn_rsqf(r2);
-                const float h = (r2 * ri) * ri;
-                ri = ri * __builtin_fmaf(-0.5f, h, 1.5f);
-                if (MASK) ri = d2 > 0.0f ? ri : 0.0f;
-                const float f = (m * ri) * (ri * ri);
-                nx = __builtin_fmaf(f, dx, nx);
-                ny = __builtin_fmaf(f, dy, ny);
-                nz = __builtin_fmaf(f, dz, nz);
-            }
-            own[0] = nx;
-            own[1] = ny;
-            own[2] = nz;
-            own[3] = 0.f;
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) kahan_add(s[q], k[q], own[q]);
-        __syncthreads();
-    }
+    double o[4];
+    mx_sweep<TJ, MASK, BF>(
+        lds, pm, iself, lane, cx, cy, cz, ax, ay, az, eps2, j0, j1,
+        [&](int t) __attribute__((always_inline)) {
+            const double* tb = tbox + (size_t)(tile0 + t) * kTileBox;
+            if (!tile_is_near(tb, lo, hi, lim2)) return kTierFar;
+            return tile_is_near(tb, lo, hi, limv2) ? kTierNear : kTierMid;
+        },
+        o);
     const int i = ibase + lane;
     if (i < n) {
-        double o[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) o[q] = (double)s[q] - (double)k[q];
         // the target offset a (FP32, as in the products), formed again here
         // so it holds no registers through the loop
         const double4 p = pm[i];
@@ -745,13 +481,13 @@ void launch_mixed(dim3 g, hipStream_t st, bool mask, const double4* pm, int n, i
 
 template <int TJ, bool BF>
 void launch_matrix(dim3 g, hipStream_t st, bool mask, const double4* pm, int n, int jchunk,
-                   float eps2, double near2, const double* tbox, double* part) {
+                   float eps2, double near2, double mid2, const double* tbox, double* part) {
     if (mask)
-        bf_matrix_kernel<TJ, true, BF><<<g, kWave, 0, st>>>(pm, n, jchunk, eps2, near2, tbox,
-                                                            part);
+        bf_matrix_kernel<TJ, true, BF><<<g, kWave, 0, st>>>(pm, n, jchunk, eps2, near2, mid2,
+                                                            tbox, part);
     else
-        bf_matrix_kernel<TJ, false, BF><<<g, kWave, 0, st>>>(pm, n, jchunk, eps2, near2, tbox,
-                                                             part);
+        bf_matrix_kernel<TJ, false, BF><<<g, kWave, 0, st>>>(pm, n, jchunk, eps2, near2, mid2,
+                                                             tbox, part);
 }
 
 template <int BLOCK, int IPT, int TJ>
@@ -917,16 +653,16 @@ gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
             const double qm = q > 0.0 ? 1.0 / (std::sqrt(2.0 * (1.0 + 1.0 / q)) - 1.0) : 0.0;
             if (tj == 128 && c->bf_matrix == 2)
                 launch_matrix<128, true>(grid, c->stream, mask, pm, ni, jchunk, eps2f, qm * qm,
-                                         tbox, part);
+                                         q * q, tbox, part);
             else if (tj == 128)
                 launch_matrix<128, false>(grid, c->stream, mask, pm, ni, jchunk, eps2f, qm * qm,
-                                          tbox, part);
+                                          q * q, tbox, part);
             else if (c->bf_matrix == 2)
                 launch_matrix<256, true>(grid, c->stream, mask, pm, ni, jchunk, eps2f, qm * qm,
-                                         tbox, part);
+                                         q * q, tbox, part);
             else
                 launch_matrix<256, false>(grid, c->stream, mask, pm, ni, jchunk, eps2f, qm * qm,
-                                          tbox, part);
+                                          q * q, tbox, part);
             timer_end(c, "bf_matrix");
             GRAVIS_TRY(check_launch(c, "bf kernel"));
             return finish_forces(c, part, S, ni, perm, fa, inv);

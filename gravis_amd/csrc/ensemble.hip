@@ -3,10 +3,12 @@
 //
 // Forward: every system runs sim-core's LeapfrogKDK (crates/sim-core/src/
 // integrator.rs:41-77) with softened brute force (gravity.rs:47-80) in the
-// gather form. All systems advance together: one kick-drift launch over the
-// concatenated arrays and ONE force launch whose workgroups cover every
-// (system, target block); the two launches of a step are captured once as a
-// hipGraph and replayed nsteps times.
+// gather form. All systems advance together. Mixed precision: ONE launch per
+// step, whose workgroups cover every (system, 64-target item), takes the
+// forces, closes the step and opens the next (double-buffered positions);
+// two steps are captured once as a hipGraph and replayed. FP64: a
+// kick-drift launch over the concatenated arrays and a force launch per
+// step, captured once and replayed nsteps times.
 //
 // Reverse: for cotangents (lx_T, lv_T) the adjoint of
 //   vh = v + h a(x);  x' = x + dt vh;  v' = vh + h a(x')       (h = dt/2)
@@ -219,41 +221,75 @@ __global__ __launch_bounds__(256) void ens_permute_kernel(EnsArrays a, const uin
     }
 }
 
-// Bounding box of every item's records [i0, i1): box[6 w .. 6 w + 5] =
-// (lo x, y, z, hi x, y, z) of item w. Items of a system cover it in
-// consecutive 64-record blocks from j0, so a source tile of kEnsTile
-// records is the union of kEnsTile / 64 consecutive items' boxes.
-__global__ __launch_bounds__(kWave) void ens_box_kernel(const EnsBlock* __restrict__ items,
-                                                        const double* __restrict__ x,
-                                                        const double* __restrict__ y,
-                                                        const double* __restrict__ z,
-                                                        double* __restrict__ box) {
-    const EnsBlock b = items[blockIdx.x];
-    const int i = b.i0 + (int)threadIdx.x;
-    const int ii = i < b.i1 ? i : b.i0;
+// ---- mixed forward, one launch per step ------------------------------------
+// The state stays in per-system Morton order. ens_pack_kernel (once per call)
+// runs the first half of step 1 -- v += h a, x += dt v, kick_drift_kernel's
+// arithmetic -- and writes the positions as (x, y, z, m) records P with every
+// item's box B. ens_wave_kernel (one launch per step) takes the forces on P
+// and applies the step's closing half kick v += h a; with DRIFT it goes on
+// with the next step's v += h a, x += dt v, written as records P' with boxes
+// B' (positions are double-buffered: every wavefront reads all of its
+// system's records while the others move theirs); without, it writes the
+// final positions to x, y, z. The rounding of every update is that of the
+// separate kernels.
+
+// Box of one item from the lanes' positions (inactive lanes: the item's
+// first target, held by lane 0); lane 0 writes bout[6 item ..].
+__device__ __forceinline__ void ens_item_box(bool active, double px, double py, double pz,
+                                             double* __restrict__ bout, unsigned item) {
+    const double qx = active ? px : __shfl(px, 0), qy = active ? py : __shfl(py, 0),
+                 qz = active ? pz : __shfl(pz, 0);
+    const double xd[1] = {qx}, yd[1] = {qy}, zd[1] = {qz};
     double lo[3], hi[3];
-    const double xd[1] = {x[ii]}, yd[1] = {y[ii]}, zd[1] = {z[ii]};
     wave_box<1>(xd, yd, zd, lo, hi);
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            box[(size_t)6 * blockIdx.x + d] = lo[d];
-            box[(size_t)6 * blockIdx.x + 3 + d] = hi[d];
+            bout[(size_t)6 * item + d] = lo[d];
+            bout[(size_t)6 * item + 3 + d] = hi[d];
         }
     }
 }
 
+__global__ __launch_bounds__(kWave) void ens_pack_kernel(
+    const EnsBlock* __restrict__ items, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ m,
+    double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
+    const double* __restrict__ ax, const double* __restrict__ ay,
+    const double* __restrict__ az, double h, double dt, double4* __restrict__ pout,
+    double* __restrict__ bout) {
+    const EnsBlock b = items[blockIdx.x];
+    const int i = b.i0 + (int)threadIdx.x;
+    const bool active = i < b.i1;
+    double nx = 0.0, ny = 0.0, nz = 0.0;
+    if (active) {
+        const double nvx = vx[i] + ax[i] * h;
+        const double nvy = vy[i] + ay[i] * h;
+        const double nvz = vz[i] + az[i] * h;
+        vx[i] = nvx;
+        vy[i] = nvy;
+        vz[i] = nvz;
+        nx = x[i] + nvx * dt;
+        ny = y[i] + nvy * dt;
+        nz = z[i] + nvz * dt;
+        pout[i] = make_double4(nx, ny, nz, m[i]);
+    }
+    ens_item_box(active, nx, ny, nz, bout, blockIdx.x);
+}
+
 // One wavefront per item: targets [i0, i1) (at most 64) of one system,
-// sources [j0, j1) = that system; assigns a, optionally kicks v by h a.
-// box: ens_box_kernel's item boxes of the same positions.
-template <bool MASK>
+// sources [j0, j1) = that system, records pin with item boxes box; sources
+// are rounded to FP32 relative to the item box's centre, 12 VALU operations
+// and one v_rsq_f32 per pair; near tiles (gap to the target box and eps both
+// below q times its diagonal, near2 = q^2) keep FP64 separations.
+template <bool MASK, bool DRIFT>
 __global__ __launch_bounds__(kWave) void ens_wave_kernel(
     const EnsBlock* __restrict__ items, const double* __restrict__ box,
-    const double* __restrict__ x,
-    const double* __restrict__ y, const double* __restrict__ z, const double* __restrict__ m,
-    float eps2, double near2, double h, int kick, double* __restrict__ ax,
-    double* __restrict__ ay, double* __restrict__ az, double* __restrict__ vx,
-    double* __restrict__ vy, double* __restrict__ vz) {
+    const double4* __restrict__ pin, float eps2, double near2, double h, double dt,
+    double* __restrict__ vx, double* __restrict__ vy, double* __restrict__ vz,
+    double* __restrict__ ax, double* __restrict__ ay, double* __restrict__ az,
+    double4* __restrict__ pout, double* __restrict__ bout, double* __restrict__ x,
+    double* __restrict__ y, double* __restrict__ z) {
     static_assert(kEnsTile % kWave == 0, "a tile is whole items");
     constexpr int LPT = kEnsTile / kWave;
     __shared__ double4 tile[kEnsTile];  // near: FP64 records; far: float4 [0, kEnsTile)
@@ -270,7 +306,8 @@ __global__ __launch_bounds__(kWave) void ens_wave_kernel(
         lo[d] = box[(size_t)6 * blockIdx.x + d];
         hi[d] = box[(size_t)6 * blockIdx.x + 3 + d];
     }
-    const double px = x[ii], py = y[ii], pz = z[ii];
+    const double4 me = pin[ii];
+    const double px = me.x, py = me.y, pz = me.z;
     const double cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]), cz = 0.5 * (lo[2] + hi[2]);
     double di2 = 0.0;
 #pragma unroll
@@ -286,8 +323,7 @@ __global__ __launch_bounds__(kWave) void ens_wave_kernel(
         for (int e = 0; e < LPT; ++e) {
             const int j = jb + e * kWave + lane;
             // padding: zero mass at the box centre (no contribution)
-            nxt[e] = j < b.j1 ? make_double4(x[j], y[j], z[j], m[j])
-                              : make_double4(cx, cy, cz, 0.0);
+            nxt[e] = j < b.j1 ? pin[j] : make_double4(cx, cy, cz, 0.0);
         }
     };
     if (b.j0 < b.j1) load(b.j0);
@@ -364,16 +400,32 @@ __global__ __launch_bounds__(kWave) void ens_wave_kernel(
         kahan_add(sz, kz, tz);
         __syncthreads();
     }
+    double nx = 0.0, ny = 0.0, nz = 0.0;
     if (active) {
         const double a0 = (double)sx - (double)kx, a1 = (double)sy - (double)ky,
                      a2 = (double)sz - (double)kz;
-        ax[i] = a0; ay[i] = a1; az[i] = a2;
-        if (kick) {
-            vx[i] = vx[i] + a0 * h;
-            vy[i] = vy[i] + a1 * h;
-            vz[i] = vz[i] + a2 * h;
+        ax[i] = a0;
+        ay[i] = a1;
+        az[i] = a2;
+        double nvx = vx[i] + a0 * h, nvy = vy[i] + a1 * h, nvz = vz[i] + a2 * h;
+        if (DRIFT) {
+            nvx = nvx + a0 * h;
+            nvy = nvy + a1 * h;
+            nvz = nvz + a2 * h;
+            nx = px + nvx * dt;
+            ny = py + nvy * dt;
+            nz = pz + nvz * dt;
+            pout[i] = make_double4(nx, ny, nz, me.w);
+        } else {
+            x[i] = px;
+            y[i] = py;
+            z[i] = pz;
         }
+        vx[i] = nvx;
+        vy[i] = nvy;
+        vz[i] = nvz;
     }
+    if (DRIFT) ens_item_box(active, nx, ny, nz, bout, blockIdx.x);
 }
 
 // lx += J_x^T u, lm += J_m^T u (FP64).
@@ -555,33 +607,6 @@ gravis_status ens_step(Ctx* c, bool mixed, const EnsBlock* d_blocks, int nblocks
     return ens_force(c, mixed, d_blocks, nblocks, a, eps2, h, 1);
 }
 
-// Mixed force on state in per-system Morton order (one wavefront per item).
-// wbox: 6 nwaves doubles (allocated by the caller, outside any capture)
-gravis_status ens_wave_force(Ctx* c, const EnsBlock* d_waves, int nwaves, double* wbox,
-                             const EnsArgs& a, double eps2, double h, int kick) {
-    if (nwaves == 0) return GRAVIS_OK;
-    const float e2 = (float)eps2;
-    const double q2 = c->bf_near_fraction * c->bf_near_fraction;
-    ens_box_kernel<<<nwaves, kWave, 0, c->stream>>>(d_waves, a.x, a.y, a.z, wbox);
-    GRAVIS_TRY(check_launch(c, "ens_box"));
-    if (!(e2 >= kMaskEps2F))
-        ens_wave_kernel<true><<<nwaves, kWave, 0, c->stream>>>(
-            d_waves, wbox, a.x, a.y, a.z, a.m, e2, q2, h, kick, a.ax, a.ay, a.az, a.vx, a.vy,
-            a.vz);
-    else
-        ens_wave_kernel<false><<<nwaves, kWave, 0, c->stream>>>(
-            d_waves, wbox, a.x, a.y, a.z, a.m, e2, q2, h, kick, a.ax, a.ay, a.az, a.vx, a.vy,
-            a.vz);
-    return check_launch(c, "ens_wave_force");
-}
-
-gravis_status ens_wave_step(Ctx* c, const EnsBlock* d_waves, int nwaves, double* wbox,
-                            size_t n, const EnsArgs& a, double eps2, double dt) {
-    const double h = 0.5 * dt;
-    GRAVIS_TRY(kick_drift(c, n, a.x, a.y, a.z, a.vx, a.vy, a.vz, a.ax, a.ay, a.az, h, dt));
-    return ens_wave_force(c, d_waves, nwaves, wbox, a, eps2, h, 1);
-}
-
 // Permutation that sorts every system's particles by Morton key within the
 // system (systems keep their ranges): perm[sorted position] = caller index.
 gravis_status ens_sort(Ctx* c, const EnsPlan& plan, const EnsBlock* d_sys, const double* x,
@@ -656,11 +681,33 @@ gravis_status ensemble_kdk(Ctx* c, const gravis_solver* s, int nsys, const uint6
     }
     const int nw = (int)plan.waves.size();
     double* wbox = nullptr;
-    if (sorted_run) GRAVIS_TRY(buf(c, "ens_wbox", (size_t)6 * std::max(nw, 1), &wbox));
-    auto step = [&]() -> gravis_status {
-        return sorted_run ? ens_wave_step(c, d_waves, nw, wbox, n, a, eps2, dt)
-                          : ens_step(c, mixed, d_blocks, nb, n, a, eps2, dt);
+    double4* rec = nullptr;
+    if (sorted_run) {
+        GRAVIS_TRY(buf(c, "ens_wbox", (size_t)12 * std::max(nw, 1), &wbox));
+        GRAVIS_TRY(buf(c, "ens_rec", 2 * n, &rec));
+    }
+    const double h = 0.5 * dt;
+    const float e2 = (float)eps2;
+    const bool mask = !(e2 >= kMaskEps2F);
+    const double q2 = c->bf_near_fraction * c->bf_near_fraction;
+    // one step on records/boxes buffer `from` (0 or 1)
+    auto wave_step = [&](int from, bool drift) -> gravis_status {
+        const double4* pin = rec + (size_t)from * n;
+        double4* pout = rec + (size_t)(1 - from) * n;
+        const double* bin = wbox + (size_t)6 * nw * from;
+        double* bout = wbox + (size_t)6 * nw * (1 - from);
+#define GRAVIS_ENS_WAVE(M, D)                                                                  \
+    ens_wave_kernel<M, D><<<nw, kWave, 0, c->stream>>>(d_waves, bin, pin, e2, q2, h, dt, a.vx, \
+                                                       a.vy, a.vz, a.ax, a.ay, a.az, pout,   \
+                                                       bout, a.x, a.y, a.z)
+        if (mask && drift) GRAVIS_ENS_WAVE(true, true);
+        else if (mask) GRAVIS_ENS_WAVE(true, false);
+        else if (drift) GRAVIS_ENS_WAVE(false, true);
+        else GRAVIS_ENS_WAVE(false, false);
+#undef GRAVIS_ENS_WAVE
+        return check_launch(c, "ens_wave");
     };
+    auto step = [&]() -> gravis_status { return ens_step(c, mixed, d_blocks, nb, n, a, eps2, dt); };
     // back to the caller's order (positions, velocities, accelerations)
     auto restore = [&]() -> gravis_status {
         if (!sorted_run) return GRAVIS_OK;
@@ -674,6 +721,46 @@ gravis_status ensemble_kdk(Ctx* c, const gravis_solver* s, int nsys, const uint6
         ens_permute_kernel<<<grid_for(n, 256, 4096), 256, 0, c->stream>>>(pa, perm, n, 1);
         return check_launch(c, "ens_permute");
     };
+    if (sorted_run) {
+        // first half step, then nsteps - 1 launches that close a step and
+        // open the next (buffers 0 -> 1 -> 0 ...), then the closing launch
+        ens_pack_kernel<<<nw, kWave, 0, c->stream>>>(d_waves, a.x, a.y, a.z, a.m, a.vx, a.vy,
+                                                     a.vz, a.ax, a.ay, a.az, h, dt, rec, wbox);
+        GRAVIS_TRY(check_launch(c, "ens_pack"));
+        uint64_t left = nsteps - 1;
+        int cur = 0;
+        if (use_graph && left >= 2) {
+            // two steps (0 -> 1, 1 -> 0) captured once and replayed
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t exec = nullptr;
+            GRAVIS_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            gravis_status st = wave_step(0, true);
+            if (st == GRAVIS_OK) st = wave_step(1, true);
+            hipError_t e = hipStreamEndCapture(c->stream, &graph);
+            if (st != GRAVIS_OK) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return st;
+            }
+            if (e != hipSuccess)
+                return fail(c, GRAVIS_HIP_ERROR, "capture: %s", hipGetErrorString(e));
+            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(graph);
+                return fail(c, GRAVIS_HIP_ERROR, "graph instantiate: %s", hipGetErrorString(e));
+            }
+            for (; left >= 2 && e == hipSuccess; left -= 2) e = hipGraphLaunch(exec, c->stream);
+            (void)hipGraphExecDestroy(exec);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess)
+                return fail(c, GRAVIS_HIP_ERROR, "graph launch: %s", hipGetErrorString(e));
+        }
+        for (; left > 0; --left) {
+            GRAVIS_TRY(wave_step(cur, true));
+            cur = 1 - cur;
+        }
+        GRAVIS_TRY(wave_step(cur, false));
+        return restore();
+    }
     if (!use_graph) {
         for (uint64_t t = 0; t < nsteps; ++t) GRAVIS_TRY(step());
         return restore();
