@@ -565,7 +565,38 @@ __global__ __launch_bounds__(256) void finish_inv_kernel(const double* __restric
     }
 }
 
+// As finish_inv_kernel for (ax, ay, az, 0) records.
+__global__ __launch_bounds__(256) void finish_inv4_kernel(const double4* __restrict__ part4,
+                                                          int n, const uint32_t* __restrict__ inv,
+                                                          FinishArgs fa) {
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= n) return;
+    const double4 p = part4[inv[o]];
+    // 0 + p, as finish_kernel's sum over one chunk forms it (-0 becomes +0)
+    const double a0 = 0.0 + p.x, a1 = 0.0 + p.y, a2 = 0.0 + p.z;
+    if (fa.mode == kAccumulate) {
+        fa.ax[o] += a0;
+        fa.ay[o] += a1;
+        fa.az[o] += a2;
+    } else {
+        fa.ax[o] = a0;
+        fa.ay[o] = a1;
+        fa.az[o] = a2;
+        fa.vx[o] = fa.vx[o] + a0 * fa.half_dt;
+        fa.vy[o] = fa.vy[o] + a1 * fa.half_dt;
+        fa.vz[o] = fa.vz[o] + a2 * fa.half_dt;
+    }
+}
+
 }  // namespace
+
+gravis_status finish_forces4(Ctx* c, const double4* part4, int n, const uint32_t* inv,
+                             const FinishArgs& fa) {
+    if (n == 0) return GRAVIS_OK;
+    if (!inv) return fail(c, GRAVIS_INTERNAL, "finish_forces4 needs the inverse permutation");
+    finish_inv4_kernel<<<grid_for((size_t)n, 256), 256, 0, c->stream>>>(part4, n, inv, fa);
+    return check_launch(c, "finish_forces");
+}
 
 gravis_status finish_forces(Ctx* c, const double* part, int nchunks, int n,
                             const uint32_t* perm, const FinishArgs& fa, const uint32_t* inv) {

@@ -90,7 +90,7 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
     const double* __restrict__ box, double theta2, double eps2_all,
     const double* __restrict__ eps2_t, uint32_t* __restrict__ frontier,
     uint32_t fcap, uint32_t* __restrict__ counter, uint32_t* __restrict__ flag,
-    double* __restrict__ part, unsigned long long* __restrict__ stats) {
+    double4* __restrict__ part, unsigned long long* __restrict__ stats) {
     // LDS per block: 32 KB (mixed: 768-entry lists, FP32 stages) for five
     // blocks per CU, 40 KB (FP64) for four
     constexpr uint32_t kList = MIXED ? kListCapMixed : kListCap;
@@ -380,12 +380,7 @@ __attribute__((amdgpu_waves_per_eu(MIXED ? 5 : 4, 8))) void bh_walk_kernel(
             ay += __shfl_xor(ay, o);
             az += __shfl_xor(az, o);
         }
-        if (active && rep == 0) {
-            const size_t i = (size_t)i0 + tl;
-            part[i] = ax;
-            part[(size_t)n + i] = ay;
-            part[2 * (size_t)n + i] = az;
-        }
+        if (active && rep == 0) part[(size_t)i0 + tl] = make_double4(ax, ay, az, 0.0);
     }
     if (lane == 0) {
         atomicAdd(&stats[0], t_nodes);
@@ -410,8 +405,8 @@ gravis_status bh_walk(Ctx* c, const gravis_solver* s, const Octree& t, const Fin
     uint32_t *front, *counter;
     GRAVIS_TRY(buf(c, "bh_front", (size_t)blocks * kWalkWaves * 2 * fcap, &front));
     GRAVIS_TRY(buf(c, "bh_counter", kXcds, &counter));
-    double* part;
-    GRAVIS_TRY(buf(c, "bh_part", 3 * n, &part));
+    double4* part;
+    GRAVIS_TRY(buf(c, "bh_part4", n, &part));
     const double theta2 = s->theta * s->theta;
     const double eps2 = s->softening * s->softening;
     unsigned long long* stats;
@@ -443,7 +438,7 @@ gravis_status bh_walk(Ctx* c, const gravis_solver* s, const Octree& t, const Fin
         return fail(c, GRAVIS_INTERNAL,
                     "Barnes-Hut walk frontier of a single target exceeded %u nodes (theta %g)",
                     fcap, s->theta);
-    return finish_forces(c, part, 1, ni, t.perm, fa, t.inv);
+    return finish_forces4(c, part, ni, t.inv, fa);
 }
 
 gravis_status bh_force(Ctx* c, const gravis_solver* s, size_t n, const double* x,

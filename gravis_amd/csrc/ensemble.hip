@@ -623,7 +623,7 @@ gravis_status ens_sort(Ctx* c, const EnsPlan& plan, const EnsBlock* d_sys, const
     int bits = 3 * kEnsMortonBits;
     while ((1ll << (bits - 3 * kEnsMortonBits)) < nsys) ++bits;
     uint64_t* sk;
-    return radix_sort(c, plan.n, (bits + 7) / 8, &sk, perm);
+    return radix_sort(c, plan.n, bits, &sk, perm);
 }
 
 gravis_status check_solver_ens(Ctx* c, const gravis_solver* s) {

@@ -355,10 +355,10 @@ gravis_status root_box(Ctx* c, size_t n, const double* x, const double* y, const
 gravis_status morton_sort(Ctx* c, size_t n, const double* x, const double* y, const double* z,
                           const double* d_box, uint64_t** d_keys_sorted, uint32_t** d_perm);
 // Stable LSD radix sort of n 64-bit keys with u32 values over the low
-// 8 * passes bits. The caller fills the buffers of sort_buffers first; the
+// `bits` bits. The caller fills the buffers of sort_buffers first; the
 // sorted keys and values are returned (either set of ping-pong buffers).
 gravis_status sort_buffers(Ctx* c, size_t n, uint64_t** keys, uint32_t** vals);
-gravis_status radix_sort(Ctx* c, size_t n, int passes, uint64_t** d_keys_sorted,
+gravis_status radix_sort(Ctx* c, size_t n, int bits, uint64_t** d_keys_sorted,
                          uint32_t** d_vals_sorted);
 // Spreads the low 21 bits of v to every third bit (Morton interleave).
 __device__ __forceinline__ uint64_t morton_spread3(uint32_t v) {
@@ -439,6 +439,11 @@ struct FinishArgs {
 gravis_status finish_forces(Ctx* c, const double* part, int nchunks, int n,
                             const uint32_t* perm, const FinishArgs& fa,
                             const uint32_t* inv = nullptr);
+// The same for one partial array of (ax, ay, az, 0) records in sorted order
+// (one line read per particle), delivered through inv (sorted position of
+// each caller index).
+gravis_status finish_forces4(Ctx* c, const double4* part4, int n, const uint32_t* inv,
+                             const FinishArgs& fa);
 
 // Brute-force force on device SoA arrays, result delivered per FinishArgs.
 // Particles up to which a FP64 brute-force KDK runs all its steps in one

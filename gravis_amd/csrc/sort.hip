@@ -7,9 +7,9 @@
 // order of BoundingBox::octant (octree.rs:67-79), so key prefixes are the
 // octree cells level by level.
 //
-// The radix sort is stable (LSD, 8-bit digits, 8 passes): equal keys keep
-// their input order, so the permutation is bit-exact with any stable CPU
-// sort of the same keys.
+// The radix sort is stable (LSD, 9-bit digits, 7 passes for the 63-bit
+// keys): equal keys keep their input order, so the permutation is bit-exact
+// with any stable CPU sort of the same keys.
 #include "internal.h"
 
 namespace gravis {
@@ -128,36 +128,47 @@ __global__ __launch_bounds__(256) void morton_kernel(const double* __restrict__ 
 }
 
 // ---- radix sort -----------------------------------------------------------
+// LSD passes of kRsBits-bit digits. Per pass: radix_hist_kernel counts the
+// digits of every 4096-key tile; radix_row_scan_kernel and
+// radix_digit_base_kernel turn the counts into each (digit, tile)'s first
+// output position; radix_scatter_kernel ranks every tile's keys by digit in
+// LDS (stable: element order within a digit) and writes each digit's run of
+// the tile to consecutive output positions, so the stores are contiguous
+// runs rather than one scattered key per lane.
+constexpr int kRsBits = 9;
+constexpr int kRsDigits = 1 << kRsBits;
+constexpr uint32_t kRsMask = kRsDigits - 1;
 constexpr int kRsBlock = 256;               // 4 wavefronts
 constexpr int kRsWaves = kRsBlock / kWave;  // 4
 constexpr int kRsRounds = 16;               // 64 keys per wave per round
 constexpr int kRsWaveTile = kWave * kRsRounds;      // 1024
 constexpr int kRsTile = kRsWaveTile * kRsWaves;     // 4096 keys per block
-constexpr int kRsDigits = 256;
+constexpr int kRsPer = kRsDigits / kRsBlock;        // digits per thread
+static_assert(kRsDigits % kRsBlock == 0, "whole digits per thread");
 
 __global__ __launch_bounds__(kRsBlock) void radix_hist_kernel(const uint64_t* __restrict__ keys,
                                                               size_t n, int shift,
                                                               uint32_t* __restrict__ counts,
                                                               int nblocks) {
     __shared__ uint32_t h[kRsDigits];
-    h[threadIdx.x] = 0;
+    for (int d = threadIdx.x; d < kRsDigits; d += kRsBlock) h[d] = 0;
     __syncthreads();
     const size_t base = (size_t)blockIdx.x * kRsTile;
     for (int e = threadIdx.x; e < kRsTile; e += kRsBlock) {
         size_t i = base + e;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & kRsMask], 1u);
     }
     __syncthreads();
-    counts[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    for (int d = threadIdx.x; d < kRsDigits; d += kRsBlock)
+        counts[(size_t)d * nblocks + blockIdx.x] = h[d];
 }
 
 // Exclusive scan of the per-tile counts in (digit, tile) order, in two
 // launches: one block per digit scans that digit's row of nblocks counts in
-// place and stores the row total; one block then turns the 256 totals into
+// place and stores the row total; one block then turns the totals into
 // exclusive digit bases. The scatter adds base[digit] to the row offsets.
 constexpr int kScanBlock = 256;
 constexpr int kScanPer = 4;  // consecutive counts per thread per chunk
-static_assert(kScanBlock == kRsDigits, "one thread per digit in the digit-base scan");
 
 __global__ __launch_bounds__(kScanBlock) void radix_row_scan_kernel(uint32_t* __restrict__ counts,
                                                                     int nblocks,
@@ -186,18 +197,28 @@ __global__ __launch_bounds__(kScanBlock) void radix_row_scan_kernel(uint32_t* __
     if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
-__global__ __launch_bounds__(kScanBlock) void radix_digit_base_kernel(
+// kRsPer consecutive digits per thread
+__global__ __launch_bounds__(kRsBlock) void radix_digit_base_kernel(
     uint32_t* __restrict__ totals) {
+    uint32_t v[kRsPer], sum = 0;
+#pragma unroll
+    for (int e = 0; e < kRsPer; ++e) {
+        v[e] = totals[threadIdx.x * kRsPer + e];
+        sum += v[e];
+    }
     uint32_t all;
-    const uint32_t v = totals[threadIdx.x];
-    const uint32_t ex = block_exclusive_scan<kScanBlock>(v, &all);
-    totals[threadIdx.x] = ex;
+    uint32_t run = block_exclusive_scan<kRsBlock>(sum, &all);
+#pragma unroll
+    for (int e = 0; e < kRsPer; ++e) {
+        totals[threadIdx.x * kRsPer + e] = run;
+        run += v[e];
+    }
 }
 
 __device__ __forceinline__ uint64_t digit_match(uint32_t d, uint64_t active) {
     uint64_t mask = active;
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < kRsBits; ++b) {
         const bool bit = (d >> b) & 1u;
         const uint64_t bal = __ballot(bit);
         mask &= bit ? bal : ~bal;
@@ -209,61 +230,98 @@ __global__ __launch_bounds__(kRsBlock) void radix_scatter_kernel(
     const uint64_t* __restrict__ kin, const uint32_t* __restrict__ vin, size_t n, int shift,
     const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ dbase, int nblocks,
     uint64_t* __restrict__ kout, uint32_t* __restrict__ vout) {
-    __shared__ uint32_t cnt[kRsWaves][kRsDigits];
+    __shared__ uint32_t cnt[kRsWaves][kRsDigits];  // per-wave counts, then LDS positions
+    __shared__ uint32_t lbase[kRsDigits];          // first LDS position of each digit
+    __shared__ uint32_t gbase[kRsDigits];          // first output position of each digit
+    __shared__ uint64_t lkey[kRsTile];             // the tile's keys, ranked by digit
+    __shared__ uint32_t lval[kRsTile];
     const int lane = threadIdx.x & (kWave - 1);
     const int w = threadIdx.x / kWave;
     const uint64_t lt = (1ull << lane) - 1ull;
     for (int e = threadIdx.x; e < kRsWaves * kRsDigits; e += kRsBlock) (&cnt[0][0])[e] = 0u;
     __syncthreads();
 
-    const size_t wbase = (size_t)blockIdx.x * kRsTile + (size_t)w * kRsWaveTile;
-    uint64_t key[kRsRounds];
-    uint32_t dig[kRsRounds];
-#pragma unroll
-    for (int r = 0; r < kRsRounds; ++r) {
-        const size_t i = wbase + (size_t)r * kWave + lane;
-        key[r] = i < n ? kin[i] : 0ull;
-        dig[r] = static_cast<uint32_t>((key[r] >> shift) & 255u);
-    }
-    // Phase A: per-wave digit counts (each digit has one leader lane per round).
-#pragma unroll
+    // consecutive tiles on one XCD: their runs of a digit are adjacent in
+    // the output and share cache lines, which then fill in one L2
+    const uint32_t tile = xcd_block(blockIdx.x, gridDim.x);
+    const size_t tbase = (size_t)tile * kRsTile;
+    const size_t wbase = tbase + (size_t)w * kRsWaveTile;
+    // Phase A: per-wave digit counts (each digit has one leader lane per
+    // round). Rounds are not unrolled: a round's ballots are used at once
+    // (unrolled, the masks of every round stayed live and the kernel ran
+    // at one wavefront per SIMD).
+#pragma unroll 1
     for (int r = 0; r < kRsRounds; ++r) {
         const size_t i = wbase + (size_t)r * kWave + lane;
         const uint64_t active = __ballot(i < n);
-        const uint64_t mask = digit_match(dig[r], active);
-        if (i < n && (mask & lt) == 0ull) cnt[w][dig[r]] += (uint32_t)__popcll(mask);
+        const uint32_t dig = i < n ? static_cast<uint32_t>((kin[i] >> shift) & kRsMask) : 0u;
+        const uint64_t mask = digit_match(dig, active);
+        if (i < n && (mask & lt) == 0ull) cnt[w][dig] += (uint32_t)__popcll(mask);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
     }
     __syncthreads();
-    // Phase B: global base of each (wave, digit): offsets + earlier waves.
+    // Phase B: digit runs of the tile in LDS (exclusive scan of the digit
+    // totals), each wave's first LDS position per digit, and each digit's
+    // first output position (the tile's row offset plus the digit base).
     {
-        const int d = threadIdx.x;
-        uint32_t run = dbase[d] + offsets[(size_t)d * nblocks + blockIdx.x];
-        for (int k = 0; k < kRsWaves; ++k) {
-            const uint32_t c = cnt[k][d];
-            cnt[k][d] = run;
-            run += c;
+        uint32_t tot[kRsPer], sum = 0;
+#pragma unroll
+        for (int e = 0; e < kRsPer; ++e) {
+            const int d = threadIdx.x * kRsPer + e;
+            tot[e] = 0;
+#pragma unroll
+            for (int k = 0; k < kRsWaves; ++k) tot[e] += cnt[k][d];
+            sum += tot[e];
+        }
+        uint32_t all;
+        uint32_t run = block_exclusive_scan<kRsBlock>(sum, &all);
+#pragma unroll
+        for (int e = 0; e < kRsPer; ++e) {
+            const int d = threadIdx.x * kRsPer + e;
+            lbase[d] = run;
+            gbase[d] = dbase[d] + offsets[(size_t)d * nblocks + tile];
+            uint32_t p = run;
+#pragma unroll
+            for (int k = 0; k < kRsWaves; ++k) {
+                const uint32_t c = cnt[k][d];
+                cnt[k][d] = p;
+                p += c;
+            }
+            run += tot[e];
         }
     }
     __syncthreads();
-    // Phase C: stable ranks in element order.
-#pragma unroll
+    // Phase C: stable ranks in element order, into LDS (keys read again:
+    // the tile is in L2).
+#pragma unroll 1
     for (int r = 0; r < kRsRounds; ++r) {
         const size_t i = wbase + (size_t)r * kWave + lane;
         const uint64_t active = __ballot(i < n);
-        const uint64_t mask = digit_match(dig[r], active);
-        uint32_t pos = 0;
+        const uint64_t key = i < n ? kin[i] : 0ull;
+        const uint32_t dig = static_cast<uint32_t>((key >> shift) & kRsMask);
+        const uint64_t mask = digit_match(dig, active);
         if (i < n) {
-            pos = cnt[w][dig[r]] + (uint32_t)__popcll(mask & lt);
-            kout[pos] = key[r];
-            vout[pos] = vin[i];
+            const uint32_t lp = cnt[w][dig] + (uint32_t)__popcll(mask & lt);
+            lkey[lp] = key;
+            lval[lp] = vin[i];
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
         __builtin_amdgcn_wave_barrier();
-        if (i < n && (mask & lt) == 0ull) cnt[w][dig[r]] += (uint32_t)__popcll(mask);
+        if (i < n && (mask & lt) == 0ull) cnt[w][dig] += (uint32_t)__popcll(mask);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // Phase D: the tile in digit order, each digit's run to consecutive
+    // output positions.
+    const int count = (int)min((size_t)kRsTile, n - tbase);
+    for (int e = threadIdx.x; e < count; e += kRsBlock) {
+        const uint64_t k = lkey[e];
+        const uint32_t d = static_cast<uint32_t>((k >> shift) & kRsMask);
+        const uint32_t gp = gbase[d] + ((uint32_t)e - lbase[d]);
+        kout[gp] = k;
+        vout[gp] = lval[e];
     }
 }
 
@@ -317,10 +375,11 @@ gravis_status sort_buffers(Ctx* c, size_t n, uint64_t** keys, uint32_t** vals) {
     return GRAVIS_OK;
 }
 
-gravis_status radix_sort(Ctx* c, size_t n, int passes, uint64_t** d_keys_sorted,
+gravis_status radix_sort(Ctx* c, size_t n, int bits, uint64_t** d_keys_sorted,
                          uint32_t** d_vals_sorted) {
     if (n > 0xFFFFFFFFull) return fail(c, GRAVIS_INVALID_ARG, "n too large for 32-bit indices");
-    if (passes < 1 || passes > 8) return fail(c, GRAVIS_INTERNAL, "radix passes must be 1..8");
+    if (bits < 1 || bits > 64) return fail(c, GRAVIS_INTERNAL, "radix sort bits must be 1..64");
+    const int passes = (bits + kRsBits - 1) / kRsBits;
     uint64_t *k0, *k1;
     uint32_t *v0, *v1, *counts, *dbase;
     const int nblocks = (int)((n + kRsTile - 1) / kRsTile);
@@ -332,10 +391,10 @@ gravis_status radix_sort(Ctx* c, size_t n, int passes, uint64_t** d_keys_sorted,
     GRAVIS_TRY(buf(c, "sort_counts", m, &counts));
     GRAVIS_TRY(buf(c, "sort_dbase", kRsDigits, &dbase));
     for (int pass = 0; pass < passes; ++pass) {
-        const int shift = 8 * pass;
+        const int shift = kRsBits * pass;
         radix_hist_kernel<<<nblocks, kRsBlock, 0, c->stream>>>(k0, n, shift, counts, nblocks);
         radix_row_scan_kernel<<<kRsDigits, kScanBlock, 0, c->stream>>>(counts, nblocks, dbase);
-        radix_digit_base_kernel<<<1, kScanBlock, 0, c->stream>>>(dbase);
+        radix_digit_base_kernel<<<1, kRsBlock, 0, c->stream>>>(dbase);
         radix_scatter_kernel<<<nblocks, kRsBlock, 0, c->stream>>>(k0, v0, n, shift, counts,
                                                                   dbase, nblocks, k1, v1);
         std::swap(k0, k1);
@@ -352,8 +411,7 @@ gravis_status morton_sort(Ctx* c, size_t n, const double* x, const double* y, co
     uint32_t* v0;
     GRAVIS_TRY(sort_buffers(c, n, &k0, &v0));
     morton_kernel<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(x, y, z, n, d_box, k0, v0);
-    // 8 passes: the result is back in the first buffers
-    return radix_sort(c, n, 8, d_keys_sorted, d_perm);
+    return radix_sort(c, n, 63, d_keys_sorted, d_perm);
 }
 
 gravis_status gather_sorted(Ctx* c, size_t n, const double* x, const double* y,
