@@ -323,7 +323,7 @@ __global__ __launch_bounds__(kWave) void ens_wave_kernel(
         for (int e = 0; e < LPT; ++e) {
             const int j = jb + e * kWave + lane;
             // padding: zero mass at the box centre (no contribution)
-            nxt[e] = j < b.j1 ? pin[j] : make_double4(cx, cy, cz, 0.0);
+            nxt[e] = record_or(pin, j, b.j1, make_double4(cx, cy, cz, 0.0));
         }
     };
     if (b.j0 < b.j1) load(b.j0);

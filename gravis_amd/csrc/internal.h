@@ -254,6 +254,15 @@ __device__ __forceinline__ float pair_f32(float dx, float dy, float dz, float m,
     return mr * ri2;
 }
 
+// Record j of a tile (j < end), else pad. The load reads record
+// min(j, end - 1) (end > 0) and the select is on values: a select on the
+// address would place pad in scratch and make the load a flat load.
+__device__ __forceinline__ double4 record_or(const double4* __restrict__ rec, int j, int end,
+                                             const double4 pad) {
+    const double4 r = rec[min(j, end - 1)];
+    return j < end ? r : pad;
+}
+
 __device__ __forceinline__ void kahan_add(float& s, float& c, float v) {
     const float y = v - c;
     const float t = s + y;
