@@ -6,10 +6,11 @@
 // cube of octree.rs:14-44. Here the same root cube is cut by 63-bit Morton
 // keys, so every node is a run of the key-sorted particles and its cell is
 // the reference's cell at that depth:
-//   build   level by level: each internal node (more than `leaf` particles,
-//           level < 21) finds its child runs by binary search on the next
-//           3-bit digit; nodes are numbered breadth first, children of a
-//           node contiguous in digit order.
+//   build   level by level, one launch per level: each internal node (more
+//           than `leaf` particles, level < 21) finds its child runs by
+//           binary search on the next 3-bit digit; the nodes of a level
+//           follow those of the level above, and the children of a node
+//           are contiguous in digit order.
 //   sums    bottom up in FP64: leaves sum their particles in sorted order,
 //           internal nodes sum their children in order (m, m*x; no FMA);
 //           com = sum(m x) / sum(m). The particle bounding box of every node
@@ -28,9 +29,10 @@ constexpr uint32_t kGroupRoot = kNodeGroupRoot;
 
 __global__ void tree_init_kernel(uint4* __restrict__ meta, uint32_t* __restrict__ lvl,
                                  uint32_t n, uint32_t* __restrict__ flags,
-                                 uint32_t* __restrict__ ngroups) {
+                                 uint32_t* __restrict__ ngroups, uint32_t* __restrict__ cnt) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     for (int k = 0; k < 4; ++k) flags[k] = 0u;
+    for (int L = 0; L <= kMaxLevel; ++L) cnt[L] = 0u;
     *ngroups = 0u;
     meta[0] = make_uint4(0u, n, 0u, kGroupRoot);
     lvl[0] = 0u;
@@ -72,15 +74,33 @@ __device__ __forceinline__ NodeRun node_split(const uint64_t* __restrict__ keys,
     return {lo, ne ? stop : lo, (uint32_t)(bal >> (lane & ~7)) & 0xFFu};
 }
 
-// Number of non-empty children of each node of level L (8 lanes per node;
-// the grid-stride loop is uniform across each block, so the wavefront-wide
-// shuffles and ballots see every lane).
-__global__ __launch_bounds__(kTreeBlock) void tree_count_kernel(const uint64_t* __restrict__ keys,
-                                                               const uint4* __restrict__ meta,
-                                                               const uint32_t* __restrict__ lvl,
-                                                               int L, uint32_t leaf,
-                                                               uint32_t* __restrict__ nch) {
-    const uint32_t b = lvl[2 * L], e = lvl[2 * L + 1];
+// One level of the build in one launch. Level L's range is the end of
+// level L-1 plus level L-1's children (cnt[L-1]); block 0 records it in lvl.
+// Each node of the level (8 lanes per node; the grid-stride loop is uniform
+// across each block, so the wavefront-wide shuffles, ballots and the block
+// scan see every lane) finds its child runs, reserves slots for its
+// children after the level (one atomic add on cnt[L] per block and
+// iteration) and writes them: the children of a node are contiguous in
+// digit order, and the order of the nodes within a level follows the
+// reservations (every consumer reads a node's children through its first
+// child and count, so none depends on that order).
+__global__ __launch_bounds__(kTreeBlock) void tree_level_kernel(
+    const uint64_t* __restrict__ keys, uint4* __restrict__ meta, uint32_t* __restrict__ lvl,
+    uint32_t* __restrict__ cnt, int L, uint32_t leaf, uint32_t cap, uint32_t gsize,
+    uint32_t* __restrict__ flag) {
+    __shared__ uint32_t sbase;
+    // a level that ran out of node capacity ends the build (the caller
+    // reports it); the levels below keep their empty ranges
+    if (__builtin_amdgcn_readfirstlane(*flag) & 2u) return;
+    uint32_t b = 0, e = 1;
+    if (L > 0) {
+        b = lvl[2 * L - 1];
+        e = min(b + cnt[L - 1], cap);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        lvl[2 * L] = b;
+        lvl[2 * L + 1] = e;
+    }
     const int shift = 3 * (kMaxLevel - (L + 1));
     const uint32_t total = 8u * (e - b);
     const int lane = threadIdx.x & (kWave - 1);
@@ -91,102 +111,17 @@ __global__ __launch_bounds__(kTreeBlock) void tree_count_kernel(const uint64_t* 
         const uint32_t d = t & 7u;
         const uint4 md = valid ? meta[v] : make_uint4(0u, 0u, 0u, 0u);
         const bool split = valid && md.y > leaf && L < kMaxLevel;
-        const NodeRun r = node_split(keys, md, shift, split, d, lane);
-        if (valid && d == 0) nch[v - b] = (uint32_t)__popc(r.bits);
-    }
-}
-
-// Exclusive scan of the level's child counts and the node range of level
-// L+1, over a fixed grid of kTsBlocks blocks whose ranges split the level
-// (sizes live on the device): per-block sums, one block scanning those, and
-// a down-sweep writing the offsets.
-constexpr int kTsBlocks = 256;
-constexpr int kTsThreads = 256;
-constexpr int kTsPer = 4;
-
-__device__ __forceinline__ void ts_range(uint32_t m, uint32_t* lo, uint32_t* hi) {
-    const uint32_t per = (m + kTsBlocks - 1) / kTsBlocks;
-    *lo = min(m, blockIdx.x * per);
-    *hi = min(m, *lo + per);
-}
-
-__global__ __launch_bounds__(kTsThreads) void tree_scan_reduce_kernel(
-    const uint32_t* __restrict__ nch, const uint32_t* __restrict__ lvl, int L,
-    uint32_t* __restrict__ bsum) {
-    const uint32_t m = lvl[2 * L + 1] - lvl[2 * L];
-    uint32_t lo, hi;
-    ts_range(m, &lo, &hi);
-    uint32_t s = 0;
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += kTsThreads) s += nch[i];
-    uint32_t all;
-    block_exclusive_scan<kTsThreads>(s, &all);
-    if (threadIdx.x == 0) bsum[blockIdx.x] = all;
-}
-
-__global__ __launch_bounds__(kTsBlocks) void tree_scan_top_kernel(uint32_t* __restrict__ bsum,
-                                                                  uint32_t* __restrict__ lvl,
-                                                                  int L) {
-    uint32_t all;
-    const uint32_t ex = block_exclusive_scan<kTsBlocks>(bsum[threadIdx.x], &all);
-    bsum[threadIdx.x] = ex;
-    if (threadIdx.x == 0) {
-        const uint32_t e = lvl[2 * L + 1];
-        lvl[2 * (L + 1)] = e;
-        lvl[2 * (L + 1) + 1] = e + all;
-    }
-}
-
-__global__ __launch_bounds__(kTsThreads) void tree_scan_down_kernel(
-    const uint32_t* __restrict__ nch, uint32_t* __restrict__ off,
-    const uint32_t* __restrict__ lvl, int L, const uint32_t* __restrict__ bsum) {
-    const uint32_t m = lvl[2 * L + 1] - lvl[2 * L];
-    uint32_t lo, hi;
-    ts_range(m, &lo, &hi);
-    uint32_t carry = bsum[blockIdx.x];
-    for (uint32_t base = lo; base < hi; base += kTsThreads * kTsPer) {
-        const uint32_t i0 = base + threadIdx.x * kTsPer;
-        uint32_t v[kTsPer];
-        uint32_t sum = 0;
-#pragma unroll
-        for (int e = 0; e < kTsPer; ++e) {
-            v[e] = i0 + e < hi ? nch[i0 + e] : 0u;
-            sum += v[e];
-        }
-        uint32_t chunk;
-        uint32_t run = carry + block_exclusive_scan<kTsThreads>(sum, &chunk);
-#pragma unroll
-        for (int e = 0; e < kTsPer; ++e)
-            if (i0 + e < hi) {
-                off[i0 + e] = run;
-                run += v[e];
-            }
-        carry += chunk;
-    }
-}
-
-static_assert(kTsBlocks % kWave == 0 && kTsThreads % kWave == 0, "whole wavefronts");
-
-__global__ __launch_bounds__(kTreeBlock) void tree_write_kernel(
-    const uint64_t* __restrict__ keys, uint4* __restrict__ meta, const uint32_t* __restrict__ lvl,
-    int L, const uint32_t* __restrict__ nch, const uint32_t* __restrict__ off, uint32_t cap,
-    uint32_t gsize, uint32_t* __restrict__ flag) {
-    const uint32_t b = lvl[2 * L], e = lvl[2 * L + 1];
-    const int shift = 3 * (kMaxLevel - (L + 1));
-    const uint32_t total = 8u * (e - b);
-    const int lane = threadIdx.x & (kWave - 1);
-    for (uint32_t t0 = blockIdx.x * kTreeBlock; t0 < total; t0 += gridDim.x * kTreeBlock) {
-        const uint32_t t = t0 + threadIdx.x;
-        const bool valid = t < total;
-        const uint32_t v = b + (valid ? t / 8u : 0u);
-        const uint32_t d = t & 7u;
-        const uint32_t nc = valid ? nch[v - b] : 0u;
-        const uint4 md = valid ? meta[v] : make_uint4(0u, 0u, 0u, 0u);
-        const uint32_t c0 = e + (valid ? off[v - b] : 0u);
-        const bool fits = c0 + nc <= cap;
         // every lane of the node reads meta[v] above before lane 0 rewrites it
-        const NodeRun r = node_split(keys, md, shift, nc > 0 && fits, d, lane);
+        const NodeRun r = node_split(keys, md, shift, split, d, lane);
+        const uint32_t nc = (uint32_t)__popc(r.bits);
+        uint32_t all;
+        const uint32_t ex = block_exclusive_scan<kTreeBlock>(d == 0 ? nc : 0u, &all);
+        if (threadIdx.x == 0) sbase = all > 0u ? atomicAdd(&cnt[L], all) : 0u;
+        __syncthreads();
+        const uint32_t c0 = e + sbase + __shfl(ex, lane & ~7);
+        __syncthreads();  // sbase is rewritten by the next iteration
         if (nc == 0) continue;
-        if (!fits) {
+        if (c0 + nc > cap) {
             if (d == 0) atomicOr(flag, 2u);
             continue;
         }
@@ -198,6 +133,15 @@ __global__ __launch_bounds__(kTreeBlock) void tree_write_kernel(
         }
         if (d == 0)
             meta[v] = make_uint4(md.x, md.y, c0, (md.w & kGroupRoot) | (nc << 8) | (uint32_t)L);
+    }
+}
+
+// Node count after the last level: lvl[2 (kMaxLevel + 1)] = lvl[.. + 1] =
+// the end of level kMaxLevel (whose nodes have no children).
+__global__ void tree_close_kernel(uint32_t* __restrict__ lvl) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        lvl[2 * (kMaxLevel + 1)] = lvl[2 * kMaxLevel + 1];
+        lvl[2 * (kMaxLevel + 1) + 1] = lvl[2 * kMaxLevel + 1];
     }
 }
 
@@ -369,7 +313,7 @@ gravis_status build_octree(Ctx* c, size_t n, const double* x, const double* y, c
     if (cap > 0x7FFFFFFFull) return fail(c, GRAVIS_INVALID_ARG, "tree too large");
     double4 *src, *sums, *bmin, *bmax;
     uint4* meta;
-    uint32_t *lvl, *nch, *off, *flags, *ngroups, *bsum;
+    uint32_t *lvl, *cnt, *flags, *ngroups;
     uint2* groups;
     GRAVIS_TRY(buf(c, "tree_src", n + cap, &src));
     GRAVIS_TRY(buf(c, "tree_sums", cap, &sums));
@@ -377,9 +321,7 @@ gravis_status build_octree(Ctx* c, size_t n, const double* x, const double* y, c
     GRAVIS_TRY(buf(c, "tree_bmax", cap, &bmax));
     GRAVIS_TRY(buf(c, "tree_meta", cap, &meta));
     GRAVIS_TRY(buf(c, "tree_lvl", 2 * (kMaxLevel + 2), &lvl));
-    GRAVIS_TRY(buf(c, "tree_nch", cap, &nch));
-    GRAVIS_TRY(buf(c, "tree_off", cap, &off));
-    GRAVIS_TRY(buf(c, "tree_bsum", kTsBlocks, &bsum));
+    GRAVIS_TRY(buf(c, "tree_cnt", kMaxLevel + 1, &cnt));
     GRAVIS_TRY(buf(c, "tree_flags", 4, &flags));
     GRAVIS_TRY(buf(c, "tree_ngroups", 1, &ngroups));
     GRAVIS_TRY(buf(c, "tree_groups", n, &groups));
@@ -388,20 +330,13 @@ gravis_status build_octree(Ctx* c, size_t n, const double* x, const double* y, c
     GRAVIS_TRY(gather_sorted(c, n, x, y, z, m, perm, src, &inv));
 
     hipStream_t st = c->stream;
-    tree_init_kernel<<<1, 64, 0, st>>>(meta, lvl, (uint32_t)n, flags, ngroups);
+    tree_init_kernel<<<1, 64, 0, st>>>(meta, lvl, (uint32_t)n, flags, ngroups, cnt);
     const unsigned tg = grid_for(cap, kTreeBlock, kTreeGrid);
-    for (int L = 0; L < kMaxLevel; ++L) {
-        tree_count_kernel<<<tg, kTreeBlock, 0, st>>>(keys, meta, lvl, L, leaf, nch);
-        tree_scan_reduce_kernel<<<kTsBlocks, kTsThreads, 0, st>>>(nch, lvl, L, bsum);
-        tree_scan_top_kernel<<<1, kTsBlocks, 0, st>>>(bsum, lvl, L);
-        tree_scan_down_kernel<<<kTsBlocks, kTsThreads, 0, st>>>(nch, off, lvl, L, bsum);
-        tree_write_kernel<<<tg, kTreeBlock, 0, st>>>(keys, meta, lvl, L, nch, off, (uint32_t)cap,
-                                                    group, flags + 1);
-    }
-    // level 21 holds leaves only (zero children); its scan closes the node range
-    tree_count_kernel<<<tg, kTreeBlock, 0, st>>>(keys, meta, lvl, kMaxLevel, leaf, nch);
-    tree_scan_reduce_kernel<<<kTsBlocks, kTsThreads, 0, st>>>(nch, lvl, kMaxLevel, bsum);
-    tree_scan_top_kernel<<<1, kTsBlocks, 0, st>>>(bsum, lvl, kMaxLevel);
+    // level 21 holds leaves only (no children); the close kernel ends the range
+    for (int L = 0; L <= kMaxLevel; ++L)
+        tree_level_kernel<<<tg, kTreeBlock, 0, st>>>(keys, meta, lvl, cnt, L, leaf,
+                                                     (uint32_t)cap, group, flags + 1);
+    tree_close_kernel<<<1, 64, 0, st>>>(lvl);
     GRAVIS_TRY(check_launch(c, "tree build"));
     for (int L = kMaxLevel; L >= 0; --L)
         tree_sums_kernel<<<tg, kTreeBlock, 0, st>>>(src, meta, lvl, L, sums, bmin, bmax);
