@@ -221,18 +221,76 @@ __global__ __launch_bounds__(kTreeBlock) void tree_com_kernel(double4* __restric
 // is cut into runs of gsize; a root holding at most gsize (or a root leaf) is
 // cut the same way. Appended in any order: each group's result depends only
 // on its own particles and the tree.
-__device__ __forceinline__ void emit_runs(uint2* groups, uint32_t* ngroups, uint32_t first,
-                                          uint32_t cnt, uint32_t gsize) {
+// Groups of a run of cnt particles from first: ceil(cnt / gsize) runs of
+// at most gsize, written at groups[at..] when EMIT; returns their number.
+template <bool EMIT>
+__device__ __forceinline__ uint32_t emit_runs(uint2* groups, uint32_t at, uint32_t first,
+                                              uint32_t cnt, uint32_t gsize) {
     const uint32_t k = (cnt + gsize - 1) / gsize;
-    const uint32_t at = atomicAdd(ngroups, k);
-    for (uint32_t q = 0; q < k; ++q)
-        groups[at + q] = make_uint2(first + q * gsize, min(gsize, cnt - q * gsize));
+    if (EMIT)
+        for (uint32_t q = 0; q < k; ++q)
+            groups[at + q] = make_uint2(first + q * gsize, min(gsize, cnt - q * gsize));
+    return k;
 }
 
 __device__ __forceinline__ double extent(double4 lo, double4 hi) {
     return fmax(fmax(hi.x - lo.x, hi.y - lo.y), hi.z - lo.z);
 }
 
+// The groups node p cuts (see above): counted, or written from groups[at].
+template <bool EMIT>
+__device__ __forceinline__ uint32_t node_groups(uint32_t p, const uint4 md,
+                                                const uint4* __restrict__ meta,
+                                                const double4* __restrict__ bmin,
+                                                const double4* __restrict__ bmax, uint32_t gsize,
+                                                uint2* __restrict__ groups, uint32_t at) {
+    const uint32_t nc = (md.w >> 8) & 255u;
+    if (p == 0 && (md.y <= gsize || nc == 0))
+        return emit_runs<EMIT>(groups, at, md.x, md.y, gsize);
+    if (md.y <= gsize || nc == 0) return 0u;
+    uint32_t k = 0;
+    uint32_t rf = 0, rc = 0;
+    double4 rlo = make_double4(0, 0, 0, 0), rhi = rlo;
+    double rmax = 0.0;
+    for (uint32_t q = 0; q < nc; ++q) {
+        const uint32_t cv = md.z + q;
+        const uint4 cm = meta[cv];
+        const uint32_t cc = (cm.w >> 8) & 255u;
+        if (cm.y <= gsize) {
+            const double4 clo = bmin[cv], chi = bmax[cv];
+            const double cext = extent(clo, chi);
+            if (rc > 0) {
+                const double4 ulo = make_double4(fmin(rlo.x, clo.x), fmin(rlo.y, clo.y),
+                                                 fmin(rlo.z, clo.z), 0.0);
+                const double4 uhi = make_double4(fmax(rhi.x, chi.x), fmax(rhi.y, chi.y),
+                                                 fmax(rhi.z, chi.z), 0.0);
+                if (rc + cm.y <= gsize && extent(ulo, uhi) <= 2.0 * fmax(rmax, cext)) {
+                    rc += cm.y;
+                    rlo = ulo;
+                    rhi = uhi;
+                    rmax = fmax(rmax, cext);
+                    continue;
+                }
+                k += emit_runs<EMIT>(groups, at + k, rf, rc, gsize);
+            }
+            rf = cm.x;
+            rc = cm.y;
+            rlo = clo;
+            rhi = chi;
+            rmax = cext;
+        } else {
+            if (rc > 0) k += emit_runs<EMIT>(groups, at + k, rf, rc, gsize);
+            rc = 0;
+            if (cc == 0) k += emit_runs<EMIT>(groups, at + k, cm.x, cm.y, gsize);
+        }
+    }
+    if (rc > 0) k += emit_runs<EMIT>(groups, at + k, rf, rc, gsize);
+    return k;
+}
+
+// Every node counts its groups, the wavefront reserves them with one atomic
+// add and each node then writes its own at its offset. The loop is uniform
+// across each block (every lane takes part in the scan).
 __global__ __launch_bounds__(kTreeBlock) void group_emit_kernel(const uint4* __restrict__ meta,
                                                                const uint32_t* __restrict__ lvl,
                                                                const double4* __restrict__ bmin,
@@ -241,51 +299,21 @@ __global__ __launch_bounds__(kTreeBlock) void group_emit_kernel(const uint4* __r
                                                                uint2* __restrict__ groups,
                                                                uint32_t* __restrict__ ngroups) {
     const uint32_t total = lvl[2 * (kMaxLevel + 1)];
-    for (uint32_t p = blockIdx.x * kTreeBlock + threadIdx.x; p < total;
-         p += gridDim.x * kTreeBlock) {
-        const uint4 md = meta[p];
-        const uint32_t nc = (md.w >> 8) & 255u;
-        if (p == 0 && (md.y <= gsize || nc == 0)) {
-            emit_runs(groups, ngroups, md.x, md.y, gsize);
-            continue;
+    const int lane = threadIdx.x & (kWave - 1);
+    for (uint32_t t0 = blockIdx.x * kTreeBlock; t0 < total; t0 += gridDim.x * kTreeBlock) {
+        const uint32_t p = t0 + threadIdx.x;
+        const bool valid = p < total;
+        const uint4 md = valid ? meta[p] : make_uint4(0u, 0u, 0u, 0u);
+        const uint32_t k = valid ? node_groups<false>(p, md, meta, bmin, bmax, gsize, groups, 0u)
+                                 : 0u;
+        const uint32_t inc = wave_inclusive_scan(k);
+        const uint32_t wtot = __builtin_amdgcn_readlane(inc, kWave - 1);
+        uint32_t base = 0;
+        if (wtot > 0) {
+            if (lane == 0) base = atomicAdd(ngroups, wtot);
+            base = __builtin_amdgcn_readfirstlane(base);
         }
-        if (md.y <= gsize || nc == 0) continue;
-        uint32_t rf = 0, rc = 0;
-        double4 rlo = make_double4(0, 0, 0, 0), rhi = rlo;
-        double rmax = 0.0;
-        for (uint32_t q = 0; q < nc; ++q) {
-            const uint32_t cv = md.z + q;
-            const uint4 cm = meta[cv];
-            const uint32_t cc = (cm.w >> 8) & 255u;
-            if (cm.y <= gsize) {
-                const double4 clo = bmin[cv], chi = bmax[cv];
-                const double cext = extent(clo, chi);
-                if (rc > 0) {
-                    const double4 ulo = make_double4(fmin(rlo.x, clo.x), fmin(rlo.y, clo.y),
-                                                     fmin(rlo.z, clo.z), 0.0);
-                    const double4 uhi = make_double4(fmax(rhi.x, chi.x), fmax(rhi.y, chi.y),
-                                                     fmax(rhi.z, chi.z), 0.0);
-                    if (rc + cm.y <= gsize && extent(ulo, uhi) <= 2.0 * fmax(rmax, cext)) {
-                        rc += cm.y;
-                        rlo = ulo;
-                        rhi = uhi;
-                        rmax = fmax(rmax, cext);
-                        continue;
-                    }
-                    emit_runs(groups, ngroups, rf, rc, gsize);
-                }
-                rf = cm.x;
-                rc = cm.y;
-                rlo = clo;
-                rhi = chi;
-                rmax = cext;
-            } else {
-                if (rc > 0) emit_runs(groups, ngroups, rf, rc, gsize);
-                rc = 0;
-                if (cc == 0) emit_runs(groups, ngroups, cm.x, cm.y, gsize);
-            }
-        }
-        if (rc > 0) emit_runs(groups, ngroups, rf, rc, gsize);
+        if (k > 0) (void)node_groups<true>(p, md, meta, bmin, bmax, gsize, groups, base + inc - k);
     }
 }
 
