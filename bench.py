@@ -125,13 +125,15 @@ def phase(bytes_, seconds, note):
 
 def tree_phase_rooflines(n, nodes, t):
     """The octree build's phases (HIP events around each, launch gaps
-    included) against their compulsory bytes: Morton keys and an 8-pass
-    8-bit radix sort of 12-byte (key, index) pairs; the gather into Morton
-    order, one read of the records for the sums and the node arrays."""
+    included) against their compulsory bytes: Morton keys and a 7-pass
+    9-bit radix sort of the 63-bit keys as 12-byte (key, index) pairs; the
+    gather into Morton order, one read of the records for the sums and the
+    node arrays."""
+    passes = 7
     return {
-        "morton_sort": phase(n * (24 + 12) + 8 * n * 24, t.get("morton_sort"),
-                             "N*(24 pos in + 12 key/index out) + 8 passes * N*24 (12 B read + "
-                             "12 B written per key)"),
+        "morton_sort": phase(n * (24 + 12) + passes * n * 24, t.get("morton_sort"),
+                             f"N*(24 pos in + 12 key/index out) + {passes} passes * N*24 (12 B "
+                             "read + 12 B written per key)"),
         "tree_build": phase(n * 68 + n * 32 + nodes * 112, t.get("tree_build"),
                             "N*68 gather (32 B in, 4 B index, 32 B record out) + N*32 record "
                             "read for the sums + nodes*112 (16 B meta, 32 B COM/mass, 64 B "

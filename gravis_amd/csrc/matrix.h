@@ -64,6 +64,18 @@ __device__ __forceinline__ void split_bf16x3(float x, __bf16& hi, __bf16& mid, _
 __device__ constexpr int kSplitB[6] = {0, 0, 1, 0, 1, 2};
 __device__ constexpr int kSplitA[6] = {0, 1, 0, 2, 1, 0};
 
+// A operands per 16-source block in LDS: the FP32 form keeps all 64 (four
+// K slots of 16 rows); the bf16 form keeps the slot groups 0..2 (48), since
+// the B operand of slot group 3 (slots 24..31) is zero for every target.
+// The lanes that supply group 3 (48..63) read 48 entries on, i.e. the next
+// block's group 0 (finite split values) or, after the tile's last block, 16
+// zero entries kept past the A operands; their products with the zero B
+// slots add exactly zero.
+template <typename T>
+constexpr int mx_a_stride() {
+    return std::is_same<T, float>::value ? 64 : 48;
+}
+
 // One source block of the matrix-core far loop: the pair forces of block
 // sbc (products dc; its source terms are read from LDS here rather than a
 // block ahead, so the split-bf16 kernel fits 128 VGPRs, four wavefronts per
@@ -84,7 +96,7 @@ __device__ __forceinline__ void mx_block(const T* __restrict__ aop,
 #pragma unroll
     for (int v = 0; v < 4; ++v) qc[v] = qop[sbc * 16 + 4 * grp + v];
     if (MORE) {
-        avn = aop[sbn * 64 + lane];
+        avn = aop[sbn * mx_a_stride<T>() + lane];
         const float4 c4 = *reinterpret_cast<const float4*>(bbq + sbn * 16 + 4 * grp);
         cbn = f32x4{c4.x, c4.y, c4.z, c4.w};
     }
@@ -114,11 +126,18 @@ __device__ __forceinline__ void mx_block(const T* __restrict__ aop,
 
 
 // LDS of one wavefront's tile in float4 units: near, TJ FP64 records; far,
-// the A operands [TJ / 16][4][16] of type T, then (m b, m) and |b|^2 per
-// source.
+// the A operands [TJ / 16][4 or 3][16] of type T (FP32: 4 floats, one
+// float4, per source; bf16: 3 bf16x8 per source, then 16 zero entries),
+// then (m b, m) and |b|^2 per source. 8.75 KB at TJ = 128 in the bf16 form:
+// 18 wavefronts' worth per CU, above the 16 that 128 VGPRs allow (with all
+// four slot groups, 10.5 KB held a CU to 15).
+template <int TJ, bool BF>
+constexpr int mx_a_f4() {
+    return BF ? 3 * TJ + 16 : TJ;
+}
 template <int TJ, bool BF>
 constexpr int mx_lds_f4() {
-    return (BF ? 4 * TJ : TJ) + TJ + TJ / 4;
+    return mx_a_f4<TJ, BF>() + TJ + TJ / 4;
 }
 
 // Tiers of a source tile.
@@ -140,12 +159,16 @@ __device__ __forceinline__ void mx_sweep(float4* lds, const double4* __restrict_
     constexpr int LPT = TJ / kWave;  // records each lane stages per tile
     constexpr int SB = TJ / 16;      // 16-source blocks per tile (even)
     using T = typename std::conditional<BF, bf16x8, float>::type;
-    constexpr int kA = BF ? 4 * TJ : TJ;  // float4 units of the A operands
+    constexpr int kA = mx_a_f4<TJ, BF>();  // float4 units of the A operands
+    constexpr int kAS = mx_a_stride<T>();  // A operands per 16-source block
     double4* tile = reinterpret_cast<double4*>(&lds[0]);
     T* aop = reinterpret_cast<T*>(&lds[0]);
     float4* qop = &lds[kA];
     float* bbq = reinterpret_cast<float*>(&lds[kA + TJ]);
     const int col = lane & 15, grp = lane >> 4;
+    // the zero entries read by slot group 3 after a tile's last block
+    // (outside every tier's staging; visible after the first tile's barrier)
+    if (BF && lane < 16) aop[SB * kAS + lane] = T{};
     // B operand of each 16-target block
     T bop[4];
 #pragma unroll
@@ -221,9 +244,9 @@ __device__ __forceinline__ void mx_sweep(float4* lds, const double4* __restrict_
                     split_bf16x3(bx, pc[0][0], pc[0][1], pc[0][2]);
                     split_bf16x3(by, pc[1][0], pc[1][1], pc[1][2]);
                     split_bf16x3(bz, pc[2][0], pc[2][1], pc[2][2]);
-                    T* a = aop + (sj >> 4) * 64 + (sj & 15);
+                    T* a = aop + (sj >> 4) * kAS + (sj & 15);
 #pragma unroll
-                    for (int kg = 0; kg < 4; ++kg) {
+                    for (int kg = 0; kg < 3; ++kg) {
                         bf16x8 v;
 #pragma unroll
                         for (int j = 0; j < 8; ++j) {
@@ -234,7 +257,7 @@ __device__ __forceinline__ void mx_sweep(float4* lds, const double4* __restrict_
                         a[16 * kg] = v;
                     }
                 } else {
-                    float* a = aop + (sj >> 4) * 64 + (sj & 15);
+                    float* a = aop + (sj >> 4) * kAS + (sj & 15);
                     a[0] = bx;
                     a[16] = by;
                     a[32] = bz;
