@@ -382,6 +382,16 @@ gravis_status gravis_kdk_device(gravis_ctx* g, const gravis_solver* s, size_t n,
         n <= c->kdk_fused_max_n)
         return gravis::kdk_small(c, n, x, y, z, vx, vy, vz, m, ax, ay, az, s->softening, dt,
                                  nsteps);
+    if (s->algorithm == GRAVIS_BRUTE_FORCE && s->precision == GRAVIS_FP64 &&
+        n > (size_t)gravis::kSmallN && n <= c->kdk_coop_max_n && nsteps >= 2) {
+        // one cooperative launch, unless the stream is being captured (the
+        // call synchronises it)
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        GRAVIS_HIP(c, hipStreamIsCapturing(c->stream, &cs));
+        if (cs == hipStreamCaptureStatusNone)
+            return gravis::kdk_coop(c, n, x, y, z, vx, vy, vz, m, ax, ay, az, s->softening, dt,
+                                    nsteps);
+    }
     for (uint64_t step = 0; step < nsteps; ++step) {
         GRAVIS_TRY(gravis::kick_drift(c, n, x, y, z, vx, vy, vz, ax, ay, az, h, dt));
         GRAVIS_TRY(force_device(c, s, n, x, y, z, m, fa));
@@ -673,6 +683,13 @@ gravis_status gravis_set_tuning(gravis_ctx* g, const char* key, int64_t value) {
         if (value < 1 || value > 4096)
             return fail(c, GRAVIS_INVALID_ARG, "sph_pool_init must be in [1, 4096]");
         c->sph_pool_init = (uint32_t)value;
+        return GRAVIS_OK;
+    }
+    if (std::strcmp(key, "kdk_coop_max_n") == 0) {
+        if (value < 0 || value > gravis::kCoopMaxN)
+            return fail(c, GRAVIS_INVALID_ARG, "kdk_coop_max_n must be in [0, %d]",
+                        gravis::kCoopMaxN);
+        c->kdk_coop_max_n = (uint32_t)value;
         return GRAVIS_OK;
     }
     if (std::strcmp(key, "kdk_fused_max_n") == 0) {

@@ -59,6 +59,10 @@ struct Ctx {
     // FP64 brute-force KDK of at most this many particles runs fused in one
     // workgroup (0 disables; at most kSmallN)
     uint32_t kdk_fused_max_n = 256;
+    // ... and of more than that, up to this many (several steps, stream not
+    // being captured) in one cooperative launch (0 disables; at most
+    // kCoopMaxN)
+    uint32_t kdk_coop_max_n = 2040;
     // Brute-force launch shape (gravis_set_tuning / gravis_autotune):
     // targets per lane of the mixed kernel (wavefront groups of 64 or 128),
     // j records per staged tile (128 or 256), and the wavefronts per CU the
@@ -461,6 +465,15 @@ constexpr int kSmallN = 256;
 gravis_status kdk_small(Ctx* c, size_t n, double* x, double* y, double* z, double* vx,
                         double* vy, double* vz, const double* m, double* ax, double* ay,
                         double* az, double softening, double dt, uint64_t nsteps);
+// Particles up to which a FP64 brute-force KDK of several steps runs all of
+// them in one cooperative launch with a grid barrier per step (kdk_coop,
+// bf.hip; the step's records are held in LDS, 32 B each, with the
+// kernel's 4-byte flag within 64 KB). The call
+// synchronises the stream to read the barrier's abort word.
+constexpr int kCoopMaxN = 2040;
+gravis_status kdk_coop(Ctx* c, size_t n, double* x, double* y, double* z, double* vx,
+                       double* vy, double* vz, const double* m, double* ax, double* ay,
+                       double* az, double softening, double dt, uint64_t nsteps);
 
 gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x,
                        const double* y, const double* z, const double* m, const FinishArgs& fa);

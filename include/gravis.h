@@ -130,7 +130,13 @@ gravis_status gravis_kdk(gravis_ctx* ctx, const gravis_solver* solver, size_t n,
                          const double* m, double* ax, double* ay, double* az, double dt,
                          uint64_t nsteps);
 
-/* Same with device pointers, asynchronous on the context stream. */
+/* Same with device pointers, asynchronous on the context stream. FP64
+ * brute force runs every step in one launch: in one workgroup for n up to
+ * "kdk_fused_max_n" (default 256, bit-identical to the step-by-step path),
+ * and, for several steps of 256 < n <= "kdk_coop_max_n" (default 2040) on a
+ * stream that is not being captured, in one cooperative launch with a grid
+ * barrier per step (forces summed in another order, so they differ from the
+ * step-by-step path in rounding); that call synchronises the stream. */
 gravis_status gravis_kdk_device(gravis_ctx* ctx, const gravis_solver* solver, size_t n,
                                 double* x, double* y, double* z, double* vx, double* vy,
                                 double* vz, const double* m, double* ax, double* ay, double* az,
@@ -362,7 +368,11 @@ gravis_status gravis_kernel_time(gravis_ctx* ctx, const char* name, double* ms_t
  * neighbour search holds (1..512, default 512; a group needing more is
  * searched one target at a time, depth first); "sph_pool_init" = initial
  * neighbour-pool capacity in entries per particle (1..4096, default 96; the
- * pool grows past it, keeping the lists written, when a round runs out). */
+ * pool grows past it, keeping the lists written, when a round runs out);
+ * "kdk_fused_max_n" (0..256, default 256) and "kdk_coop_max_n" (0..2040,
+ * default 2040) = the largest systems gravis_kdk(_device) steps in one
+ * workgroup and in one cooperative launch with FP64 brute force (0
+ * disables). */
 gravis_status gravis_set_tuning(gravis_ctx* ctx, const char* key, int64_t value);
 
 /* Launch shapes of the context (no reference counterpart). Brute force:
