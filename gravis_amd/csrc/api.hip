@@ -388,9 +388,12 @@ gravis_status gravis_kdk_device(gravis_ctx* g, const gravis_solver* s, size_t n,
         // call synchronises it)
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         GRAVIS_HIP(c, hipStreamIsCapturing(c->stream, &cs));
-        if (cs == hipStreamCaptureStatusNone)
-            return gravis::kdk_coop(c, n, x, y, z, vx, vy, vz, m, ax, ay, az, s->softening, dt,
-                                    nsteps);
+        if (cs == hipStreamCaptureStatusNone) {
+            // steps the runtime would not run cooperatively continue below
+            GRAVIS_TRY(gravis::kdk_coop(c, n, x, y, z, vx, vy, vz, m, ax, ay, az, s->softening,
+                                        dt, &nsteps));
+            if (nsteps == 0) return GRAVIS_OK;
+        }
     }
     for (uint64_t step = 0; step < nsteps; ++step) {
         GRAVIS_TRY(gravis::kick_drift(c, n, x, y, z, vx, vy, vz, ax, ay, az, h, dt));

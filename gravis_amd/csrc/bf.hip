@@ -923,8 +923,9 @@ gravis_status kdk_small(Ctx* c, size_t n, double* x, double* y, double* z, doubl
 
 gravis_status kdk_coop(Ctx* c, size_t n, double* x, double* y, double* z, double* vx,
                        double* vy, double* vz, const double* m, double* ax, double* ay,
-                       double* az, double softening, double dt, uint64_t nsteps) {
+                       double* az, double softening, double dt, uint64_t* nsteps_left) {
     if (n == 0 || n > (size_t)kCoopMaxN) return fail(c, GRAVIS_INTERNAL, "kdk_coop: n = %zu", n);
+    if (*nsteps_left == 0) return GRAVIS_OK;
     const int ni = (int)n;
     const double eps2 = softening * softening;
     const double h = 0.5 * dt;
@@ -944,8 +945,8 @@ gravis_status kdk_coop(Ctx* c, size_t n, double* x, double* y, double* z, double
         return fail(c, GRAVIS_INTERNAL, "kdk_coop: %d workgroups, %d resident", grid,
                     per_cu * c->num_cus);
     // launches of at most 2^20 steps each (the arrival count stays below 2^32)
-    while (nsteps > 0) {
-        uint32_t k = (uint32_t)std::min<uint64_t>(nsteps, 1u << 20);
+    while (*nsteps_left > 0) {
+        uint32_t k = (uint32_t)std::min<uint64_t>(*nsteps_left, 1u << 20);
         coop_reset_kernel<<<1, kWave, 0, c->stream>>>(bar);
         GRAVIS_TRY(check_launch(c, "coop_reset"));
         int na = ni;
@@ -957,8 +958,15 @@ gravis_status kdk_coop(Ctx* c, size_t n, double* x, double* y, double* z, double
         uint32_t* pbar = bar;
         void* args[] = {&na, &px, &py, &pz, &pvx, &pvy, &pvz, &pm, &pax,
                         &pay, &paz, &e2, &hh, &dd, &k, &pb, &pbar};
-        GRAVIS_HIP(c, hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kCoopThreads), args,
-                                                 (unsigned)lds, c->stream));
+        // the runtime may refuse a cooperative launch (for one, while another
+        // process holds the device's cooperative-launch resources): the
+        // state is then untouched and the caller steps the rest launch by
+        // launch
+        if (hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kCoopThreads), args, (unsigned)lds,
+                                       c->stream) != hipSuccess) {
+            (void)hipGetLastError();
+            return GRAVIS_OK;
+        }
         uint32_t abort_word = 0;
         GRAVIS_HIP(c, hipMemcpyAsync(&abort_word, bar + 1, sizeof(uint32_t),
                                      hipMemcpyDeviceToHost, c->stream));
@@ -966,7 +974,7 @@ gravis_status kdk_coop(Ctx* c, size_t n, double* x, double* y, double* z, double
         if (abort_word != 0)
             return fail(c, GRAVIS_INTERNAL,
                         "kdk_coop: a grid barrier waited past its limit (%d workgroups)", grid);
-        nsteps -= k;
+        *nsteps_left -= k;
     }
     return GRAVIS_OK;
 }

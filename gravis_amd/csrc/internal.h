@@ -468,12 +468,14 @@ gravis_status kdk_small(Ctx* c, size_t n, double* x, double* y, double* z, doubl
 // Particles up to which a FP64 brute-force KDK of several steps runs all of
 // them in one cooperative launch with a grid barrier per step (kdk_coop,
 // bf.hip; the step's records are held in LDS, 32 B each, with the
-// kernel's 4-byte flag within 64 KB). The call
-// synchronises the stream to read the barrier's abort word.
+// kernel's 4-byte flag within 64 KB). The call synchronises the stream to
+// read the barrier's abort word. *nsteps_left counts down the steps done;
+// if the runtime refuses the cooperative launch it returns GRAVIS_OK with
+// steps left, for the caller to run launch by launch.
 constexpr int kCoopMaxN = 2040;
 gravis_status kdk_coop(Ctx* c, size_t n, double* x, double* y, double* z, double* vx,
                        double* vy, double* vz, const double* m, double* ax, double* ay,
-                       double* az, double softening, double dt, uint64_t nsteps);
+                       double* az, double softening, double dt, uint64_t* nsteps_left);
 
 gravis_status bf_force(Ctx* c, const gravis_solver* s, size_t n, const double* x,
                        const double* y, const double* z, const double* m, const FinishArgs& fa);

@@ -136,7 +136,9 @@ gravis_status gravis_kdk(gravis_ctx* ctx, const gravis_solver* solver, size_t n,
  * and, for several steps of 256 < n <= "kdk_coop_max_n" (default 2040) on a
  * stream that is not being captured, in one cooperative launch with a grid
  * barrier per step (forces summed in another order, so they differ from the
- * step-by-step path in rounding); that call synchronises the stream. */
+ * step-by-step path in rounding); that call synchronises the stream, and
+ * if the runtime refuses the cooperative launch the steps run launch by
+ * launch. */
 gravis_status gravis_kdk_device(gravis_ctx* ctx, const gravis_solver* solver, size_t n,
                                 double* x, double* y, double* z, double* vx, double* vy,
                                 double* vz, const double* m, double* ax, double* ay, double* az,
