@@ -363,10 +363,13 @@ void timer_end(Ctx* c, const char* name);
 // (cx, cy, cz, half_width) on the device.
 gravis_status root_box(Ctx* c, size_t n, const double* x, const double* y, const double* z,
                        double* d_box);
-// 63-bit Morton keys over d_box and the stable sort permutation.
-// Outputs: d_keys_sorted[n], d_perm[n] (sorted position -> original index).
+// Morton keys over d_box (3 * levels bits: the top `levels` of the 21
+// levels per axis; 21 gives the octree's 63-bit keys) and the stable sort
+// permutation. Outputs: d_keys_sorted[n], d_perm[n] (sorted position ->
+// original index).
 gravis_status morton_sort(Ctx* c, size_t n, const double* x, const double* y, const double* z,
-                          const double* d_box, uint64_t** d_keys_sorted, uint32_t** d_perm);
+                          const double* d_box, uint64_t** d_keys_sorted, uint32_t** d_perm,
+                          int levels = 21);
 // Stable LSD radix sort of n 64-bit keys with u32 values over the low
 // `bits` bits. The caller fills the buffers of sort_buffers first; the
 // sorted keys and values are returned (either set of ping-pong buffers).

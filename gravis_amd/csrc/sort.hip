@@ -109,12 +109,14 @@ __device__ __forceinline__ uint32_t quantise(double v, double lo, double inv) {
     return static_cast<uint32_t>(q);
 }
 
+// 63-bit keys (21 levels per axis) shifted right by `shift` (a multiple of
+// 3: the key of the coarser grid of 21 - shift / 3 levels).
 __global__ __launch_bounds__(256) void morton_kernel(const double* __restrict__ x,
                                                      const double* __restrict__ y,
                                                      const double* __restrict__ z, size_t n,
                                                      const double* __restrict__ box,
                                                      uint64_t* __restrict__ keys,
-                                                     uint32_t* __restrict__ vals) {
+                                                     uint32_t* __restrict__ vals, int shift) {
     const double hw = box[3];
     const double inv = 2097152.0 / (2.0 * hw);
     const double lx = box[0] - hw, ly = box[1] - hw, lz = box[2] - hw;
@@ -122,7 +124,8 @@ __global__ __launch_bounds__(256) void morton_kernel(const double* __restrict__ 
          i += (size_t)gridDim.x * blockDim.x) {
         uint32_t qx = quantise(x[i], lx, inv), qy = quantise(y[i], ly, inv),
                  qz = quantise(z[i], lz, inv);
-        keys[i] = morton_spread3(qx) | (morton_spread3(qy) << 1) | (morton_spread3(qz) << 2);
+        keys[i] = (morton_spread3(qx) | (morton_spread3(qy) << 1) | (morton_spread3(qz) << 2)) >>
+                  shift;
         vals[i] = static_cast<uint32_t>(i);
     }
 }
@@ -406,12 +409,15 @@ gravis_status radix_sort(Ctx* c, size_t n, int bits, uint64_t** d_keys_sorted,
 }
 
 gravis_status morton_sort(Ctx* c, size_t n, const double* x, const double* y, const double* z,
-                          const double* d_box, uint64_t** d_keys_sorted, uint32_t** d_perm) {
+                          const double* d_box, uint64_t** d_keys_sorted, uint32_t** d_perm,
+                          int levels) {
+    if (levels < 1 || levels > 21) return fail(c, GRAVIS_INTERNAL, "morton_sort: %d levels", levels);
     uint64_t* k0;
     uint32_t* v0;
     GRAVIS_TRY(sort_buffers(c, n, &k0, &v0));
-    morton_kernel<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(x, y, z, n, d_box, k0, v0);
-    return radix_sort(c, n, 63, d_keys_sorted, d_perm);
+    morton_kernel<<<grid_for(n, 256, 8192), 256, 0, c->stream>>>(x, y, z, n, d_box, k0, v0,
+                                                                 3 * (21 - levels));
+    return radix_sort(c, n, 3 * levels, d_keys_sorted, d_perm);
 }
 
 gravis_status gather_sorted(Ctx* c, size_t n, const double* x, const double* y,
