@@ -253,11 +253,17 @@ __global__ __launch_bounds__(kRsBlock) void radix_scatter_kernel(
     // round). Rounds are not unrolled: a round's ballots are used at once
     // (unrolled, the masks of every round stayed live and the kernel ran
     // at one wavefront per SIMD).
+    // Each round's key is loaded one round ahead (a tile of few blocks has
+    // no other wavefronts to cover the load latency).
+    uint64_t knext = wbase + lane < n ? kin[wbase + lane] : 0ull;
 #pragma unroll 1
     for (int r = 0; r < kRsRounds; ++r) {
         const size_t i = wbase + (size_t)r * kWave + lane;
+        const uint64_t kcur = knext;
+        const size_t inext = i + kWave;
+        knext = r + 1 < kRsRounds && inext < n ? kin[inext] : 0ull;
         const uint64_t active = __ballot(i < n);
-        const uint32_t dig = i < n ? static_cast<uint32_t>((kin[i] >> shift) & kRsMask) : 0u;
+        const uint32_t dig = i < n ? static_cast<uint32_t>((kcur >> shift) & kRsMask) : 0u;
         const uint64_t mask = digit_match(dig, active);
         if (i < n && (mask & lt) == 0ull) cnt[w][dig] += (uint32_t)__popcll(mask);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -297,17 +303,24 @@ __global__ __launch_bounds__(kRsBlock) void radix_scatter_kernel(
     __syncthreads();
     // Phase C: stable ranks in element order, into LDS (keys read again:
     // the tile is in L2).
+    knext = wbase + lane < n ? kin[wbase + lane] : 0ull;
+    uint32_t vnext = wbase + lane < n ? vin[wbase + lane] : 0u;
 #pragma unroll 1
     for (int r = 0; r < kRsRounds; ++r) {
         const size_t i = wbase + (size_t)r * kWave + lane;
+        const uint64_t key = knext;
+        const uint32_t val = vnext;
+        const size_t inext = i + kWave;
+        const bool more = r + 1 < kRsRounds && inext < n;
+        knext = more ? kin[inext] : 0ull;
+        vnext = more ? vin[inext] : 0u;
         const uint64_t active = __ballot(i < n);
-        const uint64_t key = i < n ? kin[i] : 0ull;
         const uint32_t dig = static_cast<uint32_t>((key >> shift) & kRsMask);
         const uint64_t mask = digit_match(dig, active);
         if (i < n) {
             const uint32_t lp = cnt[w][dig] + (uint32_t)__popcll(mask & lt);
             lkey[lp] = key;
-            lval[lp] = vin[i];
+            lval[lp] = val;
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
         __builtin_amdgcn_wave_barrier();
