@@ -950,9 +950,8 @@ gravis_status kdk_coop(Ctx* c, size_t n, double* x, double* y, double* z, double
     int per_cu = 0;
     GRAVIS_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kCoopThreads, lds));
     const int grid = (ni + kCoopWaves - 1) / kCoopWaves;
-    if (grid > per_cu * c->num_cus)
-        return fail(c, GRAVIS_INTERNAL, "kdk_coop: %d workgroups, %d resident", grid,
-                    per_cu * c->num_cus);
+    // a grid that cannot be resident at once leaves the steps to the caller
+    if (grid > per_cu * c->num_cus) return GRAVIS_OK;
     // launches of at most 2^20 steps each (the arrival count stays below 2^32)
     while (*nsteps_left > 0) {
         uint32_t k = (uint32_t)std::min<uint64_t>(*nsteps_left, 1u << 20);
