@@ -1061,9 +1061,8 @@ size_t orc_bh_max_frontier(void) {
 /* Target groups (at most `group` particles each, contiguous in sorted order):
  * for every internal node holding more than `group` particles, its children
  * are visited in order and consecutive children holding at most `group`
- * particles are packed greedily while the total stays within `group` and
- * the packed particles' bounding box is no wider (largest axis extent) than
- * twice the widest packed child's; a child leaf holding more than `group`
+ * particles are packed greedily while the total stays within `group`; a
+ * child leaf holding more than `group`
  * particles (coincident points at level 21) is cut into runs of `group`; a
  * root holding at most `group` particles is one group. Every particle lies
  * in exactly one group. Group order does not affect any result. Returns the
@@ -1096,29 +1095,18 @@ static size_t lt_groups(const ltree* t, uint32_t group, uint32_t** gf, uint32_t*
         for (size_t p = 0; p < t->len; ++p) {
             if (t->cnt[p] <= group || t->nchild[p] == 0) continue;
             uint32_t rf = 0, rc = 0;
-            double rlo[3], rhi[3], rmax = 0.0;
             for (uint32_t q = 0; q < t->nchild[p]; ++q) {
                 uint32_t c = t->child[p] + q;
                 if (t->cnt[c] <= group) {
-                    const double* clo = t->blo + 3 * c;
-                    const double* chi = t->bhi + 3 * c;
-                    double cext = fmax(fmax(chi[0] - clo[0], chi[1] - clo[1]), chi[2] - clo[2]);
                     if (rc > 0) {
-                        double ulo[3], uhi[3];
-                        for (int d = 0; d < 3; ++d) { ulo[d] = fmin(rlo[d], clo[d]); uhi[d] = fmax(rhi[d], chi[d]); }
-                        double uext = fmax(fmax(uhi[0] - ulo[0], uhi[1] - ulo[1]), uhi[2] - ulo[2]);
-                        if (rc + t->cnt[c] <= group && uext <= 2.0 * fmax(rmax, cext)) {
+                        if (rc + t->cnt[c] <= group) {
                             rc += t->cnt[c];
-                            for (int d = 0; d < 3; ++d) { rlo[d] = ulo[d]; rhi[d] = uhi[d]; }
-                            rmax = fmax(rmax, cext);
                             continue;
                         }
                         gl_push(&g, rf, rc);
                     }
                     rf = t->first[c];
                     rc = t->cnt[c];
-                    for (int d = 0; d < 3; ++d) { rlo[d] = clo[d]; rhi[d] = chi[d]; }
-                    rmax = cext;
                 } else {
                     if (rc > 0) gl_push(&g, rf, rc);
                     rc = 0;
