@@ -407,8 +407,9 @@ gravis_status gravis_get_tuning(gravis_ctx* ctx, gravis_tuning* out);
 /* Autotuner (no reference counterpart): times the candidate launch shapes
  * for solver s on the given particles (host arrays, copied once) and keeps
  * the chosen one in the context. Mixed brute force: bf_waves_per_cu
- * 64..512 x (bf_ipt 2, bf_ipt 1, bf_matrix 1, bf_matrix 2) x bf_tile (32
- * shapes); FP64 brute force: bf_waves_per_cu 32..512. Each brute-force
+ * 64, 128, 256, 512, 768, 1024 x (bf_ipt 2, bf_ipt 1, bf_matrix 1,
+ * bf_matrix 2) x bf_tile (48 shapes); FP64 brute force: bf_waves_per_cu
+ * 32..1024 (7 values). Each brute-force
  * shape is timed once (a warm-up and two calls); the four fastest are timed
  * again in three interleaved rounds of four calls and the fastest total is
  * kept, a shape later in that order (more j chunks, i.e. more partial-sum
