@@ -366,7 +366,7 @@ def test_autotune_brute_force_and_barnes_hut(g):
     t = ctx.autotune(g.BruteForce(0.01, "mixed", ctx=ctx), to_particles(g, s))
     say(f"  mixed: {t}")
     assert t["bf_ipt"] in (1, 2) and t["bf_tile"] in (128, 256) and t["bf_matrix"] in (0, 1, 2)
-    assert t["bf_waves_per_cu"] in (64, 128, 256, 512) and t["best_ms"] > 0
+    assert t["bf_waves_per_cu"] in (64, 128, 256, 512, 768, 1024) and t["best_ms"] > 0
     assert ctx.tuning() == {k: v for k, v in t.items() if k != "best_ms"}
     pm = to_particles(g, s)
     g.BruteForce(0.01, "mixed", ctx=ctx).compute_accelerations(pm)
@@ -374,7 +374,7 @@ def test_autotune_brute_force_and_barnes_hut(g):
     assert rms <= 1e-6 and mx <= 1e-4
     t = ctx.autotune(g.BruteForce(0.01, ctx=ctx), to_particles(g, s))
     say(f"  fp64: {t}")
-    assert t["bf_waves_per_cu"] in (32, 64, 128, 256, 512)
+    assert t["bf_waves_per_cu"] in (32, 64, 128, 256, 512, 768, 1024)
     t = ctx.autotune(g.BarnesHut(0.01, 0.5, ctx=ctx), to_particles(g, s))
     say(f"  bh: {t}")
     assert t["bh_group_size"] in (16, 32, 64)
