@@ -447,7 +447,7 @@ gravis_status bh_force(Ctx* c, const gravis_solver* s, size_t n, const double* x
     const uint32_t leaf = s->leaf_size ? s->leaf_size : 16u;
     const uint32_t gs = s->group_size ? s->group_size : c->bh_group;
     Octree t;
-    GRAVIS_TRY(build_octree(c, n, x, y, z, m, leaf, gs, &t));
+    GRAVIS_TRY(build_octree(c, n, x, y, z, m, leaf, gs, &t, true));
     return bh_walk(c, s, t, fa);
 }
 

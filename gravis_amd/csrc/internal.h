@@ -427,8 +427,11 @@ struct Octree {
 // Root cube, Morton sort and level-by-level build of the tree of n
 // particles (device SoA), asynchronous on the context stream. The buffers
 // are the context's and are reused by the next build.
+// split_siblings: target groups cut from each run of consecutive children
+// of at most `group` particles (else whole children packed while they fit).
 gravis_status build_octree(Ctx* c, size_t n, const double* x, const double* y, const double* z,
-                           const double* m, uint32_t leaf, uint32_t group, Octree* t);
+                           const double* m, uint32_t leaf, uint32_t group, Octree* t,
+                           bool split_siblings = false);
 
 // ---- forces (bf.hip, bh.hip) --------------------------------------------
 // How the finishing kernel writes the result in the caller's order.

@@ -216,7 +216,10 @@ __global__ __launch_bounds__(kTreeBlock) void tree_com_kernel(double4* __restric
 // children of every internal node holding more than gsize particles are
 // visited in order and consecutive children holding at most gsize are packed
 // greedily while the total stays within gsize (they lie in their parent's
-// cell, so a group stays within one cell); a child leaf holding more
+// cell, so a group stays within one cell) or, with `split`, each maximal
+// run of consecutive such children is cut into runs of gsize particles
+// (fuller wavefronts; the Barnes-Hut gravity path, where it measured faster,
+// while the SPH neighbour search keeps whole children); a child leaf holding more
 // (coincident points at level 21) is cut into runs of gsize; a root holding
 // at most gsize (or a root leaf) is cut the same way. Appended in any order:
 // each group's result depends only on its own particles and the tree.
@@ -240,7 +243,8 @@ __device__ __forceinline__ uint32_t emit_runs(uint2* groups, uint32_t at, uint32
 template <bool EMIT>
 __device__ __forceinline__ uint32_t node_groups(uint32_t p, const uint4 md,
                                                 const uint4* __restrict__ meta, uint32_t gsize,
-                                                uint2* __restrict__ groups, uint32_t at) {
+                                                bool split, uint2* __restrict__ groups,
+                                                uint32_t at) {
     const uint32_t nc = (md.w >> 8) & 255u;
     if (p == 0 && (md.y <= gsize || nc == 0))
         return emit_runs<EMIT>(groups, at, md.x, md.y, gsize);
@@ -251,6 +255,13 @@ __device__ __forceinline__ uint32_t node_groups(uint32_t p, const uint4 md,
         const uint4 cm = meta[md.z + q];
         const uint32_t cc = (cm.w >> 8) & 255u;
         if (cm.y <= gsize) {
+            if (split) {
+                // the segment of consecutive small children grows; it is cut
+                // into runs of gsize when it ends
+                if (rc == 0) rf = cm.x;
+                rc += cm.y;
+                continue;
+            }
             if (rc > 0) {
                 if (rc + cm.y <= gsize) {
                     rc += cm.y;
@@ -275,7 +286,7 @@ __device__ __forceinline__ uint32_t node_groups(uint32_t p, const uint4 md,
 // across each block (every lane takes part in the scan).
 __global__ __launch_bounds__(kTreeBlock) void group_emit_kernel(const uint4* __restrict__ meta,
                                                                const uint32_t* __restrict__ lvl,
-                                                               uint32_t gsize,
+                                                               uint32_t gsize, bool split,
                                                                uint2* __restrict__ groups,
                                                                uint32_t* __restrict__ ngroups) {
     const uint32_t total = lvl[2 * (kMaxLevel + 1)];
@@ -284,7 +295,7 @@ __global__ __launch_bounds__(kTreeBlock) void group_emit_kernel(const uint4* __r
         const uint32_t p = t0 + threadIdx.x;
         const bool valid = p < total;
         const uint4 md = valid ? meta[p] : make_uint4(0u, 0u, 0u, 0u);
-        const uint32_t k = valid ? node_groups<false>(p, md, meta, gsize, groups, 0u)
+        const uint32_t k = valid ? node_groups<false>(p, md, meta, gsize, split, groups, 0u)
                                  : 0u;
         const uint32_t inc = wave_inclusive_scan(k);
         const uint32_t wtot = __builtin_amdgcn_readlane(inc, kWave - 1);
@@ -293,14 +304,15 @@ __global__ __launch_bounds__(kTreeBlock) void group_emit_kernel(const uint4* __r
             if (lane == 0) base = atomicAdd(ngroups, wtot);
             base = __builtin_amdgcn_readfirstlane(base);
         }
-        if (k > 0) (void)node_groups<true>(p, md, meta, gsize, groups, base + inc - k);
+        if (k > 0) (void)node_groups<true>(p, md, meta, gsize, split, groups, base + inc - k);
     }
 }
 
 }  // namespace
 
 gravis_status build_octree(Ctx* c, size_t n, const double* x, const double* y, const double* z,
-                           const double* m, uint32_t leaf, uint32_t group, Octree* t) {
+                           const double* m, uint32_t leaf, uint32_t group, Octree* t,
+                           bool split_siblings) {
     if (n == 0) return fail(c, GRAVIS_INVALID_ARG, "octree of zero particles");
     if (n > 0x7FFFFFFFull) return fail(c, GRAVIS_INVALID_ARG, "n too large");
     if (leaf < 1 || leaf > 4096) return fail(c, GRAVIS_INVALID_ARG, "leaf_size must be 1..4096");
@@ -349,7 +361,8 @@ gravis_status build_octree(Ctx* c, size_t n, const double* x, const double* y, c
     for (int L = kMaxLevel; L >= 0; --L)
         tree_sums_kernel<<<tg, kTreeBlock, 0, st>>>(src, meta, lvl, L, sums, bmin, bmax);
     tree_com_kernel<<<tg, kTreeBlock, 0, st>>>(src, (int)n, meta, lvl, sums);
-    group_emit_kernel<<<tg, kTreeBlock, 0, st>>>(meta, lvl, group, groups, ngroups);
+    group_emit_kernel<<<tg, kTreeBlock, 0, st>>>(meta, lvl, group, split_siblings, groups,
+                                                 ngroups);
     timer_end(c, "tree_build");
     GRAVIS_TRY(check_launch(c, "tree sums and groups"));
 

@@ -61,7 +61,7 @@ def lib():
             "orc_sort_keys": (None, [_sz, _u64p, _u64p, _u32p]),
             "orc_linear_tree": (_sz, [_sz] + [_dp] * 4 + [_u32, _sz, _u64p, _u32p] + [_u32p] * 5
                                 + [_dp, _dp]),
-            "orc_bh_group_accel": (_u64, [_sz, _d, _d, _u32, _u32, _u32] + [_dp] * 7 + [_i]),
+            "orc_bh_group_accel": (_u64, [_sz, _d, _d, _u32, _u32, _u32, _i] + [_dp] * 7 + [_i]),
             "orc_kdk_steps": (None, [_sz, _i, _d, _d] + [_dp] * 10 + [_d, _u64, _i]),
             "orc_run": (_sz, [_sz, _i, _d, _d] + [_dp] * 7 + [_d, _u64, _u64, _dp, _sz, _i]),
             "orc_galaxy_collision": (None, [C.c_void_p, _i] + [_dp] * 7 + [C.c_void_p]),
@@ -200,9 +200,12 @@ FRONTIER_CAP = 4096  # the device's group-walk frontier capacity (kGroupFrontier
 
 
 def bh_group_accel(s: State, eps: float, theta: float, leaf: int = 16, group: int = 64,
-                   nthreads: int = 0, frontier_cap: int = FRONTIER_CAP) -> int:
-    return lib().orc_bh_group_accel(s.n, eps, theta, leaf, group, frontier_cap, s.x, s.y, s.z,
-                                    s.m, s.ax, s.ay, s.az, nthreads)
+                   nthreads: int = 0, frontier_cap: int = FRONTIER_CAP, split: bool = True) -> int:
+    """The device's group walk restated. split=True: the target groups of its
+    Barnes-Hut gravity path (runs of consecutive small children cut into
+    runs of `group`); False: those of its SPH step (whole children packed)."""
+    return lib().orc_bh_group_accel(s.n, eps, theta, leaf, group, frontier_cap, int(split),
+                                    s.x, s.y, s.z, s.m, s.ax, s.ay, s.az, nthreads)
 
 
 def root_box(x, y, z) -> np.ndarray:

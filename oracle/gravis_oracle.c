@@ -1061,8 +1061,9 @@ size_t orc_bh_max_frontier(void) {
 /* Target groups (at most `group` particles each, contiguous in sorted order):
  * for every internal node holding more than `group` particles, its children
  * are visited in order and consecutive children holding at most `group`
- * particles are packed greedily while the total stays within `group`; a
- * child leaf holding more than `group`
+ * particles are packed greedily while the total stays within `group` or,
+ * with `split`, each maximal run of such consecutive children is cut into
+ * runs of `group` particles; a child leaf holding more than `group`
  * particles (coincident points at level 21) is cut into runs of `group`; a
  * root holding at most `group` particles is one group. Every particle lies
  * in exactly one group. Group order does not affect any result. Returns the
@@ -1087,7 +1088,8 @@ static void gl_runs(glist* g, uint32_t first, uint32_t cnt, uint32_t group) {
     for (uint32_t b = 0; b < cnt; b += group) gl_push(g, first + b, cnt - b < group ? cnt - b : group);
 }
 
-static size_t lt_groups(const ltree* t, uint32_t group, uint32_t** gf, uint32_t** gc) {
+static size_t lt_groups(const ltree* t, uint32_t group, int split, uint32_t** gf,
+                        uint32_t** gc) {
     glist g = {0};
     if (t->cnt[0] <= group || t->nchild[0] == 0) {
         gl_runs(&g, t->first[0], t->cnt[0], group);
@@ -1098,6 +1100,11 @@ static size_t lt_groups(const ltree* t, uint32_t group, uint32_t** gf, uint32_t*
             for (uint32_t q = 0; q < t->nchild[p]; ++q) {
                 uint32_t c = t->child[p] + q;
                 if (t->cnt[c] <= group) {
+                    if (split) {
+                        if (rc == 0) rf = t->first[c];
+                        rc += t->cnt[c];
+                        continue;
+                    }
                     if (rc > 0) {
                         if (rc + t->cnt[c] <= group) {
                             rc += t->cnt[c];
@@ -1108,12 +1115,12 @@ static size_t lt_groups(const ltree* t, uint32_t group, uint32_t** gf, uint32_t*
                     rf = t->first[c];
                     rc = t->cnt[c];
                 } else {
-                    if (rc > 0) gl_push(&g, rf, rc);
+                    if (rc > 0) gl_runs(&g, rf, rc, group);
                     rc = 0;
                     if (t->nchild[c] == 0) gl_runs(&g, t->first[c], t->cnt[c], group);
                 }
             }
-            if (rc > 0) gl_push(&g, rf, rc);
+            if (rc > 0) gl_runs(&g, rf, rc, group);
         }
     }
     *gf = g.f;
@@ -1125,9 +1132,9 @@ static size_t lt_groups(const ltree* t, uint32_t group, uint32_t** gf, uint32_t*
  * Accumulates into ax/ay/az in the caller's order. Returns the total number
  * of pair interactions evaluated (targets x sources). */
 static uint64_t bh_group(size_t n, double eps, const double* eps_i, double theta, uint32_t leaf,
-                         uint32_t group, uint32_t fcap, const double* x, const double* y,
-                         const double* z, const double* m, double* ax, double* ay, double* az,
-                         int nthreads) {
+                         uint32_t group, uint32_t fcap, int split, const double* x,
+                         const double* y, const double* z, const double* m, double* ax,
+                         double* ay, double* az, int nthreads) {
     if (n == 0) return 0;
     double box[4];
     orc_root_box(n, x, y, z, box);
@@ -1149,7 +1156,7 @@ static uint64_t bh_group(size_t n, double eps, const double* eps_i, double theta
     double side0 = 2.0 * box[3];
     double eps2 = eps * eps, theta2 = theta * theta;
     uint32_t *gfirst, *gcnt;
-    size_t ng = lt_groups(&t, group, &gfirst, &gcnt);
+    size_t ng = lt_groups(&t, group, split, &gfirst, &gcnt);
     uint64_t total = 0;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -1179,10 +1186,14 @@ static uint64_t bh_group(size_t n, double eps, const double* eps_i, double theta
     return total;
 }
 
+/* split = 1: the target groups of the device's Barnes-Hut gravity path,
+ * 0: those of its SPH step (see lt_groups). */
 uint64_t orc_bh_group_accel(size_t n, double eps, double theta, uint32_t leaf, uint32_t group,
-                            uint32_t fcap, const double* x, const double* y, const double* z,
-                            const double* m, double* ax, double* ay, double* az, int nthreads) {
-    return bh_group(n, eps, NULL, theta, leaf, group, fcap, x, y, z, m, ax, ay, az, nthreads);
+                            uint32_t fcap, int split, const double* x, const double* y,
+                            const double* z, const double* m, double* ax, double* ay, double* az,
+                            int nthreads) {
+    return bh_group(n, eps, NULL, theta, leaf, group, fcap, split, x, y, z, m, ax, ay, az,
+                    nthreads);
 }
 
 /* The same walk with Plummer softening eps_i[i] of each TARGET i
@@ -1191,7 +1202,8 @@ uint64_t orc_bh_group_accel_eps(size_t n, const double* eps_i, double theta, uin
                                 uint32_t group, uint32_t fcap, const double* x, const double* y,
                                 const double* z, const double* m, double* ax, double* ay,
                                 double* az, int nthreads) {
-    return bh_group(n, 0.0, eps_i, theta, leaf, group, fcap, x, y, z, m, ax, ay, az, nthreads);
+    return bh_group(n, 0.0, eps_i, theta, leaf, group, fcap, 0, x, y, z, m, ax, ay, az,
+                    nthreads);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1205,7 +1217,7 @@ static void orc_force(int algo, size_t n, double eps, double theta, const double
     for (size_t i = 0; i < n; ++i) { ax[i] = 0.0; ay[i] = 0.0; az[i] = 0.0; }
     if (algo == ORC_BF) orc_bf_accel(n, eps, x, y, z, m, ax, ay, az);
     else if (algo == ORC_BH_REF) orc_bh_accel(n, eps, theta, x, y, z, m, ax, ay, az, nthreads);
-    else orc_bh_group_accel(n, eps, theta, 16, 64, 16384, x, y, z, m, ax, ay, az, nthreads);
+    else orc_bh_group_accel(n, eps, theta, 16, 64, 16384, 1, x, y, z, m, ax, ay, az, nthreads);
 }
 
 void orc_kdk_steps(size_t n, int algo, double eps, double theta, double* x, double* y,

@@ -25,8 +25,9 @@ void orc_bh_accel(size_t n, double eps, double theta, const double* x, const dou
                   const double* z, const double* m, double* ax, double* ay, double* az,
                   int nthreads);
 uint64_t orc_bh_group_accel(size_t n, double eps, double theta, uint32_t leaf, uint32_t group,
-                            uint32_t fcap, const double* x, const double* y, const double* z,
-                            const double* m, double* ax, double* ay, double* az, int nthreads);
+                            uint32_t fcap, int split, const double* x, const double* y,
+                            const double* z, const double* m, double* ax, double* ay, double* az,
+                            int nthreads);
 uint64_t orc_bh_group_accel_eps(size_t n, const double* eps_i, double theta, uint32_t leaf,
                                 uint32_t group, uint32_t fcap, const double* x, const double* y,
                                 const double* z, const double* m, double* ax, double* ay,

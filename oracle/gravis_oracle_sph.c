@@ -657,7 +657,7 @@ double orc_sph_step(orc_sph_state* s, const orc_sph_params* pp, int gravity, dou
     else if (gravity == 1) orc_bf_accel(n, eps, s->x, s->y, s->z, s->m, s->ax, s->ay, s->az);
     else if (gravity == 2) orc_bh_accel(n, eps, theta, s->x, s->y, s->z, s->m, s->ax, s->ay, s->az, 0);
     else if (gravity == 3)
-        orc_bh_group_accel(n, eps, theta, 16, 64, 4096, s->x, s->y, s->z, s->m, s->ax, s->ay,
+        orc_bh_group_accel(n, eps, theta, 16, 64, 4096, 0, s->x, s->y, s->z, s->m, s->ax, s->ay,
                            s->az, 0);
     double* tmp = (double*)malloc(3 * n * sizeof(double));
     orc_sph_compute(s, pp, tmp, tmp + n, tmp + 2 * n, NULL, NULL);
