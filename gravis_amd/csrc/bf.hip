@@ -947,10 +947,15 @@ gravis_status kdk_coop(Ctx* c, size_t n, double* x, double* y, double* z, double
     const void* fn = mask ? reinterpret_cast<const void*>(&kdk_coop_kernel<true>)
                           : reinterpret_cast<const void*>(&kdk_coop_kernel<false>);
     // one workgroup per kCoopWaves targets, every one resident at once
+    // a grid that cannot be resident at once (or an occupancy query that
+    // fails) leaves the steps to the caller
     int per_cu = 0;
-    GRAVIS_HIP(c, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kCoopThreads, lds));
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kCoopThreads, lds) !=
+        hipSuccess) {
+        (void)hipGetLastError();
+        return GRAVIS_OK;
+    }
     const int grid = (ni + kCoopWaves - 1) / kCoopWaves;
-    // a grid that cannot be resident at once leaves the steps to the caller
     if (grid > per_cu * c->num_cus) return GRAVIS_OK;
     // launches of at most 2^20 steps each (the arrival count stays below 2^32)
     while (*nsteps_left > 0) {
